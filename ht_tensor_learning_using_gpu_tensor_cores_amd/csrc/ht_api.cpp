@@ -1,0 +1,606 @@
+// C ABI of the HT engine (declarations in include/ht_amd.h and include/ht_amd_debug.h).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/ht_amd_debug.h"
+#include "ht_engine.h"
+
+using namespace ht;
+
+#define CKS(expr)        \
+  do {                   \
+    int _s = (expr);     \
+    if (_s) return (ht_status)_s; \
+  } while (0)
+
+extern "C" {
+
+const char* ht_version(void) { return "ht_amd 0.1 (gfx950)"; }
+
+void ht_default_options(ht_options* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->method = HT_METHOD_LTR;
+  o->precision = HT_PREC_BF16;
+  o->oversample = 8;
+  o->power_iters = 1;
+  o->err_mode = HT_ERR_MODE_RESIDUAL;
+  o->max_eig_iters = 200;
+  o->eig_tol = 1e-10;
+  o->seed = 1234;
+  o->want_interior_u = 0;
+  o->check_finite = 1;
+}
+
+ht_status ht_create(ht_handle_t* handle, int32_t device) {
+  if (!handle) return HT_ERR_INVALID_ARG;
+  *handle = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return HT_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return HT_ERR_INVALID_ARG;
+  if (hipSetDevice(device) != hipSuccess) return HT_ERR_HIP;
+  ht_handle_s* h = new ht_handle_s();
+  h->device = device;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return HT_ERR_HIP;
+  }
+  for (auto& e : h->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete h;
+      return HT_ERR_HIP;
+    }
+  *handle = h;
+  return HT_OK;
+}
+
+ht_status ht_destroy(ht_handle_t h) {
+  if (!h) return HT_ERR_INVALID_ARG;
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  h->arena.release();
+  if (h->xbuf) (void)hipFree(h->xbuf);
+  if (h->obuf) (void)hipFree(h->obuf);
+  for (auto& e : h->ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return HT_OK;
+}
+
+void* ht_get_stream(ht_handle_t h) { return h ? (void*)h->stream : nullptr; }
+
+ht_status ht_validate_tree(int32_t order, int32_t n_nodes, const int32_t* lo, const int32_t* hi,
+                           const int32_t* left, const int32_t* right) {
+  return (ht_status)validate_tree(order, n_nodes, lo, hi, left, right);
+}
+
+ht_status ht_output_layout(int32_t order, const int64_t* dims, int32_t n_nodes, const int32_t* lo,
+                           const int32_t* hi, const int32_t* left, const int32_t* right, const int32_t* ranks,
+                           int32_t want_interior_u, int64_t* u_off, int64_t* b_off, int64_t* s_off,
+                           int64_t* total) {
+  Problem p;
+  CKS(p.setup(order, dims, n_nodes, lo, hi, left, right, ranks, want_interior_u));
+  for (int t = 0; t < n_nodes; ++t) {
+    if (u_off) u_off[t] = p.u_off[t];
+    if (b_off) b_off[t] = p.b_off[t];
+    if (s_off) s_off[t] = p.s_off[t];
+  }
+  if (total) *total = p.total;
+  return HT_OK;
+}
+
+static int ensure(float** buf, size_t* have, size_t bytes) {
+  if (*have >= bytes) return 0;
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *have = 0;
+  if (hipMalloc((void**)buf, bytes) != hipSuccess) return HT_ERR_ALLOC;
+  *have = bytes;
+  return 0;
+}
+
+ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32_t order, const int64_t* dims,
+                       int32_t n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                       const int32_t* right, const int32_t* ranks, const ht_options* opt_in, float* out,
+                       int32_t out_on_device, double* rel_err, ht_stats* stats) {
+  if (!h || !x) return HT_ERR_INVALID_ARG;
+  if (rel_err) *rel_err = NAN;
+  ht_options opt;
+  if (opt_in) opt = *opt_in;
+  else ht_default_options(&opt);
+  if (opt.method < 0 || opt.method > 2 || opt.precision < 0 || opt.precision > 2 || opt.oversample < 0 ||
+      opt.power_iters < 0 || opt.max_eig_iters < 1 || !(opt.eig_tol > 0))
+    return HT_ERR_INVALID_ARG;
+  Problem p;
+  CKS(p.setup(order, dims, n_nodes, lo, hi, left, right, ranks, opt.want_interior_u));
+  if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
+  const size_t xbytes = (size_t)p.N * sizeof(float);
+  float h2d_ms = 0.f;
+  const float* xd = x;
+  if (!x_on_device) {
+    CKS(ensure(&h->xbuf, &h->xbuf_bytes, xbytes));
+    if (hipEventRecord(h->ev[2], h->stream) != hipSuccess) return HT_ERR_HIP;
+    if (hipMemcpyAsync(h->xbuf, x, xbytes, hipMemcpyHostToDevice, h->stream) != hipSuccess) return HT_ERR_HIP;
+    if (hipEventRecord(h->ev[3], h->stream) != hipSuccess) return HT_ERR_HIP;
+    xd = h->xbuf;
+  }
+  float* od = out;
+  if (out && !out_on_device) {
+    CKS(ensure(&h->obuf, &h->obuf_bytes, (size_t)p.total * sizeof(float)));
+    od = h->obuf;
+  }
+  h->arena.reset();
+  Engine e;
+  e.st = h->stream;
+  e.arena = &h->arena;
+  e.prob = &p;
+  e.opt = opt;
+  if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
+  int s = e.run(xd, od);
+  if (s && s != HT_ERR_NOCONV) {
+    (void)hipStreamSynchronize(h->stream);
+    return (ht_status)s;
+  }
+  if (hipEventRecord(h->ev[1], h->stream) != hipSuccess) return HT_ERR_HIP;
+  if (hipEventSynchronize(h->ev[1]) != hipSuccess) return HT_ERR_HIP;
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+  if (!x_on_device) (void)hipEventElapsedTime(&h2d_ms, h->ev[2], h->ev[3]);
+  if (out && !out_on_device) {
+    if (hipMemcpy(out, od, (size_t)p.total * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return HT_ERR_HIP;
+  }
+  if (rel_err) *rel_err = e.rel_err;
+  if (stats) {
+    stats->total_ms = ms;
+    stats->h2d_ms = h2d_ms;
+    stats->error_ms = 0.0;
+    stats->bytes_x_passes = e.passes;
+    stats->eig_iters_max = e.eig_iters_max;
+    stats->eig_calls = e.eig_calls;
+  }
+  if (!std::isfinite(e.x_sq)) return HT_ERR_NONFINITE;
+  return (ht_status)s;
+}
+
+// Load an HT block from the packed layout into an engine (FP64 copies on the device).
+static int load_packed(Engine& e, const Problem& p, const float* packed) {
+  for (int t = 0; t < HT_MAX_NODES; ++t) {
+    e.Ufull[t] = nullptr;
+    e.Bd[t] = nullptr;
+    e.SVd[t] = nullptr;
+  }
+  for (int t = 0; t < p.n_nodes; ++t) {
+    if (p.left[t] < 0) {
+      const long n = p.rows[t] * p.rank[t];
+      e.Ufull[t] = (double*)e.arena->get(n * sizeof(double));
+      if (!e.Ufull[t]) return HT_ERR_ALLOC;
+      CKS(cast_f32_f64_launch(e.st, packed + p.u_off[t], e.Ufull[t], n));
+    } else {
+      const long n = (long)p.rank[p.left[t]] * p.rank[p.right[t]] * p.rank[t];
+      e.Bd[t] = (double*)e.arena->get(n * sizeof(double));
+      if (!e.Bd[t]) return HT_ERR_ALLOC;
+      CKS(cast_f32_f64_launch(e.st, packed + p.b_off[t], e.Bd[t], n));
+    }
+  }
+  return 0;
+}
+
+ht_status ht_residual(ht_handle_t h, const float* x, int32_t order, const int64_t* dims, int32_t n_nodes,
+                      const int32_t* lo, const int32_t* hi, const int32_t* left, const int32_t* right,
+                      const int32_t* ranks, int32_t want_interior_u, const float* packed, double* rel_err) {
+  if (!h || !x || !packed || !rel_err) return HT_ERR_INVALID_ARG;
+  Problem p;
+  CKS(p.setup(order, dims, n_nodes, lo, hi, left, right, ranks, want_interior_u));
+  if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
+  h->arena.reset();
+  Engine e;
+  e.st = h->stream;
+  e.arena = &h->arena;
+  e.prob = &p;
+  ht_default_options(&e.opt);
+  CKS(load_packed(e, p, packed));
+  double esq = 0, xsq = 0;
+  CKS(e.residual(x, &esq, &xsq));
+  *rel_err = xsq > 0 ? std::sqrt(esq / xsq) : 0.0;
+  return HT_OK;
+}
+
+ht_status ht_reconstruct(ht_handle_t h, int32_t order, const int64_t* dims, int32_t n_nodes, const int32_t* lo,
+                         const int32_t* hi, const int32_t* left, const int32_t* right, const int32_t* ranks,
+                         int32_t want_interior_u, const float* packed, float* x_out) {
+  if (!h || !packed || !x_out) return HT_ERR_INVALID_ARG;
+  Problem p;
+  CKS(p.setup(order, dims, n_nodes, lo, hi, left, right, ranks, want_interior_u));
+  if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
+  h->arena.reset();
+  Engine e;
+  e.st = h->stream;
+  e.arena = &h->arena;
+  e.prob = &p;
+  ht_default_options(&e.opt);
+  CKS(load_packed(e, p, packed));
+  const int l = p.left[0], r = p.right[0];
+  const long nl = p.rows[l], nr = p.rows[r];
+  const int kl = p.rank[l], kr = p.rank[r];
+  double *Ul, *Ur;
+  CKS(e.expand(l, &Ul));
+  CKS(e.expand(r, &Ur));
+  double* Rr = (double*)h->arena.get(sizeof(double) * nr * kl);
+  float* Lf = (float*)h->arena.get(sizeof(float) * nl * kl);
+  float* Rf = (float*)h->arena.get(sizeof(float) * nr * kl);
+  if (!Rr || !Lf || !Rf) return HT_ERR_ALLOC;
+  CKS(dgemm_small_launch(e.st, 0, 1, (int)nr, kl, kr, 1.0, Ur, (int)nr, e.Bd[0], kl, 0.0, Rr, (int)nr));
+  CKS(cast_f64_f32_launch(e.st, Ul, Lf, nl * kl));
+  CKS(cast_f64_f32_launch(e.st, Rr, Rf, nr * kl));
+  CKS(lowrank_launch(e.st, 0, Lf, Rf, nl, nr, kl, nullptr, x_out, nullptr, nullptr));
+  if (hipStreamSynchronize(e.st) != hipSuccess) return HT_ERR_HIP;
+  return HT_OK;
+}
+
+// ------------------------------------------------------------------ generators
+// Q of the QR factorisation of the n x r Philox Gaussian (seed, stream), diag(R) > 0,
+// by Gram-Schmidt with reorthogonalisation in FP64 (the Q factor with positive diag(R)
+// is unique, so it matches any stable QR up to rounding).
+static std::vector<double> orthonormal_factor(uint64_t seed, uint32_t stream, long n, int r) {
+  std::vector<double> A((size_t)n * r);
+  for (long i = 0; i < n * r; ++i) A[i] = philox_normal(seed, stream, (uint64_t)i);
+  for (int j = 0; j < r; ++j) {
+    double* a = &A[(size_t)n * j];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = 0; i < j; ++i) {
+        const double* q = &A[(size_t)n * i];
+        double d = 0.0;
+        for (long t = 0; t < n; ++t) d += q[t] * a[t];
+        for (long t = 0; t < n; ++t) a[t] -= d * q[t];
+      }
+    double nr = 0.0;
+    for (long t = 0; t < n; ++t) nr += a[t] * a[t];
+    nr = std::sqrt(nr);
+    for (long t = 0; t < n; ++t) a[t] /= nr;
+  }
+  return A;
+}
+
+// x = core x_n U_n (+ noise) with the modes split into [0, m) and [m, d): X_(left) = L R^T,
+// L = U_{m-1} (x) ... (x) U_0, R = (core x_{n>=m} U_n)_(left)^T.
+static ht_status generate_from_core(ht_handle_t h, int order, const int64_t* dims, const int32_t* core_dims,
+                                    std::vector<double> core, std::vector<std::vector<double>> U, uint64_t seed,
+                                    double noise_rel, float* x_dev) {
+  const int m = (order + 1) / 2;
+  long nl = 1, kl = 1, nr = 1;
+  for (int k = 0; k < m; ++k) { nl *= dims[k]; kl *= core_dims[k]; }
+  for (int k = m; k < order; ++k) nr *= dims[k];
+  if (kl > 4096) return HT_ERR_UNSUPPORTED;
+  // L (nl x kl)
+  std::vector<float> L((size_t)nl * kl);
+  for (long col = 0; col < kl; ++col)
+    for (long row = 0; row < nl; ++row) {
+      long rr = row, cc = col;
+      double v = 1.0;
+      for (int k = 0; k < m; ++k) {
+        const long i = rr % dims[k], j = cc % core_dims[k];
+        rr /= dims[k];
+        cc /= core_dims[k];
+        v *= U[k][i + dims[k] * j];
+      }
+      L[row + nl * col] = (float)v;
+    }
+  // T = core x_{k>=m} U_k, modes (core_dims[0..m), dims[m..d)).
+  std::vector<long> td(order);
+  for (int k = 0; k < order; ++k) td[k] = core_dims[k];
+  std::vector<double> T = std::move(core);
+  for (int k = m; k < order; ++k) {
+    long P = 1, S = 1;
+    for (int q = 0; q < k; ++q) P *= td[q];
+    for (int q = k + 1; q < order; ++q) S *= td[q];
+    const long rk = td[k], nk = dims[k];
+    std::vector<double> T2((size_t)P * nk * S, 0.0);
+    for (long s = 0; s < S; ++s)
+      for (long j = 0; j < rk; ++j)
+        for (long i = 0; i < nk; ++i) {
+          const double u = U[k][i + nk * j];
+          const double* src = &T[(size_t)P * (j + rk * s)];
+          double* dst = &T2[(size_t)P * (i + nk * s)];
+          for (long p = 0; p < P; ++p) dst[p] += u * src[p];
+        }
+    T.swap(T2);
+    td[k] = nk;
+  }
+  // R[i_right, j_left] = T[j_left + kl * i_right]
+  std::vector<float> R((size_t)nr * kl);
+  for (long i = 0; i < nr; ++i)
+    for (long j = 0; j < kl; ++j) R[i + nr * j] = (float)T[j + kl * i];
+  float *Ld = nullptr, *Rd = nullptr;
+  double* acc = nullptr;
+  const size_t accn = 2048;
+  if (hipMalloc((void**)&Ld, L.size() * 4) != hipSuccess) return HT_ERR_ALLOC;
+  if (hipMalloc((void**)&Rd, R.size() * 4) != hipSuccess) { (void)hipFree(Ld); return HT_ERR_ALLOC; }
+  if (hipMalloc((void**)&acc, (accn + 8192) * 8) != hipSuccess) { (void)hipFree(Ld); (void)hipFree(Rd); return HT_ERR_ALLOC; }
+  ht_status st = HT_OK;
+  do {
+    if (hipMemcpy(Ld, L.data(), L.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { st = HT_ERR_HIP; break; }
+    if (hipMemcpy(Rd, R.data(), R.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { st = HT_ERR_HIP; break; }
+    if (lowrank_launch(h->stream, 0, Ld, Rd, nl, nr, (int)kl, nullptr, x_dev, nullptr, nullptr)) { st = HT_ERR_HIP; break; }
+    if (noise_rel > 0.0) {
+      if (sumsq_f32_launch(h->stream, x_dev, nl * nr, acc)) { st = HT_ERR_HIP; break; }
+      int nb = 0;
+      if (gen_noise_sumsq_launch(h->stream, nl * nr, seed, 64, acc + 2048, &nb)) { st = HT_ERR_HIP; break; }
+      if (gram_reduce_norms_launch(h->stream, acc + 2048, nb, acc + 2047)) { st = HT_ERR_HIP; break; }
+      double hv[2];
+      if (hipMemcpyAsync(&hv[0], acc, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
+      if (hipMemcpyAsync(&hv[1], acc + 2047, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
+      if (hipStreamSynchronize(h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
+      const double scale = noise_rel * std::sqrt(hv[0]) / std::sqrt(hv[1]);
+      if (gen_add_noise_launch(h->stream, x_dev, nl * nr, seed, 64, scale)) { st = HT_ERR_HIP; break; }
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) st = HT_ERR_HIP;
+  } while (0);
+  (void)hipFree(Ld);
+  (void)hipFree(Rd);
+  (void)hipFree(acc);
+  return st;
+}
+
+ht_status ht_generate_tucker(ht_handle_t h, int32_t order, const int64_t* dims, const int32_t* core_dims,
+                             uint64_t seed, double noise_rel, float* x_dev) {
+  if (!h || !dims || !core_dims || !x_dev || order < 2 || order > HT_MAX_ORDER || noise_rel < 0) return HT_ERR_INVALID_ARG;
+  if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
+  long nc = 1;
+  for (int k = 0; k < order; ++k) {
+    if (dims[k] < 1 || core_dims[k] < 1 || core_dims[k] > dims[k]) return HT_ERR_INVALID_ARG;
+    nc *= core_dims[k];
+  }
+  std::vector<double> core(nc);
+  for (long i = 0; i < nc; ++i) core[i] = philox_normal(seed, 1, (uint64_t)i);
+  std::vector<std::vector<double>> U(order);
+  for (int k = 0; k < order; ++k) U[k] = orthonormal_factor(seed, 16 + k, dims[k], core_dims[k]);
+  return generate_from_core(h, order, dims, core_dims, std::move(core), std::move(U), seed, noise_rel, x_dev);
+}
+
+ht_status ht_generate_cp_decay(ht_handle_t h, int32_t order, const int64_t* dims, int32_t rank, double exponent,
+                               uint64_t seed, double noise_rel, float* x_dev) {
+  if (!h || !dims || !x_dev || order < 2 || order > HT_MAX_ORDER || rank < 1 || noise_rel < 0) return HT_ERR_INVALID_ARG;
+  if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
+  std::vector<int32_t> cd(order, rank);
+  long nc = 1, stride = 0, pk = 1;
+  for (int k = 0; k < order; ++k) {
+    if (dims[k] < rank) return HT_ERR_INVALID_ARG;
+    nc *= rank;
+    stride += pk;
+    pk *= rank;
+  }
+  std::vector<double> core(nc, 0.0);
+  for (int k = 0; k < rank; ++k) core[(size_t)k * stride] = std::pow(k + 1.0, -exponent);
+  std::vector<std::vector<double>> U(order);
+  for (int k = 0; k < order; ++k) U[k] = orthonormal_factor(seed, 16 + k, dims[k], rank);
+  return generate_from_core(h, order, dims, cd.data(), std::move(core), std::move(U), seed, noise_rel, x_dev);
+}
+
+ht_status ht_generate_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t n_bumps, uint64_t seed, int64_t sample,
+                            float* x_dev) {
+  if (!h || !x_dev || order < 1 || order > HT_MAX_ORDER || n < 2 || n_bumps < 1 || sample < 0) return HT_ERR_INVALID_ARG;
+  if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
+  const int per = n_bumps * (2 + order);
+  std::vector<float> amp(n_bumps), fac((size_t)n_bumps * order * n);
+  for (int j = 0; j < n_bumps; ++j) {
+    auto u = [&](int c) { return philox_uniform(seed, 96, (uint64_t)sample * per + (uint64_t)j * (2 + order) + c); };
+    const double a = 0.5 + 1.5 * u(0), w = 0.1 + 0.3 * u(1);
+    amp[j] = (float)a;
+    for (int m = 0; m < order; ++m) {
+      const double c = u(2 + m);
+      for (long i = 0; i < n; ++i) {
+        const double g = (double)i / (double)(n - 1);
+        fac[((size_t)j * order + m) * n + i] = (float)std::exp(-(g - c) * (g - c) / (2.0 * w * w));
+      }
+    }
+  }
+  float *fd = nullptr, *ad = nullptr;
+  if (hipMalloc((void**)&fd, fac.size() * 4) != hipSuccess) return HT_ERR_ALLOC;
+  if (hipMalloc((void**)&ad, amp.size() * 4) != hipSuccess) { (void)hipFree(fd); return HT_ERR_ALLOC; }
+  ht_status st = HT_OK;
+  if (hipMemcpy(fd, fac.data(), fac.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(ad, amp.data(), amp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      gen_bumps_launch(h->stream, x_dev, order, n, fd, ad, n_bumps) || hipStreamSynchronize(h->stream) != hipSuccess)
+    st = HT_ERR_HIP;
+  (void)hipFree(fd);
+  (void)hipFree(ad);
+  return st;
+}
+
+// ------------------------------------------------------------------ compat wrappers
+static ht_handle_t default_handle() {
+  static std::once_flag once;
+  static ht_handle_t h = nullptr;
+  std::call_once(once, [] {
+    if (ht_create(&h, 0) != HT_OK) h = nullptr;
+  });
+  return h;
+}
+
+static double compat_run(float* x, int order, const int64_t* dims, const int32_t* lo, const int32_t* hi,
+                         const int32_t* left, const int32_t* right, const int* k, int method, double* ms) {
+  ht_handle_t h = default_handle();
+  if (!h || !x || !k) return -1.0;
+  const int nn = 2 * order - 1;
+  std::vector<int32_t> ranks(k, k + nn);
+  ht_options opt;
+  ht_default_options(&opt);
+  opt.method = method;
+  opt.precision = method == HT_METHOD_PLAIN ? HT_PREC_FP32 : HT_PREC_BF16;
+  double err = NAN;
+  ht_stats stats;
+  const ht_status s = ht_decompose(h, x, 0, order, dims, nn, lo, hi, left, right, ranks.data(), &opt, nullptr, 0,
+                                   &err, &stats);
+  if (s != HT_OK) return -1.0 - (double)s;
+  *ms = stats.total_ms + stats.h2d_ms;
+  return err;
+}
+
+// Heap-numbered reference trees.
+static const int32_t T3_LO[5] = {0, 0, 2, 0, 1}, T3_HI[5] = {3, 2, 3, 1, 2}, T3_L[5] = {1, 3, -1, -1, -1},
+                     T3_R[5] = {2, 4, -1, -1, -1};
+static const int32_t T4_LO[7] = {0, 0, 2, 0, 1, 2, 3}, T4_HI[7] = {4, 2, 4, 1, 2, 3, 4},
+                     T4_L[7] = {1, 3, 5, -1, -1, -1, -1}, T4_R[7] = {2, 4, 6, -1, -1, -1, -1};
+
+float htd(float* x, long a, long b, long c, int* k, float rel_eps, float max_rank) {
+  (void)rel_eps;
+  (void)max_rank;
+  const int64_t dims[3] = {a, b, c};
+  double ms = 0;
+  const double err = compat_run(x, 3, dims, T3_LO, T3_HI, T3_L, T3_R, k, HT_METHOD_PLAIN, &ms);
+  if (err >= 0) {
+    printf("cost time :%gs\n", ms / 1000.0);
+    printf("error rate %g\n", err);
+  }
+  return (float)err;
+}
+
+float htd_tc(float* x, long a, long b, long c, int* k) {
+  const int64_t dims[3] = {a, b, c};
+  double ms = 0;
+  const double err = compat_run(x, 3, dims, T3_LO, T3_HI, T3_L, T3_R, k, HT_METHOD_LTR, &ms);
+  if (err >= 0) {
+    printf("cost time :%gs\n", ms / 1000.0);
+    printf("error rate %g\n", err);
+  }
+  return (float)err;
+}
+
+void htd4(float* x, long a, long b, long c, long d, int* k) {
+  const int64_t dims[4] = {a, b, c, d};
+  double ms = 0;
+  const double err = compat_run(x, 4, dims, T4_LO, T4_HI, T4_L, T4_R, k, HT_METHOD_PLAIN, &ms);
+  if (err >= 0) printf("error rate %g\n", err);
+  else printf("htd4 failed (status %d)\n", (int)(-1.0 - err));
+}
+
+void htd4_tc(float* x, long a, long b, long c, long d, int* k, void* blas_handle, void* solver_handle) {
+  (void)blas_handle;
+  (void)solver_handle;
+  const int64_t dims[4] = {a, b, c, d};
+  double ms = 0;
+  const double err = compat_run(x, 4, dims, T4_LO, T4_HI, T4_L, T4_R, k, HT_METHOD_LTR, &ms);
+  if (err >= 0) printf("error rate %g\n", err);
+  else printf("htd4_tc failed (status %d)\n", (int)(-1.0 - err));
+}
+
+// ------------------------------------------------------------------ component entry points
+ht_status ht_debug_mfma_probe(ht_handle_t h, const float* a, const float* b, float* o_bf16_32, float* o_f32_32,
+                              float* o_f32_16, double* o_f64_16, float* o_bf16_16) {
+  if (!h) return HT_ERR_INVALID_ARG;
+  CKS(mfma_probe_launch(h->stream, a, b, o_bf16_32, o_f32_32, o_f32_16, o_f64_16, o_bf16_16));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+ht_status ht_mode_gram(ht_handle_t h, const float* x, int32_t order, const int64_t* dims, int32_t lo, int32_t hi,
+                       int32_t precision, double* G, double* sumsq) {
+  if (!h || !x || !dims || !G || order < 1 || order > HT_MAX_ORDER || lo < 0 || hi > order || lo >= hi)
+    return HT_ERR_INVALID_ARG;
+  PMS v{1, 1, 1};
+  for (int m = 0; m < lo; ++m) v.P *= dims[m];
+  for (int m = lo; m < hi; ++m) v.M *= dims[m];
+  for (int m = hi; m < order; ++m) v.S *= dims[m];
+  if (v.M > 1024) return HT_ERR_UNSUPPORTED;
+  const int prec = precision == HT_PREC_FP32 ? PREC_F32 : (precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
+  h->arena.reset();
+  Engine e;
+  e.st = h->stream;
+  e.arena = &h->arena;
+  double* s = sumsq ? (double*)h->arena.get(8) : nullptr;
+  CKS(e.gram(x, v, prec, G, s));
+  if (sumsq) CKS(e.d2h(sumsq, s, 8));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+ht_status ht_slab_gram(ht_handle_t h, const float* x, int32_t n1, int32_t n2, int64_t S, int32_t precision,
+                       double* G1, double* G2, double* sumsq) {
+  if (!h || !x || !G1 || !G2 || n1 < 1 || n2 < 1 || n1 > 128 || n2 > 128 || S < 1) return HT_ERR_INVALID_ARG;
+  const int prec = precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16;
+  h->arena.reset();
+  const int nb = (int)std::min<int64_t>(S, 1024);
+  float* p1 = (float*)h->arena.get((size_t)nb * 128 * 128 * 4);
+  float* p2 = (float*)h->arena.get((size_t)nb * 128 * 128 * 4);
+  double* norms = (double*)h->arena.get((size_t)nb * 8 + 64);
+  if (!p1 || !p2 || !norms) return HT_ERR_ALLOC;
+  CKS(slab_gram_launch(h->stream, x, n1, n2, S, prec, nb, p1, p2, norms));
+  CKS(gram_reduce_launch(h->stream, p1, n1, nb, G1));
+  CKS(gram_reduce_launch(h->stream, p2, n2, nb, G2));
+  if (sumsq) {
+    CKS(gram_reduce_norms_launch(h->stream, norms, nb, norms + nb));
+    if (hipMemcpyAsync(sumsq, norms + nb, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess) return HT_ERR_HIP;
+  }
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+ht_status ht_slab_project(ht_handle_t h, const float* x, int32_t n1, int32_t n2, int64_t S, const float* U1, int32_t r1,
+                          const float* U2, int32_t r2, float* z) {
+  if (!h || !x || !U1 || !U2 || !z) return HT_ERR_INVALID_ARG;
+  CKS(slab_proj_launch(h->stream, x, n1, n2, S, U1, r1, U2, r2, z));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+ht_status ht_sym_eig(ht_handle_t h, const double* A, int32_t n, double* w, double* V) {
+  if (!h || !A || !w || !V || n < 1 || n > 64) return HT_ERR_INVALID_ARG;
+  h->arena.reset();
+  int* info = (int*)h->arena.get(64);
+  CKS(jacobi_eig_launch(h->stream, A, n, n, w, V, n, info, 60, 1e-15));
+  int hinfo = 0;
+  if (hipMemcpyAsync(&hinfo, info, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess) return HT_ERR_HIP;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return HT_ERR_HIP;
+  return hinfo ? HT_ERR_NOCONV : HT_OK;
+}
+
+ht_status ht_sym_topk(ht_handle_t h, const double* G, int32_t n, int32_t k, double tol, double* U, double* w,
+                      int32_t* iters) {
+  if (!h || !G || !U || !w || n < 1 || k < 1 || k > n || k > 64) return HT_ERR_INVALID_ARG;
+  h->arena.reset();
+  Engine e;
+  e.st = h->stream;
+  e.arena = &h->arena;
+  ht_default_options(&e.opt);
+  e.eig_iters_max = 0;
+  e.noconv = 0;
+  CKS(e.sym_topk(G, n, k, U, w, tol, 500, 77));
+  if (iters) *iters = e.eig_iters_max;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return HT_ERR_HIP;
+  return e.noconv ? HT_ERR_NOCONV : HT_OK;
+}
+
+ht_status ht_tsqr(ht_handle_t h, float* W, int64_t m, int32_t b, float* Q, double* R) {
+  if (!h || !W || !Q || !R || m < 1 || b < 1 || b > 64) return HT_ERR_INVALID_ARG;
+  h->arena.reset();
+  void* ws = h->arena.get(tsqr_ws_bytes(m, b));
+  if (!ws) return HT_ERR_ALLOC;
+  CKS(tsqr_launch(h->stream, W, m, b, Q, R, ws));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+ht_status ht_skinny_gemm(ht_handle_t h, const float* A, int64_t M, int64_t K, int32_t transA, const float* B, int32_t b,
+                         int32_t precision, float* Y) {
+  if (!h || !A || !B || !Y || M < 1 || K < 1 || b < 1 || b > 64) return HT_ERR_INVALID_ARG;
+  const int prec = precision == HT_PREC_FP32 ? PREC_F32 : (precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
+  h->arena.reset();
+  const size_t wsf = gemm_ws_floats(M, K, b);
+  float* ws = (float*)h->arena.get(wsf * 4);
+  if (!ws) return HT_ERR_ALLOC;
+  if (transA) CKS(gemm_atb_launch(h->stream, A, M, K, B, b, prec, Y, ws, wsf));
+  else CKS(gemm_ab_launch(h->stream, A, M, K, B, b, prec, Y, ws, wsf));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+ht_status ht_philox_normals(ht_handle_t h, uint64_t seed, uint32_t stream, int64_t n, double* out_dev) {
+  if (!h || !out_dev || n < 0) return HT_ERR_INVALID_ARG;
+  CKS(fill_normals_f64_launch(h->stream, out_dev, n, seed, stream));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+void ht_philox_normals_host(uint64_t seed, uint32_t stream, int64_t n, double* out) {
+  for (int64_t i = 0; i < n; ++i) out[i] = philox_normal(seed, stream, (uint64_t)i);
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// Shared device helpers and kernel launcher declarations of the HT engine (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HT_HIP_CHECK(expr)                                  \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) return ht::hip_fail(_e, #expr);   \
+  } while (0)
+
+namespace ht {
+
+int hip_fail(hipError_t e, const char* what);
+
+// ---------------------------------------------------------------- MFMA operand types
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+enum Prec { PREC_F32 = 0, PREC_BF16 = 1, PREC_F16 = 2 };
+
+// FP32 -> BF16 bits, round to nearest even (inputs are finite: the engine checks them).
+__device__ __forceinline__ uint16_t f2bf_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ uint16_t f2h_bits(float f) {
+  _Float16 h = (_Float16)f;
+  return __builtin_bit_cast(uint16_t, h);
+}
+template <int PREC>
+__device__ __forceinline__ uint16_t to16(float f) {
+  if constexpr (PREC == PREC_F16) return f2h_bits(f);
+  else return f2bf_bits(f);
+}
+
+// 32x32x16 MFMA on 16-bit operands held as raw bits.
+template <int PREC>
+__device__ __forceinline__ f32x16 mfma32x32x16(u16x8 a, u16x8 b, f32x16 c) {
+  if constexpr (PREC == PREC_F16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// Output position of element e (0..15) of a 32x32 MFMA accumulator in lane l:
+//   row = 8*(e/4) + 4*(l/32) + (e%4), col = l%32.
+__device__ __forceinline__ int acc32_row(int e, int lane) { return 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3); }
+// 16x16 accumulators (f32x4 / f64x4): row = 4*(l/16) + e, col = l%16.
+__device__ __forceinline__ int acc16_row(int e, int lane) { return 4 * (lane >> 4) + e; }
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11). Key = (seed, 0x5EED0000 ^ stream),
+// counter = (block lo, block hi, 0, 0); a block yields 4 normals by Box-Muller.
+struct Philox4 {
+  uint32_t v[4];
+};
+__host__ __device__ inline Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                 uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  Philox4 o;
+  o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+  return o;
+}
+// The i-th standard normal of (seed, stream), in double precision.
+__host__ __device__ inline double philox_normal(uint64_t seed, uint32_t stream, uint64_t i) {
+  uint64_t blk = i >> 2;
+  Philox4 w = philox4x32_10((uint32_t)blk, (uint32_t)(blk >> 32), 0u, 0u, (uint32_t)seed,
+                            0x5EED0000u ^ stream);
+  int j = (int)(i & 3);
+  double ua = ((double)w.v[(j >> 1) * 2] + 0.5) * 2.3283064365386963e-10;
+  double ub = ((double)w.v[(j >> 1) * 2 + 1] + 0.5) * 2.3283064365386963e-10;
+  double r = sqrt(-2.0 * log(ua));
+  double t = 6.283185307179586 * ub;
+  return (j & 1) ? r * sin(t) : r * cos(t);
+}
+__host__ __device__ inline double philox_uniform(uint64_t seed, uint32_t stream, uint64_t i) {
+  uint64_t blk = i >> 2;
+  Philox4 w = philox4x32_10((uint32_t)blk, (uint32_t)(blk >> 32), 0u, 0u, (uint32_t)seed,
+                            0x5EED0000u ^ stream);
+  return ((double)w.v[i & 3] + 0.5) * 2.3283064365386963e-10;
+}
+
+// ------------------------------------------------------------------ tensor views
+// (P, M, S) view of a column-major tensor: element (m, c) of the matricisation with
+// c = p + P*s lives at x[p + P*m + P*M*s].
+struct PMS {
+  long P, M, S;
+};
+
+// ------------------------------------------------------------------ launchers
+// All launchers enqueue on `stream` and return 0 or an ht_status code.
+
+// Leaf Gram partials of an M-row matricisation (M <= 1024) in 128x128 tiles.
+// part: [n_pairs][n_split][128*128] FP32; norms: [n_pairs*n_split] FP64 sum of squares
+// of the diagonal-pair tiles (their sum is ||X||^2).
+int gram_pms_launch(hipStream_t st, const float* x, PMS v, int prec, int n_split, float* part,
+                    double* norms);
+int gram_pms_nsplit(PMS v, int target_blocks);
+// Sum partials into a full symmetric FP64 Gram (ld = M).
+int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G);
+int gram_reduce_norms_launch(hipStream_t st, const double* norms, int n, double* out);
+
+// Fused slab kernels on the two leading modes (x as n1 x n2 x S, n1,n2 <= 128).
+// GRAM: part1/part2 [n_blocks][128*128] partials of X_(1)X_(1)^T and X_(2)X_(2)^T.
+int slab_gram_launch(hipStream_t st, const float* x, int n1, int n2, long S, int prec, int n_blocks,
+                     float* part1, float* part2, double* norms);
+// PROJ: z[a + r1*b + r1*r2*s] = sum_ij U1[i,a] X[i,j,s] U2[j,b]  (r1, r2 <= 64, FP32).
+int slab_proj_launch(hipStream_t st, const float* x, int n1, int n2, long S, const float* U1, int r1,
+                     const float* U2, int r2, float* z);
+
+// Generic mode projection y = x x_t U^T on a (P,M,S) view: y[p + P*j + P*r*s] =
+// sum_m U[m + M*j] x[p + P*m + P*M*s]. T = float or double.
+int proj_pms_f32_launch(hipStream_t st, const float* x, PMS v, const float* U, int r, float* y);
+int proj_pms_f64_launch(hipStream_t st, const double* x, PMS v, const double* U, int r, double* y);
+int proj_pms_f32in_f64_launch(hipStream_t st, const float* x, PMS v, const double* U, int r, double* y);
+
+// Skinny FP32 GEMMs over a big column-major A (M x K, lda = M), b <= 64:
+//   Y(K x b) = A^T B(M x b)       (gemm_atb)
+//   Y(M x b) = A B(K x b)         (gemm_ab)
+// With prec = BF16/F16 the A and B tiles are rounded to 16 bits (FP32 accumulation).
+int gemm_atb_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec,
+                    float* Y, float* ws, size_t ws_floats);
+int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec,
+                   float* Y, float* ws, size_t ws_floats);
+size_t gemm_ws_floats(long M, long K, int b);
+
+// Low-rank tiles C = L R^T (L: M x k, R: N x k, both column-major, k <= 256).
+//   mode 0: store C into out (ld M);  mode 1: accumulate sum (X - C)^2 and sum X^2 (FP64)
+//   into acc[2*block] per block; the host sums the blocks.
+int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, long M, long N, int k,
+                   const float* X, float* out, double* acc, int* n_blocks_out);
+
+// Small dense FP64 kernels (single work-group unless noted).
+int jacobi_eig_launch(hipStream_t st, const double* A, int n, int lda, double* w, double* V, int ldv,
+                      int* info, int max_sweeps, double tol);
+int jacobi_eig_batched_launch(hipStream_t st, const double* A, int n, long strideA, double* w, long stridew,
+                              double* V, long strideV, int batch, int* info, int max_sweeps, double tol);
+// C(m x n) = op(A) op(B) with op = transpose flags; multi-block; all FP64, column-major.
+int dgemm_small_launch(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double* A,
+                       int lda, const double* B, int ldb, double beta, double* C, int ldc);
+// Cholesky QR with column replacement (Z: m x b FP64, in place); R out (b x b).
+int cholqr_launch(hipStream_t st, double* Z, int m, int b, double* W, double* R, uint64_t seed,
+                  uint32_t stream, int* info);
+int ritz_resid_launch(hipStream_t st, const double* GU, const double* U, const double* w, int n, int k,
+                      double* out);
+int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G);
+int sum_strided_launch(hipStream_t st, const double* acc, int n, double* out);
+int transpose_launch(hipStream_t st, const double* A, long m, long n, double* AT);
+int sqrt_clamp_launch(hipStream_t st, const double* w, int k, double* s);
+int scale_cols_inv_launch(hipStream_t st, double* A, int m, int n, const double* s);
+// Casts and sign canonicalisation.
+int cast_f64_f32_launch(hipStream_t st, const double* a, float* b, long n);
+int cast_f32_f64_launch(hipStream_t st, const float* a, double* b, long n);
+int canon_signs_launch(hipStream_t st, double* U, int m, int k, int ldu);
+int fill_normals_f64_launch(hipStream_t st, double* a, long n, uint64_t seed, uint32_t stream);
+int fill_normals_f32_launch(hipStream_t st, float* a, long n, uint64_t seed, uint32_t stream);
+int copy_cols_f64_launch(hipStream_t st, const double* a, int lda, double* b, int ldb, int m, int n);
+int sumsq_f64_launch(hipStream_t st, const double* a, long n, double* out);
+int sumsq_f32_launch(hipStream_t st, const float* a, long n, double* out);
+
+// TSQR of a tall-skinny FP32 W (m x b, ld m, b <= 64): explicit Q (m x b, FP32) and R
+// (b x b, FP64). ws must hold tsqr_ws_bytes(m, b) bytes.
+size_t tsqr_ws_bytes(long m, int b);
+int tsqr_launch(hipStream_t st, float* W, long m, int b, float* Q, double* R, void* ws);
+
+// Generators.
+int gen_noise_sumsq_launch(hipStream_t st, long n, uint64_t seed, uint32_t stream, double* acc, int* nb);
+int gen_add_noise_launch(hipStream_t st, float* x, long n, uint64_t seed, uint32_t stream, double scale);
+int gen_bumps_launch(hipStream_t st, float* x, int order, long n, const float* factors, const float* amp,
+                     int n_bumps);
+
+// MFMA lane-layout probe (diagnostic): runs each MFMA shape once on given inputs.
+int mfma_probe_launch(hipStream_t st, const float* a, const float* b, float* out_bf16_32, float* out_f32_32,
+                      float* out_f32_16, double* out_f64_16, float* out_bf16_16);
+
+}  // namespace ht

@@ -1,0 +1,861 @@
+// Host driver of the HT engine: validation, workspace, and the three decomposition
+// methods built from the gfx950 kernels.
+//
+// Reference drivers this replaces (paths relative to the reference repository):
+//   plain  -- third_order_tensor_decompsition/CUDA/Baseline/htd_unopt.cu:2-297 and
+//             fourth_order_tensor_decomposition/single_GPU/CUDA/Baseline/htd4.cu:2-220
+//   ltr    -- no counterpart (the fast path: leaf Grams, one projection pass, FP64 core)
+//   rsvd   -- fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/htd4_tensorcore.cu:2-306
+#include "ht_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace ht {
+
+int hip_fail(hipError_t e, const char* what) {
+  if (getenv("HT_DEBUG")) fprintf(stderr, "ht: HIP error %d (%s) in %s\n", (int)e, hipGetErrorString(e), what);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return HT_ERR_ALLOC;
+  return HT_ERR_HIP;
+}
+
+// ------------------------------------------------------------------ arena
+void* Arena::get(size_t bytes) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  while (cur < chunks.size()) {
+    if (off + bytes <= chunks[cur].second) {
+      void* p = (char*)chunks[cur].first + off;
+      off += bytes;
+      return p;
+    }
+    ++cur;
+    off = 0;
+  }
+  size_t sz = std::max(bytes, (size_t)256 << 20);
+  void* p = nullptr;
+  if (hipMalloc(&p, sz) != hipSuccess) return nullptr;
+  chunks.push_back({p, sz});
+  cur = chunks.size() - 1;
+  off = bytes;
+  return p;
+}
+void Arena::reset() {
+  cur = 0;
+  off = 0;
+}
+void Arena::release() {
+  for (auto& c : chunks) (void)hipFree(c.first);
+  chunks.clear();
+  reset();
+}
+
+// ------------------------------------------------------------------ tree validation
+int validate_tree(int order, int n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                  const int32_t* right) {
+  if (order < 2 || order > HT_MAX_ORDER || n_nodes != 2 * order - 1) return HT_ERR_INVALID_TREE;
+  if (!lo || !hi || !left || !right) return HT_ERR_INVALID_ARG;
+  if (lo[0] != 0 || hi[0] != order) return HT_ERR_INVALID_TREE;
+  std::vector<int> seen(order, 0), parent(n_nodes, -1);
+  for (int t = 0; t < n_nodes; ++t) {
+    if (!(0 <= lo[t] && lo[t] < hi[t] && hi[t] <= order)) return HT_ERR_INVALID_TREE;
+    const int l = left[t], r = right[t];
+    if ((l < 0) != (r < 0)) return HT_ERR_INVALID_TREE;
+    if (l < 0) {
+      if (hi[t] != lo[t] + 1) return HT_ERR_INVALID_TREE;
+      if (seen[lo[t]]++) return HT_ERR_INVALID_TREE;
+    } else {
+      if (l >= n_nodes || r >= n_nodes || l == t || r == t || l == 0 || r == 0) return HT_ERR_INVALID_TREE;
+      if (parent[l] >= 0 || parent[r] >= 0) return HT_ERR_INVALID_TREE;
+      parent[l] = t;
+      parent[r] = t;
+      if (!(lo[l] == lo[t] && hi[l] == lo[r] && hi[r] == hi[t])) return HT_ERR_INVALID_TREE;
+    }
+  }
+  for (int m = 0; m < order; ++m)
+    if (seen[m] != 1) return HT_ERR_INVALID_TREE;
+  for (int t = 1; t < n_nodes; ++t)
+    if (parent[t] < 0) return HT_ERR_INVALID_TREE;
+  return HT_OK;
+}
+
+int Problem::setup(int order_, const int64_t* dims_, int n_nodes_, const int32_t* lo_, const int32_t* hi_,
+                   const int32_t* left_, const int32_t* right_, const int32_t* ranks_, int want_int_u) {
+  if (!dims_ || !ranks_) return HT_ERR_INVALID_ARG;
+  int st = validate_tree(order_, n_nodes_, lo_, hi_, left_, right_);
+  if (st) return st;
+  order = order_;
+  n_nodes = n_nodes_;
+  N = 1;
+  for (int m = 0; m < order; ++m) {
+    if (dims_[m] < 1) return HT_ERR_INVALID_ARG;
+    dims[m] = dims_[m];
+    if (N > ((int64_t)1 << 50) / dims[m]) return HT_ERR_INVALID_ARG;
+    N *= dims[m];
+  }
+  for (int t = 0; t < n_nodes; ++t) {
+    lo[t] = lo_[t];
+    hi[t] = hi_[t];
+    left[t] = left_[t];
+    right[t] = right_[t];
+    rank[t] = ranks_[t];
+    rows[t] = 1;
+    for (int m = lo[t]; m < hi[t]; ++m) rows[t] *= dims[m];
+    parent[t] = -1;
+  }
+  for (int t = 0; t < n_nodes; ++t)
+    if (left[t] >= 0) parent[left[t]] = parent[right[t]] = t;
+  if (rank[0] != 1) return HT_ERR_RANK;
+  for (int t = 1; t < n_nodes; ++t) {
+    const int64_t k = rank[t];
+    const int64_t cols = N / rows[t];
+    if (k < 1 || k > HT_MAX_RANK || k > rows[t] || k > cols) return HT_ERR_RANK;
+    if (left[t] >= 0 && k > (int64_t)rank[left[t]] * rank[right[t]]) return HT_ERR_RANK;
+  }
+  // Packed output layout.
+  int64_t off = 0;
+  want_interior_u = want_int_u;
+  for (int t = 0; t < n_nodes; ++t) {
+    u_off[t] = b_off[t] = s_off[t] = -1;
+    if (t != 0 && (left[t] < 0 || want_int_u)) {
+      u_off[t] = off;
+      off += rows[t] * rank[t];
+    }
+  }
+  for (int t = 0; t < n_nodes; ++t)
+    if (left[t] >= 0) {
+      b_off[t] = off;
+      off += (int64_t)rank[left[t]] * rank[right[t]] * rank[t];
+    }
+  for (int t = 1; t < n_nodes; ++t) {
+    s_off[t] = off;
+    off += rank[t];
+  }
+  total = off;
+  return HT_OK;
+}
+
+int Problem::depth(int t) const {
+  int d = 0;
+  while (t != 0) {
+    t = parent[t];
+    ++d;
+  }
+  return d;
+}
+
+int Problem::leaf_of_mode(int m) const {
+  for (int t = 0; t < n_nodes; ++t)
+    if (left[t] < 0 && lo[t] == m) return t;
+  return -1;
+}
+
+// ------------------------------------------------------------------ engine helpers
+#define CK(expr)                 \
+  do {                           \
+    int _s = (expr);             \
+    if (_s) return _s;           \
+  } while (0)
+
+static PMS view_of(const int64_t* dims, int order, int lo, int hi) {
+  PMS v{1, 1, 1};
+  for (int m = 0; m < lo; ++m) v.P *= dims[m];
+  for (int m = lo; m < hi; ++m) v.M *= dims[m];
+  for (int m = hi; m < order; ++m) v.S *= dims[m];
+  return v;
+}
+
+int Engine::sync() {
+  HT_HIP_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int Engine::d2h(void* dst, const void* src, size_t bytes) {
+  HT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  HT_HIP_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+template <typename T>
+T* Engine::alloc(size_t n) {
+  return (T*)arena->get(n * sizeof(T) + 16);
+}
+
+// Gram of an (P,M,S) view of FP32 data, reduced to FP64 (M x M). Returns ||x||^2 of the
+// data in *sumsq (device scalar) when sumsq != nullptr.
+int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
+  const int ns = gram_pms_nsplit(v, 512);
+  const int nt = (int)((v.M + 127) / 128);
+  const int pairs = nt * (nt + 1) / 2;
+  float* part = alloc<float>((size_t)pairs * ns * 128 * 128);
+  double* norms = alloc<double>((size_t)pairs * ns);
+  if (!part || !norms) return HT_ERR_ALLOC;
+  CK(gram_pms_launch(st, x, v, prec, ns, part, norms));
+  CK(gram_reduce_launch(st, part, (int)v.M, ns, G));
+  if (sumsq) CK(gram_reduce_norms_launch(st, norms, pairs * ns, sumsq));
+  passes += 1.0;
+  return 0;
+}
+
+// Top-k eigenpairs (descending) of the symmetric PSD n x n FP64 matrix G.
+// n <= 64: one Jacobi solve. Otherwise subspace iteration with a b-column block
+// (b <= 64), Cholesky-QR, and a Rayleigh-Ritz step with a Jacobi solve of the b x b
+// projected matrix; it stops when every top-k Ritz residual ||G u - theta u|| is below
+// tol * theta_1.
+int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream) {
+  int* info = alloc<int>(4);
+  if (n <= 64) {
+    double* V = alloc<double>((size_t)n * n);
+    double* wa = alloc<double>(n);
+    CK(jacobi_eig_launch(st, G, n, n, wa, V, n, info, 60, 1e-15));
+    HT_HIP_CHECK(hipMemcpyAsync(U, V, sizeof(double) * n * k, hipMemcpyDeviceToDevice, st));
+    HT_HIP_CHECK(hipMemcpyAsync(w, wa, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
+    ++eig_calls;
+    return 0;
+  }
+  int b = std::min(64, std::max(2 * k, k + 16));
+  b = std::min(b, n);
+  double* Q = alloc<double>((size_t)n * b);
+  double* Y = alloc<double>((size_t)n * b);
+  double* T = alloc<double>((size_t)b * b);
+  double* S = alloc<double>((size_t)b * b);
+  double* th = alloc<double>(b);
+  double* W = alloc<double>((size_t)b * b);
+  double* res = alloc<double>(k);
+  if (!Q || !Y || !T || !S || !th || !W || !res) return HT_ERR_ALLOC;
+  CK(fill_normals_f64_launch(st, Q, (long)n * b, opt.seed, stream));
+  CK(cholqr_launch(st, Q, n, b, W, nullptr, opt.seed, stream + 1, info));
+  int it = 0, steps = 2;
+  std::vector<double> hres(k), hth(b);
+  bool done = false;
+  while (!done) {
+    for (int s = 0; s < steps; ++s) {
+      CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, G, n, Q, n, 0.0, Y, n));
+      CK(cholqr_launch(st, Y, n, b, W, nullptr, opt.seed, stream + 2 + it, info));
+      std::swap(Q, Y);
+      ++it;
+    }
+    // Rayleigh-Ritz: Y = G Q, T = Q^T Y, T = S diag(th) S^T, U_full = Q S, G U_full = Y S.
+    CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, G, n, Q, n, 0.0, Y, n));
+    CK(dgemm_small_launch(st, 1, 0, b, b, n, 1.0, Q, n, Y, n, 0.0, T, b));
+    CK(jacobi_eig_launch(st, T, b, b, th, S, b, info, 60, 1e-15));
+    double* QS = alloc<double>((size_t)n * b);
+    double* YS = alloc<double>((size_t)n * b);
+    if (!QS || !YS) return HT_ERR_ALLOC;
+    CK(dgemm_small_launch(st, 0, 0, n, b, b, 1.0, Q, n, S, b, 0.0, QS, n));
+    CK(dgemm_small_launch(st, 0, 0, n, b, b, 1.0, Y, n, S, b, 0.0, YS, n));
+    CK(ritz_resid_launch(st, YS, QS, th, n, k, res));
+    CK(d2h(hres.data(), res, sizeof(double) * k));
+    CK(d2h(hth.data(), th, sizeof(double) * b));
+    double rmax = 0.0;
+    for (int i = 0; i < k; ++i) rmax = std::max(rmax, hres[i]);
+    const double scale = std::max(std::fabs(hth[0]), 1e-300);
+    done = (rmax <= tol * scale) || (it >= maxit) || (hth[0] == 0.0);
+    std::swap(Q, QS);  // continue from the Ritz basis
+    if (done) {
+      HT_HIP_CHECK(hipMemcpyAsync(U, Q, sizeof(double) * n * k, hipMemcpyDeviceToDevice, st));
+      HT_HIP_CHECK(hipMemcpyAsync(w, th, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
+      if (rmax > tol * scale && hth[0] != 0.0) noconv = 1;
+    }
+    steps = std::min(8, steps * 2);
+  }
+  CK(canon_signs_launch(st, U, n, k, n));
+  eig_iters_max = std::max(eig_iters_max, it);
+  ++eig_calls;
+  return 0;
+}
+
+// Orthonormal basis of the columns of Y (m x b, FP32) by TSQR; Q (FP32) and R (FP64).
+int Engine::tsqr(float* Y, long m, int b, float* Q, double* R) {
+  void* ws = arena->get(tsqr_ws_bytes(m, b));
+  if (!ws) return HT_ERR_ALLOC;
+  return tsqr_launch(st, Y, m, b, Q, R, ws);
+}
+
+// Skinny products with a (P,M,S) view where P == 1 (A = X_(t), lda = M) or S == 1
+// (A^T = X viewed as P x M, lda = P). op 0: Y = A B (B: cols x b); op 1: Y = A^T B.
+int Engine::view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y) {
+  float* ws = nullptr;
+  if (v.P == 1) {
+    const long M = v.M, K = v.S;
+    const size_t wsf = gemm_ws_floats(M, K, b);
+    ws = alloc<float>(wsf);
+    if (!ws) return HT_ERR_ALLOC;
+    if (op == 0) CK(gemm_ab_launch(st, x, M, K, B, b, prec, Y, ws, wsf));
+    else CK(gemm_atb_launch(st, x, M, K, B, b, prec, Y, ws, wsf));
+  } else if (v.S == 1) {
+    const long M = v.P, K = v.M;  // X viewed as P x M; A = X^T
+    const size_t wsf = gemm_ws_floats(M, K, b);
+    ws = alloc<float>(wsf);
+    if (!ws) return HT_ERR_ALLOC;
+    if (op == 0) CK(gemm_atb_launch(st, x, M, K, B, b, prec, Y, ws, wsf));  // A B = X^T B
+    else CK(gemm_ab_launch(st, x, M, K, B, b, prec, Y, ws, wsf));           // A^T B = X B
+  } else {
+    return HT_ERR_UNSUPPORTED;
+  }
+  passes += 1.0;
+  return 0;
+}
+
+// Exact (to tolerance) top-k left and right singular vectors of A = X_(t) for a node
+// whose rows and columns both exceed the Gram limit: block subspace iteration with
+// TSQR orthonormalisation, FP32 products, FP64 projected problem.
+//   U (rows x kU, FP64), Wr (cols x kW, FP64, may be null), sv (max(kU,kW), FP64)
+int Engine::exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U, double* Wr, double* sv,
+                               uint32_t stream) {
+  const long M = v.P == 1 ? v.M : v.P;  // not used for general views
+  (void)M;
+  const long rows = v.M, cols = v.P * v.S;
+  const int kk = std::max(kU, kW);
+  int b = std::min(64, std::max(2 * kk, kk + 16));
+  b = (int)std::min<long>(b, std::min(rows, cols));
+  float* Q = alloc<float>((size_t)rows * b);
+  float* Y = alloc<float>((size_t)rows * b);
+  float* Z = alloc<float>((size_t)cols * b);
+  float* Qz = alloc<float>((size_t)cols * b);
+  double* R = alloc<double>((size_t)b * b);
+  double* RRt = alloc<double>((size_t)b * b);
+  double* Sv = alloc<double>((size_t)b * b);
+  double* th = alloc<double>(b);
+  int* info = alloc<int>(4);
+  if (!Q || !Y || !Z || !Qz || !R || !RRt || !Sv || !th) return HT_ERR_ALLOC;
+  CK(fill_normals_f32_launch(st, Y, rows * b, opt.seed, stream));
+  CK(tsqr(Y, rows, b, Q, R));
+  std::vector<double> prev(kk, 0.0), cur(b);
+  int it = 0;
+  const double tol = std::max(opt.eig_tol, 3e-7);
+  for (;;) {
+    CK(view_gemm(x, v, 1, Q, b, PREC_F32, Z));    // Z = A^T Q
+    CK(tsqr(Z, cols, b, Qz, R));                  // Z = Qz Rz
+    CK(view_gemm(x, v, 0, Qz, b, PREC_F32, Y));   // Y = A Qz
+    CK(tsqr(Y, rows, b, Q, R));                   // Y = Q R, R = Q^T A Qz
+    ++it;
+    // sigma^2 = eig(R R^T)
+    CK(dgemm_small_launch(st, 0, 1, b, b, b, 1.0, R, b, R, b, 0.0, RRt, b));
+    CK(jacobi_eig_launch(st, RRt, b, b, th, Sv, b, info, 60, 1e-15));
+    CK(d2h(cur.data(), th, sizeof(double) * b));
+    double tot = 0.0, diff = 0.0;
+    for (int i = 0; i < kk; ++i) {
+      tot += std::max(cur[i], 0.0);
+      diff = std::max(diff, std::fabs(cur[i] - prev[i]));
+      prev[i] = cur[i];
+    }
+    const bool conv = (it >= 2 && diff <= tol * std::max(cur[0], 1e-300)) || cur[0] == 0.0;
+    if (conv || it >= opt.max_eig_iters) {
+      if (!conv) noconv = 1;
+      break;
+    }
+  }
+  eig_iters_max = std::max(eig_iters_max, it);
+  ++eig_calls;
+  // U = Q S (left vectors of R: R R^T S = S Th), W = Qz R^T S Th^{-1/2}.
+  double* Qd = alloc<double>((size_t)rows * b);
+  if (!Qd) return HT_ERR_ALLOC;
+  CK(cast_f32_f64_launch(st, Q, Qd, rows * b));
+  CK(dgemm_small_launch(st, 0, 0, (int)rows, kU, b, 1.0, Qd, (int)rows, Sv, b, 0.0, U, (int)rows));
+  CK(canon_signs_launch(st, U, (int)rows, kU, (int)rows));
+  std::vector<double> hs(b);
+  for (int i = 0; i < b; ++i) hs[i] = std::sqrt(std::max(cur[i], 0.0));
+  double* svd = alloc<double>(b);
+  HT_HIP_CHECK(hipMemcpyAsync(svd, hs.data(), sizeof(double) * b, hipMemcpyHostToDevice, st));
+  HT_HIP_CHECK(hipMemcpyAsync(sv, svd, sizeof(double) * kk, hipMemcpyDeviceToDevice, st));
+  if (Wr && kW > 0) {
+    // V_R = R^T S diag(1/sigma) (b x kW), W = Qz V_R.
+    double* RtS = alloc<double>((size_t)b * kW);
+    double* Qzd = alloc<double>((size_t)cols * b);
+    if (!RtS || !Qzd) return HT_ERR_ALLOC;
+    std::vector<double> inv(b * kW, 0.0);
+    for (int i = 0; i < kW; ++i) inv[i + (size_t)b * i] = hs[i] > 0 ? 1.0 / hs[i] : 0.0;
+    double* D = alloc<double>((size_t)b * kW);
+    HT_HIP_CHECK(hipMemcpyAsync(D, inv.data(), sizeof(double) * b * kW, hipMemcpyHostToDevice, st));
+    double* RtSfull = alloc<double>((size_t)b * b);
+    CK(dgemm_small_launch(st, 1, 0, b, b, b, 1.0, R, b, Sv, b, 0.0, RtSfull, b));
+    CK(dgemm_small_launch(st, 0, 0, b, kW, b, 1.0, RtSfull, b, D, b, 0.0, RtS, b));
+    CK(cast_f32_f64_launch(st, Qz, Qzd, cols * b));
+    CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, b, 1.0, Qzd, (int)cols, RtS, b, 0.0, Wr, (int)cols));
+    CK(canon_signs_launch(st, Wr, (int)cols, kW, (int)cols));
+  }
+  return sync();
+}
+
+// B_t = (U_l (x) U_r)^T U_t (reshaped k_l x k_r x k_t), all FP64 on the device.
+int Engine::contract_transfer(const double* Ut, const double* Ul, const double* Ur, long nl, long nr, int kl,
+                              int kr, int kt, double* B) {
+  double* T1 = alloc<double>((size_t)kl * nr * kt);
+  if (!T1) return HT_ERR_ALLOC;
+  CK(proj_pms_f64_launch(st, Ut, PMS{1, nl, nr * kt}, Ul, kl, T1));       // (kl, nr, kt)
+  CK(proj_pms_f64_launch(st, T1, PMS{kl, nr, kt}, Ur, kr, B));           // (kl, kr, kt)
+  return 0;
+}
+
+// Expand the full (rows_t x k_t) basis of node t from the HT representation (FP64).
+int Engine::expand(int t, double** out) {
+  const Problem& p = *prob;
+  if (p.left[t] < 0) {
+    *out = Ufull[t];
+    return 0;
+  }
+  double *Ul, *Ur;
+  CK(expand(p.left[t], &Ul));
+  CK(expand(p.right[t], &Ur));
+  const int l = p.left[t], r = p.right[t];
+  const long nl = p.rows[l], nr = p.rows[r];
+  const int kl = p.rank[l], kr = p.rank[r], kt = p.rank[t];
+  // out[il + nl ir, c] = sum_ab Ul[il,a] Ur[ir,b] B[a,b,c]: expand with U^T copies.
+  double* UlT = alloc<double>((size_t)nl * kl);
+  double* UrT = alloc<double>((size_t)nr * kr);
+  double* T1 = alloc<double>((size_t)nl * kr * kt);
+  double* E = alloc<double>((size_t)nl * nr * kt);
+  if (!UlT || !UrT || !T1 || !E) return HT_ERR_ALLOC;
+  CK(transpose_f64(Ul, nl, kl, UlT));
+  CK(transpose_f64(Ur, nr, kr, UrT));
+  CK(proj_pms_f64_launch(st, Bd[t], PMS{1, kl, (long)kr * kt}, UlT, (int)nl, T1));  // (nl, kr, kt)
+  CK(proj_pms_f64_launch(st, T1, PMS{nl, kr, kt}, UrT, (int)nr, E));                // (nl, nr, kt)
+  *out = E;
+  return 0;
+}
+
+int Engine::transpose_f64(const double* A, long m, long n, double* AT) {
+  // AT (n x m) via a copy with swapped strides: use dgemm with identity-free form.
+  return transpose_launch(st, A, m, n, AT);
+}
+
+// ||X - X_HT||^2 via one low-rank residual pass over X_(root-left) (N_l x N_r).
+int Engine::residual(const float* x, double* err_sq, double* x_sq) {
+  const Problem& p = *prob;
+  const int l = p.left[0], r = p.right[0];
+  const long nl = p.rows[l], nr = p.rows[r];
+  const int kl = p.rank[l], kr = p.rank[r];
+  double *Ul, *Ur;
+  CK(expand(l, &Ul));
+  CK(expand(r, &Ur));
+  // Rr = Ur B_root^T (nr x kl)
+  double* Rr = alloc<double>((size_t)nr * kl);
+  float* Lf = alloc<float>((size_t)nl * kl);
+  float* Rf = alloc<float>((size_t)nr * kl);
+  if (!Rr || !Lf || !Rf) return HT_ERR_ALLOC;
+  CK(dgemm_small_launch(st, 0, 1, (int)nr, kl, kr, 1.0, Ur, (int)nr, Bd[0], kl, 0.0, Rr, (int)nr));
+  CK(cast_f64_f32_launch(st, Ul, Lf, nl * kl));
+  CK(cast_f64_f32_launch(st, Rr, Rf, nr * kl));
+  const long nbx = (nl + 127) / 128, nby = (nr + 127) / 128;
+  double* acc = alloc<double>((size_t)2 * nbx * nby);
+  if (!acc) return HT_ERR_ALLOC;
+  int nb = 0;
+  CK(lowrank_launch(st, 1, Lf, Rf, nl, nr, kl, x, nullptr, acc, &nb));
+  double* red = alloc<double>(2 + 2048);
+  CK(sum_strided_launch(st, acc, nb, red));
+  double h[2];
+  CK(d2h(h, red, sizeof(h)));
+  *err_sq = h[0];
+  *x_sq = h[1];
+  passes += 1.0;
+  return 0;
+}
+
+// ------------------------------------------------------------------ methods
+int Engine::leaves_from_x_plain(const float* x, double* xsq_dev) {
+  const Problem& p = *prob;
+  bool have_norm = false;
+  for (int t = 0; t < p.n_nodes; ++t) {
+    if (p.left[t] >= 0) continue;
+    const int m = p.lo[t];
+    const long n = p.dims[m];
+    if (n > 1024) return HT_ERR_UNSUPPORTED;
+    const PMS v = view_of(p.dims, p.order, m, m + 1);
+    double* G = alloc<double>((size_t)n * n);
+    if (!G) return HT_ERR_ALLOC;
+    CK(gram(x, v, PREC_F32, G, have_norm ? nullptr : xsq_dev));
+    have_norm = true;
+    Ufull[t] = alloc<double>((size_t)n * p.rank[t]);
+    double* w = alloc<double>(p.rank[t]);
+    CK(sym_topk(G, (int)n, p.rank[t], Ufull[t], w, opt.eig_tol * 1e3, opt.max_eig_iters, 1000 + t * 16));
+    CK(eig_to_sv(w, p.rank[t], SVd[t]));
+  }
+  return 0;
+}
+
+int Engine::eig_to_sv(const double* w, int k, double*& sv) {
+  sv = alloc<double>(k);
+  return sqrt_clamp_launch(st, w, k, sv);
+}
+
+int Engine::run_plain(const float* x) {
+  const Problem& p = *prob;
+  double* xsq = alloc<double>(1);
+  CK(leaves_from_x_plain(x, xsq));
+  // Interior non-root nodes: exact basis of X_(t).
+  std::vector<int> done(p.n_nodes, 0);
+  for (int t = 1; t < p.n_nodes; ++t) {
+    if (p.left[t] < 0 || done[t]) continue;
+    const PMS v = view_of(p.dims, p.order, p.lo[t], p.hi[t]);
+    const long rows = v.M, cols = v.P * v.S;
+    const int k = p.rank[t];
+    Ufull[t] = alloc<double>((size_t)rows * k);
+    if (rows <= 1024) {
+      double* G = alloc<double>((size_t)rows * rows);
+      double* w = alloc<double>(k);
+      CK(gram(x, v, PREC_F32, G, nullptr));
+      CK(sym_topk(G, (int)rows, k, Ufull[t], w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16));
+      CK(eig_to_sv(w, k, SVd[t]));
+    } else if (cols <= 1024 && (p.lo[t] == 0 || p.hi[t] == p.order)) {
+      // Gram of the complementary (contiguous) mode range, then U = X_(t) V / sigma via TSQR.
+      const PMS vc = p.lo[t] == 0 ? PMS{rows, cols, 1} : PMS{1, cols, rows};
+      double* G = alloc<double>((size_t)cols * cols);
+      double* Vc = alloc<double>((size_t)cols * k);
+      double* w = alloc<double>(k);
+      CK(gram(x, vc, PREC_F32, G, nullptr));
+      CK(sym_topk(G, (int)cols, k, Vc, w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16));
+      CK(eig_to_sv(w, k, SVd[t]));
+      float* Vf = alloc<float>((size_t)cols * k);
+      float* Y = alloc<float>((size_t)rows * k);
+      float* Qf = alloc<float>((size_t)rows * k);
+      double* R = alloc<double>((size_t)k * k);
+      CK(cast_f64_f32_launch(st, Vc, Vf, cols * k));
+      CK(view_gemm(x, v, 0, Vf, k, PREC_F32, Y));
+      CK(tsqr(Y, rows, k, Qf, R));
+      CK(cast_f32_f64_launch(st, Qf, Ufull[t], rows * k));
+      CK(canon_signs_launch(st, Ufull[t], (int)rows, k, (int)rows));
+    } else if (p.lo[t] == 0 || p.hi[t] == p.order) {
+      // Both dimensions large: subspace iteration; the complementary sibling (a root
+      // child) takes the right singular vectors of the same problem.
+      int sib = -1;
+      if (p.parent[t] == 0) sib = (p.left[0] == t) ? p.right[0] : p.left[0];
+      const int kW = (sib >= 0 && p.left[sib] >= 0) ? p.rank[sib] : 0;
+      double* Wr = kW ? alloc<double>((size_t)cols * kW) : nullptr;
+      const int kk = std::max(k, kW);
+      double* sv = alloc<double>(kk);
+      CK(exact_svd_subspace(x, v, k, kW, Ufull[t], Wr, sv, 3000 + t * 64));
+      SVd[t] = sv;
+      if (kW) {
+        Ufull[sib] = Wr;
+        SVd[sib] = sv;
+        done[sib] = 1;
+      }
+    } else {
+      return HT_ERR_UNSUPPORTED;
+    }
+    done[t] = 1;
+  }
+  // Transfer tensors of interior non-root nodes.
+  for (int t = 1; t < p.n_nodes; ++t) {
+    if (p.left[t] < 0) continue;
+    const int l = p.left[t], r = p.right[t];
+    Bd[t] = alloc<double>((size_t)p.rank[l] * p.rank[r] * p.rank[t]);
+    CK(contract_transfer(Ufull[t], Ufull[l], Ufull[r], p.rows[l], p.rows[r], p.rank[l], p.rank[r], p.rank[t], Bd[t]));
+  }
+  // Root: B_root = U_l^T X_(l) U_r (one pass over X).
+  CK(root_transfer(x));
+  return finish_error(x, xsq);
+}
+
+int Engine::root_transfer(const float* x) {
+  const Problem& p = *prob;
+  const int l = p.left[0], r = p.right[0];
+  const long nl = p.rows[l], nr = p.rows[r];
+  const int kl = p.rank[l], kr = p.rank[r];
+  float* Ulf = alloc<float>((size_t)nl * kl);
+  float* Y = alloc<float>((size_t)nr * kl);
+  double* Yd = alloc<double>((size_t)nr * kl);
+  Bd[0] = alloc<double>((size_t)kl * kr);
+  if (!Ulf || !Y || !Yd || !Bd[0]) return HT_ERR_ALLOC;
+  CK(cast_f64_f32_launch(st, Ufull[l], Ulf, nl * kl));
+  const PMS v{1, nl, nr};  // X as (nl x nr), A^T U_l
+  CK(view_gemm(x, v, 1, Ulf, kl, PREC_F32, Y));
+  CK(cast_f32_f64_launch(st, Y, Yd, nr * kl));
+  CK(dgemm_small_launch(st, 1, 0, kl, kr, (int)nr, 1.0, Yd, (int)nr, Ufull[r], (int)nr, 0.0, Bd[0], kl));
+  return 0;
+}
+
+int Engine::finish_error(const float* x, const double* xsq_dev) {
+  double xsq = 0.0;
+  CK(d2h(&xsq, xsq_dev, sizeof(double)));
+  x_sq = xsq;
+  if (!std::isfinite(xsq)) return HT_ERR_NONFINITE;
+  rel_err = NAN;
+  if (opt.err_mode == HT_ERR_MODE_IDENTITY) {
+    const Problem& p = *prob;
+    const int n = p.rank[p.left[0]] * p.rank[p.right[0]];
+    double* out = alloc<double>(2 + 1024);
+    CK(sumsq_f64_launch(st, Bd[0], n, out));
+    double bsq = 0.0;
+    CK(d2h(&bsq, out, sizeof(double)));
+    rel_err = xsq > 0 ? std::sqrt(std::max(xsq - bsq, 0.0) / xsq) : 0.0;
+  } else if (opt.err_mode == HT_ERR_MODE_RESIDUAL) {
+    double esq = 0.0, xs2 = 0.0;
+    CK(residual(x, &esq, &xs2));
+    rel_err = xs2 > 0 ? std::sqrt(esq / xs2) : 0.0;
+  }
+  return 0;
+}
+
+// Leaves-to-root method.
+int Engine::run_ltr(const float* x) {
+  const Problem& p = *prob;
+  const int d = p.order;
+  const int prec = opt.precision == HT_PREC_FP32 ? PREC_F32 : (opt.precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
+  const int t0 = p.leaf_of_mode(0), t1 = p.leaf_of_mode(1);
+  const long n0 = p.dims[0], n1 = p.dims[1];
+  const int r0 = p.rank[t0], r1 = p.rank[t1];
+  double* xsq = alloc<double>(1);
+  const bool slab = n0 <= 128 && n1 <= 128 && n0 % 16 == 0 && n1 % 16 == 0 && r0 <= 32 && r1 <= 32 &&
+                    prec != PREC_F32 && (((uintptr_t)x & 15) == 0);
+  const long S = p.N / (n0 * n1);
+  // Pass 1: Grams of the two leading modes.
+  double* G0 = alloc<double>((size_t)n0 * n0);
+  double* G1 = alloc<double>((size_t)n1 * n1);
+  if (!G0 || !G1) return HT_ERR_ALLOC;
+  if (slab) {
+    int nb = (int)std::min<long>(S, 1024);
+    float* p1 = alloc<float>((size_t)nb * 128 * 128);
+    float* p2 = alloc<float>((size_t)nb * 128 * 128);
+    double* norms = alloc<double>(nb);
+    if (!p1 || !p2 || !norms) return HT_ERR_ALLOC;
+    CK(slab_gram_launch(st, x, (int)n0, (int)n1, S, prec, nb, p1, p2, norms));
+    CK(gram_reduce_launch(st, p1, (int)n0, nb, G0));
+    CK(gram_reduce_launch(st, p2, (int)n1, nb, G1));
+    CK(gram_reduce_norms_launch(st, norms, nb, xsq));
+    passes += 1.0;
+  } else {
+    if (n0 > 1024 || n1 > 1024) return HT_ERR_UNSUPPORTED;
+    CK(gram(x, view_of(p.dims, d, 0, 1), prec, G0, xsq));
+    CK(gram(x, view_of(p.dims, d, 1, 2), prec, G1, nullptr));
+  }
+  Ufull[t0] = alloc<double>((size_t)n0 * r0);
+  Ufull[t1] = alloc<double>((size_t)n1 * r1);
+  double* w0 = alloc<double>(r0);
+  double* w1 = alloc<double>(r1);
+  CK(sym_topk(G0, (int)n0, r0, Ufull[t0], w0, 1e-9, opt.max_eig_iters, 1000 + t0 * 16));
+  CK(sym_topk(G1, (int)n1, r1, Ufull[t1], w1, 1e-9, opt.max_eig_iters, 1000 + t1 * 16));
+  CK(eig_to_sv(w0, r0, SVd[t0]));
+  CK(eig_to_sv(w1, r1, SVd[t1]));
+  // Pass 2: Z = X x_1 U_0^T x_2 U_1^T  (r0, r1, n_2, ...), FP32.
+  float* U0f = alloc<float>((size_t)n0 * r0);
+  float* U1f = alloc<float>((size_t)n1 * r1);
+  float* Z = alloc<float>((size_t)r0 * r1 * S);
+  if (!U0f || !U1f || !Z) return HT_ERR_ALLOC;
+  CK(cast_f64_f32_launch(st, Ufull[t0], U0f, n0 * r0));
+  CK(cast_f64_f32_launch(st, Ufull[t1], U1f, n1 * r1));
+  if (slab) {
+    CK(slab_proj_launch(st, x, (int)n0, (int)n1, S, U0f, r0, U1f, r1, Z));
+  } else {
+    float* Z1 = alloc<float>((size_t)r0 * n1 * S);
+    if (!Z1) return HT_ERR_ALLOC;
+    CK(proj_pms_f32_launch(st, x, PMS{1, n0, n1 * S}, U0f, r0, Z1));
+    CK(proj_pms_f32_launch(st, Z1, PMS{r0, n1, S}, U1f, r1, Z));
+  }
+  passes += 1.0;
+  // Remaining leaves from Z; then the core C (FP64).
+  int64_t zd[HT_MAX_ORDER];
+  zd[0] = r0;
+  zd[1] = r1;
+  for (int m = 2; m < d; ++m) zd[m] = p.dims[m];
+  for (int m = 2; m < d; ++m) {
+    const int t = p.leaf_of_mode(m);
+    const long n = p.dims[m];
+    if (n > 1024) return HT_ERR_UNSUPPORTED;
+    double* G = alloc<double>((size_t)n * n);
+    CK(gram(Z, view_of(zd, d, m, m + 1), PREC_F32, G, nullptr));
+    passes -= 1.0;  // Z is small; not a pass over X
+    Ufull[t] = alloc<double>((size_t)n * p.rank[t]);
+    double* w = alloc<double>(p.rank[t]);
+    CK(sym_topk(G, (int)n, p.rank[t], Ufull[t], w, 1e-9, opt.max_eig_iters, 1000 + t * 16));
+    CK(eig_to_sv(w, p.rank[t], SVd[t]));
+  }
+  long csize = r0 * r1 * S;
+  double* C = alloc<double>((size_t)csize);
+  CK(cast_f32_f64_launch(st, Z, C, csize));
+  int64_t cd[HT_MAX_ORDER];
+  for (int m = 0; m < d; ++m) cd[m] = zd[m];
+  for (int m = 2; m < d; ++m) {
+    const int t = p.leaf_of_mode(m);
+    const int k = p.rank[t];
+    const PMS v = view_of(cd, d, m, m + 1);
+    double* C2 = alloc<double>((size_t)(csize / cd[m]) * k);
+    CK(proj_pms_f64_launch(st, C, v, Ufull[t], k, C2));
+    csize = csize / cd[m] * k;
+    cd[m] = k;
+    C = C2;
+  }
+  return ltr_interior(C, cd, xsq, x);
+}
+
+// Interior nodes on the leaf core (FP64), deepest first; then the root.
+int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const float* x) {
+  const Problem& p = *prob;
+  int d = p.order;
+  std::vector<int64_t> cd(cd_in, cd_in + d);
+  std::vector<int> slots(d);
+  for (int m = 0; m < d; ++m) slots[m] = p.leaf_of_mode(m);
+  std::vector<int> todo;
+  for (int t = 1; t < p.n_nodes; ++t)
+    if (p.left[t] >= 0) todo.push_back(t);
+  std::sort(todo.begin(), todo.end(), [&](int a, int b) {
+    const int da = p.depth(a), db = p.depth(b);
+    return da != db ? da > db : p.lo[a] < p.lo[b];
+  });
+  for (int t : todo) {
+    const int l = p.left[t], r = p.right[t];
+    int i = -1;
+    for (int s = 0; s + 1 < (int)slots.size(); ++s)
+      if (slots[s] == l && slots[s + 1] == r) i = s;
+    if (i < 0) return HT_ERR_INVALID_TREE;
+    PMS v{1, cd[i] * cd[i + 1], 1};
+    for (int s = 0; s < i; ++s) v.P *= cd[s];
+    for (int s = i + 2; s < (int)cd.size(); ++s) v.S *= cd[s];
+    const int k = p.rank[t];
+    const long M = v.M;
+    if (M > 4096) return HT_ERR_UNSUPPORTED;
+    double* G = alloc<double>((size_t)M * M);
+    CK(gram_pms_f64_launch(st, C, v, G));
+    double* V = alloc<double>((size_t)M * k);
+    double* w = alloc<double>(k);
+    CK(sym_topk(G, (int)M, k, V, w, opt.eig_tol, opt.max_eig_iters, 4000 + t * 16));
+    CK(eig_to_sv(w, k, SVd[t]));
+    Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself
+    double* C2 = alloc<double>((size_t)v.P * k * v.S);
+    CK(proj_pms_f64_launch(st, C, v, V, k, C2));
+    C = C2;
+    cd.erase(cd.begin() + i + 1);
+    cd[i] = k;
+    slots.erase(slots.begin() + i + 1);
+    slots[i] = t;
+  }
+  if (slots.size() != 2) return HT_ERR_INVALID_TREE;
+  Bd[0] = C;  // (k_l, k_r)
+  return finish_error(x, xsq);
+}
+
+// Randomised root-to-leaves method (4O semantics with oversampling, power iterations,
+// TSQR and an FP64 projected SVD).
+int Engine::run_rsvd(const float* x) {
+  const Problem& p = *prob;
+  const int prec = opt.precision == HT_PREC_FP32 ? PREC_F32 : (opt.precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
+  double* xsq = alloc<double>(1);
+  bool have_norm = false;
+  for (int t = 0; t < p.n_nodes; ++t) {
+    if (p.left[t] >= 0) continue;
+    const int m = p.lo[t];
+    const long n = p.dims[m];
+    if (n > 1024) return HT_ERR_UNSUPPORTED;
+    double* G = alloc<double>((size_t)n * n);
+    CK(gram(x, view_of(p.dims, p.order, m, m + 1), prec, G, have_norm ? nullptr : xsq));
+    have_norm = true;
+    Ufull[t] = alloc<double>((size_t)n * p.rank[t]);
+    double* w = alloc<double>(p.rank[t]);
+    CK(sym_topk(G, (int)n, p.rank[t], Ufull[t], w, 1e-9, opt.max_eig_iters, 1000 + t * 16));
+    CK(eig_to_sv(w, p.rank[t], SVd[t]));
+  }
+  std::vector<int> done(p.n_nodes, 0);
+  for (int t = 1; t < p.n_nodes; ++t) {
+    if (p.left[t] < 0 || done[t]) continue;
+    if (!(p.lo[t] == 0 || p.hi[t] == p.order)) return HT_ERR_UNSUPPORTED;
+    const PMS v = view_of(p.dims, p.order, p.lo[t], p.hi[t]);
+    const long rows = v.M, cols = v.P * v.S;
+    int sib = -1;
+    if (p.parent[t] == 0) sib = (p.left[0] == t) ? p.right[0] : p.left[0];
+    const int k = p.rank[t];
+    const int kW = (sib >= 0 && p.left[sib] >= 0) ? p.rank[sib] : 0;
+    const int kk = std::max(k, kW);
+    const int l = (int)std::min<long>(std::min<long>(kk + opt.oversample, 64), std::min(rows, cols));
+    float* Om = alloc<float>((size_t)cols * l);
+    float* Y = alloc<float>((size_t)rows * l);
+    float* Q = alloc<float>((size_t)rows * l);
+    float* Z = alloc<float>((size_t)cols * l);
+    float* Qz = alloc<float>((size_t)cols * l);
+    double* R = alloc<double>((size_t)l * l);
+    if (!Om || !Y || !Q || !Z || !Qz || !R) return HT_ERR_ALLOC;
+    CK(fill_normals_f32_launch(st, Om, cols * l, opt.seed, 256 + t));
+    CK(view_gemm(x, v, 0, Om, l, prec, Y));
+    CK(tsqr(Y, rows, l, Q, R));
+    for (int q = 0; q < opt.power_iters; ++q) {
+      CK(view_gemm(x, v, 1, Q, l, prec, Z));
+      CK(tsqr(Z, cols, l, Qz, R));
+      CK(view_gemm(x, v, 0, Qz, l, prec, Y));
+      CK(tsqr(Y, rows, l, Q, R));
+    }
+    // A^T Q = Qz Rz  =>  Q^T A = Rz^T Qz^T; SVD of Rz^T via eig(Rz^T Rz).
+    CK(view_gemm(x, v, 1, Q, l, prec, Z));
+    CK(tsqr(Z, cols, l, Qz, R));
+    double* M = alloc<double>((size_t)l * l);
+    double* Sv = alloc<double>((size_t)l * l);
+    double* th = alloc<double>(l);
+    int* info = alloc<int>(4);
+    CK(dgemm_small_launch(st, 1, 0, l, l, l, 1.0, R, l, R, l, 0.0, M, l));  // Rz^T Rz
+    CK(jacobi_eig_launch(st, M, l, l, th, Sv, l, info, 60, 1e-15));
+    ++eig_calls;
+    // Left vectors of A: Q Sv; right vectors: Qz Rz Sv / sigma.
+    double* Qd = alloc<double>((size_t)rows * l);
+    Ufull[t] = alloc<double>((size_t)rows * k);
+    CK(cast_f32_f64_launch(st, Q, Qd, rows * l));
+    CK(dgemm_small_launch(st, 0, 0, (int)rows, k, l, 1.0, Qd, (int)rows, Sv, l, 0.0, Ufull[t], (int)rows));
+    CK(canon_signs_launch(st, Ufull[t], (int)rows, k, (int)rows));
+    double* sv = alloc<double>(l);
+    CK(sqrt_clamp_launch(st, th, l, sv));
+    SVd[t] = sv;
+    if (kW) {
+      double* RS = alloc<double>((size_t)l * kW);
+      double* Qzd = alloc<double>((size_t)cols * l);
+      Ufull[sib] = alloc<double>((size_t)cols * kW);
+      CK(dgemm_small_launch(st, 0, 0, l, kW, l, 1.0, R, l, Sv, l, 0.0, RS, l));
+      CK(scale_cols_inv_launch(st, RS, l, kW, sv));
+      CK(cast_f32_f64_launch(st, Qz, Qzd, cols * l));
+      CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, l, 1.0, Qzd, (int)cols, RS, l, 0.0, Ufull[sib], (int)cols));
+      CK(canon_signs_launch(st, Ufull[sib], (int)cols, kW, (int)cols));
+      SVd[sib] = sv;
+      done[sib] = 1;
+    }
+    done[t] = 1;
+  }
+  for (int t = 1; t < p.n_nodes; ++t) {
+    if (p.left[t] < 0) continue;
+    const int l = p.left[t], r = p.right[t];
+    Bd[t] = alloc<double>((size_t)p.rank[l] * p.rank[r] * p.rank[t]);
+    CK(contract_transfer(Ufull[t], Ufull[l], Ufull[r], p.rows[l], p.rows[r], p.rank[l], p.rank[r], p.rank[t], Bd[t]));
+  }
+  CK(root_transfer(x));
+  return finish_error(x, xsq);
+}
+
+// Write the packed outputs (FP32) into `out` (device).
+int Engine::write_outputs(float* out) {
+  const Problem& p = *prob;
+  for (int t = 0; t < p.n_nodes; ++t) {
+    if (p.u_off[t] >= 0) {
+      double* U = Ufull[t];
+      if (p.left[t] >= 0) CK(expand(t, &U));
+      CK(cast_f64_f32_launch(st, U, out + p.u_off[t], p.rows[t] * p.rank[t]));
+    }
+    if (p.b_off[t] >= 0) {
+      const long n = (long)p.rank[p.left[t]] * p.rank[p.right[t]] * p.rank[t];
+      CK(cast_f64_f32_launch(st, Bd[t], out + p.b_off[t], n));
+    }
+    if (p.s_off[t] >= 0 && SVd[t]) CK(cast_f64_f32_launch(st, SVd[t], out + p.s_off[t], p.rank[t]));
+  }
+  return 0;
+}
+
+int Engine::run(const float* x, float* out) {
+  for (int t = 0; t < HT_MAX_NODES; ++t) {
+    Ufull[t] = nullptr;
+    Bd[t] = nullptr;
+    SVd[t] = nullptr;
+  }
+  passes = 0.0;
+  eig_iters_max = 0;
+  eig_calls = 0;
+  noconv = 0;
+  int s = 0;
+  if (opt.method == HT_METHOD_PLAIN) s = run_plain(x);
+  else if (opt.method == HT_METHOD_LTR) s = run_ltr(x);
+  else if (opt.method == HT_METHOD_RSVD) s = run_rsvd(x);
+  else return HT_ERR_INVALID_ARG;
+  if (s) return s;
+  if (out) CK(write_outputs(out));
+  CK(sync());
+  if (noconv) return HT_ERR_NOCONV;
+  return 0;
+}
+
+}  // namespace ht

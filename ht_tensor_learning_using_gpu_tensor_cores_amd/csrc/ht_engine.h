@@ -1,0 +1,91 @@
+// Internal host-side types of the HT engine.
+#pragma once
+#include <utility>
+#include <vector>
+
+#include "../../include/ht_amd.h"
+#include "ht_common.h"
+
+namespace ht {
+
+// Bump allocator over persistent device chunks; reset() rewinds without freeing, so a
+// repeated call of the same shape allocates nothing after the first.
+struct Arena {
+  std::vector<std::pair<void*, size_t>> chunks;
+  size_t cur = 0, off = 0;
+  void* get(size_t bytes);
+  void reset();
+  void release();
+};
+
+int validate_tree(int order, int n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                  const int32_t* right);
+
+// Validated problem description and packed output layout.
+struct Problem {
+  int order = 0, n_nodes = 0, want_interior_u = 0;
+  int64_t dims[HT_MAX_ORDER] = {};
+  int64_t N = 0;
+  int lo[HT_MAX_NODES], hi[HT_MAX_NODES], left[HT_MAX_NODES], right[HT_MAX_NODES], parent[HT_MAX_NODES];
+  int rank[HT_MAX_NODES];
+  int64_t rows[HT_MAX_NODES];
+  int64_t u_off[HT_MAX_NODES], b_off[HT_MAX_NODES], s_off[HT_MAX_NODES];
+  int64_t total = 0;
+  int setup(int order, const int64_t* dims, int n_nodes, const int32_t* lo, const int32_t* hi,
+            const int32_t* left, const int32_t* right, const int32_t* ranks, int want_interior_u);
+  int depth(int t) const;
+  int leaf_of_mode(int m) const;
+};
+
+struct Engine {
+  hipStream_t st;
+  Arena* arena;
+  const Problem* prob;
+  ht_options opt;
+  // per-node device data (FP64): full bases (leaves; exact interior bases for plain),
+  // transfer tensors, singular values
+  double* Ufull[HT_MAX_NODES];
+  double* Bd[HT_MAX_NODES];
+  double* SVd[HT_MAX_NODES];
+  double passes = 0.0;
+  int eig_iters_max = 0, eig_calls = 0, noconv = 0;
+  double rel_err = 0.0, x_sq = 0.0;
+
+  template <typename T>
+  T* alloc(size_t n);
+  int sync();
+  int d2h(void* dst, const void* src, size_t bytes);
+  int gram(const float* x, PMS v, int prec, double* G, double* sumsq);
+  int sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream);
+  int tsqr(float* Y, long m, int b, float* Q, double* R);
+  int view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y);
+  int exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U, double* Wr, double* sv, uint32_t stream);
+  int contract_transfer(const double* Ut, const double* Ul, const double* Ur, long nl, long nr, int kl, int kr,
+                        int kt, double* B);
+  int expand(int t, double** out);
+  int transpose_f64(const double* A, long m, long n, double* AT);
+  int residual(const float* x, double* err_sq, double* x_sq);
+  int leaves_from_x_plain(const float* x, double* xsq_dev);
+  int eig_to_sv(const double* w, int k, double*& sv);
+  int root_transfer(const float* x);
+  int finish_error(const float* x, const double* xsq_dev);
+  int run_plain(const float* x);
+  int run_ltr(const float* x);
+  int ltr_interior(double* C, int64_t* cd, const double* xsq, const float* x);
+  int run_rsvd(const float* x);
+  int write_outputs(float* out);
+  int run(const float* x, float* out);
+};
+
+}  // namespace ht
+
+struct ht_handle_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  ht::Arena arena;
+  float* xbuf = nullptr;   // device copy of host inputs
+  size_t xbuf_bytes = 0;
+  float* obuf = nullptr;   // device output staging for host outputs
+  size_t obuf_bytes = 0;
+};

@@ -1,0 +1,520 @@
+// Small dense FP64 kernels of the HT engine: the projected eigen problems and helpers.
+//
+// jacobi_eig: symmetric eigen decomposition of an n x n (n <= 64) FP64 matrix by the
+// cyclic parallel Jacobi method (round-robin pair ordering, one work-group, LDS), sorted
+// descending with canonical signs. It is the (k+p)-sized projected solver that replaces
+// cusolverDnSsyevd/Sgesvd at every node (third_order_tensor_decompsition/CUDA/Optimized/
+// kernel.cu:682-718, fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/
+// kernel.cu:267-287, :457-481).
+//
+// cholqr: Cholesky QR (two passes) of a tall FP64 block with replacement of numerically
+// dependent columns by fresh Philox vectors -- the orthonormalisation step of the
+// subspace iteration on small Gram matrices (the reference's QR: geqrf + orgqr,
+// .../Optimized/kernel.cu:289-346).
+#include "ht_common.h"
+
+namespace ht {
+
+constexpr int JMAX = 64;
+constexpr int JLD = JMAX + 1;
+
+__global__ __launch_bounds__(256) void jacobi_eig_kernel(const double* __restrict__ A, int n, int lda,
+                                                         long strideA, double* __restrict__ w, long stridew,
+                                                         double* __restrict__ V, int ldv, long strideV,
+                                                         int* __restrict__ info, int max_sweeps, double tol) {
+  __shared__ double a[JMAX * JLD];
+  __shared__ double v[JMAX * JLD];
+  __shared__ double cs[JMAX / 2], sn[JMAX / 2];
+  __shared__ int pp[JMAX / 2], qq[JMAX / 2];
+  __shared__ double red[8];
+  __shared__ int order[JMAX];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int bidx = blockIdx.x;
+  A += (long)bidx * strideA;
+  w += (long)bidx * stridew;
+  V += (long)bidx * strideV;
+  const int np = (n + 1) & ~1;
+
+  // Load; the padding index (odd n) gets a large negative diagonal so that it stays
+  // decoupled (zero off-diagonals are never rotated) and sorts last.
+  double amax = 0.0;
+  for (int e = tid; e < n * n; e += 256) {
+    const int i = e % n, j = e / n;
+    const double x = A[i + (long)lda * j];
+    a[i * JLD + j] = x;
+    amax = fmax(amax, fabs(x));
+  }
+  {
+    double m = amax;
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (lane == 0) red[wv] = m;
+  }
+  for (int e = tid; e < np * np; e += 256) {
+    const int i = e % np, j = e / np;
+    v[i * JLD + j] = (i == j) ? 1.0 : 0.0;
+    if (i >= n || j >= n) a[i * JLD + j] = 0.0;
+  }
+  __syncthreads();
+  const double scale = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  if (np != n && tid == 0) a[(np - 1) * JLD + (np - 1)] = -(scale * (double)n + 1.0) * 4.0;
+  __syncthreads();
+
+  const int half = np / 2;
+  int converged = 0;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    double off = 0.0, dg = 0.0;
+    for (int e = tid; e < n * n; e += 256) {
+      const int i = e % n, j = e / n;
+      const double x = a[i * JLD + j];
+      if (i == j) dg += x * x;
+      else off += x * x;
+    }
+    off = wave_sum(off);
+    dg = wave_sum(dg);
+    __syncthreads();
+    if (lane == 0) { red[wv] = off; red[4 + wv] = dg; }
+    __syncthreads();
+    off = red[0] + red[1] + red[2] + red[3];
+    dg = red[4] + red[5] + red[6] + red[7];
+    __syncthreads();
+    if (off <= tol * tol * dg || off == 0.0) { converged = 1; break; }
+    for (int step = 0; step < np - 1; ++step) {
+      if (tid < half) {
+        int p, q;
+        if (tid == 0) { p = np - 1; q = step; }
+        else {
+          p = (step + tid) % (np - 1);
+          q = (step - tid + (np - 1)) % (np - 1);
+        }
+        if (p > q) { const int t = p; p = q; q = t; }
+        const double app = a[p * JLD + p], aqq = a[q * JLD + q], apq = a[p * JLD + q];
+        double c = 1.0, s = 0.0;
+        if (apq != 0.0 && fabs(apq) > 1e-300) {
+          const double tau = (aqq - app) / (2.0 * apq);
+          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          c = 1.0 / sqrt(1.0 + t * t);
+          s = t * c;
+        }
+        pp[tid] = p; qq[tid] = q; cs[tid] = c; sn[tid] = s;
+      }
+      __syncthreads();
+      // A <- A J and V <- V J (columns p, q of every pair).
+      for (int e = tid; e < np * half; e += 256) {
+        const int r = e % np, i = e / np;
+        const int p = pp[i], q = qq[i];
+        const double c = cs[i], s = sn[i];
+        const double xp = a[r * JLD + p], xq = a[r * JLD + q];
+        a[r * JLD + p] = c * xp - s * xq;
+        a[r * JLD + q] = s * xp + c * xq;
+        const double yp = v[r * JLD + p], yq = v[r * JLD + q];
+        v[r * JLD + p] = c * yp - s * yq;
+        v[r * JLD + q] = s * yp + c * yq;
+      }
+      __syncthreads();
+      // A <- J^T A (rows p, q).
+      for (int e = tid; e < np * half; e += 256) {
+        const int r = e % np, i = e / np;
+        const int p = pp[i], q = qq[i];
+        const double c = cs[i], s = sn[i];
+        const double xp = a[p * JLD + r], xq = a[q * JLD + r];
+        a[p * JLD + r] = c * xp - s * xq;
+        a[q * JLD + r] = s * xp + c * xq;
+      }
+      __syncthreads();
+      // Exact zeros on the rotated pairs (removes rounding residue).
+      if (tid < half) {
+        const int p = pp[tid], q = qq[tid];
+        if (sn[tid] != 0.0) { a[p * JLD + q] = 0.0; a[q * JLD + p] = 0.0; }
+      }
+      __syncthreads();
+    }
+  }
+  // Sort descending (n <= 64: one thread).
+  if (tid == 0) {
+    for (int i = 0; i < np; ++i) order[i] = i;
+    for (int i = 0; i < np; ++i) {
+      int best = i;
+      for (int j = i + 1; j < np; ++j)
+        if (a[order[j] * JLD + order[j]] > a[order[best] * JLD + order[best]]) best = j;
+      const int t = order[i]; order[i] = order[best]; order[best] = t;
+    }
+    if (info) info[bidx] = converged ? 0 : 1;
+  }
+  __syncthreads();
+  // Output column j = eigenvector order[j], signed so its largest-magnitude entry is > 0.
+  for (int j = wv; j < n; j += 4) {
+    const int src = order[j];
+    double best = -1.0;
+    int bi = 0;
+    for (int r = lane; r < n; r += 64) {
+      const double x = fabs(v[r * JLD + src]);
+      if (x > best) { best = x; bi = r; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+    const double sg = v[bi * JLD + src] < 0.0 ? -1.0 : 1.0;
+    for (int r = lane; r < n; r += 64) V[r + (long)ldv * j] = sg * v[r * JLD + src];
+    if (lane == 0) w[j] = a[src * JLD + src];
+  }
+}
+
+int jacobi_eig_launch(hipStream_t st, const double* A, int n, int lda, double* w, double* V, int ldv, int* info,
+                      int max_sweeps, double tol) {
+  if (n < 1 || n > JMAX) return 8;
+  hipLaunchKernelGGL(jacobi_eig_kernel, dim3(1), dim3(256), 0, st, A, n, lda, 0L, w, 0L, V, ldv, 0L, info,
+                     max_sweeps, tol);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+int jacobi_eig_batched_launch(hipStream_t st, const double* A, int n, long strideA, double* w, long stridew,
+                              double* V, long strideV, int batch, int* info, int max_sweeps, double tol) {
+  if (n < 1 || n > JMAX || batch < 1) return 8;
+  hipLaunchKernelGGL(jacobi_eig_kernel, dim3(batch), dim3(256), 0, st, A, n, n, strideA, w, stridew, V, n,
+                     strideV, info, max_sweeps, tol);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ dgemm_small
+__global__ void dgemm_small_kernel(int ta, int tb, int m, int n, int k, double alpha, const double* __restrict__ A,
+                                   int lda, const double* __restrict__ B, int ldb, double beta,
+                                   double* __restrict__ C, int ldc) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  const int i = (int)(e % m), j = (int)(e / m);
+  double s = 0.0;
+  if (!ta && !tb) for (int l = 0; l < k; ++l) s += A[i + (long)lda * l] * B[l + (long)ldb * j];
+  else if (ta && !tb) for (int l = 0; l < k; ++l) s += A[l + (long)lda * i] * B[l + (long)ldb * j];
+  else if (!ta && tb) for (int l = 0; l < k; ++l) s += A[i + (long)lda * l] * B[j + (long)ldb * l];
+  else for (int l = 0; l < k; ++l) s += A[l + (long)lda * i] * B[j + (long)ldb * l];
+  double* c = C + i + (long)ldc * j;
+  *c = alpha * s + (beta == 0.0 ? 0.0 : beta * (*c));
+}
+
+int dgemm_small_launch(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double* A, int lda,
+                       const double* B, int ldb, double beta, double* C, int ldc) {
+  if (m < 1 || n < 1) return 0;
+  const long tot = (long)m * n;
+  hipLaunchKernelGGL(dgemm_small_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ta, tb, m, n, k,
+                     alpha, A, lda, B, ldb, beta, C, ldc);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ cholqr
+// One work-group. Z (m x b, FP64, ld m) is orthonormalised in place by two Cholesky-QR
+// passes; a column whose Cholesky pivot falls below 1e-13 of the largest diagonal is
+// replaced by a Philox Gaussian vector and the pass restarts. W (b x b) is scratch;
+// R (b x b) receives the product of the two triangular factors.
+__global__ __launch_bounds__(256) void cholqr_kernel(double* __restrict__ Z, int m, int b, double* __restrict__ W,
+                                                     double* __restrict__ Rout, uint64_t seed, uint32_t stream,
+                                                     int* __restrict__ info) {
+  __shared__ double R[JMAX * JLD];
+  __shared__ double Racc[JMAX * JLD];
+  __shared__ int bad;
+  __shared__ double dmax;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < b * b; e += 256) Racc[(e % b) * JLD + e / b] = (e % b == e / b) ? 1.0 : 0.0;
+  int passes = 0, restarts = 0;
+  while (passes < 2 && restarts < 4 * JMAX) {
+    __syncthreads();
+    // W = Z^T Z (upper), one thread per (i <= j).
+    for (int e = tid; e < b * b; e += 256) {
+      const int i = e % b, j = e / b;
+      if (i > j) continue;
+      double s = 0.0;
+      for (int r = 0; r < m; ++r) s += Z[r + (long)m * i] * Z[r + (long)m * j];
+      R[i * JLD + j] = s;
+      R[j * JLD + i] = s;
+    }
+    if (tid == 0) bad = -1;
+    __syncthreads();
+    if (tid == 0) {
+      double d = 0.0;
+      for (int i = 0; i < b; ++i) d = fmax(d, R[i * JLD + i]);
+      dmax = d;
+    }
+    __syncthreads();
+    // Right-looking Cholesky W = R^T R in LDS.
+    for (int j = 0; j < b; ++j) {
+      if (tid == 0) {
+        const double d = R[j * JLD + j];
+        if (!(d > 1e-13 * dmax) || !(dmax > 0.0)) bad = j;
+        else R[j * JLD + j] = sqrt(d);
+      }
+      __syncthreads();
+      if (bad >= 0) break;
+      const double rjj = R[j * JLD + j];
+      for (int c = j + 1 + tid; c < b; c += 256) R[j * JLD + c] /= rjj;
+      __syncthreads();
+      for (int e = tid; e < (b - j - 1) * (b - j - 1); e += 256) {
+        const int c1 = j + 1 + e % (b - j - 1), c2 = j + 1 + e / (b - j - 1);
+        if (c1 <= c2) R[c1 * JLD + c2] -= R[j * JLD + c1] * R[j * JLD + c2];
+      }
+      __syncthreads();
+    }
+    if (bad >= 0) {
+      const int j = bad;
+      for (int r = tid; r < m; r += 256)
+        Z[r + (long)m * j] = philox_normal(seed, stream, (uint64_t)(restarts + 1) * (uint64_t)m * JMAX + (uint64_t)j * m + r);
+      ++restarts;
+      passes = 0;
+      for (int e = tid; e < b * b; e += 256) Racc[(e % b) * JLD + e / b] = (e % b == e / b) ? 1.0 : 0.0;
+      __syncthreads();
+      continue;
+    }
+    // Z <- Z R^{-1}: each thread solves rows independently (forward substitution).
+    for (int r = tid; r < m; r += 256) {
+      double x[JMAX];
+      for (int j = 0; j < b; ++j) {
+        double s = Z[r + (long)m * j];
+        for (int l = 0; l < j; ++l) s -= x[l] * R[l * JLD + j];
+        x[j] = s / R[j * JLD + j];
+      }
+      for (int j = 0; j < b; ++j) Z[r + (long)m * j] = x[j];
+    }
+    __syncthreads();
+    // Racc <- R * Racc (upper triangular).
+    for (int e = tid; e < b * b; e += 256) {
+      const int i = e % b, j = e / b;
+      double s = 0.0;
+      if (i <= j)
+        for (int l = i; l <= j; ++l) s += R[i * JLD + l] * Racc[l * JLD + j];
+      W[i + (long)b * j] = s;
+    }
+    __syncthreads();
+    for (int e = tid; e < b * b; e += 256) Racc[(e % b) * JLD + e / b] = W[(e % b) + (long)b * (e / b)];
+    ++passes;
+  }
+  __syncthreads();
+  if (Rout)
+    for (int e = tid; e < b * b; e += 256) Rout[e] = Racc[(e % b) * JLD + e / b];
+  if (tid == 0 && info) *info = (passes == 2) ? restarts : -1;
+}
+
+int cholqr_launch(hipStream_t st, double* Z, int m, int b, double* W, double* R, uint64_t seed, uint32_t stream,
+                  int* info) {
+  if (b < 1 || b > JMAX || m < b) return 8;
+  hipLaunchKernelGGL(cholqr_kernel, dim3(1), dim3(256), 0, st, Z, m, b, W, R, seed, stream, info);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ elementwise helpers
+__global__ void cast_f64_f32_kernel(const double* __restrict__ a, float* __restrict__ b, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+__global__ void cast_f32_f64_kernel(const float* __restrict__ a, double* __restrict__ b, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = (double)a[i];
+}
+__global__ void fill_normals_f64_kernel(double* __restrict__ a, long n, uint64_t seed, uint32_t stream) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    a[i] = philox_normal(seed, stream, (uint64_t)i);
+}
+__global__ void fill_normals_f32_kernel(float* __restrict__ a, long n, uint64_t seed, uint32_t stream) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    a[i] = (float)philox_normal(seed, stream, (uint64_t)i);
+}
+__global__ void copy_cols_f64_kernel(const double* __restrict__ a, int lda, double* __restrict__ b, int ldb, int m,
+                                     int n) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  const int i = (int)(e % m), j = (int)(e / m);
+  b[i + (long)ldb * j] = a[i + (long)lda * j];
+}
+__global__ void canon_signs_kernel(double* __restrict__ U, int m, int ldu) {
+  __shared__ double bv[4];
+  __shared__ int bix[4];
+  const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double best = -1.0;
+  int bi = 0;
+  for (int r = tid; r < m; r += 256) {
+    const double x = fabs(U[r + (long)ldu * j]);
+    if (x > best) { best = x; bi = r; }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+  }
+  if (lane == 0) { bv[wv] = best; bix[wv] = bi; }
+  __syncthreads();
+  if (tid == 0) {
+    for (int k = 1; k < 4; ++k)
+      if (bv[k] > bv[0] || (bv[k] == bv[0] && bix[k] < bix[0])) { bv[0] = bv[k]; bix[0] = bix[k]; }
+  }
+  __syncthreads();
+  const double sg = U[bix[0] + (long)ldu * j] < 0.0 ? -1.0 : 1.0;
+  if (sg < 0.0)
+    for (int r = tid; r < m; r += 256) U[r + (long)ldu * j] = -U[r + (long)ldu * j];
+}
+template <typename T>
+__global__ void sumsq_stage1(const T* __restrict__ a, long n, double* __restrict__ part) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double x = (double)a[i];
+    s += x * x;
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ void sum_stage2(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+static unsigned grid_for(long n) {
+  long b = (n + 255) / 256;
+  if (b > 16384) b = 16384;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+int cast_f64_f32_launch(hipStream_t st, const double* a, float* b, long n) {
+  hipLaunchKernelGGL(cast_f64_f32_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, b, n);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int cast_f32_f64_launch(hipStream_t st, const float* a, double* b, long n) {
+  hipLaunchKernelGGL(cast_f32_f64_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, b, n);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int fill_normals_f64_launch(hipStream_t st, double* a, long n, uint64_t seed, uint32_t stream) {
+  hipLaunchKernelGGL(fill_normals_f64_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, n, seed, stream);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int fill_normals_f32_launch(hipStream_t st, float* a, long n, uint64_t seed, uint32_t stream) {
+  hipLaunchKernelGGL(fill_normals_f32_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, n, seed, stream);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int copy_cols_f64_launch(hipStream_t st, const double* a, int lda, double* b, int ldb, int m, int n) {
+  const long tot = (long)m * n;
+  if (tot <= 0) return 0;
+  hipLaunchKernelGGL(copy_cols_f64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a, lda, b, ldb, m, n);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int canon_signs_launch(hipStream_t st, double* U, int m, int k, int ldu) {
+  if (k < 1) return 0;
+  hipLaunchKernelGGL(canon_signs_kernel, dim3(k), dim3(256), 0, st, U, m, ldu);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+// out must hold 1 + 1024 doubles (part scratch after the result).
+int sumsq_f64_launch(hipStream_t st, const double* a, long n, double* out) {
+  unsigned g = grid_for(n);
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(sumsq_stage1<double>, dim3(g), dim3(256), 0, st, a, n, out + 1);
+  hipLaunchKernelGGL(sum_stage2, dim3(1), dim3(256), 0, st, out + 1, (int)g, out);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int sumsq_f32_launch(hipStream_t st, const float* a, long n, double* out) {
+  unsigned g = grid_for(n);
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(sumsq_stage1<float>, dim3(g), dim3(256), 0, st, a, n, out + 1);
+  hipLaunchKernelGGL(sum_stage2, dim3(1), dim3(256), 0, st, out + 1, (int)g, out);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ Ritz residuals
+// out[i] = ||GU[:,i] - w[i] U[:,i]||_2 for i < k (one block per column).
+__global__ void ritz_resid_kernel(const double* __restrict__ GU, const double* __restrict__ U,
+                                  const double* __restrict__ w, int n, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int i = blockIdx.x;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const double d = GU[r + (long)n * i] - w[i] * U[r + (long)n * i];
+    s += d * d;
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[i] = sqrt(red[0] + red[1] + red[2] + red[3]);
+}
+int ritz_resid_launch(hipStream_t st, const double* GU, const double* U, const double* w, int n, int k,
+                      double* out) {
+  if (k < 1) return 0;
+  hipLaunchKernelGGL(ritz_resid_kernel, dim3(k), dim3(256), 0, st, GU, U, w, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ FP64 Gram of a view
+// G(i,j) = sum_{p,s} x[p + P i + P M s] x[p + P j + P M s] for small tensors (M <= 4096).
+__global__ void gram_pms_f64_kernel(const double* __restrict__ x, PMS v, double* __restrict__ G) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= v.M * v.M) return;
+  const long i = e % v.M, j = e / v.M;
+  if (i > j) return;
+  double s = 0.0;
+  for (long q = 0; q < v.S; ++q) {
+    const double* b = x + v.P * v.M * q;
+    for (long p = 0; p < v.P; ++p) s += b[p + v.P * i] * b[p + v.P * j];
+  }
+  G[i + v.M * j] = s;
+  G[j + v.M * i] = s;
+}
+int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G) {
+  const long tot = v.M * v.M;
+  hipLaunchKernelGGL(gram_pms_f64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, x, v, G);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ misc small kernels
+__global__ void sum_pairs_kernel(const double* __restrict__ acc, int n, double* __restrict__ out) {
+  __shared__ double r0[4], r1[4];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) { a += acc[2 * i]; b += acc[2 * i + 1]; }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) { r0[threadIdx.x >> 6] = a; r1[threadIdx.x >> 6] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = r0[0] + r0[1] + r0[2] + r0[3];
+    out[1] = r1[0] + r1[1] + r1[2] + r1[3];
+  }
+}
+int sum_strided_launch(hipStream_t st, const double* acc, int n, double* out) {
+  hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(256), 0, st, acc, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+__global__ void transpose_f64_kernel(const double* __restrict__ A, long m, long n, double* __restrict__ AT) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m * n) return;
+  const long i = e % m, j = e / m;
+  AT[j + n * i] = A[e];
+}
+int transpose_launch(hipStream_t st, const double* A, long m, long n, double* AT) {
+  const long tot = m * n;
+  if (tot <= 0) return 0;
+  hipLaunchKernelGGL(transpose_f64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A, m, n, AT);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+__global__ void sqrt_clamp_kernel(const double* __restrict__ w, int k, double* __restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < k) s[i] = sqrt(fmax(w[i], 0.0));
+}
+int sqrt_clamp_launch(hipStream_t st, const double* w, int k, double* s) {
+  if (k < 1) return 0;
+  hipLaunchKernelGGL(sqrt_clamp_kernel, dim3((k + 255) / 256), dim3(256), 0, st, w, k, s);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+__global__ void scale_cols_inv_kernel(double* __restrict__ A, int m, int n, const double* __restrict__ s) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)m * n) return;
+  const int j = (int)(e / m);
+  A[e] = s[j] > 0.0 ? A[e] / s[j] : 0.0;
+}
+int scale_cols_inv_launch(hipStream_t st, double* A, int m, int n, const double* s) {
+  const long tot = (long)m * n;
+  if (tot <= 0) return 0;
+  hipLaunchKernelGGL(scale_cols_inv_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A, m, n, s);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+}  // namespace ht

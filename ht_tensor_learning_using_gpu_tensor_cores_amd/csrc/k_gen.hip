@@ -1,0 +1,70 @@
+// Synthetic-input kernels: Philox Gaussian noise and the Gaussian-bump family.
+// The Tucker part X0 = core x_n U_n is a low-rank product built with the lowrank kernel.
+// These replace genHtensor (third_order_tensor_decompsition/CUDA/Baseline/kernel.cu:94-202),
+// whose srand(time(NULL)) input changes from run to run.
+#include "ht_common.h"
+
+namespace ht {
+
+__global__ void noise_sumsq_kernel(long n, uint64_t seed, uint32_t stream, double* __restrict__ part) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double z = philox_normal(seed, stream, (uint64_t)i);
+    s += z * z;
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void add_noise_kernel(float* __restrict__ x, long n, uint64_t seed, uint32_t stream, double scale) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = (float)((double)x[i] + scale * philox_normal(seed, stream, (uint64_t)i));
+}
+
+// x[i1 + n i2 + ...] = sum_j amp[j] prod_m factors[(j*order + m)*n + i_m]
+__global__ void bumps_kernel(float* __restrict__ x, int order, long n, const float* __restrict__ factors,
+                             const float* __restrict__ amp, int n_bumps) {
+  long total = 1;
+  for (int m = 0; m < order; ++m) total *= n;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    long idx[8];
+    long r = e;
+    for (int m = 0; m < order; ++m) { idx[m] = r % n; r /= n; }
+    double s = 0.0;
+    for (int j = 0; j < n_bumps; ++j) {
+      double t = amp[j];
+      for (int m = 0; m < order; ++m) t *= factors[((long)j * order + m) * n + idx[m]];
+      s += t;
+    }
+    x[e] = (float)s;
+  }
+}
+
+int gen_noise_sumsq_launch(hipStream_t st, long n, uint64_t seed, uint32_t stream, double* acc, int* nb) {
+  long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  *nb = (int)blocks;
+  hipLaunchKernelGGL(noise_sumsq_kernel, dim3((unsigned)blocks), dim3(256), 0, st, n, seed, stream, acc);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int gen_add_noise_launch(hipStream_t st, float* x, long n, uint64_t seed, uint32_t stream, double scale) {
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(add_noise_kernel, dim3((unsigned)blocks), dim3(256), 0, st, x, n, seed, stream, scale);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int gen_bumps_launch(hipStream_t st, float* x, int order, long n, const float* factors, const float* amp,
+                     int n_bumps) {
+  if (order < 1 || order > 8) return 8;
+  long total = 1;
+  for (int m = 0; m < order; ++m) total *= n;
+  long blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(bumps_kernel, dim3((unsigned)blocks), dim3(256), 0, st, x, order, n, factors, amp, n_bumps);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+}  // namespace ht

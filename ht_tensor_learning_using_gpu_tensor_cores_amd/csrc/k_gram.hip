@@ -1,0 +1,640 @@
+// Leaf Gram kernels of the HT engine (gfx950, MFMA).
+//
+// gram_pms: G = X_(t) X_(t)^T for the (P,M,S) view of a mode range t, read straight
+// from the column-major tensor into LDS tiles with coalesced strided loads -- no
+// permuted copy of X is ever written (the reference materialises mode-2/3 unfoldings
+// first: third_order_tensor_decompsition/CUDA/Baseline/kernel.cu:45-69,
+// fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/kernel.cu:36-63, then
+// calls GemmEx: .../Optimized/htd4_tensorcore.cu:61-103).
+//
+// slab_gram: the Grams of the two leading modes from one read of each n1 x n2 slab.
+// slab_proj: z_s = U1^T S_s U2 per slab (the leaf-core projection of the two leading
+// modes), one wavefront per slab, FP32 MFMA, no LDS.
+#include "ht_common.h"
+
+namespace ht {
+
+// ------------------------------------------------------------------ MFMA layout probe
+// Runs each MFMA shape used by the engine once, on a 64-lane wave, so a test can check
+// the assumed operand/accumulator lane layouts against a host matmul.
+//   a, b: 32x16 (row-major) operands for the 16-bit shapes; the first 32x2 / 16x4 part
+//   (row-major, k fastest) is used for the f32/f64 shapes.
+__global__ void mfma_probe_kernel(const float* a, const float* b, float* o_bf16_32, float* o_f32_32,
+                                  float* o_f32_16, double* o_f64_16, float* o_bf16_16) {
+  const int l = threadIdx.x;
+  // 32x32x16 bf16: A[row=l%32][k=8*(l/32)+i], B[k=8*(l/32)+i][col=l%32]; b is stored as
+  // B^T (col-major B) row-major 32x16, i.e. b[col*16 + k].
+  {
+    u16x8 av, bv;
+    for (int i = 0; i < 8; ++i) {
+      av[i] = f2bf_bits(a[(l & 31) * 16 + 8 * (l >> 5) + i]);
+      bv[i] = f2bf_bits(b[(l & 31) * 16 + 8 * (l >> 5) + i]);
+    }
+    f32x16 acc = {};
+    acc = mfma32x32x16<PREC_BF16>(av, bv, acc);
+    for (int e = 0; e < 16; ++e) o_bf16_32[acc32_row(e, l) * 32 + (l & 31)] = acc[e];
+  }
+  // 32x32x2 f32: A[row=l%32][k=l/32], B[k=l/32][col=l%32].
+  {
+    float av = a[(l & 31) * 16 + (l >> 5)];
+    float bv = b[(l & 31) * 16 + (l >> 5)];
+    f32x16 acc = {};
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+    for (int e = 0; e < 16; ++e) o_f32_32[acc32_row(e, l) * 32 + (l & 31)] = acc[e];
+  }
+  // 16x16x4 f32: A[row=l%16][k=l/16], B[k=l/16][col=l%16].
+  {
+    float av = a[(l & 15) * 16 + (l >> 4)];
+    float bv = b[(l & 15) * 16 + (l >> 4)];
+    f32x4 acc = {};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+    for (int e = 0; e < 4; ++e) o_f32_16[acc16_row(e, l) * 16 + (l & 15)] = acc[e];
+  }
+  // 16x16x4 f64: same operand layout.
+  {
+    double av = a[(l & 15) * 16 + (l >> 4)];
+    double bv = b[(l & 15) * 16 + (l >> 4)];
+    f64x4 acc = {};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    for (int e = 0; e < 4; ++e) o_f64_16[acc16_row(e, l) * 16 + (l & 15)] = acc[e];
+  }
+  // 16x16x32 bf16: A[row=l%16][k=8*(l/16)+i], B[k=8*(l/16)+i][col=l%16]; a, b as 16x32.
+  {
+    u16x8 av, bv;
+    for (int i = 0; i < 8; ++i) {
+      av[i] = f2bf_bits(a[(l & 15) * 32 + 8 * (l >> 4) + i]);
+      bv[i] = f2bf_bits(b[(l & 15) * 32 + 8 * (l >> 4) + i]);
+    }
+    f32x4 acc = {};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
+                                                  __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
+    for (int e = 0; e < 4; ++e) o_bf16_16[acc16_row(e, l) * 16 + (l & 15)] = acc[e];
+  }
+}
+
+int mfma_probe_launch(hipStream_t st, const float* a, const float* b, float* o1, float* o2, float* o3,
+                      double* o4, float* o5) {
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, st, a, b, o1, o2, o3, o4, o5);
+  return (int)hipGetLastError() == 0 ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ gram_pms
+// Tile: 128 rows of the matricisation x 64 columns per chunk; 256 threads = 4 waves,
+// wave w owns the 64x64 quadrant (w>>1, w&1) of the 128x128 output tile pair.
+constexpr int GT_ROWS = 128;
+constexpr int GT_K = 64;
+
+enum Loader { LD_ROWS_CONTIG = 0, LD_COLS_CONTIG = 1, LD_GENERIC = 2 };
+
+template <int PREC>
+struct GramLds;
+template <>
+struct GramLds<PREC_F32> {
+  static constexpr int LD = GT_K + 4;  // floats per row
+  typedef float T;
+};
+template <>
+struct GramLds<PREC_BF16> {
+  static constexpr int LD = GT_K + 8;  // 16-bit elements per row (row = 144 bytes)
+  typedef uint16_t T;
+};
+template <>
+struct GramLds<PREC_F16> {
+  static constexpr int LD = GT_K + 8;
+  typedef uint16_t T;
+};
+
+// Global -> registers for one 128 x 64 tile (rows m0.., columns col0..), 32 floats.
+template <int LOADER>
+__device__ __forceinline__ void gram_load(const float* __restrict__ x, PMS v, long K, int m0, long col0,
+                                          int tid, float (&r)[32]) {
+  if constexpr (LOADER == LD_COLS_CONTIG) {
+    // P % 64 == 0: the 64 columns share one s; each row is 64 contiguous floats.
+    const long s = col0 / v.P;
+    const long p0 = col0 - s * v.P;
+    const float* base = x + p0 + v.P * v.M * s;
+    const int kq = tid & 15;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int row = (tid >> 4) + 16 * i;
+      const long m = m0 + row;
+      float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (m < v.M) q = *reinterpret_cast<const float4*>(base + v.P * m + 4 * kq);
+      r[4 * i + 0] = q.x; r[4 * i + 1] = q.y; r[4 * i + 2] = q.z; r[4 * i + 3] = q.w;
+    }
+  } else if constexpr (LOADER == LD_ROWS_CONTIG) {
+    // P == 1, M % 4 == 0: column c is M contiguous floats; 4 rows per float4.
+    const int rq = tid & 31;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = (tid >> 5) + 8 * i;
+      const long c = col0 + k;
+      const long m = m0 + 4 * rq;
+      float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (m < v.M && c < K) q = *reinterpret_cast<const float4*>(x + m + v.M * c);
+      r[4 * i + 0] = q.x; r[4 * i + 1] = q.y; r[4 * i + 2] = q.z; r[4 * i + 3] = q.w;
+    }
+  } else {
+#pragma unroll 4
+    for (int i = 0; i < 32; ++i) {
+      const int e = tid + 256 * i;
+      const int row = e >> 6, k = e & 63;
+      const long m = m0 + row, c = col0 + k;
+      float val = 0.f;
+      if (m < v.M && c < K) {
+        const long s = c / v.P, p = c - s * v.P;
+        val = x[p + v.P * m + v.P * v.M * s];
+      }
+      r[i] = val;
+    }
+  }
+}
+
+// Registers -> LDS tile [row][k].
+template <int PREC, int LOADER>
+__device__ __forceinline__ void gram_store(typename GramLds<PREC>::T* __restrict__ T, int tid,
+                                           const float (&r)[32]) {
+  constexpr int LD = GramLds<PREC>::LD;
+  if constexpr (LOADER == LD_COLS_CONTIG) {
+    const int kq = tid & 15;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int row = (tid >> 4) + 16 * i;
+      if constexpr (PREC == PREC_F32) {
+        *reinterpret_cast<float4*>(&T[row * LD + 4 * kq]) =
+            make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+      } else {
+        uint2 w;
+        w.x = (uint32_t)to16<PREC>(r[4 * i]) | ((uint32_t)to16<PREC>(r[4 * i + 1]) << 16);
+        w.y = (uint32_t)to16<PREC>(r[4 * i + 2]) | ((uint32_t)to16<PREC>(r[4 * i + 3]) << 16);
+        *reinterpret_cast<uint2*>(&T[row * LD + 4 * kq]) = w;
+      }
+    }
+  } else if constexpr (LOADER == LD_ROWS_CONTIG) {
+    const int rq = tid & 31;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = (tid >> 5) + 8 * i;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if constexpr (PREC == PREC_F32) T[(4 * rq + j) * LD + k] = r[4 * i + j];
+        else T[(4 * rq + j) * LD + k] = to16<PREC>(r[4 * i + j]);
+      }
+    }
+  } else {
+#pragma unroll 4
+    for (int i = 0; i < 32; ++i) {
+      const int e = tid + 256 * i;
+      const int row = e >> 6, k = e & 63;
+      if constexpr (PREC == PREC_F32) T[row * LD + k] = r[i];
+      else T[row * LD + k] = to16<PREC>(r[i]);
+    }
+  }
+}
+
+template <int PREC>
+__device__ __forceinline__ void gram_compute(const typename GramLds<PREC>::T* __restrict__ Ai,
+                                             const typename GramLds<PREC>::T* __restrict__ Aj, int wr,
+                                             int wc, int lane, f32x16 (&acc)[2][2]) {
+  constexpr int LD = GramLds<PREC>::LD;
+  if constexpr (PREC == PREC_F32) {
+#pragma unroll 4
+    for (int kk = 0; kk < GT_K / 2; ++kk) {
+      float af[2], bf[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) af[a] = Ai[(64 * wr + 32 * a + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bf[b] = Aj[(64 * wc + 32 * b + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < GT_K / 16; ++kk) {
+      u16x8 af[2], bf[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        af[a] = *reinterpret_cast<const u16x8*>(&Ai[(64 * wr + 32 * a + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+        bf[b] = *reinterpret_cast<const u16x8*>(&Aj[(64 * wc + 32 * b + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = mfma32x32x16<PREC>(af[a], bf[b], acc[a][b]);
+    }
+  }
+}
+
+template <int PREC, int LOADER>
+__global__ __launch_bounds__(256) void gram_pms_kernel(const float* __restrict__ x, PMS v, long K,
+                                                       long n_chunks, int n_split, int n_tiles,
+                                                       float* __restrict__ part, double* __restrict__ norms) {
+  typedef typename GramLds<PREC>::T T;
+  constexpr int LD = GramLds<PREC>::LD;
+  __shared__ __attribute__((aligned(16))) T Ai[GT_ROWS * LD];
+  __shared__ __attribute__((aligned(16))) T Aj[GT_ROWS * LD];
+  __shared__ double red[4];
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const int split = blockIdx.x, pair = blockIdx.y;
+  int bi = 0, bj = 0;
+  {
+    int p = pair;
+    for (bi = 0; bi < n_tiles; ++bi) {
+      const int cnt = n_tiles - bi;
+      if (p < cnt) { bj = bi + p; break; }
+      p -= cnt;
+    }
+  }
+  const bool diag = (bi == bj);
+  const long c_begin = (long)split * n_chunks / n_split;
+  const long c_end = (long)(split + 1) * n_chunks / n_split;
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x16{};
+
+  float ri[32], rj[32];
+  double sumsq = 0.0;
+  if (c_begin < c_end) {
+    gram_load<LOADER>(x, v, K, GT_ROWS * bi, c_begin * GT_K, tid, ri);
+    if (!diag) gram_load<LOADER>(x, v, K, GT_ROWS * bj, c_begin * GT_K, tid, rj);
+  }
+  for (long c = c_begin; c < c_end; ++c) {
+    __syncthreads();
+    gram_store<PREC, LOADER>(Ai, tid, ri);
+    if (!diag) gram_store<PREC, LOADER>(Aj, tid, rj);
+    if (diag) {
+      float s = 0.f;
+#pragma unroll
+      for (int i = 0; i < 32; ++i) s += ri[i] * ri[i];
+      sumsq += (double)s;
+    }
+    __syncthreads();
+    if (c + 1 < c_end) {
+      gram_load<LOADER>(x, v, K, GT_ROWS * bi, (c + 1) * GT_K, tid, ri);
+      if (!diag) gram_load<LOADER>(x, v, K, GT_ROWS * bj, (c + 1) * GT_K, tid, rj);
+    }
+    gram_compute<PREC>(Ai, diag ? Ai : Aj, wr, wc, lane, acc);
+  }
+
+  float* out = part + ((long)pair * n_split + split) * (GT_ROWS * GT_ROWS);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int row = 64 * wr + 32 * a + acc32_row(e, lane);
+        const int col = 64 * wc + 32 * b + (lane & 31);
+        out[row * GT_ROWS + col] = acc[a][b][e];
+      }
+  sumsq = wave_sum(sumsq);
+  if (lane == 0) red[w] = sumsq;
+  __syncthreads();
+  if (tid == 0) norms[(long)pair * n_split + split] = red[0] + red[1] + red[2] + red[3];
+}
+
+int gram_pms_nsplit(PMS v, int target_blocks) {
+  const long K = v.P * v.S;
+  const long n_chunks = (K + GT_K - 1) / GT_K;
+  const int nt = (int)((v.M + GT_ROWS - 1) / GT_ROWS);
+  const int pairs = nt * (nt + 1) / 2;
+  long ns = target_blocks / pairs;
+  if (ns < 1) ns = 1;
+  if (ns > n_chunks) ns = n_chunks;
+  return (int)ns;
+}
+
+template <int PREC>
+static void gram_dispatch(hipStream_t st, const float* x, PMS v, int n_split, float* part, double* norms) {
+  const long K = v.P * v.S;
+  const long n_chunks = (K + GT_K - 1) / GT_K;
+  const int nt = (int)((v.M + GT_ROWS - 1) / GT_ROWS);
+  const int pairs = nt * (nt + 1) / 2;
+  dim3 grid(n_split, pairs);
+  const bool aligned = ((uintptr_t)x & 15) == 0;
+  if (aligned && v.P % 64 == 0)
+    hipLaunchKernelGGL((gram_pms_kernel<PREC, LD_COLS_CONTIG>), grid, dim3(256), 0, st, x, v, K, n_chunks,
+                       n_split, nt, part, norms);
+  else if (aligned && v.P == 1 && v.M % 4 == 0)
+    hipLaunchKernelGGL((gram_pms_kernel<PREC, LD_ROWS_CONTIG>), grid, dim3(256), 0, st, x, v, K, n_chunks,
+                       n_split, nt, part, norms);
+  else
+    hipLaunchKernelGGL((gram_pms_kernel<PREC, LD_GENERIC>), grid, dim3(256), 0, st, x, v, K, n_chunks, n_split,
+                       nt, part, norms);
+}
+
+int gram_pms_launch(hipStream_t st, const float* x, PMS v, int prec, int n_split, float* part, double* norms) {
+  if (v.M <= 0 || v.M > 1024 || v.P <= 0 || v.S <= 0 || n_split < 1) return 8;
+  if (prec == PREC_F32) gram_dispatch<PREC_F32>(st, x, v, n_split, part, norms);
+  else if (prec == PREC_F16) gram_dispatch<PREC_F16>(st, x, v, n_split, part, norms);
+  else gram_dispatch<PREC_BF16>(st, x, v, n_split, part, norms);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// Sum the split partials of every tile pair into the full symmetric FP64 Gram.
+__global__ void gram_reduce_kernel(const float* __restrict__ part, int M, int nt, int n_split,
+                                   double* __restrict__ G) {
+  const int pair = blockIdx.y;
+  int bi = 0, bj = 0;
+  {
+    int p = pair;
+    for (bi = 0; bi < nt; ++bi) {
+      const int cnt = nt - bi;
+      if (p < cnt) { bj = bi + p; break; }
+      p -= cnt;
+    }
+  }
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= GT_ROWS * GT_ROWS) return;
+  const int r = e / GT_ROWS, c = e % GT_ROWS;
+  const int i = GT_ROWS * bi + r, j = GT_ROWS * bj + c;
+  if (i >= M || j >= M) return;
+  if (bi == bj && r > c) return;
+  const float* p = part + (long)pair * n_split * (GT_ROWS * GT_ROWS) + e;
+  double s = 0.0;
+  for (int k = 0; k < n_split; ++k) s += (double)p[(long)k * (GT_ROWS * GT_ROWS)];
+  G[i + (long)M * j] = s;
+  G[j + (long)M * i] = s;
+}
+
+int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G) {
+  const int nt = (M + GT_ROWS - 1) / GT_ROWS;
+  const int pairs = nt * (nt + 1) / 2;
+  dim3 grid((GT_ROWS * GT_ROWS + 255) / 256, pairs);
+  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(256), 0, st, part, M, nt, n_split, G);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+__global__ void sum_f64_kernel(const double* __restrict__ a, int n, double* __restrict__ out) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += a[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = red[0] + red[1] + red[2] + red[3];
+}
+
+int gram_reduce_norms_launch(hipStream_t st, const double* norms, int n, double* out) {
+  hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, st, norms, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ slab_gram
+// 512 threads = 8 waves. Waves 0-3: quadrants of G1 = sum_s S S^T (A operand rows i,
+// k = j, from the row-major copy); waves 4-7: quadrants of G2 = sum_s S^T S (rows j,
+// k = i, from the column-major copy). Slabs are padded to 128 x 128 with zeros.
+template <int PREC>
+__global__ __launch_bounds__(512) void slab_gram_kernel(const float* __restrict__ x, int n1, int n2, long S,
+                                                        int n_blocks, float* __restrict__ part1,
+                                                        float* __restrict__ part2, double* __restrict__ norms) {
+  constexpr int LD = 128 + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t Srm[128 * LD];  // [i][j]
+  __shared__ __attribute__((aligned(16))) uint16_t Scm[128 * LD];  // [j][i]
+  __shared__ double red[8];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int g = w >> 2, q = w & 3, wr = q >> 1, wc = q & 1;
+  for (int i = tid; i < 128 * LD; i += 512) { Srm[i] = 0; Scm[i] = 0; }
+
+  const long s_begin = (long)blockIdx.x * S / n_blocks;
+  const long s_end = (long)(blockIdx.x + 1) * S / n_blocks;
+  const long slab = (long)n1 * n2;
+  const bool vec = ((n1 & 3) == 0) && (((uintptr_t)x & 15) == 0);
+  const int n4 = (int)(slab / 4);
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x16{};
+  float r[32];
+  double sumsq = 0.0;
+
+  auto load = [&](long s) {
+    const float* base = x + s * slab;
+    if (vec) {
+#pragma unroll
+      for (int it = 0; it < 8; ++it) {
+        const int f = tid + 512 * it;
+        float4 v4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (f < n4) v4 = reinterpret_cast<const float4*>(base)[f];
+        r[4 * it] = v4.x; r[4 * it + 1] = v4.y; r[4 * it + 2] = v4.z; r[4 * it + 3] = v4.w;
+      }
+    } else {
+#pragma unroll 4
+      for (int it = 0; it < 32; ++it) {
+        const int e = tid + 512 * it;
+        r[it] = (e < slab) ? base[e] : 0.f;
+      }
+    }
+  };
+  auto store = [&]() {
+    if (vec) {
+#pragma unroll
+      for (int it = 0; it < 8; ++it) {
+        const int f = tid + 512 * it;
+        if (f < n4) {
+          const int e = 4 * f;
+          const int i = e % n1, j = e / n1;
+          uint16_t h0 = to16<PREC>(r[4 * it]), h1 = to16<PREC>(r[4 * it + 1]);
+          uint16_t h2 = to16<PREC>(r[4 * it + 2]), h3 = to16<PREC>(r[4 * it + 3]);
+          uint2 wv;
+          wv.x = (uint32_t)h0 | ((uint32_t)h1 << 16);
+          wv.y = (uint32_t)h2 | ((uint32_t)h3 << 16);
+          *reinterpret_cast<uint2*>(&Scm[j * LD + i]) = wv;
+          Srm[(i + 0) * LD + j] = h0;
+          Srm[(i + 1) * LD + j] = h1;
+          Srm[(i + 2) * LD + j] = h2;
+          Srm[(i + 3) * LD + j] = h3;
+          sumsq += (double)(r[4 * it] * r[4 * it] + r[4 * it + 1] * r[4 * it + 1] +
+                            r[4 * it + 2] * r[4 * it + 2] + r[4 * it + 3] * r[4 * it + 3]);
+        }
+      }
+    } else {
+#pragma unroll 4
+      for (int it = 0; it < 32; ++it) {
+        const int e = tid + 512 * it;
+        if (e < slab) {
+          const int i = e % n1, j = e / n1;
+          const uint16_t h = to16<PREC>(r[it]);
+          Scm[j * LD + i] = h;
+          Srm[i * LD + j] = h;
+          sumsq += (double)(r[it] * r[it]);
+        }
+      }
+    }
+  };
+
+  if (s_begin < s_end) load(s_begin);
+  for (long s = s_begin; s < s_end; ++s) {
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (s + 1 < s_end) load(s + 1);
+    const uint16_t* T = g == 0 ? Srm : Scm;
+    const int kmax = g == 0 ? n2 : n1;
+    for (int kk = 0; kk < (kmax + 15) / 16; ++kk) {
+      u16x8 af[2], bf[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        af[a] = *reinterpret_cast<const u16x8*>(&T[(64 * wr + 32 * a + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+        bf[b] = *reinterpret_cast<const u16x8*>(&T[(64 * wc + 32 * b + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = mfma32x32x16<PREC>(af[a], bf[b], acc[a][b]);
+    }
+  }
+  float* out = (g == 0 ? part1 : part2) + (long)blockIdx.x * (128 * 128);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int row = 64 * wr + 32 * a + acc32_row(e, lane);
+        const int col = 64 * wc + 32 * b + (lane & 31);
+        out[row * 128 + col] = acc[a][b][e];
+      }
+  sumsq = wave_sum(sumsq);
+  if (lane == 0) red[w] = sumsq;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int i = 0; i < 8; ++i) t += red[i];
+    norms[blockIdx.x] = t;
+  }
+}
+
+int slab_gram_launch(hipStream_t st, const float* x, int n1, int n2, long S, int prec, int n_blocks,
+                     float* part1, float* part2, double* norms) {
+  if (n1 < 1 || n2 < 1 || n1 > 128 || n2 > 128 || S < 1 || n_blocks < 1) return 8;
+  if (prec == PREC_F16)
+    hipLaunchKernelGGL(slab_gram_kernel<PREC_F16>, dim3(n_blocks), dim3(512), 0, st, x, n1, n2, S, n_blocks,
+                       part1, part2, norms);
+  else
+    hipLaunchKernelGGL(slab_gram_kernel<PREC_BF16>, dim3(n_blocks), dim3(512), 0, st, x, n1, n2, S, n_blocks,
+                       part1, part2, norms);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ slab_proj
+// One wavefront per slab S (n1 x n2, column-major, n1 % 16 == 0, n2 % 16 == 0,
+// r1, r2 <= 32). FP32 MFMA 16x16x4 with the reduction index permuted so that every
+// lane reads float4 runs of a slab column:
+//   T^T (n2 x r1) = S^T U1   -- A operand S^T: lane (j = l%16, g = l/16) at step s uses
+//                               i = i0 + 4g + s, i.e. S[i0+4g .. i0+4g+3][j] (one float4);
+//   Z^T (r2 x r1) = U2^T T^T -- the T^T accumulators are already in B-operand layout
+//                               (their element e is row 4g + e = the slot of step e).
+template <int RB1, int RB2>
+__global__ __launch_bounds__(256) void slab_proj_kernel(const float* __restrict__ x, int n1, int n2, long S,
+                                                        const float* __restrict__ U1, int r1,
+                                                        const float* __restrict__ U2, int r2,
+                                                        float* __restrict__ z) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, g = lane >> 4;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long n_waves = ((long)gridDim.x * blockDim.x) >> 6;
+  const int n1g = n1 / 16;  // row groups of 16 (<= 8)
+  const int nb2 = n2 / 16;  // column blocks of 16 (<= 8)
+
+  // u1r[ig][st][a] = U1[16 ig + 4g + st][16 a + col]; u2r[bb][st][b] = U2[16 bb + 4g + st][16 b + col].
+  float u1r[8][4][RB1], u2r[8][4][RB2];
+#pragma unroll
+  for (int ig = 0; ig < 8; ++ig)
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const int i = 16 * ig + 4 * g + st;
+#pragma unroll
+      for (int a = 0; a < RB1; ++a) {
+        const int ac = 16 * a + col;
+        u1r[ig][st][a] = (ig < n1g && ac < r1) ? U1[i + (long)n1 * ac] : 0.f;
+      }
+#pragma unroll
+      for (int b = 0; b < RB2; ++b) {
+        const int bc = 16 * b + col;
+        u2r[ig][st][b] = (ig < nb2 && bc < r2) ? U2[i + (long)n2 * bc] : 0.f;
+      }
+    }
+
+  for (long s = wave; s < S; s += n_waves) {
+    const float* Sb = x + s * (long)n1 * n2;
+    f32x4 zt[RB2][RB1];
+#pragma unroll
+    for (int b = 0; b < RB2; ++b)
+#pragma unroll
+      for (int a = 0; a < RB1; ++a) zt[b][a] = f32x4{};
+#pragma unroll
+    for (int bb = 0; bb < 8; ++bb) {
+      if (bb >= nb2) break;
+      const float* colp = Sb + (long)(16 * bb + col) * n1 + 4 * g;
+      float4 sv[8];
+#pragma unroll
+      for (int ig = 0; ig < 8; ++ig)
+        sv[ig] = (ig < n1g) ? *reinterpret_cast<const float4*>(colp + 16 * ig) : make_float4(0.f, 0.f, 0.f, 0.f);
+      f32x4 tt[RB1];
+#pragma unroll
+      for (int a = 0; a < RB1; ++a) tt[a] = f32x4{};
+#pragma unroll
+      for (int ig = 0; ig < 8; ++ig) {
+        const float svs[4] = {sv[ig].x, sv[ig].y, sv[ig].z, sv[ig].w};
+#pragma unroll
+        for (int st = 0; st < 4; ++st)
+#pragma unroll
+          for (int a = 0; a < RB1; ++a)
+            tt[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(svs[st], u1r[ig][st][a], tt[a], 0, 0, 0);
+      }
+      // tt[a] element e: T^T[row 16bb + 4g + e][col 16a + col].
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+#pragma unroll
+        for (int b = 0; b < RB2; ++b)
+#pragma unroll
+          for (int a = 0; a < RB1; ++a)
+            zt[b][a] = __builtin_amdgcn_mfma_f32_16x16x4f32(u2r[bb][st][b], tt[a][st], zt[b][a], 0, 0, 0);
+    }
+    // zt[b][a] element e: Z^T[row 16b + 4g + e][col 16a + col] = Z[a'][b'].
+    float* zs = z + s * (long)r1 * r2;
+#pragma unroll
+    for (int b = 0; b < RB2; ++b)
+#pragma unroll
+      for (int a = 0; a < RB1; ++a)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int bp = 16 * b + 4 * g + e, ap = 16 * a + col;
+          if (ap < r1 && bp < r2) zs[ap + (long)r1 * bp] = zt[b][a][e];
+        }
+  }
+}
+
+int slab_proj_launch(hipStream_t st, const float* x, int n1, int n2, long S, const float* U1, int r1,
+                     const float* U2, int r2, float* z) {
+  if (n1 % 16 || n2 % 16 || n1 > 128 || n2 > 128 || r1 < 1 || r2 < 1 || r1 > 32 || r2 > 32) return 8;
+  if (((uintptr_t)x & 15) != 0) return 8;
+  long waves = S;
+  long blocks = (waves + 3) / 4;
+  if (blocks > 4096) blocks = 4096;
+  dim3 grid((unsigned)blocks);
+  if (r1 <= 16 && r2 <= 16)
+    hipLaunchKernelGGL((slab_proj_kernel<1, 1>), grid, dim3(256), 0, st, x, n1, n2, S, U1, r1, U2, r2, z);
+  else if (r1 <= 16)
+    hipLaunchKernelGGL((slab_proj_kernel<1, 2>), grid, dim3(256), 0, st, x, n1, n2, S, U1, r1, U2, r2, z);
+  else if (r2 <= 16)
+    hipLaunchKernelGGL((slab_proj_kernel<2, 1>), grid, dim3(256), 0, st, x, n1, n2, S, U1, r1, U2, r2, z);
+  else
+    hipLaunchKernelGGL((slab_proj_kernel<2, 2>), grid, dim3(256), 0, st, x, n1, n2, S, U1, r1, U2, r2, z);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+}  // namespace ht

@@ -1,0 +1,321 @@
+// TSQR of tall-skinny FP32 blocks (m x b, b <= 64) as a Householder reduction tree.
+//
+// The reference's tsqr_svd splits A into 4 row blocks on the host with a CPU loop over
+// every element, QRs each block after a host-to-device copy, combines [R1;R2], [R3;R4]
+// and the root, then assembles Q with GEMMs (third_order_tensor_decompsition/CUDA/TSQR/
+// kernel.cu:500-677). Here the whole tree runs on the device:
+//   leaves  -- balanced blocks of <= 256 rows, one work-group each, Householder QR with
+//              FP64 arithmetic in LDS; the reflectors overwrite the block in W (FP32),
+//              R goes up;
+//   tree    -- 4-ary: the stacked R factors of up to four children (<= 256 x b) are
+//              QR-factored in FP64 by one work-group; the root R is FP64;
+//   Q       -- formed explicitly top-down: each node applies its reflectors to [E; 0]
+//              with E the b x b block its parent handed down, leaves write Q rows.
+// Thread layout inside a block QR: lane = column c, wave q = quarter of the rows.
+#include "ht_common.h"
+
+namespace ht {
+
+constexpr int TQ_ROWS = 256;
+constexpr int TQ_B = 64;
+constexpr int TQ_FAN = 4;
+
+struct TsqrLevel {
+  long n_nodes;
+  double* R;    // [n_nodes][b*b] (column-major b x b)
+  double* V;    // [n_nodes][TQ_ROWS*b] reflectors (row-major rows x b), tree levels only
+  double* tau;  // [n_nodes][b]
+  double* E;    // [n_nodes][b*b] block handed down by the parent (column-major)
+  int* rows;    // [n_nodes] stacked rows of the node
+};
+
+// Householder QR of the rows x b matrix in LDS A[r*TQ_B + c]; on return the upper
+// triangle holds R, the strict lower part the reflector tails (unit diagonal implied).
+__device__ void block_householder(double* A, double* tau, double* red, int rows, int b) {
+  const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
+  const int rq = (rows + 3) / 4;
+  const int r_lo = q * rq, r_hi = min(rows, r_lo + rq);
+  __shared__ double sh_beta, sh_scale, sh_tau;
+  for (int j = 0; j < b && j < rows; ++j) {
+    // norm^2 of column j from row j down
+    if (c == j) {
+      double s = 0.0;
+      for (int r = max(r_lo, j + 1); r < r_hi; ++r) s += A[r * TQ_B + j] * A[r * TQ_B + j];
+      red[q * TQ_B + j] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const double sub = red[0 * TQ_B + j] + red[1 * TQ_B + j] + red[2 * TQ_B + j] + red[3 * TQ_B + j];
+      const double alpha = A[j * TQ_B + j];
+      if (sub == 0.0) {
+        sh_tau = 0.0; sh_beta = alpha; sh_scale = 0.0;
+      } else {
+        const double nrm = sqrt(alpha * alpha + sub);
+        const double beta = alpha >= 0.0 ? -nrm : nrm;
+        sh_tau = (beta - alpha) / beta;
+        sh_scale = 1.0 / (alpha - beta);
+        sh_beta = beta;
+      }
+      tau[j] = sh_tau;
+    }
+    __syncthreads();
+    const double tj = sh_tau;
+    if (c == j) {
+      for (int r = max(r_lo, j + 1); r < r_hi; ++r) A[r * TQ_B + j] *= sh_scale;
+    }
+    __syncthreads();
+    if (tid == 0) A[j * TQ_B + j] = sh_beta;
+    // w_c = sum_{r>=j} v_r A[r][c] for c > j
+    if (c > j && c < b && tj != 0.0) {
+      double s = 0.0;
+      for (int r = max(r_lo, j); r < r_hi; ++r) {
+        const double vr = (r == j) ? 1.0 : A[r * TQ_B + j];
+        s += vr * A[r * TQ_B + c];
+      }
+      red[q * TQ_B + c] = s;
+    }
+    __syncthreads();
+    if (c > j && c < b && tj != 0.0) {
+      const double wc = tj * (red[0 * TQ_B + c] + red[1 * TQ_B + c] + red[2 * TQ_B + c] + red[3 * TQ_B + c]);
+      for (int r = max(r_lo, j); r < r_hi; ++r) {
+        const double vr = (r == j) ? 1.0 : A[r * TQ_B + j];
+        A[r * TQ_B + c] -= vr * wc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Reflector accessors: tree nodes keep FP64 reflectors in global memory (row-major
+// rows x TQ_B), leaves keep them in place in W (FP32, column-major, ld m).
+struct VGlobal64 {
+  const double* V;
+  __device__ double operator()(int r, int j) const { return V[r * TQ_B + j]; }
+};
+struct VLeaf32 {
+  const float* W;
+  long r0, m;
+  __device__ double operator()(int r, int j) const { return (double)W[r0 + r + m * j]; }
+};
+
+// M (rows x b in LDS) <- H_0 ... H_{b-1} M with the reflectors given by V.
+template <class VA>
+__device__ void apply_q(VA V, const double* tau, double* M, double* red, int rows, int b) {
+  const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
+  const int rq = (rows + 3) / 4;
+  const int r_lo = q * rq, r_hi = min(rows, r_lo + rq);
+  for (int j = min(b, rows) - 1; j >= 0; --j) {
+    const double tj = tau[j];
+    if (tj == 0.0) continue;  // uniform across the block
+    if (c < b) {
+      double s = 0.0;
+      for (int r = max(r_lo, j); r < r_hi; ++r) {
+        const double vr = (r == j) ? 1.0 : V(r, j);
+        s += vr * M[r * TQ_B + c];
+      }
+      red[q * TQ_B + c] = s;
+    }
+    __syncthreads();
+    if (c < b) {
+      const double wc = tj * (red[0 * TQ_B + c] + red[1 * TQ_B + c] + red[2 * TQ_B + c] + red[3 * TQ_B + c]);
+      for (int r = max(r_lo, j); r < r_hi; ++r) {
+        const double vr = (r == j) ? 1.0 : V(r, j);
+        M[r * TQ_B + c] -= vr * wc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Leaf QR. With n_leaf = ceil(m/256) blocks, block i holds rows [i m / n_leaf,
+// (i+1) m / n_leaf): at most 256 and, when n_leaf > 1, at least 128 >= b rows, so every
+// reflector lives in rows the block owns and can be stored in place in W (FP32). A
+// single block (m <= 256) keeps its reflectors in FP64 (V1, needed when m < b).
+__global__ __launch_bounds__(256) void tsqr_leaf_kernel(float* __restrict__ W, long m, int b, long n_leaf,
+                                                        double* __restrict__ Rout, double* __restrict__ tauout,
+                                                        double* __restrict__ V1) {
+  __shared__ double A[TQ_ROWS * TQ_B];
+  __shared__ double red[4 * TQ_B];
+  __shared__ double tau[TQ_B];
+  const int tid = threadIdx.x;
+  const long r0 = (long)blockIdx.x * m / n_leaf;
+  const long r1 = (long)(blockIdx.x + 1) * m / n_leaf;
+  const int nr = (int)(r1 - r0);
+  const int rows = max(nr, b);
+  for (int e = tid; e < TQ_ROWS * TQ_B; e += 256) {
+    const int r = e / TQ_B, c = e % TQ_B;
+    A[e] = (r < nr && c < b) ? (double)W[r0 + r + m * c] : 0.0;
+  }
+  __syncthreads();
+  block_householder(A, tau, red, rows, b);
+  double* R = Rout + (long)blockIdx.x * b * b;
+  for (int e = tid; e < b * b; e += 256) {
+    const int i = e % b, j = e / b;
+    R[e] = (i <= j) ? A[i * TQ_B + j] : 0.0;
+  }
+  if (n_leaf == 1) {
+    for (int e = tid; e < TQ_ROWS * TQ_B; e += 256) V1[e] = A[e];
+  } else {
+    for (int e = tid; e < nr * b; e += 256) {
+      const int r = e % nr, c = e / nr;
+      W[r0 + r + m * c] = (float)A[r * TQ_B + c];
+    }
+  }
+  if (tid < b) tauout[(long)blockIdx.x * b + tid] = tau[tid];
+}
+
+// Tree node QR: node i stacks the R factors of children FAN*i .. FAN*i+FAN-1.
+__global__ __launch_bounds__(256) void tsqr_tree_kernel(const double* __restrict__ Rchild, long n_child, int b,
+                                                        double* __restrict__ Rout, double* __restrict__ Vout,
+                                                        double* __restrict__ tauout, int* __restrict__ rows_out) {
+  __shared__ double A[TQ_ROWS * TQ_B];
+  __shared__ double red[4 * TQ_B];
+  __shared__ double tau[TQ_B];
+  const int tid = threadIdx.x;
+  const long c0 = (long)blockIdx.x * TQ_FAN;
+  const int nch = (int)min((long)TQ_FAN, n_child - c0);
+  const int rows = nch * b;
+  for (int e = tid; e < TQ_ROWS * TQ_B; e += 256) {
+    const int r = e / TQ_B, c = e % TQ_B;
+    double val = 0.0;
+    if (r < rows && c < b) {
+      const int ch = r / b, i = r % b;
+      val = Rchild[(c0 + ch) * b * b + i + (long)b * c];
+    }
+    A[e] = val;
+  }
+  __syncthreads();
+  block_householder(A, tau, red, rows, b);
+  double* R = Rout + (long)blockIdx.x * b * b;
+  for (int e = tid; e < b * b; e += 256) {
+    const int i = e % b, j = e / b;
+    R[e] = (i <= j) ? A[i * TQ_B + j] : 0.0;
+  }
+  double* V = Vout + (long)blockIdx.x * TQ_ROWS * TQ_B;
+  for (int e = tid; e < rows * TQ_B; e += 256) V[e] = A[e];
+  if (tid < b) tauout[(long)blockIdx.x * b + tid] = tau[tid];
+  if (tid == 0) rows_out[blockIdx.x] = rows;
+}
+
+// Top-down through a tree level: node i receives E_i (b x b) and hands the blocks of
+// H_i [E_i; 0] to its children.
+__global__ __launch_bounds__(256) void tsqr_down_kernel(const double* __restrict__ V, const double* __restrict__ tauv,
+                                                        const int* __restrict__ rows_in, const double* __restrict__ E,
+                                                        int b, int identity, double* __restrict__ Echild) {
+  __shared__ double M[TQ_ROWS * TQ_B];
+  __shared__ double red[4 * TQ_B];
+  __shared__ double tau[TQ_B];
+  const int tid = threadIdx.x;
+  const long node = blockIdx.x;
+  const int rows = rows_in[node];
+  for (int e = tid; e < TQ_ROWS * TQ_B; e += 256) {
+    const int r = e / TQ_B, c = e % TQ_B;
+    double val = 0.0;
+    if (r < b && c < b) val = identity ? (r == c ? 1.0 : 0.0) : E[node * b * b + r + (long)b * c];
+    M[e] = val;
+  }
+  if (tid < b) tau[tid] = tauv[node * b + tid];
+  __syncthreads();
+  apply_q(VGlobal64{V + node * TQ_ROWS * TQ_B}, tau, M, red, rows, b);
+  const int nch = rows / b;
+  for (int e = tid; e < nch * b * b; e += 256) {
+    const int ch = e / (b * b), rem = e % (b * b);
+    const int i = rem % b, c = rem / b;
+    Echild[(node * TQ_FAN + ch) * b * b + i + (long)b * c] = M[(ch * b + i) * TQ_B + c];
+  }
+}
+
+// Leaves: Q rows = H_leaf [E_leaf; 0].
+__global__ __launch_bounds__(256) void tsqr_leafq_kernel(const float* __restrict__ W, long m, int b, long n_leaf,
+                                                         const double* __restrict__ tauv, const double* __restrict__ E,
+                                                         int identity, const double* __restrict__ V1,
+                                                         float* __restrict__ Q) {
+  __shared__ double M[TQ_ROWS * TQ_B];
+  __shared__ double red[4 * TQ_B];
+  __shared__ double tau[TQ_B];
+  const int tid = threadIdx.x;
+  const long r0 = (long)blockIdx.x * m / n_leaf;
+  const long r1 = (long)(blockIdx.x + 1) * m / n_leaf;
+  const int nr = (int)(r1 - r0);
+  const int rows = max(nr, b);
+  for (int e = tid; e < TQ_ROWS * TQ_B; e += 256) {
+    const int r = e / TQ_B, c = e % TQ_B;
+    double val = 0.0;
+    if (r < b && c < b) val = identity ? (r == c ? 1.0 : 0.0) : E[(long)blockIdx.x * b * b + r + (long)b * c];
+    M[e] = val;
+  }
+  if (tid < b) tau[tid] = tauv[(long)blockIdx.x * b + tid];
+  __syncthreads();
+  if (n_leaf == 1) apply_q(VGlobal64{V1}, tau, M, red, rows, b);
+  else apply_q(VLeaf32{W, r0, m}, tau, M, red, rows, b);
+  for (int e = tid; e < nr * b; e += 256) {
+    const int r = e % nr, c = e / nr;
+    Q[r0 + r + m * c] = (float)M[r * TQ_B + c];
+  }
+}
+
+static long ceil_div(long a, long b) { return (a + b - 1) / b; }
+
+size_t tsqr_ws_bytes(long m, int b) {
+  size_t bytes = 0;
+  long n = ceil_div(m, TQ_ROWS);
+  bytes += (size_t)n * (b * b + b + b * b) * sizeof(double) + 256 * 3;  // leaf R, tau, E (+ alignment)
+  bytes += (size_t)TQ_ROWS * TQ_B * sizeof(double) + 256;                // single-leaf reflectors
+  while (n > 1) {
+    n = ceil_div(n, TQ_FAN);
+    bytes += (size_t)n * ((size_t)b * b * 2 + b + (size_t)TQ_ROWS * TQ_B) * sizeof(double) + n * sizeof(int) + 5 * 256;
+  }
+  return bytes + 4096;
+}
+
+int tsqr_launch(hipStream_t st, float* W, long m, int b, float* Q, double* Rfinal, void* ws) {
+  if (b < 1 || b > TQ_B || m < 1) return 8;
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) {
+    char* r = p;
+    p += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  const long n_leaf = ceil_div(m, TQ_ROWS);
+  double* leafR = (double*)take((size_t)n_leaf * b * b * sizeof(double));
+  double* leafTau = (double*)take((size_t)n_leaf * b * sizeof(double));
+  double* leafE = (double*)take((size_t)n_leaf * b * b * sizeof(double));
+  double* V1 = (double*)take((size_t)TQ_ROWS * TQ_B * sizeof(double));
+  TsqrLevel lv[24];
+  int nl = 0;
+  long n = n_leaf;
+  while (n > 1) {
+    const long nn = ceil_div(n, TQ_FAN);
+    TsqrLevel& L = lv[nl++];
+    L.n_nodes = nn;
+    L.R = (double*)take((size_t)nn * b * b * sizeof(double));
+    L.V = (double*)take((size_t)nn * TQ_ROWS * TQ_B * sizeof(double));
+    L.tau = (double*)take((size_t)nn * b * sizeof(double));
+    L.E = (double*)take((size_t)nn * b * b * sizeof(double));
+    L.rows = (int*)take((size_t)nn * sizeof(int));
+    n = nn;
+    if (nl >= 24) return 8;
+  }
+  hipLaunchKernelGGL(tsqr_leaf_kernel, dim3((unsigned)n_leaf), dim3(256), 0, st, W, m, b, n_leaf, leafR, leafTau, V1);
+  const double* Rc = leafR;
+  long nc = n_leaf;
+  for (int i = 0; i < nl; ++i) {
+    hipLaunchKernelGGL(tsqr_tree_kernel, dim3((unsigned)lv[i].n_nodes), dim3(256), 0, st, Rc, nc, b, lv[i].R,
+                       lv[i].V, lv[i].tau, lv[i].rows);
+    Rc = lv[i].R;
+    nc = lv[i].n_nodes;
+  }
+  // Root R.
+  if (Rfinal) hipMemcpyAsync(Rfinal, Rc, (size_t)b * b * sizeof(double), hipMemcpyDeviceToDevice, st);
+  // Down the tree.
+  for (int i = nl - 1; i >= 0; --i) {
+    double* Echild = (i == 0) ? leafE : lv[i - 1].E;
+    hipLaunchKernelGGL(tsqr_down_kernel, dim3((unsigned)lv[i].n_nodes), dim3(256), 0, st, lv[i].V, lv[i].tau,
+                       lv[i].rows, lv[i].E, b, (i == nl - 1) ? 1 : 0, Echild);
+  }
+  hipLaunchKernelGGL(tsqr_leafq_kernel, dim3((unsigned)n_leaf), dim3(256), 0, st, W, m, b, n_leaf, leafTau, leafE,
+                     nl == 0 ? 1 : 0, V1, Q);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+}  // namespace ht

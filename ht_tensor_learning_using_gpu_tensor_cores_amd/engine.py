@@ -1,0 +1,279 @@
+"""Python front end of the HT engine: a thin layer over the C ABI.
+
+Tensors are column-major (mode 1 fastest), as in the reference
+(third_order_tensor_decompsition/CUDA/Optimized/htd_tensorcore.cu:98-104). Device
+buffers are torch tensors on a ROCm device (torch is used for memory and streams only;
+every computation runs in libht_amd.so). Host inputs are numpy arrays.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _lib
+from .tree import Tree
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Handle:
+    """Owns a HIP stream, a device workspace and events (ht_create / ht_destroy)."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        self._h = ctypes.c_void_p()
+        _lib.check(self.lib.ht_create(ctypes.byref(self._h), int(device)), "ht_create")
+        self.device = int(device)
+
+    @property
+    def ptr(self):
+        if self._h is None or not self._h.value:
+            raise RuntimeError("handle is closed")
+        return self._h
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            self.lib.ht_destroy(self._h)
+        self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default = {}
+
+
+def default_handle(device=0):
+    h = _default.get(device)
+    if h is None:
+        h = Handle(device)
+        _default[device] = h
+    return h
+
+
+def make_options(method="ltr", precision="bf16", err="residual", oversample=8, power_iters=1, seed=1234,
+                 eig_tol=1e-10, max_eig_iters=200, want_interior_u=False):
+    lib = _lib.load()
+    o = _lib.Options()
+    lib.ht_default_options(ctypes.byref(o))
+    o.method = _lib.METHOD[method]
+    o.precision = _lib.PRECISION[precision]
+    o.err_mode = _lib.ERR_MODE[err]
+    o.oversample = int(oversample)
+    o.power_iters = int(power_iters)
+    o.seed = int(seed)
+    o.eig_tol = float(eig_tol)
+    o.max_eig_iters = int(max_eig_iters)
+    o.want_interior_u = int(bool(want_interior_u))
+    return o
+
+
+def output_layout(dims, tree: Tree, ranks, want_interior_u=False):
+    lib = _lib.load()
+    n = tree.n_nodes
+    u = (ctypes.c_int64 * n)()
+    b = (ctypes.c_int64 * n)()
+    s = (ctypes.c_int64 * n)()
+    tot = ctypes.c_int64()
+    _lib.check(lib.ht_output_layout(len(dims), _lib.i64_array(dims), n, _lib.i32_array(tree.lo),
+                                    _lib.i32_array(tree.hi), _lib.i32_array(tree.left), _lib.i32_array(tree.right),
+                                    _lib.i32_array(ranks), int(bool(want_interior_u)), u, b, s,
+                                    ctypes.byref(tot)), "ht_output_layout")
+    return list(u), list(b), list(s), tot.value
+
+
+@dataclass
+class HTensor:
+    """Result of a decomposition: U[t] (rows_t x k_t), B[t] (k_l, k_r, k_t, column-major),
+    sv[t] (k_t singular values), the relative error and the timing statistics."""
+    dims: list
+    tree: Tree
+    ranks: list
+    packed: object
+    U: Dict[int, object] = field(default_factory=dict)
+    B: Dict[int, object] = field(default_factory=dict)
+    sv: Dict[int, object] = field(default_factory=dict)
+    rel_err: float = float("nan")
+    stats: Optional[dict] = None
+    want_interior_u: bool = False
+
+
+def _unpack(res: HTensor, layout):
+    u_off, b_off, s_off, _ = layout
+    tr, rk, pk = res.tree, res.ranks, res.packed
+    rows = {}
+    for t in range(tr.n_nodes):
+        r = 1
+        for m in range(tr.lo[t], tr.hi[t]):
+            r *= res.dims[m]
+        rows[t] = r
+    for t in range(tr.n_nodes):
+        if u_off[t] >= 0:
+            n = rows[t] * rk[t]
+            res.U[t] = pk[u_off[t]:u_off[t] + n].reshape(rk[t], rows[t]).T
+        if b_off[t] >= 0:
+            kl, kr = rk[tr.left[t]], rk[tr.right[t]]
+            n = kl * kr * rk[t]
+            res.B[t] = pk[b_off[t]:b_off[t] + n].reshape(rk[t], kr, kl).permute(2, 1, 0) \
+                if hasattr(pk, "permute") else pk[b_off[t]:b_off[t] + n].reshape((kl, kr, rk[t]), order="F")
+        if s_off[t] >= 0:
+            res.sv[t] = pk[s_off[t]:s_off[t] + rk[t]]
+
+
+def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="residual", handle=None,
+              want_interior_u=False, **kw):
+    """HT decomposition of the column-major FP32 tensor x (torch CUDA tensor or numpy).
+
+    Mirrors the reference drivers htd/htd4 (third_order_tensor_decompsition/CUDA/Baseline/
+    function.h:6, fourth_order_tensor_decomposition/single_GPU/CUDA/Baseline/function.h:6)
+    with an explicit tree and the outputs returned instead of freed.
+    """
+    lib = _lib.load()
+    h = handle or default_handle()
+    tree.validate()
+    dims = [int(d) for d in dims]
+    ranks = [int(r) for r in ranks]
+    opt = make_options(method, precision, err, want_interior_u=want_interior_u, **kw)
+    layout = output_layout(dims, tree, ranks, want_interior_u)
+    total = layout[3]
+    on_dev = hasattr(x, "is_cuda") and x.is_cuda
+    if on_dev:
+        torch = _torch()
+        if x.dtype != torch.float32 or not x.is_contiguous():
+            raise ValueError("x must be a contiguous float32 tensor (column-major data)")
+        out = torch.empty(total, dtype=torch.float32, device=x.device)
+        xptr, optr = x.data_ptr(), out.data_ptr()
+        torch.cuda.synchronize(x.device)
+    else:
+        xa = np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1))
+        out = np.empty(total, dtype=np.float32)
+        xptr, optr = xa.ctypes.data, out.ctypes.data
+    if int(np.prod(dims)) != (x.numel() if on_dev else xa.size):
+        raise ValueError("x has the wrong number of elements")
+    rel = ctypes.c_double()
+    st = _lib.Stats()
+    status = lib.ht_decompose(h.ptr, ctypes.c_void_p(xptr), int(on_dev), len(dims), _lib.i64_array(dims),
+                              tree.n_nodes, _lib.i32_array(tree.lo), _lib.i32_array(tree.hi),
+                              _lib.i32_array(tree.left), _lib.i32_array(tree.right), _lib.i32_array(ranks),
+                              ctypes.byref(opt), ctypes.c_void_p(optr), int(on_dev), ctypes.byref(rel),
+                              ctypes.byref(st))
+    _lib.check(status, "ht_decompose")
+    res = HTensor(dims=dims, tree=tree, ranks=ranks, packed=out, rel_err=rel.value, want_interior_u=want_interior_u,
+                  stats=dict(total_ms=st.total_ms, h2d_ms=st.h2d_ms, passes=st.bytes_x_passes,
+                             eig_iters_max=st.eig_iters_max, eig_calls=st.eig_calls))
+    _unpack(res, layout)
+    return res
+
+
+def residual(x, res: HTensor, handle=None):
+    """||X - X_HT||_F / ||X||_F in one device pass (x and res.packed on the device)."""
+    lib = _lib.load()
+    h = handle or default_handle()
+    tr = res.tree
+    rel = ctypes.c_double()
+    _lib.check(lib.ht_residual(h.ptr, ctypes.c_void_p(x.data_ptr()), len(res.dims), _lib.i64_array(res.dims),
+                               tr.n_nodes, _lib.i32_array(tr.lo), _lib.i32_array(tr.hi), _lib.i32_array(tr.left),
+                               _lib.i32_array(tr.right), _lib.i32_array(res.ranks), int(res.want_interior_u),
+                               ctypes.c_void_p(res.packed.data_ptr()), ctypes.byref(rel)), "ht_residual")
+    return rel.value
+
+
+def reconstruct(res: HTensor, handle=None):
+    """Explicit X_HT (flat, column-major) on the device."""
+    torch = _torch()
+    lib = _lib.load()
+    h = handle or default_handle()
+    tr = res.tree
+    out = torch.empty(int(np.prod(res.dims)), dtype=torch.float32, device=res.packed.device)
+    _lib.check(lib.ht_reconstruct(h.ptr, len(res.dims), _lib.i64_array(res.dims), tr.n_nodes,
+                                  _lib.i32_array(tr.lo), _lib.i32_array(tr.hi), _lib.i32_array(tr.left),
+                                  _lib.i32_array(tr.right), _lib.i32_array(res.ranks), int(res.want_interior_u),
+                                  ctypes.c_void_p(res.packed.data_ptr()), ctypes.c_void_p(out.data_ptr())),
+               "ht_reconstruct")
+    return out
+
+
+def generate_tucker(dims, core_dims, seed, noise_rel, device=0, handle=None):
+    """Seeded Tucker tensor + Gaussian noise, generated on the device (flat, column-major)."""
+    torch = _torch()
+    lib = _lib.load()
+    h = handle or default_handle(device)
+    x = torch.empty(int(np.prod(dims)), dtype=torch.float32, device=f"cuda:{device}")
+    torch.cuda.synchronize(x.device)
+    _lib.check(lib.ht_generate_tucker(h.ptr, len(dims), _lib.i64_array(dims), _lib.i32_array(core_dims),
+                                      int(seed), float(noise_rel), ctypes.c_void_p(x.data_ptr())),
+               "ht_generate_tucker")
+    return x
+
+
+def generate_cp_decay(dims, rank, seed, noise_rel, exponent=1.5, device=0, handle=None):
+    torch = _torch()
+    lib = _lib.load()
+    h = handle or default_handle(device)
+    x = torch.empty(int(np.prod(dims)), dtype=torch.float32, device=f"cuda:{device}")
+    torch.cuda.synchronize(x.device)
+    _lib.check(lib.ht_generate_cp_decay(h.ptr, len(dims), _lib.i64_array(dims), int(rank), float(exponent),
+                                        int(seed), float(noise_rel), ctypes.c_void_p(x.data_ptr())),
+               "ht_generate_cp_decay")
+    return x
+
+
+def generate_bumps(n, order, seed, sample, n_bumps=6, device=0, handle=None):
+    torch = _torch()
+    lib = _lib.load()
+    h = handle or default_handle(device)
+    x = torch.empty(int(n) ** int(order), dtype=torch.float32, device=f"cuda:{device}")
+    torch.cuda.synchronize(x.device)
+    _lib.check(lib.ht_generate_bumps(h.ptr, int(order), int(n), int(n_bumps), int(seed), int(sample),
+                                     ctypes.c_void_p(x.data_ptr())), "ht_generate_bumps")
+    return x
+
+
+def _host_f32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1))
+
+
+def htd(x, a, b, c, k, rel_eps=0.0, max_rank=0.0):
+    """float htd(dt *x, long a, long b, long c, int *k, dt rel_eps, dt max_rank) --
+    third_order_tensor_decompsition/CUDA/Baseline/function.h:6 (plain FP32, tree {1,2},{3},
+    k heap-indexed with k[0]=1). Prints "cost time"/"error rate" like the reference and
+    returns the relative error (negative on failure)."""
+    lib = _lib.load()
+    xa = _host_f32(x)
+    return float(lib.htd(ctypes.c_void_p(xa.ctypes.data), a, b, c, _lib.i32_array(k), float(rel_eps),
+                         float(max_rank)))
+
+
+def htd_tc(x, a, b, c, k):
+    """The mixed-precision variant of htd (third_order_tensor_decompsition/CUDA/Optimized/function.h:6)."""
+    lib = _lib.load()
+    xa = _host_f32(x)
+    return float(lib.htd_tc(ctypes.c_void_p(xa.ctypes.data), a, b, c, _lib.i32_array(k)))
+
+
+def htd4(x, a, b, c, d, k):
+    """void htd4(dt *x, long a, long b, long c, long d, int *k) --
+    fourth_order_tensor_decomposition/single_GPU/CUDA/Baseline/function.h:6."""
+    lib = _lib.load()
+    xa = _host_f32(x)
+    lib.htd4(ctypes.c_void_p(xa.ctypes.data), a, b, c, d, _lib.i32_array(k))
+
+
+def htd4_tc(x, a, b, c, d, k):
+    """fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/function.h:6 (handles ignored)."""
+    lib = _lib.load()
+    xa = _host_f32(x)
+    lib.htd4_tc(ctypes.c_void_p(xa.ctypes.data), a, b, c, d, _lib.i32_array(k), None, None)

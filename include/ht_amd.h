@@ -1,0 +1,200 @@
+/*
+ * ht_amd.h -- C ABI of the MI355X hierarchical Tucker (HT) decomposition engine.
+ *
+ * A dense FP32 tensor (column-major, mode 1 fastest), a binary dimension tree and one
+ * rank per tree node go in; leaf bases U_t, transfer tensors B_t and the relative
+ * reconstruction error come out. Plain pointers and sizes only: no HIP, torch or
+ * library types cross this boundary.
+ *
+ * Reference surface this replaces (paths relative to the reference repository):
+ *   float htd(dt*,long,long,long,int*,dt,dt)         third_order_tensor_decompsition/CUDA/Baseline/function.h:6
+ *   float htd(dt*,long,long,long,int*)               third_order_tensor_decompsition/CUDA/Optimized/function.h:6
+ *   void  htd(dt*,long,long,long,int*)               third_order_tensor_decompsition/CUDA/TSQR/function.h:6
+ *   void  htd4(dt*,long,long,long,long,int*)         fourth_order_tensor_decomposition/single_GPU/CUDA/Baseline/function.h:6
+ *   void  htd4(dt*,long,long,long,long,int*,cublasHandle_t,cusolverDnHandle_t)
+ *                                                    fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/function.h:6
+ * The compatibility wrappers at the end keep those names and argument meanings; the
+ * general entry point ht_decompose adds what the reference hard-codes (the tree), what
+ * it throws away (U_t, B_t are returned) and status codes (the reference reports none).
+ *
+ * Conventions shared by every entry point:
+ *   - Tree nodes are contiguous mode ranges [lo, hi); node 0 is the root; a leaf holds
+ *     one mode; an interior node's left child holds its lower modes. Nodes are numbered
+ *     breadth-first, which is the reference's heap numbering for its complete trees
+ *     (third_order_tensor_decompsition/Python/HT_TensorLy.py:23).
+ *   - rank[0] must be 1 (k[0]=1, third_order_tensor_decompsition/CUDA/Baseline/ht_init.cu:16-19).
+ *   - U_t is (rows_t x k_t) column-major, columns in descending singular-value order,
+ *     each column signed so that its largest-magnitude entry is positive.
+ *   - B_t is (k_l x k_r x k_t) column-major (left child's rank index fastest), the layout
+ *     of the reference's d_B buffers (third_order_tensor_decompsition/CUDA/Baseline/htd_unopt.cu:105-108).
+ */
+#ifndef HT_AMD_H
+#define HT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HT_MAX_ORDER 8
+#define HT_MAX_NODES (2 * HT_MAX_ORDER - 1)
+#define HT_MAX_RANK 64
+
+typedef enum {
+  HT_OK = 0,
+  HT_ERR_INVALID_ARG = 1,   /* null pointer, bad order, non-positive size            */
+  HT_ERR_INVALID_TREE = 2,  /* not a binary contiguous tree covering every mode       */
+  HT_ERR_RANK = 3,          /* rank < 1, rank > rows, rank > k_l*k_r, rank > HT_MAX_RANK */
+  HT_ERR_ALLOC = 4,         /* device or host allocation failed                       */
+  HT_ERR_HIP = 5,           /* a HIP runtime call failed                              */
+  HT_ERR_NONFINITE = 6,     /* the input holds NaN or Inf                             */
+  HT_ERR_NOCONV = 7,        /* an eigensolver did not reach its tolerance             */
+  HT_ERR_UNSUPPORTED = 8,   /* a shape this build does not handle                     */
+  HT_ERR_NO_DEVICE = 9      /* no usable GPU                                          */
+} ht_status;
+
+/* Method selection. */
+typedef enum {
+  /* Reference semantics: every non-root node's basis is the exact truncated SVD of its
+   * own matricisation of X, computed in FP32 on the GPU (FP64 for the small eigen
+   * problems). third_order_tensor_decompsition/CUDA/Baseline/htd_unopt.cu:2-297. */
+  HT_METHOD_PLAIN = 0,
+  /* Leaves-to-root: leaf Grams on MFMA (BF16/FP16 in, FP32 accumulate), then a single
+   * projection pass to the leaf core and exact FP64 interior problems on that core. */
+  HT_METHOD_LTR = 1,
+  /* Root-to-leaves randomised: every non-root node by a range finder (oversampling p,
+   * q power iterations) on MFMA, TSQR orthonormalisation, FP64 projected SVD.
+   * fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/kernel.cu:419-504. */
+  HT_METHOD_RSVD = 2
+} ht_method;
+
+typedef enum { HT_PREC_FP32 = 0, HT_PREC_BF16 = 1, HT_PREC_FP16 = 2 } ht_precision;
+
+typedef enum {
+  HT_ERR_MODE_NONE = 0,      /* do not compute the error                                   */
+  HT_ERR_MODE_IDENTITY = 1,  /* sqrt(||X||^2 - ||B_root||^2)/||X|| (orthonormal bases)     */
+  HT_ERR_MODE_RESIDUAL = 2   /* one fused pass: ||X - X_HT||_F / ||X||_F                   */
+} ht_err_mode;
+
+typedef struct {
+  int32_t method;          /* ht_method                                                 */
+  int32_t precision;       /* ht_precision of the large GEMMs (plain always FP32)        */
+  int32_t oversample;      /* p for HT_METHOD_RSVD (default 8)                           */
+  int32_t power_iters;     /* q for HT_METHOD_RSVD (default 1)                           */
+  int32_t err_mode;        /* ht_err_mode (default HT_ERR_MODE_IDENTITY)                 */
+  int32_t max_eig_iters;   /* cap on subspace iterations of the exact solvers (200)      */
+  double eig_tol;          /* relative Ritz-value tolerance of the exact solvers (1e-10) */
+  uint64_t seed;           /* seed of the Philox test matrices (1234)                    */
+  int32_t want_interior_u; /* also return U_t of interior non-root nodes                 */
+  int32_t check_finite;    /* scan the input for NaN/Inf first (default 1)               */
+} ht_options;
+
+/* Timing and accounting of the last call (filled when stats != NULL). */
+typedef struct {
+  double total_ms;         /* device time of the decomposition (HIP events)              */
+  double h2d_ms;           /* host-to-device copy of X when x_on_device == 0             */
+  double error_ms;         /* residual pass when err_mode == HT_ERR_MODE_RESIDUAL        */
+  double bytes_x_passes;   /* number of full passes over X made by the decomposition     */
+  int32_t eig_iters_max;   /* largest subspace-iteration count of any node               */
+  int32_t eig_calls;       /* number of node eigen problems solved                       */
+} ht_stats;
+
+typedef struct ht_handle_s* ht_handle_t;
+
+/* Fill *opt with the defaults listed above. */
+void ht_default_options(ht_options* opt);
+
+/* Create a handle bound to `device`: it owns a HIP stream, a growable device workspace
+ * and HIP events. Handles are independent; one handle must not be used concurrently. */
+ht_status ht_create(ht_handle_t* handle, int32_t device);
+ht_status ht_destroy(ht_handle_t handle);
+/* The HIP stream (hipStream_t) the handle launches on, as an opaque pointer. */
+void* ht_get_stream(ht_handle_t handle);
+
+/* Validate a tree (per-node lo/hi/left/right arrays, node 0 = root) for `order` modes. */
+ht_status ht_validate_tree(int32_t order, int32_t n_nodes, const int32_t* lo, const int32_t* hi,
+                           const int32_t* left, const int32_t* right);
+
+/* Packed output layout. For node t: U_t (leaves, and interior non-root nodes when
+ * want_interior_u) starts at u_off[t] (else -1), B_t (interior nodes) at b_off[t]
+ * (else -1), the node's singular values (k_t doubles stored as floats) at s_off[t]
+ * (non-root nodes, else -1). *total receives the number of floats of the whole block. */
+ht_status ht_output_layout(int32_t order, const int64_t* dims, int32_t n_nodes, const int32_t* lo,
+                           const int32_t* hi, const int32_t* left, const int32_t* right,
+                           const int32_t* ranks, int32_t want_interior_u, int64_t* u_off,
+                           int64_t* b_off, int64_t* s_off, int64_t* total);
+
+/* General entry point.
+ *   x            FP32 tensor, column-major; a device pointer when x_on_device != 0,
+ *                otherwise host memory that is copied in (timed separately as h2d_ms).
+ *   dims         `order` mode sizes (order 2..HT_MAX_ORDER).
+ *   tree, ranks  as above (ranks[0] == 1).
+ *   out          packed output block of ht_output_layout's size; device memory when
+ *                out_on_device != 0, otherwise host memory.
+ *   rel_err      receives the relative error (NaN when err_mode == NONE); may be NULL.
+ * The call is synchronous with respect to the host. */
+ht_status ht_decompose(ht_handle_t handle, const float* x, int32_t x_on_device, int32_t order,
+                       const int64_t* dims, int32_t n_nodes, const int32_t* lo, const int32_t* hi,
+                       const int32_t* left, const int32_t* right, const int32_t* ranks,
+                       const ht_options* opt, float* out, int32_t out_on_device, double* rel_err,
+                       ht_stats* stats);
+
+/* ||X - X_HT||_F / ||X||_F in one fused device pass for an HT block in the packed layout
+ * (the reconstruction + error of third_order_tensor_decompsition/CUDA/Baseline/htd_unopt.cu:202-266
+ * and fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/htd4_tensorcore.cu:232-291,
+ * without materialising X_HT). x and packed are device pointers. */
+ht_status ht_residual(ht_handle_t handle, const float* x, int32_t order, const int64_t* dims,
+                      int32_t n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                      const int32_t* right, const int32_t* ranks, int32_t want_interior_u,
+                      const float* packed, double* rel_err);
+
+/* Explicit reconstruction X_HT (device pointers). */
+ht_status ht_reconstruct(ht_handle_t handle, int32_t order, const int64_t* dims, int32_t n_nodes,
+                         const int32_t* lo, const int32_t* hi, const int32_t* left,
+                         const int32_t* right, const int32_t* ranks, int32_t want_interior_u,
+                         const float* packed, float* x_out);
+
+/* ---- synthetic inputs (seeded Philox4x32-10, generated on the device) ---------------
+ * Tucker: core (prod core_dims, i.i.d. N(0,1)) x_n U_n, U_n orthonormal (Q of QR of a
+ * Gaussian, diag R > 0), plus Gaussian noise with ||E||_F = noise_rel * ||X0||_F.
+ * Replaces genHtensor (third_order_tensor_decompsition/CUDA/Baseline/kernel.cu:94-202,
+ * fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/kernel.cu:106-248), whose
+ * srand(time(NULL)) input is not reproducible. x_dev is a device pointer. */
+ht_status ht_generate_tucker(ht_handle_t handle, int32_t order, const int64_t* dims,
+                             const int32_t* core_dims, uint64_t seed, double noise_rel,
+                             float* x_dev);
+/* Diagonal core sigma_k = k^-exponent (every mode spectrum decays as k^-exponent). */
+ht_status ht_generate_cp_decay(ht_handle_t handle, int32_t order, const int64_t* dims, int32_t rank,
+                               double exponent, uint64_t seed, double noise_rel, float* x_dev);
+/* Gaussian-bump family f = sum_j a_j exp(-||x-c_j||^2/(2 w_j^2)) on the [0,1]^order grid. */
+ht_status ht_generate_bumps(ht_handle_t handle, int32_t order, int64_t n, int32_t n_bumps,
+                            uint64_t seed, int64_t sample, float* x_dev);
+
+/* ---- compatibility wrappers keeping the reference names --------------------------- */
+
+/* third_order_tensor_decompsition/CUDA/Baseline/function.h:6: plain FP32 HT of the
+ * reference's tree {1,2},{3}; x is host memory (a*b*c floats), k[5] heap-indexed ranks
+ * with k[0]=1. rel_eps and max_rank are accepted and ignored, as in the reference.
+ * Returns the relative error, or a negative value on failure. */
+float htd(float* x, long a, long b, long c, int* k, float rel_eps, float max_rank);
+/* third_order_tensor_decompsition/CUDA/Optimized/function.h:6: the mixed-precision
+ * variant (LTR, BF16 MFMA Grams) of the same tree. */
+float htd_tc(float* x, long a, long b, long c, int* k);
+/* fourth_order_tensor_decomposition/single_GPU/CUDA/Baseline/function.h:6: plain FP32
+ * HT of the balanced tree {1,2},{3,4}; k[7] heap-indexed. The error is printed as in the
+ * reference ("error rate ..."). */
+void htd4(float* x, long a, long b, long c, long d, int* k);
+/* fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/function.h:6: the
+ * mixed-precision variant; the two BLAS/solver handles are accepted and ignored. */
+void htd4_tc(float* x, long a, long b, long c, long d, int* k, void* blas_handle,
+             void* solver_handle);
+
+/* Library version string. */
+const char* ht_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HT_AMD_H */

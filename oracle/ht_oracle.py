@@ -216,8 +216,12 @@ def canon_signs(U):
 
 
 def top_left_singular(A, k):
-    """Exact truncated SVD (FP64 LAPACK): top-k left singular vectors and values."""
-    U, s, _ = np.linalg.svd(np.asarray(A, dtype=np.float64), full_matrices=False)
+    """Exact truncated SVD (FP64 LAPACK): top-k left singular vectors and values. When k
+    exceeds min(A.shape) the basis is completed with orthonormal null-space directions
+    (zero singular values)."""
+    A = np.asarray(A, dtype=np.float64)
+    U, s, _ = np.linalg.svd(A, full_matrices=k > min(A.shape))
+    s = np.concatenate([s, np.zeros(max(0, k - s.size))])
     return canon_signs(U[:, :k]), s[:k]
 
 

@@ -1,0 +1,170 @@
+"""End-to-end parity of the HT engine with the FP64 oracle on identical seeded inputs.
+
+Gates (from the task's accuracy contract): relative reconstruction error within 1e-4
+absolute of the oracle's, leaf-subspace principal angles small, and the returned U_t /
+B_t reproducing the reported error when reconstructed on the CPU.
+"""
+import numpy as np
+import pytest
+from scipy.linalg import subspace_angles
+
+import ht_tensor_learning_using_gpu_tensor_cores_amd as ht
+from oracle import gen as G
+from oracle import ht_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def to_np(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def oracle_from_result(res, dims, tree_np):
+    """Rebuild an oracle HT from the engine's U/B and reconstruct on the CPU (FP64)."""
+    h = O.HT(dims, tree_np, res.ranks)
+    for t in range(tree_np.n_nodes):
+        if tree_np.is_leaf(t):
+            h.U[t] = to_np(res.U[t]).astype(np.float64)
+        else:
+            h.B[t] = to_np(res.B[t]).astype(np.float64)
+    return h
+
+
+def as_oracle_tree(tr):
+    return O.Tree(tr.lo, tr.hi, tr.left, tr.right)
+
+
+def run(x_np, dims, tr, ranks, **kw):
+    import torch
+    x = torch.from_numpy(x_np).cuda()
+    return ht.decompose(x, dims, tr, ranks, **kw), x
+
+
+CASES_3D = [
+    # dims, core, tree, ranks (heap order)
+    ((32, 24, 20), (4, 5, 3), "ref", [1, 6, 4, 4, 5]),
+    ((48, 32, 16), (8, 8, 8), "first", [1, 8, 8, 8, 8]),
+    ((128, 64, 40), (8, 8, 8), "first", [1, 8, 8, 8, 8]),
+]
+
+
+def make_tree(kind):
+    return {"ref": ht.tree.reference_3d(), "first": ht.tree.first_vs_rest_3d(), "bal4": ht.tree.balanced_4d()}[kind]
+
+
+@pytest.mark.parametrize("dims,core,kind,ranks", CASES_3D)
+@pytest.mark.parametrize("method", ["plain", "ltr", "rsvd"])
+def test_3d_against_oracle(dims, core, kind, ranks, method):
+    x = G.tucker(list(dims), list(core), seed=0, noise_rel=1e-3)
+    tr = make_tree(kind)
+    otr = as_oracle_tree(tr)
+    exact = O.plain_ht(x, list(dims), otr, ranks)
+    e_exact = O.rel_error(x, exact.reconstruct())
+    prec = "fp32" if method == "plain" else "bf16"
+    print(f"case dims={dims} tree={kind} method={method}", flush=True)
+    res, _ = run(x, list(dims), tr, ranks, method=method, precision=prec)
+    assert abs(res.rel_err - e_exact) <= 1e-4, (res.rel_err, e_exact)
+    # U/B reproduce the error on the CPU
+    h = oracle_from_result(res, list(dims), otr)
+    e_cpu = O.rel_error(x, h.reconstruct())
+    assert abs(e_cpu - res.rel_err) <= 1e-5
+    for t in otr.leaves():
+        Ug = to_np(res.U[t]).astype(np.float64)
+        np.testing.assert_allclose(Ug.T @ Ug, np.eye(Ug.shape[1]), atol=1e-5)
+        ang = np.max(subspace_angles(Ug, exact.U[t]))
+        assert ang < 1e-3, (t, ang)
+
+
+@pytest.mark.parametrize("n,core,r", [(24, 8, 8), (32, 8, 6), (40, 16, 16)])
+@pytest.mark.parametrize("method", ["plain", "ltr", "rsvd"])
+def test_4d_against_oracle(n, core, r, method):
+    dims = [n] * 4
+    x = G.tucker(dims, [core] * 4, seed=1, noise_rel=1e-4)
+    tr = ht.tree.balanced_4d()
+    otr = as_oracle_tree(tr)
+    ranks = [1] + [r] * 6
+    exact = O.plain_ht(x, dims, otr, ranks)
+    e_exact = O.rel_error(x, exact.reconstruct())
+    prec = "fp32" if method == "plain" else "bf16"
+    print(f"case n={n} core={core} r={r} method={method}", flush=True)
+    res, _ = run(x, dims, tr, ranks, method=method, precision=prec)
+    tol = 1e-4 if method != "rsvd" or core <= r + 8 else 0.1
+    assert abs(res.rel_err - e_exact) <= tol, (res.rel_err, e_exact)
+    h = oracle_from_result(res, dims, otr)
+    assert abs(O.rel_error(x, h.reconstruct()) - res.rel_err) <= 1e-5
+    for t in otr.leaves():
+        ang = np.max(subspace_angles(to_np(res.U[t]).astype(np.float64), exact.U[t]))
+        assert ang < 1e-3
+
+
+def test_plain_subspace_iteration_path():
+    """Interior nodes with more than 1024 rows and columns take the exact subspace solver."""
+    dims = [36, 36, 36, 36]
+    x = G.tucker(dims, [6] * 4, seed=5, noise_rel=1e-2)
+    tr = ht.tree.balanced_4d()
+    ranks = [1, 5, 5, 5, 5, 5, 5]
+    exact = O.plain_ht(x, dims, as_oracle_tree(tr), ranks)
+    e_exact = O.rel_error(x, exact.reconstruct())
+    res, _ = run(x, dims, tr, ranks, method="plain", precision="fp32", want_interior_u=True)
+    assert abs(res.rel_err - e_exact) <= 1e-5
+    for t in (1, 2):
+        ang = np.max(subspace_angles(to_np(res.U[t]).astype(np.float64), exact.full_basis(t)))
+        assert ang < 1e-3
+
+
+def test_identity_error_matches_residual_ltr():
+    dims = [32] * 4
+    x = G.tucker(dims, [8] * 4, seed=1, noise_rel=1e-1)
+    tr = ht.tree.balanced_4d()
+    ranks = [1] + [8] * 6
+    a, _ = run(x, dims, tr, ranks, method="ltr", err="identity")
+    b, _ = run(x, dims, tr, ranks, method="ltr", err="residual")
+    assert abs(a.rel_err - b.rel_err) < 1e-6
+
+
+def test_compat_wrappers_match_general_entry():
+    dims = [24, 20, 16]
+    x = G.tucker(dims, [4, 4, 4], seed=0, noise_rel=1e-3)
+    k = [1, 8, 4, 4, 4]
+    e1 = ht.htd(x, 24, 20, 16, k)
+    otr = O.tree_reference_3d()
+    e_exact = O.rel_error(x, O.plain_ht(x, dims, otr, k).reconstruct())
+    assert abs(e1 - e_exact) < 1e-5
+    e2 = ht.htd_tc(x, 24, 20, 16, k)
+    assert abs(e2 - e_exact) < 1e-4
+
+
+def test_generators_match_oracle(handle):
+    dims, core = [16, 12, 20], [3, 4, 5]
+    xg = to_np(ht.generate_tucker(dims, core, seed=3, noise_rel=1e-3))
+    xo = G.tucker(dims, core, seed=3, noise_rel=1e-3)
+    np.testing.assert_allclose(xg, xo, rtol=1e-5, atol=1e-6 * np.abs(xo).max())
+    xg = to_np(ht.generate_cp_decay([20, 20, 20], 5, seed=2, noise_rel=1e-5))
+    xo = G.cp_decay([20, 20, 20], 5, seed=2, noise_rel=1e-5)
+    np.testing.assert_allclose(xg, xo, rtol=1e-5, atol=1e-6 * np.abs(xo).max())
+    xg = to_np(ht.generate_bumps(12, 4, seed=4, sample=7))
+    xo = G.bumps(12, seed=4, sample=7)
+    np.testing.assert_allclose(xg, xo, rtol=2e-5, atol=1e-6)
+
+
+def test_nonfinite_input_is_reported():
+    dims = [16, 16, 16]
+    x = G.tucker(dims, [4, 4, 4], seed=0, noise_rel=1e-3)
+    x[100] = np.nan
+    with pytest.raises(ht.HTError) as e:
+        run(x, dims, ht.tree.first_vs_rest_3d(), [1, 4, 4, 4, 4], method="ltr")
+    assert e.value.status == 6
+
+
+def test_reconstruct_and_residual_entry_points():
+    import torch
+    dims = [24] * 4
+    x = G.tucker(dims, [6] * 4, seed=9, noise_rel=1e-2)
+    tr = ht.tree.balanced_4d()
+    res, xd = run(x, dims, tr, [1] + [6] * 6, method="ltr")
+    xr = ht.reconstruct(res)
+    e = float(torch.linalg.norm(xd - xr) / torch.linalg.norm(xd))
+    assert abs(e - res.rel_err) < 1e-6
+    assert abs(ht.residual(xd, res) - res.rel_err) < 1e-7

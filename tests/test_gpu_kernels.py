@@ -1,0 +1,212 @@
+"""Component parity of the gfx950 kernels against numpy FP64 (through the C ABI)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from ht_tensor_learning_using_gpu_tensor_cores_amd import _lib
+from oracle import ht_oracle as O
+from oracle import philox
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=None):
+    import torch
+    a = np.asarray(a)
+    t = torch.from_numpy(np.ascontiguousarray(a if dtype is None else a.astype(dtype)))
+    return t.cuda()
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def bf16_round(a):
+    a = np.asarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def test_mfma_lane_layouts(lib, handle, rng):
+    """The operand/accumulator lane layouts the kernels assume, checked per shape."""
+    import torch
+    a = rng.standard_normal((32, 16)).astype(np.float32)
+    b = rng.standard_normal((32, 16)).astype(np.float32)
+    da, db = dev(a), dev(b)
+    o1 = torch.zeros(32 * 32, device="cuda")
+    o2 = torch.zeros(32 * 32, device="cuda")
+    o3 = torch.zeros(16 * 16, device="cuda")
+    o4 = torch.zeros(16 * 16, dtype=torch.float64, device="cuda")
+    o5 = torch.zeros(16 * 16, device="cuda")
+    _lib.check(lib.ht_debug_mfma_probe(handle.ptr, ptr(da), ptr(db), ptr(o1), ptr(o2), ptr(o3), ptr(o4), ptr(o5)),
+               "probe")
+    # 32x32x16 bf16: D = A B with A = a (32x16), B = b^T (16x32)
+    ab, bb = bf16_round(a).astype(np.float64), bf16_round(b).astype(np.float64)
+    np.testing.assert_allclose(host(o1).reshape(32, 32), ab @ bb.T, rtol=1e-5, atol=1e-4)
+    # 32x32x2 f32: A = a[:, :2], B = b[:, :2]^T
+    np.testing.assert_allclose(host(o2).reshape(32, 32), a[:, :2].astype(np.float64) @ b[:, :2].T, rtol=1e-5,
+                               atol=1e-5)
+    # 16x16x4 f32 / f64: A = a[:16, :4], B = b[:16, :4]^T
+    ref = a[:16, :4].astype(np.float64) @ b[:16, :4].T
+    np.testing.assert_allclose(host(o3).reshape(16, 16), ref, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(host(o4).reshape(16, 16), ref, rtol=1e-12, atol=1e-12)
+    # 16x16x32 bf16: A = a viewed as 16x32, B = (b viewed 16x32)^T
+    a2, b2 = bf16_round(a.reshape(16, 32)).astype(np.float64), bf16_round(b.reshape(16, 32)).astype(np.float64)
+    np.testing.assert_allclose(host(o5).reshape(16, 16), a2 @ b2.T, rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.parametrize("dims,lo,hi", [((24, 20, 18), 0, 1), ((24, 20, 18), 1, 2), ((24, 20, 18), 2, 3),
+                                        ((64, 32, 40), 1, 2), ((128, 64, 8, 4), 0, 1), ((16, 128, 8, 4), 1, 2),
+                                        ((8, 8, 130, 4), 2, 3), ((12, 10, 6), 0, 2), ((3, 5, 7, 2), 1, 3)])
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_mode_gram(lib, handle, rng, dims, lo, hi, precision):
+    x = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
+    A = O.unfold(x.astype(np.float64), list(dims), lo, hi)
+    M = A.shape[0]
+    import torch
+    G = torch.zeros(M * M, dtype=torch.float64, device="cuda")
+    s = ctypes.c_double()
+    _lib.check(lib.ht_mode_gram(handle.ptr, ptr(dev(x)), len(dims), _lib.i64_array(dims), lo, hi,
+                                _lib.PRECISION[precision], ptr(G), ctypes.byref(s)), "gram")
+    Gh = host(G).reshape((M, M), order="F")
+    if precision == "fp32":
+        ref = A @ A.T
+        np.testing.assert_allclose(Gh, ref, rtol=1e-5, atol=1e-4 * np.abs(ref).max())
+    else:
+        Ab = O.unfold(bf16_round(x).astype(np.float64), list(dims), lo, hi)
+        ref = Ab @ Ab.T
+        np.testing.assert_allclose(Gh, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+        # forward error bound against the exact FP64 Gram: |G_bf16 - G| <= 2^-7 |A||A|^T
+        bound = 2.0 ** -7 * (np.abs(A) @ np.abs(A).T) + 1e-6
+        assert np.all(np.abs(Gh - A @ A.T) <= bound)
+    np.testing.assert_allclose(s.value, float(np.sum(x.astype(np.float64) ** 2)), rtol=1e-6)
+    assert np.allclose(Gh, Gh.T)
+
+
+@pytest.mark.parametrize("n1,n2,S", [(128, 128, 6), (64, 32, 10), (20, 12, 7), (128, 16, 3)])
+def test_slab_gram(lib, handle, rng, n1, n2, S):
+    import torch
+    x = rng.standard_normal(n1 * n2 * S).astype(np.float32)
+    G1 = torch.zeros(n1 * n1, dtype=torch.float64, device="cuda")
+    G2 = torch.zeros(n2 * n2, dtype=torch.float64, device="cuda")
+    s = ctypes.c_double()
+    _lib.check(lib.ht_slab_gram(handle.ptr, ptr(dev(x)), n1, n2, S, _lib.PRECISION["bf16"], ptr(G1), ptr(G2),
+                                ctypes.byref(s)), "slab_gram")
+    xb = bf16_round(x).astype(np.float64)
+    dims = [n1, n2, S]
+    A1, A2 = O.unfold(xb, dims, 0, 1), O.unfold(xb, dims, 1, 2)
+    np.testing.assert_allclose(host(G1).reshape(n1, n1), A1 @ A1.T, rtol=1e-4, atol=1e-3)
+    np.testing.assert_allclose(host(G2).reshape(n2, n2), A2 @ A2.T, rtol=1e-4, atol=1e-3)
+    np.testing.assert_allclose(s.value, float(np.sum(x.astype(np.float64) ** 2)), rtol=1e-6)
+
+
+@pytest.mark.parametrize("n1,n2,S,r1,r2", [(128, 128, 5, 16, 16), (64, 48, 9, 8, 12), (32, 16, 4, 20, 30),
+                                           (16, 128, 3, 32, 5)])
+def test_slab_project(lib, handle, rng, n1, n2, S, r1, r2):
+    import torch
+    x = rng.standard_normal(n1 * n2 * S).astype(np.float32)
+    U1 = rng.standard_normal((n1, r1)).astype(np.float32)
+    U2 = rng.standard_normal((n2, r2)).astype(np.float32)
+    z = torch.zeros(r1 * r2 * S, device="cuda")
+    _lib.check(lib.ht_slab_project(handle.ptr, ptr(dev(x)), n1, n2, S, ptr(dev(U1.reshape(-1, order="F"))), r1,
+                                   ptr(dev(U2.reshape(-1, order="F"))), r2, ptr(z)), "slab_project")
+    X = x.astype(np.float64).reshape((n1, n2, S), order="F")
+    ref = np.einsum("ia,ijs,jb->abs", U1.astype(np.float64), X, U2.astype(np.float64))
+    np.testing.assert_allclose(host(z).reshape((r1, r2, S), order="F"), ref, rtol=1e-4, atol=1e-3)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 16, 31, 64])
+def test_sym_eig(lib, handle, rng, n):
+    import torch
+    A = rng.standard_normal((n, n))
+    A = A + A.T
+    w = torch.zeros(n, dtype=torch.float64, device="cuda")
+    V = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    _lib.check(lib.ht_sym_eig(handle.ptr, ptr(dev(A.reshape(-1, order="F"))), n, ptr(w), ptr(V)), "eig")
+    wr, Vr = np.linalg.eigh(A)
+    wr, Vr = wr[::-1], O.canon_signs(Vr[:, ::-1])
+    np.testing.assert_allclose(host(w), wr, rtol=1e-12, atol=1e-12 * np.abs(wr).max())
+    Vh = host(V).reshape((n, n), order="F")
+    np.testing.assert_allclose(Vh.T @ Vh, np.eye(n), atol=1e-12)
+    np.testing.assert_allclose(Vh @ np.diag(host(w)) @ Vh.T, A, atol=1e-11 * np.abs(A).max())
+    gaps = np.abs(np.diff(wr))
+    if n > 1 and gaps.min() > 1e-3:
+        np.testing.assert_allclose(Vh, Vr, atol=1e-9)
+
+
+@pytest.mark.parametrize("n,k,decay", [(128, 16, "gap"), (256, 16, "flat"), (100, 8, "gap"), (40, 5, "flat")])
+def test_sym_topk(lib, handle, rng, n, k, decay):
+    import torch
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    if decay == "gap":
+        lam = np.concatenate([np.linspace(10, 5, k), 1e-6 * rng.random(n - k)])
+    else:
+        lam = np.sort(rng.random(n) * 10)[::-1]
+    G = (Q * lam) @ Q.T
+    G = (G + G.T) / 2
+    U = torch.zeros(n * k, dtype=torch.float64, device="cuda")
+    w = torch.zeros(k, dtype=torch.float64, device="cuda")
+    it = ctypes.c_int32()
+    _lib.check(lib.ht_sym_topk(handle.ptr, ptr(dev(G.reshape(-1, order="F"))), n, k, 1e-10, ptr(U), ptr(w),
+                               ctypes.byref(it)), "topk")
+    np.testing.assert_allclose(host(w), lam[:k], rtol=1e-9)
+    Uh = host(U).reshape((n, k), order="F")
+    Ur = O.canon_signs(Q[:, :k])
+    from scipy.linalg import subspace_angles
+    assert np.max(subspace_angles(Uh, Ur)) < 1e-6
+
+
+@pytest.mark.parametrize("m,b", [(16384, 64), (1000, 24), (300, 17), (256, 64), (40, 40), (30, 40), (70000, 40)])
+def test_tsqr(lib, handle, rng, m, b):
+    import torch
+    W = rng.standard_normal((m, b)).astype(np.float32)
+    if m >= b:
+        W[:, -1] = W[:, 0] * 1e-3 + W[:, 1]  # mild ill-conditioning
+    Wd = dev(W.reshape(-1, order="F"))
+    Q = torch.zeros(m * b, device="cuda")
+    R = torch.zeros(b * b, dtype=torch.float64, device="cuda")
+    _lib.check(lib.ht_tsqr(handle.ptr, ptr(Wd), m, b, ptr(Q), ptr(R)), "tsqr")
+    Qh = host(Q).reshape((m, b), order="F").astype(np.float64)
+    Rh = host(R).reshape((b, b), order="F")
+    kk = min(m, b)
+    np.testing.assert_allclose(Qh @ Rh, W, atol=2e-5 * np.abs(W).max())
+    if m >= b:
+        np.testing.assert_allclose(Qh.T @ Qh, np.eye(b), atol=5e-6)
+        Rn = np.linalg.qr(W.astype(np.float64))[1]
+        np.testing.assert_allclose(np.abs(Rh), np.abs(Rn), atol=1e-4 * np.abs(Rn).max())
+    assert np.allclose(np.tril(Rh[:kk, :kk], -1), 0)
+
+
+@pytest.mark.parametrize("M,K,b,trans", [(4096, 3000, 64, 1), (4096, 3000, 24, 0), (1000, 777, 17, 1),
+                                         (130, 5000, 8, 0), (16384, 512, 40, 1)])
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_skinny_gemm(lib, handle, rng, M, K, b, trans, precision):
+    import torch
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal(((M if trans else K), b)).astype(np.float32)
+    Y = torch.zeros((K if trans else M) * b, device="cuda")
+    _lib.check(lib.ht_skinny_gemm(handle.ptr, ptr(dev(A.reshape(-1, order="F"))), M, K, trans,
+                                  ptr(dev(B.reshape(-1, order="F"))), b, _lib.PRECISION[precision], ptr(Y)), "gemm")
+    rows = K if trans else M
+    Yh = host(Y).reshape((rows, b), order="F")
+    if precision == "bf16":
+        A, B = bf16_round(A), bf16_round(B)
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    ref = A64.T @ B64 if trans else A64 @ B64
+    np.testing.assert_allclose(Yh, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+
+
+def test_philox_device_matches_oracle(lib, handle):
+    import torch
+    n = 4099
+    out = torch.zeros(n, dtype=torch.float64, device="cuda")
+    _lib.check(lib.ht_philox_normals(handle.ptr, 7, 64, n, ptr(out)), "philox")
+    np.testing.assert_allclose(host(out), philox.normals(7, 64, n), rtol=1e-13, atol=1e-13)
