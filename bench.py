@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X HT engine: decompositions/s of the 4th-order 128^4 workload.
+
+Workload: FP32 tensor 128 x 128 x 128 x 128 (1 GiB), Tucker core 16^4 (i.i.d. N(0,1))
+times orthonormal Gaussian factors plus 1e-4 relative noise, seed 1; balanced tree
+{1,2},{3,4}; all ranks 16. The tensor is generated on the device and is resident in HBM
+before the timed region starts.
+
+One step = one complete decomposition (leaf bases, transfer tensors and the relative
+error by a residual pass over X) with the mixed-precision leaves-to-root method (BF16
+MFMA Grams, FP32 projection, FP64 small solvers). The plain FP32 exact truncated-SVD HT
+is timed on the same GPU as the comparator; the reference's CPU path (its TensorLy
+semantics, restated in FP32 numpy) is timed on a bounded sample on the host.
+
+Prints one JSON line (rank 0, N = 1).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--rank", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-sample-n", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                                           "pmc_traffic_c2.txt"))
+    return ap.parse_args()
+
+
+def read_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC summary (lines
+    'kernel FETCH_KiB WRITE_KiB scale'), or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                p = line.split()
+                if len(p) >= 4 and p[0] == kernel:
+                    return (float(p[1]) + float(p[2])) * 1024.0 * float(p[3])
+    except OSError:
+        return None
+    return None
+
+
+def cpu_baseline(n, rank):
+    """The reference's CPU-path semantics (oracle.tensorly_ht_fp32) on an n^4 member of
+    the same family, timed on the host cores: one warm-up, then one timed run."""
+    import numpy as np
+    from oracle import gen as G
+    from oracle import ht_oracle as O
+    dims = [n] * 4
+    x = G.tucker(dims, [16] * 4, seed=1, noise_rel=1e-4)
+    tr = O.tree_balanced_4d()
+    ranks = [1] + [rank] * 6
+    O.tensorly_ht_fp32(x, dims, tr, ranks)
+    t0 = time.perf_counter()
+    _, err = O.tensorly_ht_fp32(x, dims, tr, ranks)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / dt, "unit": "decomps/s", "cores": threads, "kind": "port",
+            "sample": f"full decomposition of a {n}^4 tensor of the same family (core 16^4, ranks {rank}, "
+                      f"seed 1) by the FP32 numpy restatement of the reference's TensorLy path; "
+                      f"{dt:.2f} s, rel err {err:.6f}; not scaled to 128^4"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one MI355X; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import ht_tensor_learning_using_gpu_tensor_cores_amd as ht
+    from ht_tensor_learning_using_gpu_tensor_cores_amd import _lib
+
+    lib = _lib.load()
+    n, r = args.n, args.rank
+    dims = [n] * 4
+    tree = ht.tree.balanced_4d()
+    ranks = [1] + [r] * 6
+    h = ht.default_handle(0)
+    x = ht.generate_tucker(dims, [16] * 4, seed=1, noise_rel=1e-4, handle=h)
+    torch.cuda.synchronize()
+    N = int(np.prod(dims))
+    xbytes = 4.0 * N
+
+    def fast():
+        return ht.decompose(x, dims, tree, ranks, method="ltr", precision="bf16", err="residual", handle=h)
+
+    for _ in range(args.warmup):
+        res = fast()
+    nk = len(_lib.PROFILE_KERNELS)
+    ms = (ctypes.c_double * nk)()
+    cnt = (ctypes.c_int64 * nk)()
+    lib.ht_profile_read(h.ptr, ms, cnt, nk, 1)
+    lib.ht_profile_enable(h.ptr, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = fast()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lib.ht_profile_enable(h.ptr, 0)
+    lib.ht_profile_read(h.ptr, ms, cnt, nk, 1)
+    step_ms = (t1 - t0) * 1000.0 / args.steps
+    value = args.steps / (t1 - t0)
+    err_fast = res.rel_err
+    kern = {}
+    for i, name in enumerate(_lib.PROFILE_KERNELS):
+        if cnt[i] > 0:
+            kern[name] = {"avg_ms": ms[i] / cnt[i], "launches_per_step": cnt[i] / args.steps,
+                          "share": ms[i] / args.steps / step_ms}
+    # HBM-pass kernels: each reads X once (algorithmic bytes = |X|_FP32).
+    flops = {"slab_gram": 2.0 * 2.0 * n * n * (N / n), "slab_proj": 2.0 * N * r + 2.0 * (N / n) * r * r,
+             "residual": 2.0 * N * r}
+    for name in ("slab_gram", "slab_proj", "residual"):
+        if name in kern:
+            k = kern[name]
+            k["algorithmic_bytes"] = xbytes
+            k["GBps"] = xbytes / (k["avg_ms"] * 1e-3) / 1e9
+            k["hbm_frac"] = k["GBps"] / 8000.0
+            k["TFLOPs"] = flops[name] / (k["avg_ms"] * 1e-3) / 1e12
+    dominant = max((nm for nm in ("slab_gram", "slab_proj", "residual") if nm in kern),
+                   key=lambda nm: kern[nm]["share"], default=None)
+
+    # Plain FP32 exact truncated-SVD HT on the same GPU (the comparator).
+    plain = None
+    if not args.no_plain:
+        def slow():
+            return ht.decompose(x, dims, tree, ranks, method="plain", precision="fp32", err="residual", handle=h)
+        pr = slow()
+        torch.cuda.synchronize()
+        p0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            pr = slow()
+        torch.cuda.synchronize()
+        p1 = time.perf_counter()
+        plain = {"ms_per_decomp": (p1 - p0) * 1000.0 / args.plain_steps, "rel_err": pr.rel_err,
+                 "passes_over_x": pr.stats["passes"], "eig_iters_max": pr.stats["eig_iters_max"]}
+
+    roofline = None
+    if dominant:
+        k = kern[dominant]
+        roofline = {"bound": "hbm", "achieved": round(k["GBps"], 1), "peak": 8000.0, "unit": "GB/s",
+                    "frac": round(k["hbm_frac"], 4), "traffic": read_traffic(args.traffic_file, dominant),
+                    "kernel": dominant, "avg_ms": round(k["avg_ms"], 4), "algorithmic_bytes": xbytes}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_sample_n, r)
+
+    out = {
+        "metric": "HT decomps/s at rel-err \u22641e-3: BF16-MFMA rSVD vs FP32 plain, 4D 128^4",
+        "value": round(value, 3),
+        "unit": "decomps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded Philox Tucker tensor generated on the device)",
+        "config": {"workload": f"4th-order {n}^4 FP32 Tucker(16^4)+1e-4 noise, seed 1, tree {{1,2}},{{3,4}}, "
+                               f"ranks {r}; leaves-to-root, BF16 MFMA Grams, FP32 projection, FP64 small "
+                               f"solvers, residual error pass", "tensor_GB": xbytes / 1e9},
+        "tensor_GBps": round(xbytes / (step_ms * 1e-3) / 1e9, 1),
+        "rel_err": err_fast,
+        "plain_fp32": plain,
+        "speedup_vs_plain": round(plain["ms_per_decomp"] / step_ms, 2) if plain else None,
+        "err_gap_vs_plain": abs(err_fast - plain["rel_err"]) if plain else None,
+        "kernels": kern,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -80,7 +80,12 @@ SIGNATURES = {
     "ht_skinny_gemm": (I32, [H, P, I64, I64, I32, P, I32, I32, P]),
     "ht_philox_normals": (I32, [H, U64, U32, I64, P]),
     "ht_philox_normals_host": (None, [U64, U32, I64, PD]),
+    "ht_profile_enable": (I32, [H, I32]),
+    "ht_profile_read": (I32, [H, PD, PI64, I32, I32]),
 }
+
+PROFILE_KERNELS = ["slab_gram", "slab_proj", "mode_gram", "residual", "skinny_gemm", "tsqr", "small_eig",
+                   "small_contract", "generator"]
 
 _lib = None
 
@@ -90,6 +95,14 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: torch ships its own libamdhip64 and opens it by path,
+    # so if the library (linked against the system runtime) were loaded first, torch
+    # would start a second runtime that cannot see the device. Importing torch first
+    # makes the library bind to the runtime torch already loaded (same soname).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} not found: build it with "
                           "`python -m ht_tensor_learning_using_gpu_tensor_cores_amd.csrc.build`")

@@ -64,6 +64,7 @@ ht_status ht_destroy(ht_handle_t h) {
   (void)hipSetDevice(h->device);
   (void)hipStreamSynchronize(h->stream);
   h->arena.release();
+  h->prof.destroy();
   if (h->xbuf) (void)hipFree(h->xbuf);
   if (h->obuf) (void)hipFree(h->obuf);
   for (auto& e : h->ev) (void)hipEventDestroy(e);
@@ -140,6 +141,7 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
   e.arena = &h->arena;
   e.prob = &p;
   e.opt = opt;
+  e.prof = &h->prof;
   if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
   int s = e.run(xd, od);
   if (s && s != HT_ERR_NOCONV) {
@@ -487,6 +489,27 @@ void htd4_tc(float* x, long a, long b, long c, long d, int* k, void* blas_handle
   const double err = compat_run(x, 4, dims, T4_LO, T4_HI, T4_L, T4_R, k, HT_METHOD_LTR, &ms);
   if (err >= 0) printf("error rate %g\n", err);
   else printf("htd4_tc failed (status %d)\n", (int)(-1.0 - err));
+}
+
+// ------------------------------------------------------------------ profiling
+ht_status ht_profile_enable(ht_handle_t h, int32_t on) {
+  if (!h) return HT_ERR_INVALID_ARG;
+  h->prof.enabled = on != 0;
+  return HT_OK;
+}
+
+ht_status ht_profile_read(ht_handle_t h, double* ms, int64_t* launches, int32_t n, int32_t reset) {
+  if (!h || n < 0) return HT_ERR_INVALID_ARG;
+  for (int i = 0; i < n && i < PK_COUNT; ++i) {
+    if (ms) ms[i] = h->prof.ms[i];
+    if (launches) launches[i] = h->prof.cnt[i];
+  }
+  if (reset)
+    for (int i = 0; i < PK_COUNT; ++i) {
+      h->prof.ms[i] = 0.0;
+      h->prof.cnt[i] = 0;
+    }
+  return HT_OK;
 }
 
 // ------------------------------------------------------------------ component entry points

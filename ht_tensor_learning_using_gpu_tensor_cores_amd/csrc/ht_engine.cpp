@@ -52,6 +52,48 @@ void Arena::release() {
   reset();
 }
 
+// ------------------------------------------------------------------ profiler
+int Profiler::begin(hipStream_t st, int k) {
+  if (!enabled) return -1;
+  if (!created) {
+    for (auto& x : ev)
+      if (hipEventCreate(&x) != hipSuccess) return -1;
+    created = true;
+  }
+  if (n >= NREC) return -1;
+  if (hipEventRecord(ev[2 * n], st) != hipSuccess) return -1;
+  kid[n] = k;
+  return n++;
+}
+void Profiler::end(hipStream_t st, int slot) {
+  if (slot >= 0) (void)hipEventRecord(ev[2 * slot + 1], st);
+}
+void Profiler::collect() {
+  for (int i = 0; i < n; ++i) {
+    float t = 0.f;
+    if (hipEventSynchronize(ev[2 * i + 1]) == hipSuccess && hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) == hipSuccess) {
+      ms[kid[i]] += t;
+      cnt[kid[i]] += 1;
+    }
+  }
+  n = 0;
+}
+void Profiler::destroy() {
+  if (created)
+    for (auto& x : ev) (void)hipEventDestroy(x);
+  created = false;
+}
+
+struct ProfScope {
+  Profiler* p;
+  hipStream_t st;
+  int slot;
+  ProfScope(Profiler* p_, hipStream_t s, int k) : p(p_), st(s), slot(p_ ? p_->begin(s, k) : -1) {}
+  ~ProfScope() {
+    if (p) p->end(st, slot);
+  }
+};
+
 // ------------------------------------------------------------------ tree validation
 int validate_tree(int order, int n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
                   const int32_t* right) {
@@ -192,7 +234,10 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
   float* part = alloc<float>((size_t)pairs * ns * 128 * 128);
   double* norms = alloc<double>((size_t)pairs * ns);
   if (!part || !norms) return HT_ERR_ALLOC;
-  CK(gram_pms_launch(st, x, v, prec, ns, part, norms));
+  {
+    ProfScope ps(prof, st, PK_GRAM);
+    CK(gram_pms_launch(st, x, v, prec, ns, part, norms));
+  }
   CK(gram_reduce_launch(st, part, (int)v.M, ns, G));
   if (sumsq) CK(gram_reduce_norms_launch(st, norms, pairs * ns, sumsq));
   passes += 1.0;
@@ -205,6 +250,7 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
 // projected matrix; it stops when every top-k Ritz residual ||G u - theta u|| is below
 // tol * theta_1.
 int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream) {
+  ProfScope ps(prof, st, PK_EIG);
   int* info = alloc<int>(4);
   if (n <= 64) {
     double* V = alloc<double>((size_t)n * n);
@@ -271,6 +317,7 @@ int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double
 int Engine::tsqr(float* Y, long m, int b, float* Q, double* R) {
   void* ws = arena->get(tsqr_ws_bytes(m, b));
   if (!ws) return HT_ERR_ALLOC;
+  ProfScope ps(prof, st, PK_TSQR);
   return tsqr_launch(st, Y, m, b, Q, R, ws);
 }
 
@@ -278,6 +325,7 @@ int Engine::tsqr(float* Y, long m, int b, float* Q, double* R) {
 // (A^T = X viewed as P x M, lda = P). op 0: Y = A B (B: cols x b); op 1: Y = A^T B.
 int Engine::view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y) {
   float* ws = nullptr;
+  ProfScope ps(prof, st, PK_GEMM);
   if (v.P == 1) {
     const long M = v.M, K = v.S;
     const size_t wsf = gemm_ws_floats(M, K, b);
@@ -443,7 +491,10 @@ int Engine::residual(const float* x, double* err_sq, double* x_sq) {
   double* acc = alloc<double>((size_t)2 * nbx * nby);
   if (!acc) return HT_ERR_ALLOC;
   int nb = 0;
-  CK(lowrank_launch(st, 1, Lf, Rf, nl, nr, kl, x, nullptr, acc, &nb));
+  {
+    ProfScope ps(prof, st, PK_RESIDUAL);
+    CK(lowrank_launch(st, 1, Lf, Rf, nl, nr, kl, x, nullptr, acc, &nb));
+  }
   double* red = alloc<double>(2 + 2048);
   CK(sum_strided_launch(st, acc, nb, red));
   double h[2];
@@ -612,7 +663,10 @@ int Engine::run_ltr(const float* x) {
     float* p2 = alloc<float>((size_t)nb * 128 * 128);
     double* norms = alloc<double>(nb);
     if (!p1 || !p2 || !norms) return HT_ERR_ALLOC;
-    CK(slab_gram_launch(st, x, (int)n0, (int)n1, S, prec, nb, p1, p2, norms));
+    {
+      ProfScope ps(prof, st, PK_SLAB_GRAM);
+      CK(slab_gram_launch(st, x, (int)n0, (int)n1, S, prec, nb, p1, p2, norms));
+    }
     CK(gram_reduce_launch(st, p1, (int)n0, nb, G0));
     CK(gram_reduce_launch(st, p2, (int)n1, nb, G1));
     CK(gram_reduce_norms_launch(st, norms, nb, xsq));
@@ -638,6 +692,7 @@ int Engine::run_ltr(const float* x) {
   CK(cast_f64_f32_launch(st, Ufull[t0], U0f, n0 * r0));
   CK(cast_f64_f32_launch(st, Ufull[t1], U1f, n1 * r1));
   if (slab) {
+    ProfScope ps(prof, st, PK_SLAB_PROJ);
     CK(slab_proj_launch(st, x, (int)n0, (int)n1, S, U0f, r0, U1f, r1, Z));
   } else {
     float* Z1 = alloc<float>((size_t)r0 * n1 * S);
@@ -854,6 +909,7 @@ int Engine::run(const float* x, float* out) {
   if (s) return s;
   if (out) CK(write_outputs(out));
   CK(sync());
+  if (prof && prof->enabled) prof->collect();
   if (noconv) return HT_ERR_NOCONV;
   return 0;
 }

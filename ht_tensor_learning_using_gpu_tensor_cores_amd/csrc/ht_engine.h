@@ -37,7 +37,26 @@ struct Problem {
   int leaf_of_mode(int m) const;
 };
 
+// Per-kernel device timing with HIP events recorded on the launching stream.
+enum ProfKernel {
+  PK_SLAB_GRAM = 0, PK_SLAB_PROJ, PK_GRAM, PK_RESIDUAL, PK_GEMM, PK_TSQR, PK_EIG, PK_SMALL, PK_GEN, PK_COUNT
+};
+struct Profiler {
+  static constexpr int NREC = 1024;
+  hipEvent_t ev[2 * NREC];
+  int kid[NREC];
+  int n = 0;
+  bool created = false, enabled = false;
+  double ms[PK_COUNT] = {};
+  long cnt[PK_COUNT] = {};
+  int begin(hipStream_t st, int k);
+  void end(hipStream_t st, int slot);
+  void collect();
+  void destroy();
+};
+
 struct Engine {
+  Profiler* prof = nullptr;
   hipStream_t st;
   Arena* arena;
   const Problem* prob;
@@ -88,4 +107,5 @@ struct ht_handle_s {
   size_t xbuf_bytes = 0;
   float* obuf = nullptr;   // device output staging for host outputs
   size_t obuf_bytes = 0;
+  ht::Profiler prof;
 };

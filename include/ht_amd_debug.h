@@ -64,6 +64,15 @@ ht_status ht_skinny_gemm(ht_handle_t h, const float* A, int64_t M, int64_t K, in
 ht_status ht_philox_normals(ht_handle_t h, uint64_t seed, uint32_t stream, int64_t n, double* out_dev);
 void ht_philox_normals_host(uint64_t seed, uint32_t stream, int64_t n, double* out);
 
+/* Per-kernel device timing. When enabled, every decomposition records HIP events on
+ * the handle's stream around its main kernels and accumulates their durations. Kernel
+ * ids: 0 slab Gram, 1 slab projection, 2 mode Gram, 3 residual, 4 skinny GEMM, 5 TSQR,
+ * 6 small eigen solves, 7 small contractions, 8 generators. ht_profile_read copies the
+ * accumulated milliseconds and launch counts of the first n ids (and clears them when
+ * reset != 0). */
+ht_status ht_profile_enable(ht_handle_t h, int32_t on);
+ht_status ht_profile_read(ht_handle_t h, double* ms, int64_t* launches, int32_t n, int32_t reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -508,3 +508,60 @@ def ht_error_bound(x, dims, tree, ranks):
         s = np.linalg.svd(A, compute_uv=False)
         total += float(np.sum(s[ranks[t]:] ** 2))
     return float(np.sqrt(total) / np.linalg.norm(x))
+
+
+# ------------------------------------------------------------- randomised RTL HT
+
+def rsvd_ht(x, dims, tree, ranks, p=8, q=1, seed=1234, max_block=64):
+    """Root-to-leaves randomised HT as the engine's HT_METHOD_RSVD runs it (the 4O
+    structure, fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/htd4_tensorcore.cu:
+    174-228, with oversampling p, q power iterations and the full projected SVD):
+    leaves by the Gram eigenvectors of X_(n); each interior node t whose mode range starts
+    at mode 0 or ends at the last mode by a range finder on X_(t) with the FP32-rounded
+    Philox test matrix of stream 256+t (l = min(k + p, 64, rows, cols) columns, k the
+    larger of the node's and its complementary sibling's rank); the complementary sibling
+    of a root child takes the right singular vectors of the same projected SVD."""
+    validate_ranks(dims, tree, ranks)
+    x = np.asarray(x, dtype=np.float64)
+    ht = HT(dims, tree, ranks)
+    full = {}
+    for t in tree.leaves():
+        A = unfold(x, dims, tree.lo[t], tree.hi[t])
+        U, w = gram_topk(A @ A.T, ranks[t])
+        full[t] = ht.U[t] = U
+        ht.sv[t] = np.sqrt(np.maximum(w, 0))
+    parent = tree.parents()
+    done = set()
+    for t in range(1, tree.n_nodes):
+        if tree.is_leaf(t) or t in done:
+            continue
+        A = unfold(x, dims, tree.lo[t], tree.hi[t])
+        rows, cols = A.shape
+        sib = -1
+        if parent[t] == 0:
+            sib = tree.right[0] if tree.left[0] == t else tree.left[0]
+        kW = ranks[sib] if sib >= 0 and not tree.is_leaf(sib) else 0
+        k = ranks[t]
+        l = int(min(max(k, kW) + p, max_block, rows, cols))
+        om = philox.normals(seed, 256 + t, cols * l).astype(np.float32).astype(np.float64)
+        om = om.reshape((cols, l), order="F")
+        Q = qr_q(A @ om)
+        for _ in range(q):
+            Qz = qr_q(A.T @ Q)
+            Q = qr_q(A @ Qz)
+        Ub, s, Vbt = np.linalg.svd(Q.T @ A, full_matrices=False)
+        full[t] = ht.U[t] = canon_signs((Q @ Ub)[:, :k])
+        ht.sv[t] = s[:k]
+        if kW:
+            full[sib] = ht.U[sib] = canon_signs(Vbt.T[:, :kW])
+            ht.sv[sib] = s[:kW]
+            done.add(sib)
+        done.add(t)
+    for t in tree.interior():
+        l, r = tree.left[t], tree.right[t]
+        nl, nr = node_rows(dims, tree, l), node_rows(dims, tree, r)
+        if t == 0:
+            ht.B[0] = (full[l].T @ x.reshape((nl, nr), order="F") @ full[r])[:, :, None]
+        else:
+            ht.B[t] = contract_transfer(full[t], full[l], full[r], nl, nr)
+    return ht
