@@ -545,7 +545,7 @@ ht_status ht_slab_gram(ht_handle_t h, const float* x, int32_t n1, int32_t n2, in
   if (!h || !x || !G1 || !G2 || n1 < 1 || n2 < 1 || n1 > 128 || n2 > 128 || S < 1) return HT_ERR_INVALID_ARG;
   const int prec = precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16;
   h->arena.reset();
-  const int nb = (int)std::min<int64_t>(S, 1024);
+  const int nb = (int)std::min<int64_t>(S, 256);
   float* p1 = (float*)h->arena.get((size_t)nb * 128 * 128 * 4);
   float* p2 = (float*)h->arena.get((size_t)nb * 128 * 128 * 4);
   double* norms = (double*)h->arena.get((size_t)nb * 8 + 64);

@@ -244,23 +244,96 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
   return 0;
 }
 
-// Top-k eigenpairs (descending) of the symmetric PSD n x n FP64 matrix G.
-// n <= 64: one Jacobi solve. Otherwise subspace iteration with a b-column block
-// (b <= 64), Cholesky-QR, and a Rayleigh-Ritz step with a Jacobi solve of the b x b
-// projected matrix; it stops when every top-k Ritz residual ||G u - theta u|| is below
-// tol * theta_1.
-int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream) {
+// Top-k eigenpairs (descending, canonical signs) of `batch` symmetric PSD n x n FP64
+// matrices G (stride n*n), written to U (stride n*k) and w (stride k).
+// n <= 64: one Jacobi solve each. n <= 256: rounds of block subspace iteration (block b,
+// q powers, Householder re-orthonormalisation, Rayleigh-Ritz with a b x b Jacobi solve)
+// until every top-k Ritz residual is below tol * theta_1; q doubles between rounds and
+// each round restarts from the previous Ritz basis.
+int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
+                           uint32_t stream) {
   ProfScope ps(prof, st, PK_EIG);
-  int* info = alloc<int>(4);
   if (n <= 64) {
-    double* V = alloc<double>((size_t)n * n);
-    double* wa = alloc<double>(n);
-    CK(jacobi_eig_launch(st, G, n, n, wa, V, n, info, 60, 1e-15));
-    HT_HIP_CHECK(hipMemcpyAsync(U, V, sizeof(double) * n * k, hipMemcpyDeviceToDevice, st));
-    HT_HIP_CHECK(hipMemcpyAsync(w, wa, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
-    ++eig_calls;
+    double* V = alloc<double>((size_t)batch * n * n);
+    double* wa = alloc<double>((size_t)batch * n);
+    int* info = alloc<int>(batch);
+    if (!V || !wa || !info) return HT_ERR_ALLOC;
+    CK(jacobi_full_launch(st, G, (long)n * n, n, wa, n, V, (long)n * n, batch, info, 60, 1e-15));
+    HT_HIP_CHECK(hipMemcpy2DAsync(U, sizeof(double) * n * k, V, sizeof(double) * n * n, sizeof(double) * n * k, batch,
+                                  hipMemcpyDeviceToDevice, st));
+    HT_HIP_CHECK(hipMemcpy2DAsync(w, sizeof(double) * k, wa, sizeof(double) * n, sizeof(double) * k, batch,
+                                  hipMemcpyDeviceToDevice, st));
+    eig_calls += batch;
     return 0;
   }
+  if (n > 256) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream);
+  // Block width: k + 8 by default; the Householder step keeps n x b reflectors and Q in
+  // LDS, so n * b <= 9400 (b <= 36 at n = 256). A round that does not converge widens
+  // the block to that limit.
+  const int bmax = std::min(std::min(64, n), 9400 / n);
+  if (b <= 0) b = k + 8;
+  b = std::min(b, bmax);
+  if (k > b) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream);
+  const int b0 = b;
+  b = bmax;  // buffers sized for the widest block
+  long nb = (long)n * b;
+  double* W = alloc<double>((size_t)batch * nb);
+  double* Y = alloc<double>((size_t)batch * nb);
+  double* Q = alloc<double>((size_t)batch * nb);
+  double* Z = alloc<double>((size_t)batch * nb);
+  double* Uf = alloc<double>((size_t)batch * nb);
+  double* Vs = alloc<double>((size_t)batch * 256 * 64);
+  double* wall = alloc<double>((size_t)batch * 64);
+  int* conv = alloc<int>(batch);
+  if (!W || !Y || !Q || !Z || !Uf || !Vs || !wall || !conv) return HT_ERR_ALLOC;
+  CK(fill_normals_f64_launch(st, W, batch * nb, opt.seed, stream));
+  std::vector<int> hconv(batch);
+  int q = std::max(1, q0), iters = 0;
+  b = b0;
+  nb = (long)n * b;
+  for (int round = 0;; ++round) {
+    CK(power_launch(st, G, (long)n * n, W, nb, Y, nb, n, b, q, 1, batch));
+    CK(hhqr_launch(st, Y, nb, Vs, Q, nb, n, b, batch));
+    CK(power_launch(st, G, (long)n * n, Q, nb, Z, nb, n, b, 1, 0, batch));
+    CK(rr_launch(st, Q, Z, nb, n, b, k, Uf, U, (long)n * k, w, k, wall, tol, conv, batch));
+    iters += q;
+    CK(d2h(hconv.data(), conv, sizeof(int) * batch));
+    bool all = true;
+    for (int i = 0; i < batch; ++i) all = all && hconv[i];
+    if (all) break;
+    if (iters >= opt.max_eig_iters) {
+      noconv = 1;
+      break;
+    }
+    // next round: restart from the Ritz basis; widen the block after the first miss
+    if (b < bmax) {
+      const int bn = bmax;
+      const long nbn = (long)n * bn;
+      // Ritz vectors first, fresh Gaussian columns after them (per problem)
+      CK(fill_normals_f64_launch(st, W, batch * nbn, opt.seed, stream + 100 + round));
+      HT_HIP_CHECK(hipMemcpy2DAsync(W, sizeof(double) * nbn, Uf, sizeof(double) * nb, sizeof(double) * nb, batch,
+                                    hipMemcpyDeviceToDevice, st));
+      b = bn;
+      nb = nbn;
+    } else {
+      std::swap(W, Uf);
+    }
+    q = std::min(2 * q, 64);
+  }
+  eig_iters_max = std::max(eig_iters_max, iters);
+  eig_calls += batch;
+  return 0;
+}
+
+int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream) {
+  (void)maxit;
+  return sym_topk_batch(G, n, k, 1, U, w, tol, 2, 0, stream);
+}
+
+// Subspace iteration for n > 256 (multi-kernel, host-checked convergence).
+int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream) {
+  ProfScope ps(prof, st, PK_EIG);
+  int* info = alloc<int>(4);
   int b = std::min(64, std::max(2 * k, k + 16));
   b = std::min(b, n);
   double* Q = alloc<double>((size_t)n * b);
@@ -507,22 +580,41 @@ int Engine::residual(const float* x, double* err_sq, double* x_sq) {
 
 // ------------------------------------------------------------------ methods
 int Engine::leaves_from_x_plain(const float* x, double* xsq_dev) {
+  return leaves_from_x(x, PREC_F32, xsq_dev);
+}
+
+// Leaf bases from Grams of X (one pass per leaf), eigen problems of equal shape batched.
+int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
   const Problem& p = *prob;
+  std::vector<int> leaves;
+  for (int t = 0; t < p.n_nodes; ++t)
+    if (p.left[t] < 0) leaves.push_back(t);
+  std::vector<int> done(p.n_nodes, 0);
   bool have_norm = false;
-  for (int t = 0; t < p.n_nodes; ++t) {
-    if (p.left[t] >= 0) continue;
-    const int m = p.lo[t];
-    const long n = p.dims[m];
+  for (int t : leaves) {
+    if (done[t]) continue;
+    const long n = p.dims[p.lo[t]];
+    const int k = p.rank[t];
     if (n > 1024) return HT_ERR_UNSUPPORTED;
-    const PMS v = view_of(p.dims, p.order, m, m + 1);
-    double* G = alloc<double>((size_t)n * n);
-    if (!G) return HT_ERR_ALLOC;
-    CK(gram(x, v, PREC_F32, G, have_norm ? nullptr : xsq_dev));
-    have_norm = true;
-    Ufull[t] = alloc<double>((size_t)n * p.rank[t]);
-    double* w = alloc<double>(p.rank[t]);
-    CK(sym_topk(G, (int)n, p.rank[t], Ufull[t], w, opt.eig_tol * 1e3, opt.max_eig_iters, 1000 + t * 16));
-    CK(eig_to_sv(w, p.rank[t], SVd[t]));
+    std::vector<int> grp;
+    for (int u : leaves)
+      if (!done[u] && p.dims[p.lo[u]] == n && p.rank[u] == k) grp.push_back(u);
+    const int nb = (int)grp.size();
+    double* G = alloc<double>((size_t)nb * n * n);
+    double* U = alloc<double>((size_t)nb * n * k);
+    double* w = alloc<double>((size_t)nb * k);
+    if (!G || !U || !w) return HT_ERR_ALLOC;
+    for (int i = 0; i < nb; ++i) {
+      const int m = p.lo[grp[i]];
+      CK(gram(x, view_of(p.dims, p.order, m, m + 1), prec, G + (size_t)i * n * n, have_norm ? nullptr : xsq_dev));
+      have_norm = true;
+    }
+    CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16));
+    for (int i = 0; i < nb; ++i) {
+      Ufull[grp[i]] = U + (size_t)i * n * k;
+      CK(eig_to_sv(w + (size_t)i * k, k, SVd[grp[i]]));
+      done[grp[i]] = 1;
+    }
   }
   return 0;
 }
@@ -654,11 +746,11 @@ int Engine::run_ltr(const float* x) {
                     prec != PREC_F32 && (((uintptr_t)x & 15) == 0);
   const long S = p.N / (n0 * n1);
   // Pass 1: Grams of the two leading modes.
-  double* G0 = alloc<double>((size_t)n0 * n0);
-  double* G1 = alloc<double>((size_t)n1 * n1);
+  double* G0 = alloc<double>((size_t)n0 * n0 + (size_t)n1 * n1);
+  double* G1 = G0 + (size_t)n0 * n0;
   if (!G0 || !G1) return HT_ERR_ALLOC;
   if (slab) {
-    int nb = (int)std::min<long>(S, 1024);
+    int nb = (int)std::min<long>(S, 256);
     float* p1 = alloc<float>((size_t)nb * 128 * 128);
     float* p2 = alloc<float>((size_t)nb * 128 * 128);
     double* norms = alloc<double>(nb);
@@ -676,12 +768,16 @@ int Engine::run_ltr(const float* x) {
     CK(gram(x, view_of(p.dims, d, 0, 1), prec, G0, xsq));
     CK(gram(x, view_of(p.dims, d, 1, 2), prec, G1, nullptr));
   }
-  Ufull[t0] = alloc<double>((size_t)n0 * r0);
-  Ufull[t1] = alloc<double>((size_t)n1 * r1);
-  double* w0 = alloc<double>(r0);
-  double* w1 = alloc<double>(r1);
-  CK(sym_topk(G0, (int)n0, r0, Ufull[t0], w0, 1e-9, opt.max_eig_iters, 1000 + t0 * 16));
-  CK(sym_topk(G1, (int)n1, r1, Ufull[t1], w1, 1e-9, opt.max_eig_iters, 1000 + t1 * 16));
+  Ufull[t0] = alloc<double>((size_t)n0 * r0 + (size_t)n1 * r1);
+  Ufull[t1] = Ufull[t0] + (size_t)n0 * r0;
+  double* w0 = alloc<double>(r0 + r1);
+  double* w1 = w0 + r0;
+  if (n0 == n1 && r0 == r1 && G1 == G0 + (size_t)n0 * n0) {
+    CK(sym_topk_batch(G0, (int)n0, r0, 2, Ufull[t0], w0, 1e-9, 2, 0, 1000 + t0 * 16));
+  } else {
+    CK(sym_topk_batch(G0, (int)n0, r0, 1, Ufull[t0], w0, 1e-9, 2, 0, 1000 + t0 * 16));
+    CK(sym_topk_batch(G1, (int)n1, r1, 1, Ufull[t1], w1, 1e-9, 2, 0, 1000 + t1 * 16));
+  }
   CK(eig_to_sv(w0, r0, SVd[t0]));
   CK(eig_to_sv(w1, r1, SVd[t1]));
   // Pass 2: Z = X x_1 U_0^T x_2 U_1^T  (r0, r1, n_2, ...), FP32.
@@ -706,17 +802,33 @@ int Engine::run_ltr(const float* x) {
   zd[0] = r0;
   zd[1] = r1;
   for (int m = 2; m < d; ++m) zd[m] = p.dims[m];
-  for (int m = 2; m < d; ++m) {
-    const int t = p.leaf_of_mode(m);
-    const long n = p.dims[m];
-    if (n > 1024) return HT_ERR_UNSUPPORTED;
-    double* G = alloc<double>((size_t)n * n);
-    CK(gram(Z, view_of(zd, d, m, m + 1), PREC_F32, G, nullptr));
-    passes -= 1.0;  // Z is small; not a pass over X
-    Ufull[t] = alloc<double>((size_t)n * p.rank[t]);
-    double* w = alloc<double>(p.rank[t]);
-    CK(sym_topk(G, (int)n, p.rank[t], Ufull[t], w, 1e-9, opt.max_eig_iters, 1000 + t * 16));
-    CK(eig_to_sv(w, p.rank[t], SVd[t]));
+  {
+    std::vector<int> done(d, 0);
+    for (int m = 2; m < d; ++m) {
+      if (done[m]) continue;
+      const long n = p.dims[m];
+      const int k = p.rank[p.leaf_of_mode(m)];
+      if (n > 1024) return HT_ERR_UNSUPPORTED;
+      std::vector<int> grp;
+      for (int m2 = m; m2 < d; ++m2)
+        if (!done[m2] && p.dims[m2] == n && p.rank[p.leaf_of_mode(m2)] == k) grp.push_back(m2);
+      const int nb = (int)grp.size();
+      double* G = alloc<double>((size_t)nb * n * n);
+      double* U = alloc<double>((size_t)nb * n * k);
+      double* w = alloc<double>((size_t)nb * k);
+      if (!G || !U || !w) return HT_ERR_ALLOC;
+      for (int i = 0; i < nb; ++i) {
+        CK(gram(Z, view_of(zd, d, grp[i], grp[i] + 1), PREC_F32, G + (size_t)i * n * n, nullptr));
+        passes -= 1.0;  // Z is small; not a pass over X
+      }
+      CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16));
+      for (int i = 0; i < nb; ++i) {
+        const int t = p.leaf_of_mode(grp[i]);
+        Ufull[t] = U + (size_t)i * n * k;
+        CK(eig_to_sv(w + (size_t)i * k, k, SVd[t]));
+        done[grp[i]] = 1;
+      }
+    }
   }
   long csize = r0 * r1 * S;
   double* C = alloc<double>((size_t)csize);
@@ -766,7 +878,7 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
     CK(gram_pms_f64_launch(st, C, v, G));
     double* V = alloc<double>((size_t)M * k);
     double* w = alloc<double>(k);
-    CK(sym_topk(G, (int)M, k, V, w, opt.eig_tol, opt.max_eig_iters, 4000 + t * 16));
+    CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-7, 24, 64, 4000 + t * 16));
     CK(eig_to_sv(w, k, SVd[t]));
     Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself
     double* C2 = alloc<double>((size_t)v.P * k * v.S);
@@ -788,20 +900,7 @@ int Engine::run_rsvd(const float* x) {
   const Problem& p = *prob;
   const int prec = opt.precision == HT_PREC_FP32 ? PREC_F32 : (opt.precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
   double* xsq = alloc<double>(1);
-  bool have_norm = false;
-  for (int t = 0; t < p.n_nodes; ++t) {
-    if (p.left[t] >= 0) continue;
-    const int m = p.lo[t];
-    const long n = p.dims[m];
-    if (n > 1024) return HT_ERR_UNSUPPORTED;
-    double* G = alloc<double>((size_t)n * n);
-    CK(gram(x, view_of(p.dims, p.order, m, m + 1), prec, G, have_norm ? nullptr : xsq));
-    have_norm = true;
-    Ufull[t] = alloc<double>((size_t)n * p.rank[t]);
-    double* w = alloc<double>(p.rank[t]);
-    CK(sym_topk(G, (int)n, p.rank[t], Ufull[t], w, 1e-9, opt.max_eig_iters, 1000 + t * 16));
-    CK(eig_to_sv(w, p.rank[t], SVd[t]));
-  }
+  CK(leaves_from_x(x, prec, xsq));
   std::vector<int> done(p.n_nodes, 0);
   for (int t = 1; t < p.n_nodes; ++t) {
     if (p.left[t] < 0 || done[t]) continue;

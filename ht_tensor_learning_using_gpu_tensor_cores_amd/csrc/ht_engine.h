@@ -76,6 +76,9 @@ struct Engine {
   int d2h(void* dst, const void* src, size_t bytes);
   int gram(const float* x, PMS v, int prec, double* G, double* sumsq);
   int sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream);
+  int sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
+                     uint32_t stream);
+  int sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream);
   int tsqr(float* Y, long m, int b, float* Q, double* R);
   int view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y);
   int exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U, double* Wr, double* sv, uint32_t stream);
@@ -85,6 +88,7 @@ struct Engine {
   int transpose_f64(const double* A, long m, long n, double* AT);
   int residual(const float* x, double* err_sq, double* x_sq);
   int leaves_from_x_plain(const float* x, double* xsq_dev);
+  int leaves_from_x(const float* x, int prec, double* xsq_dev);
   int eig_to_sv(const double* w, int k, double*& sv);
   int root_transfer(const float* x);
   int finish_error(const float* x, const double* xsq_dev);

@@ -393,6 +393,9 @@ __global__ __launch_bounds__(256) void lowrank_kernel(const float* __restrict__ 
   }
 }
 
+__global__ void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M, long N, int k,
+                                const float* __restrict__ X, double* __restrict__ acc_out);
+
 int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, long M, long N, int k,
                    const float* X, float* out, double* acc, int* n_blocks_out) {
   if (M < 1 || N < 1 || k < 1 || k > 4096) return 8;
@@ -401,8 +404,110 @@ int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, lon
   if (mode == 0)
     hipLaunchKernelGGL(lowrank_kernel<0>, grid, dim3(256), 0, st, L, R, M, N, k, X, out, acc);
   else
-    hipLaunchKernelGGL(lowrank_kernel<1>, grid, dim3(256), 0, st, L, R, M, N, k, X, out, acc);
+    hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
   return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// Residual pass: acc[2*block] = sum (X - L R^T)^2, acc[2*block+1] = sum X^2 over one
+// 128 x 128 tile of X (ld M). The X tile is requested (float4, coalesced along m) before
+// the low-rank tile is formed with MFMA, then compared against it through LDS.
+__global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M,
+                                                       long N, int k, const float* __restrict__ X,
+                                                       double* __restrict__ acc_out) {
+  __shared__ float Ls[LR_K * LR_T];
+  __shared__ float Rs[LR_K * LR_T];
+  __shared__ float Cs[LR_T * (LR_T + 4)];
+  __shared__ double red[8];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const long m0 = (long)blockIdx.x * LR_T, n0 = (long)blockIdx.y * LR_T;
+  const bool vec = (M % 4 == 0) && (((uintptr_t)X & 15) == 0);
+  // X tile prefetch: element (m0 + 4*mq .. +3, n0 + col), 16 float4 per thread.
+  float4 xv[16];
+  const int mq = tid & 31;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int col = (tid >> 5) + 8 * i;
+    const long n = n0 + col, m = m0 + 4 * mq;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (n < N) {
+      const float* p = X + m + M * n;
+      if (vec && m + 3 < M) v = *reinterpret_cast<const float4*>(p);
+      else {
+        if (m < M) v.x = p[0];
+        if (m + 1 < M) v.y = p[1];
+        if (m + 2 < M) v.z = p[2];
+        if (m + 3 < M) v.w = p[3];
+      }
+    }
+    xv[i] = v;
+  }
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x16{};
+  for (int k0 = 0; k0 < k; k0 += LR_K) {
+    __syncthreads();
+    for (int e = tid; e < LR_K * LR_T; e += 256) {
+      const int kk = e / LR_T, row = e % LR_T;
+      const long m = m0 + row, n = n0 + row;
+      Ls[e] = (k0 + kk < k && m < M) ? L[m + M * (long)(k0 + kk)] : 0.f;
+      Rs[e] = (k0 + kk < k && n < N) ? R[n + N * (long)(k0 + kk)] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < LR_K / 2; ++kk) {
+      float af[2], bf[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) af[a] = Rs[(2 * kk + (lane >> 5)) * LR_T + 64 * wr + 32 * a + (lane & 31)];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bf[b] = Ls[(2 * kk + (lane >> 5)) * LR_T + 64 * wc + 32 * b + (lane & 31)];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+  }
+  // C^T tile -> LDS: Cs[n_local][m_local]
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int nl = 64 * wr + 32 * a + acc32_row(e, lane);
+        const int ml = 64 * wc + 32 * b + (lane & 31);
+        Cs[nl * (LR_T + 4) + ml] = acc[a][b][e];
+      }
+  __syncthreads();
+  double se = 0.0, sx = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int col = (tid >> 5) + 8 * i;
+    if (n0 + col >= N) continue;
+    const float4 c4 = *reinterpret_cast<const float4*>(&Cs[col * (LR_T + 4) + 4 * mq]);
+    const float xs[4] = {xv[i].x, xv[i].y, xv[i].z, xv[i].w};
+    const float cc[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (m0 + 4 * mq + j < M) {
+        const float d = xs[j] - cc[j];
+        se += (double)(d * d);
+        sx += (double)(xs[j] * xs[j]);
+      }
+    }
+  }
+  se = wave_sum(se);
+  sx = wave_sum(sx);
+  if (lane == 0) { red[w] = se; red[4 + w] = sx; }
+  __syncthreads();
+  if (tid == 0) {
+    const long bid = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    acc_out[2 * bid] = red[0] + red[1] + red[2] + red[3];
+    acc_out[2 * bid + 1] = red[4] + red[5] + red[6] + red[7];
+  }
 }
 
 }  // namespace ht

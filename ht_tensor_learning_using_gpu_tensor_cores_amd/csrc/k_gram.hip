@@ -361,6 +361,7 @@ __global__ void gram_reduce_kernel(const float* __restrict__ part, int M, int nt
   if (bi == bj && r > c) return;
   const float* p = part + (long)pair * n_split * (GT_ROWS * GT_ROWS) + e;
   double s = 0.0;
+#pragma unroll 8
   for (int k = 0; k < n_split; ++k) s += (double)p[(long)k * (GT_ROWS * GT_ROWS)];
   G[i + (long)M * j] = s;
   G[j + (long)M * i] = s;

@@ -12,12 +12,10 @@
 //   Q       -- formed explicitly top-down: each node applies its reflectors to [E; 0]
 //              with E the b x b block its parent handed down, leaves write Q rows.
 // Thread layout inside a block QR: lane = column c, wave q = quarter of the rows.
-#include "ht_common.h"
+#include "k_hh.h"
 
 namespace ht {
 
-constexpr int TQ_ROWS = 256;
-constexpr int TQ_B = 64;
 constexpr int TQ_FAN = 4;
 
 struct TsqrLevel {
@@ -28,104 +26,6 @@ struct TsqrLevel {
   double* E;    // [n_nodes][b*b] block handed down by the parent (column-major)
   int* rows;    // [n_nodes] stacked rows of the node
 };
-
-// Householder QR of the rows x b matrix in LDS A[r*TQ_B + c]; on return the upper
-// triangle holds R, the strict lower part the reflector tails (unit diagonal implied).
-__device__ void block_householder(double* A, double* tau, double* red, int rows, int b) {
-  const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
-  const int rq = (rows + 3) / 4;
-  const int r_lo = q * rq, r_hi = min(rows, r_lo + rq);
-  __shared__ double sh_beta, sh_scale, sh_tau;
-  for (int j = 0; j < b && j < rows; ++j) {
-    // norm^2 of column j from row j down
-    if (c == j) {
-      double s = 0.0;
-      for (int r = max(r_lo, j + 1); r < r_hi; ++r) s += A[r * TQ_B + j] * A[r * TQ_B + j];
-      red[q * TQ_B + j] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      const double sub = red[0 * TQ_B + j] + red[1 * TQ_B + j] + red[2 * TQ_B + j] + red[3 * TQ_B + j];
-      const double alpha = A[j * TQ_B + j];
-      if (sub == 0.0) {
-        sh_tau = 0.0; sh_beta = alpha; sh_scale = 0.0;
-      } else {
-        const double nrm = sqrt(alpha * alpha + sub);
-        const double beta = alpha >= 0.0 ? -nrm : nrm;
-        sh_tau = (beta - alpha) / beta;
-        sh_scale = 1.0 / (alpha - beta);
-        sh_beta = beta;
-      }
-      tau[j] = sh_tau;
-    }
-    __syncthreads();
-    const double tj = sh_tau;
-    if (c == j) {
-      for (int r = max(r_lo, j + 1); r < r_hi; ++r) A[r * TQ_B + j] *= sh_scale;
-    }
-    __syncthreads();
-    if (tid == 0) A[j * TQ_B + j] = sh_beta;
-    // w_c = sum_{r>=j} v_r A[r][c] for c > j
-    if (c > j && c < b && tj != 0.0) {
-      double s = 0.0;
-      for (int r = max(r_lo, j); r < r_hi; ++r) {
-        const double vr = (r == j) ? 1.0 : A[r * TQ_B + j];
-        s += vr * A[r * TQ_B + c];
-      }
-      red[q * TQ_B + c] = s;
-    }
-    __syncthreads();
-    if (c > j && c < b && tj != 0.0) {
-      const double wc = tj * (red[0 * TQ_B + c] + red[1 * TQ_B + c] + red[2 * TQ_B + c] + red[3 * TQ_B + c]);
-      for (int r = max(r_lo, j); r < r_hi; ++r) {
-        const double vr = (r == j) ? 1.0 : A[r * TQ_B + j];
-        A[r * TQ_B + c] -= vr * wc;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Reflector accessors: tree nodes keep FP64 reflectors in global memory (row-major
-// rows x TQ_B), leaves keep them in place in W (FP32, column-major, ld m).
-struct VGlobal64 {
-  const double* V;
-  __device__ double operator()(int r, int j) const { return V[r * TQ_B + j]; }
-};
-struct VLeaf32 {
-  const float* W;
-  long r0, m;
-  __device__ double operator()(int r, int j) const { return (double)W[r0 + r + m * j]; }
-};
-
-// M (rows x b in LDS) <- H_0 ... H_{b-1} M with the reflectors given by V.
-template <class VA>
-__device__ void apply_q(VA V, const double* tau, double* M, double* red, int rows, int b) {
-  const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
-  const int rq = (rows + 3) / 4;
-  const int r_lo = q * rq, r_hi = min(rows, r_lo + rq);
-  for (int j = min(b, rows) - 1; j >= 0; --j) {
-    const double tj = tau[j];
-    if (tj == 0.0) continue;  // uniform across the block
-    if (c < b) {
-      double s = 0.0;
-      for (int r = max(r_lo, j); r < r_hi; ++r) {
-        const double vr = (r == j) ? 1.0 : V(r, j);
-        s += vr * M[r * TQ_B + c];
-      }
-      red[q * TQ_B + c] = s;
-    }
-    __syncthreads();
-    if (c < b) {
-      const double wc = tj * (red[0 * TQ_B + c] + red[1 * TQ_B + c] + red[2 * TQ_B + c] + red[3 * TQ_B + c]);
-      for (int r = max(r_lo, j); r < r_hi; ++r) {
-        const double vr = (r == j) ? 1.0 : V(r, j);
-        M[r * TQ_B + c] -= vr * wc;
-      }
-    }
-    __syncthreads();
-  }
-}
 
 // Leaf QR. With n_leaf = ceil(m/256) blocks, block i holds rows [i m / n_leaf,
 // (i+1) m / n_leaf): at most 256 and, when n_leaf > 1, at least 128 >= b rows, so every
