@@ -258,7 +258,7 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     double* wa = alloc<double>((size_t)batch * n);
     int* info = alloc<int>(batch);
     if (!V || !wa || !info) return HT_ERR_ALLOC;
-    CK(jacobi_full_launch(st, G, (long)n * n, n, wa, n, V, (long)n * n, batch, info, 60, 1e-15));
+    CK(jacobi_full_launch(st, G, (long)n * n, n, wa, n, V, (long)n * n, batch, info, 30, 1e-14));
     HT_HIP_CHECK(hipMemcpy2DAsync(U, sizeof(double) * n * k, V, sizeof(double) * n * n, sizeof(double) * n * k, batch,
                                   hipMemcpyDeviceToDevice, st));
     HT_HIP_CHECK(hipMemcpy2DAsync(w, sizeof(double) * k, wa, sizeof(double) * n, sizeof(double) * k, batch,
@@ -848,6 +848,48 @@ int Engine::run_ltr(const float* x) {
   return ltr_interior(C, cd, xsq, x);
 }
 
+// Top-k left singular vectors of the FP64 matricisation A (M x cols) of the view v when
+// the column side is the small one (cols < M, cols <= 64, P == 1 or S == 1): the exact
+// eigen decomposition of A^T A (Jacobi), then U = A V diag(1/sigma) with canonical signs
+// and w = sigma^2. *done = 0 (nothing written) when the view does not qualify or
+// sigma_k / sigma_1 <= 1e-12, where the division would lose the basis.
+int Engine::small_side_svd(const double* C, PMS v, int k, double* U, double* w, int* done) {
+  *done = 0;
+  const long M = v.M, cols = v.P * v.S;
+  if (!(cols < M && cols <= 64 && k <= cols && (v.P == 1 || v.S == 1))) return 0;
+  ProfScope ps(prof, st, PK_EIG);
+  const int c = (int)cols;
+  // A^T A through the transposed view: P == 1 -> element (s, m) at m + M s;
+  // S == 1 -> element (p, m) at p + P m.
+  const PMS vs = v.P == 1 ? PMS{M, v.S, 1} : PMS{1, v.P, M};
+  double* H = alloc<double>((size_t)c * c);
+  double* Vh = alloc<double>((size_t)c * c);
+  double* wh = alloc<double>(c);
+  double* sv = alloc<double>(c);
+  int* info = alloc<int>(1);
+  double* AV = alloc<double>((size_t)M * k);
+  if (!H || !Vh || !wh || !sv || !info || !AV) return HT_ERR_ALLOC;
+  CK(gram_pms_f64_launch(st, C, vs, H));
+  CK(jacobi_full_launch(st, H, (long)c * c, c, wh, c, Vh, (long)c * c, 1, info, 30, 1e-14));
+  std::vector<double> hw(c);
+  CK(d2h(hw.data(), wh, sizeof(double) * c));
+  if (!(hw[0] > 0.0) || !(hw[k - 1] > 1e-24 * hw[0])) return 0;
+  // A V: contraction over the small side of the transposed view.
+  if (v.P == 1) {
+    CK(proj_pms_f64_launch(st, C, vs, Vh, k, U));  // (M, k) column-major
+  } else {
+    CK(proj_pms_f64_launch(st, C, vs, Vh, k, AV));  // (k, M)
+    CK(transpose_launch(st, AV, k, M, U));
+  }
+  CK(sqrt_clamp_launch(st, wh, k, sv));
+  CK(scale_cols_inv_launch(st, U, (int)M, k, sv));
+  CK(canon_signs_launch(st, U, (int)M, k, (int)M));
+  HT_HIP_CHECK(hipMemcpyAsync(w, wh, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
+  ++eig_calls;
+  *done = 1;
+  return 0;
+}
+
 // Interior nodes on the leaf core (FP64), deepest first; then the root.
 int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const float* x) {
   const Problem& p = *prob;
@@ -874,11 +916,17 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
     const int k = p.rank[t];
     const long M = v.M;
     if (M > 4096) return HT_ERR_UNSUPPORTED;
-    double* G = alloc<double>((size_t)M * M);
-    CK(gram_pms_f64_launch(st, C, v, G));
     double* V = alloc<double>((size_t)M * k);
     double* w = alloc<double>(k);
-    CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-7, 24, 64, 4000 + t * 16));
+    if (!V || !w) return HT_ERR_ALLOC;
+    int small = 0;
+    CK(small_side_svd(C, v, k, V, w, &small));
+    if (!small) {
+      double* G = alloc<double>((size_t)M * M);
+      if (!G) return HT_ERR_ALLOC;
+      CK(gram_pms_f64_launch(st, C, v, G));
+      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-5, 48, 64, 4000 + t * 16));
+    }
     CK(eig_to_sv(w, k, SVd[t]));
     Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself
     double* C2 = alloc<double>((size_t)v.P * k * v.S);

@@ -95,6 +95,7 @@ struct Engine {
   int run_plain(const float* x);
   int run_ltr(const float* x);
   int ltr_interior(double* C, int64_t* cd, const double* xsq, const float* x);
+  int small_side_svd(const double* C, PMS v, int k, double* U, double* w, int* done);
   int run_rsvd(const float* x);
   int write_outputs(float* out);
   int run(const float* x, float* out);

@@ -12,7 +12,8 @@ constexpr int TQ_B = 64;
 // triangle holds R, the strict lower part the reflector tails (unit diagonal implied).
 __device__ void block_householder(double* A, double* tau, double* red, int rows, int b, int ld = TQ_B) {
   const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
-  const int rq = (rows + 3) / 4;
+  const int ng = blockDim.x >> 6;  // row groups (red holds ng x TQ_B partials)
+  const int rq = (rows + ng - 1) / ng;
   const int r_lo = q * rq, r_hi = min(rows, r_lo + rq);
   __shared__ double sh_beta, sh_scale, sh_tau;
   for (int j = 0; j < b && j < rows; ++j) {
@@ -24,7 +25,8 @@ __device__ void block_householder(double* A, double* tau, double* red, int rows,
     }
     __syncthreads();
     if (tid == 0) {
-      const double sub = red[0 * TQ_B + j] + red[1 * TQ_B + j] + red[2 * TQ_B + j] + red[3 * TQ_B + j];
+      double sub = 0.0;
+      for (int g = 0; g < ng; ++g) sub += red[g * TQ_B + j];
       const double alpha = A[j * ld + j];
       if (sub == 0.0) {
         sh_tau = 0.0; sh_beta = alpha; sh_scale = 0.0;
@@ -55,7 +57,9 @@ __device__ void block_householder(double* A, double* tau, double* red, int rows,
     }
     __syncthreads();
     if (c > j && c < b && tj != 0.0) {
-      const double wc = tj * (red[0 * TQ_B + c] + red[1 * TQ_B + c] + red[2 * TQ_B + c] + red[3 * TQ_B + c]);
+      double wsum = 0.0;
+      for (int g = 0; g < ng; ++g) wsum += red[g * TQ_B + c];
+      const double wc = tj * wsum;
       for (int r = max(r_lo, j); r < r_hi; ++r) {
         const double vr = (r == j) ? 1.0 : A[r * ld + j];
         A[r * ld + c] -= vr * wc;
@@ -86,7 +90,8 @@ struct VLeaf32 {
 template <class VA>
 __device__ void apply_q(VA V, const double* tau, double* M, double* red, int rows, int b, int ld = TQ_B) {
   const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
-  const int rq = (rows + 3) / 4;
+  const int ng = blockDim.x >> 6;
+  const int rq = (rows + ng - 1) / ng;
   const int r_lo = q * rq, r_hi = min(rows, r_lo + rq);
   for (int j = min(b, rows) - 1; j >= 0; --j) {
     const double tj = tau[j];
@@ -101,7 +106,9 @@ __device__ void apply_q(VA V, const double* tau, double* M, double* red, int row
     }
     __syncthreads();
     if (c < b) {
-      const double wc = tj * (red[0 * TQ_B + c] + red[1 * TQ_B + c] + red[2 * TQ_B + c] + red[3 * TQ_B + c]);
+      double wsum = 0.0;
+      for (int g = 0; g < ng; ++g) wsum += red[g * TQ_B + c];
+      const double wc = tj * wsum;
       for (int r = max(r_lo, j); r < r_hi; ++r) {
         const double vr = (r == j) ? 1.0 : V(r, j);
         M[r * ld + c] -= vr * wc;

@@ -175,88 +175,99 @@ __global__ __launch_bounds__(256) void jacobi_full_kernel(const double* __restri
   if (tid == 0 && info) info[bidx] = conv ? 0 : 1;
 }
 
-// Y = G^q W for a batch of problems; column block of 8 per work-group (n <= 256).
-__global__ __launch_bounds__(256) void power_kernel(const double* __restrict__ G, long sG, const double* __restrict__ W,
-                                                    long sW, double* __restrict__ Y, long sY, int n, int b, int q,
-                                                    int normalise) {
-  __shared__ double cur[256 * 8];
-  __shared__ double red[4][8];
-  __shared__ double scl[8];
+// Y = G^q W for a batch of problems (n <= 256): two columns per work-group of 1024
+// threads; thread (row r, part p) sums the p-th quarter of the row's dot products, the
+// quarters are added through LDS, and each power is followed by a column normalisation.
+constexpr int PW_COLS = 2;
+__global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ G, long sG, const double* __restrict__ W,
+                                                     long sW, double* __restrict__ Y, long sY, int n, int b, int q,
+                                                     int normalise) {
+  __shared__ double cur[256 * PW_COLS];
+  __shared__ double part[4][256 * PW_COLS];
+  __shared__ double red[16][PW_COLS];
+  __shared__ double scl[PW_COLS];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int c0 = blockIdx.x * 8, bidx = blockIdx.y;
-  const int nc = min(8, b - c0);
+  const int r = tid & 255, p = tid >> 8;
+  const int c0 = blockIdx.x * PW_COLS, bidx = blockIdx.y;
+  const int nc = min(PW_COLS, b - c0);
   G += bidx * sG;
   W += bidx * sW;
   Y += bidx * sY;
-  for (int e = tid; e < 256 * 8; e += 256) {
-    const int r = e / 8, c = e % 8;
-    cur[e] = (r < n && c < nc) ? W[r + (long)n * (c0 + c)] : 0.0;
+  for (int e = tid; e < 256 * PW_COLS; e += 1024) {
+    const int rr = e / PW_COLS, c = e % PW_COLS;
+    cur[e] = (rr < n && c < nc) ? W[rr + (long)n * (c0 + c)] : 0.0;
   }
   __syncthreads();
-  double acc[8];
+  const int l0 = (p * n) / 4, l1 = ((p + 1) * n) / 4;
   for (int it = 0; it < q; ++it) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) acc[c] = 0.0;
-    if (tid < n) {
-#pragma unroll 8
-      for (int l = 0; l < n; ++l) {
-        const double g = G[tid + (long)n * l];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) acc[c] += g * cur[l * 8 + c];
+    double a0 = 0.0, a1 = 0.0;
+    if (r < n) {
+      const double* gp = G + r;
+#pragma unroll 16
+      for (int l = l0; l < l1; ++l) {
+        const double g = gp[(long)n * l];
+        a0 += g * cur[l * PW_COLS + 0];
+        a1 += g * cur[l * PW_COLS + 1];
       }
     }
+    part[p][r * PW_COLS + 0] = a0;
+    part[p][r * PW_COLS + 1] = a1;
     __syncthreads();
+    double y = 0.0;
+    if (tid < 256 * PW_COLS) {
+      y = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    }
     if (normalise) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const double s = wave_sum(acc[c] * acc[c]);
-        if (lane == 0) red[wv][c] = s;
-      }
+      // tid < 512: row tid / 2, column tid % 2; reduce squares per column
+      const double y2 = (tid < 256 * PW_COLS) ? y * y : 0.0;
+      double s0 = (tid % 2 == 0) ? y2 : 0.0, s1 = (tid % 2 == 1) ? y2 : 0.0;
+      s0 = wave_sum(s0);
+      s1 = wave_sum(s1);
+      if (lane == 0) { red[wv][0] = s0; red[wv][1] = s1; }
       __syncthreads();
-      if (tid < 8) {
-        const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+      if (tid < PW_COLS) {
+        double s = 0.0;
+        for (int g = 0; g < 16; ++g) s += red[g][tid];
         scl[tid] = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
       }
       __syncthreads();
+      if (tid < 256 * PW_COLS) y *= scl[tid % PW_COLS];
     }
-    if (tid < n) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) cur[tid * 8 + c] = normalise ? acc[c] * scl[c] : acc[c];
-    }
+    if (tid < 256 * PW_COLS) cur[tid] = y;
     __syncthreads();
   }
-  for (int e = tid; e < n * nc; e += 256) {
-    const int r = e % n, c = e / n;
-    Y[r + (long)n * (c0 + c)] = cur[r * 8 + c];
+  for (int e = tid; e < n * nc; e += 1024) {
+    const int rr = e % n, c = e / n;
+    Y[rr + (long)n * (c0 + c)] = cur[rr * PW_COLS + c];
   }
 }
 
 // Q = orth(Y) (n x b, b <= n, n * b <= HH_MAX) by Householder QR. The reflectors and the
 // explicitly formed Q both stay in LDS (row stride b).
 constexpr int HH_MAX = 9400;
-__global__ __launch_bounds__(256) void hhqr_kernel(const double* __restrict__ Y, long sY, double* __restrict__ Q,
-                                                   long sQ, int n, int b) {
+__global__ __launch_bounds__(1024) void hhqr_kernel(const double* __restrict__ Y, long sY, double* __restrict__ Q,
+                                                    long sQ, int n, int b) {
   __shared__ double smem[2 * HH_MAX];
-  __shared__ double red[4 * TQ_B];
+  __shared__ double red[16 * TQ_B];
   __shared__ double tau[TQ_B];
   const int tid = threadIdx.x, bidx = blockIdx.x;
   Y += bidx * sY;
   Q += bidx * sQ;
   double* A = smem;
   double* M = smem + HH_MAX;
-  for (int e = tid; e < n * b; e += 256) {
+  for (int e = tid; e < n * b; e += 1024) {
     const int r = e % n, c = e / n;
     A[r * b + c] = Y[r + (long)n * c];
   }
   __syncthreads();
   block_householder(A, tau, red, n, b, b);
-  for (int e = tid; e < n * b; e += 256) {
+  for (int e = tid; e < n * b; e += 1024) {
     const int r = e / b, c = e % b;
     M[e] = (r == c) ? 1.0 : 0.0;
   }
   __syncthreads();
   apply_q(VLds{A, b}, tau, M, red, n, b, b);
-  for (int e = tid; e < n * b; e += 256) {
+  for (int e = tid; e < n * b; e += 1024) {
     const int r = e % n, c = e / n;
     Q[r + (long)n * c] = M[r * b + c];
   }
@@ -345,7 +356,7 @@ __global__ __launch_bounds__(256) void rr_kernel(const double* __restrict__ Q, c
     a[(np - 1) * SJLD + (np - 1)] = -(m * (double)b + 1.0) * 4.0;
   }
   __syncthreads();
-  jacobi_sweeps(a, v, b, np, max_sweeps, 1e-15);
+  jacobi_sweeps(a, v, b, np, max_sweeps, 1e-14);
   jacobi_order(a, np, order);
   if (tid < SJ) res[tid] = 0.0;
   __syncthreads();
@@ -419,14 +430,15 @@ int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* 
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, long sW, double* Y, long sY, int n, int b,
                  int q, int normalise, int batch) {
   if (n < 1 || n > 256 || b < 1 || b > 64 || batch < 1) return 8;
-  hipLaunchKernelGGL(power_kernel, dim3((b + 7) / 8, batch), dim3(256), 0, st, G, sG, W, sW, Y, sY, n, b, q, normalise);
+  hipLaunchKernelGGL(power_kernel, dim3((b + PW_COLS - 1) / PW_COLS, batch), dim3(1024), 0, st, G, sG, W, sW, Y, sY, n, b,
+                     q, normalise);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
 int hhqr_launch(hipStream_t st, const double* Y, long sY, double* Vscratch, double* Q, long sQ, int n, int b, int batch) {
   (void)Vscratch;
   if (n < b || (long)n * b > HH_MAX || b < 1 || b > TQ_B || batch < 1) return 8;
-  hipLaunchKernelGGL(hhqr_kernel, dim3(batch), dim3(256), 0, st, Y, sY, Q, sQ, n, b);
+  hipLaunchKernelGGL(hhqr_kernel, dim3(batch), dim3(1024), 0, st, Y, sY, Q, sQ, n, b);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
@@ -434,7 +446,7 @@ int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, int n, 
               long sU, double* w, long sw, double* wall, double tol, int* converged, int batch) {
   if (n < 1 || b < 1 || b > SJ || k < 1 || k > b || batch < 1) return 8;
   hipLaunchKernelGGL(rr_kernel, dim3(batch), dim3(256), 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, tol, converged,
-                     60);
+                     20);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
