@@ -247,9 +247,11 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
 // Top-k eigenpairs (descending, canonical signs) of `batch` symmetric PSD n x n FP64
 // matrices G (stride n*n), written to U (stride n*k) and w (stride k).
 // n <= 64: one Jacobi solve each. n <= 256: rounds of block subspace iteration (block b,
-// q powers, Householder re-orthonormalisation, Rayleigh-Ritz with a b x b Jacobi solve)
-// until every top-k Ritz residual is below tol * theta_1; q doubles between rounds and
-// each round restarts from the previous Ritz basis.
+// Householder re-orthonormalisation, Rayleigh-Ritz with a b x b Jacobi solve) until
+// every top-k Ritz residual is below tol * theta_1. Round 0 applies q0 normalised powers
+// to a Gaussian block; later rounds restart from the previous Ritz basis and apply a
+// Chebyshev filter (degree 8, 16, ...) that damps [0, smallest Ritz value]. A missed
+// round also widens a block narrower than 2k by 16 columns (up to min(64, n)).
 int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
                            uint32_t stream) {
   ProfScope ps(prof, st, PK_EIG);
@@ -267,36 +269,29 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     return 0;
   }
   if (n > 256) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream);
-  // Block width: k + 8 by default; the Householder step keeps n x b reflectors and Q in
-  // LDS, so n * b <= 9400 (b <= 36 at n = 256). A round that does not converge widens
-  // the block to that limit.
-  const int bmax = std::min(std::min(64, n), 9400 / n);
+  const int bmax = std::min(64, n);
   if (b <= 0) b = k + 8;
-  b = std::min(b, bmax);
-  if (k > b) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream);
-  const int b0 = b;
-  b = bmax;  // buffers sized for the widest block
-  long nb = (long)n * b;
-  double* W = alloc<double>((size_t)batch * nb);
-  double* Y = alloc<double>((size_t)batch * nb);
-  double* Q = alloc<double>((size_t)batch * nb);
-  double* Z = alloc<double>((size_t)batch * nb);
-  double* Uf = alloc<double>((size_t)batch * nb);
-  double* Vs = alloc<double>((size_t)batch * 256 * 64);
+  b = std::min(std::max(b, k), bmax);
+  const size_t nbmax = (size_t)n * bmax;
+  double* W = alloc<double>(batch * nbmax);
+  double* Y = alloc<double>(batch * nbmax);
+  double* Q = alloc<double>(batch * nbmax);
+  double* Z = alloc<double>(batch * nbmax);
+  double* Uf = alloc<double>(batch * nbmax);
   double* wall = alloc<double>((size_t)batch * 64);
+  double* cut = alloc<double>(batch);
   int* conv = alloc<int>(batch);
-  if (!W || !Y || !Q || !Z || !Uf || !Vs || !wall || !conv) return HT_ERR_ALLOC;
+  if (!W || !Y || !Q || !Z || !Uf || !wall || !cut || !conv) return HT_ERR_ALLOC;
+  long nb = (long)n * b;
   CK(fill_normals_f64_launch(st, W, batch * nb, opt.seed, stream));
   std::vector<int> hconv(batch);
-  int q = std::max(1, q0), iters = 0;
-  b = b0;
-  nb = (long)n * b;
+  int q = std::max(1, q0), mode = 0, iters = 0;
   for (int round = 0;; ++round) {
-    CK(power_launch(st, G, (long)n * n, W, nb, Y, nb, n, b, q, 1, batch));
-    CK(hhqr_launch(st, Y, nb, Vs, Q, nb, n, b, batch));
-    CK(power_launch(st, G, (long)n * n, Q, nb, Z, nb, n, b, 1, 0, batch));
-    CK(rr_launch(st, Q, Z, nb, n, b, k, Uf, U, (long)n * k, w, k, wall, tol, conv, batch));
-    iters += q;
+    CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, batch));
+    CK(qr_launch(st, Y, Q, nb, n, b, batch));
+    CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, batch));
+    CK(rr_launch(st, Q, Z, nb, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, conv, batch));
+    iters += q + 1;
     CK(d2h(hconv.data(), conv, sizeof(int) * batch));
     bool all = true;
     for (int i = 0; i < batch; ++i) all = all && hconv[i];
@@ -305,20 +300,20 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
       noconv = 1;
       break;
     }
-    // next round: restart from the Ritz basis; widen the block after the first miss
-    if (b < bmax) {
-      const int bn = bmax;
+    if (b < std::min(bmax, 2 * k)) {
+      // Ritz vectors first, fresh Gaussian columns after them (row-major blocks)
+      const int bn = std::min(bmax, b + 16);
       const long nbn = (long)n * bn;
-      // Ritz vectors first, fresh Gaussian columns after them (per problem)
       CK(fill_normals_f64_launch(st, W, batch * nbn, opt.seed, stream + 100 + round));
-      HT_HIP_CHECK(hipMemcpy2DAsync(W, sizeof(double) * nbn, Uf, sizeof(double) * nb, sizeof(double) * nb, batch,
-                                    hipMemcpyDeviceToDevice, st));
+      HT_HIP_CHECK(hipMemcpy2DAsync(W, sizeof(double) * bn, Uf, sizeof(double) * b, sizeof(double) * b,
+                                    (size_t)n * batch, hipMemcpyDeviceToDevice, st));
       b = bn;
       nb = nbn;
     } else {
       std::swap(W, Uf);
     }
-    q = std::min(2 * q, 64);
+    q = round == 0 ? 8 : std::min(2 * q, 64);
+    mode = 2;
   }
   eig_iters_max = std::max(eig_iters_max, iters);
   eig_calls += batch;
@@ -925,7 +920,7 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
       double* G = alloc<double>((size_t)M * M);
       if (!G) return HT_ERR_ALLOC;
       CK(gram_pms_f64_launch(st, C, v, G));
-      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-5, 48, 64, 4000 + t * 16));
+      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-5, 4, 32, 4000 + t * 16));
     }
     CK(eig_to_sv(w, k, SVd[t]));
     Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself

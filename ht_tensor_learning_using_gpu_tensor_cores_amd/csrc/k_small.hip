@@ -4,36 +4,163 @@
 // leaf core's matricisations (k_l k_r x k_l k_r). Their top-k eigenpairs replace the
 // reference's cusolverDnSsyevd / Sgesvd calls (third_order_tensor_decompsition/CUDA/
 // Optimized/kernel.cu:682-718; fourth_order_tensor_decomposition/single_GPU/CUDA/
-// Optimized/kernel.cu:267-287, :457-481). One round of the solver is
-//   power_kernel  Y = G^q W          column blocks of 8 in parallel, each power followed
-//                                    by a column normalisation (the span is unchanged)
-//   hhqr_kernel   Q = orth(Y)        Householder QR in FP64 (stable for any conditioning,
-//                                    rank-deficient blocks included)
-//   power_kernel  Z = G Q            (q = 1, no normalisation)
-//   rr_kernel     T = Q^T Z = S Th S^T by parallel Jacobi; U = Q S, the Ritz residuals
-//                 ||G u_i - th_i u_i|| and a converged flag per problem
+// Optimized/kernel.cu:267-287, :457-481). One round of the block solver is
+//   power_kernel  Y = f(G) W         f = G^q (normalised powers) or, from the second round
+//                                    on, the degree-q Chebyshev filter that damps [0, cut]
+//                                    (cut = smallest Ritz value of the previous round)
+//   qr_kernel     Q = orth(Y)        Householder QR in FP64, rows in registers
+//   power_kernel  Z = G Q
+//   rr_kernel     T = sym(Q^T Z) = S Th S^T by parallel Jacobi; U = Q S, the Ritz
+//                 residuals ||G u_i - th_i u_i||, canonical signs and a converged flag
 // and the host only reads the flags between rounds.
-#include "k_hh.h"
+//
+// Block layout: the n x b blocks W, Y, Q, Z and Uf are row-major (ld = b), one problem
+// after another. The row-distributed kernels (qr, rr) run 16 waves; thread (wave g,
+// lane c) holds column c of rows r = i * 16 + g, i < 16, in registers.
+#include "ht_common.h"
 
 namespace ht {
 
-constexpr int SJ = 64;
-constexpr int SJLD = SJ + 1;
+constexpr int SJ = 64;        // largest Jacobi problem
+constexpr int SJLD = SJ + 1;  // LDS row stride of the Jacobi matrices
+constexpr int SW = 16;        // waves of the row-distributed kernels
+constexpr int SR = 16;        // rows per thread there: n <= SW * SR
+constexpr int SNT = SW * 64;  // threads of the row-distributed kernels
+
+__device__ __forceinline__ double lane_bcast(double x, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Round-robin pairing for np (even) indices: step s pairs (np-1, s) and
+// ((s+t) mod (np-1), (s-t) mod (np-1)), t = 1 .. np/2-1; every pair meets once per sweep.
+// Entry (s, t) = p | q << 8 with p < q.
+__device__ __forceinline__ void jacobi_schedule(unsigned short* sched, int np) {
+  const int half = np / 2;
+  for (int e = threadIdx.x; e < (np - 1) * half; e += blockDim.x) {
+    const int s = e / half, t = e % half;
+    int p, q;
+    if (t == 0) {
+      p = np - 1;
+      q = s;
+    } else {
+      p = (s + t) % (np - 1);
+      q = (s - t + (np - 1)) % (np - 1);
+    }
+    if (p > q) {
+      const int x = p;
+      p = q;
+      q = x;
+    }
+    sched[e] = (unsigned short)(p | (q << 8));
+  }
+}
+
+// 1/x and 1/sqrt(x) in FP64 from the hardware approximations (v_rcp_f64, v_rsq_f64) and
+// two Newton steps each (quadratic convergence from ~2^-22 leaves ~1 ulp).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  r = fma(0.5 * r, fma(-x * r, r, 1.0), r);
+  r = fma(0.5 * r, fma(-x * r, r, 1.0), r);
+  return r;
+}
+
+// Jacobi rotation (c, s) that annihilates a_pq (Rutishauser's form: t = tan(phi) of the
+// smaller angle, c = 1/sqrt(1 + t^2), s = t c; c^2 + s^2 = 1 to rounding).
+__device__ __forceinline__ void jacobi_rotation(double app, double aqq, double apq, double& c, double& s) {
+  c = 1.0;
+  s = 0.0;
+  if (apq == 0.0) return;
+  const double tau = (aqq - app) * rcp_nr(2.0 * apq);
+  const double at = fabs(tau);
+  double t;
+  if (at > 1e150) {
+    t = 0.5 * rcp_nr(at);
+  } else {
+    const double h = 1.0 + tau * tau;
+    t = rcp_nr(at + h * rsq_nr(h));
+  }
+  if (tau < 0.0) t = -t;
+  c = rsq_nr(1.0 + t * t);
+  s = t * c;
+}
 
 // Cyclic parallel Jacobi on the np x np symmetric matrix a (LDS, ld SJLD), accumulating
-// the rotations into v. Round-robin pair ordering; each step rotates np/2 disjoint pairs
-// and updates every 2x2 block (rows of pair i, columns of pair j) as R_i^T A_ij R_j in a
-// single phase. Returns 1 when the off-diagonal mass fell below tol^2 * diagonal mass.
-__device__ int jacobi_sweeps(double* a, double* v, int n, int np, int max_sweeps, double tol) {
-  __shared__ double cs[SJ / 2], sn[SJ / 2];
-  __shared__ int pp[SJ / 2], qq[SJ / 2];
-  __shared__ double red[16];
+// the rotations into v, for a block of SNT = 1024 threads. Each step rotates np/2
+// disjoint pairs (phase A, one thread per pair) and then updates every 2x2 block (rows of
+// pair i, columns of pair j) as R_i^T A_ij R_j and the two columns of v per pair (phase
+// B, over one combined item space). Returns 1 when the off-diagonal mass fell below
+// tol^2 * diagonal mass. The schedule must be in sched (jacobi_schedule + barrier).
+__device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np, int max_sweeps, double tol,
+                                             const unsigned short* sched) {
+  __shared__ double2 rcs[SJ / 2];
+  __shared__ int rpq[SJ / 2];
+  __shared__ double red[2][SW];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int nt = blockDim.x;
-  const int half = np / 2;
+  const int half = np / 2, nblk = half * half, nitem = nblk + np * half;
+  // items of this thread (e = tid, tid + SNT, tid + 2 SNT): kind 0 = 2x2 block
+  // (x0, x1) = (bi, bj); kind 1 = row x0 of v for pair x1; kind -1 = none
+  auto item = [&](int e, int& kind, int& x0, int& x1) {
+    if (e < nblk) {
+      kind = 0;
+      x0 = e % half;
+      x1 = e / half;
+    } else if (e < nitem) {
+      kind = 1;
+      x0 = (e - nblk) % np;
+      x1 = (e - nblk) / np;
+    } else {
+      kind = -1;
+      x0 = 0;
+      x1 = 0;
+    }
+  };
+  int k0, a0, b0, k1, a1, b1, k2, a2, b2;
+  item(tid, k0, a0, b0);
+  item(tid + SNT, k1, a1, b1);
+  item(tid + 2 * SNT, k2, a2, b2);
+  auto apply = [&](int kind, int x0, int x1) {
+    if (kind == 0) {
+      const int bi = x0, bj = x1;
+      const int pq = rpq[bi], PQ = rpq[bj];
+      const int p = pq & 255, q = pq >> 8, P = PQ & 255, Q = PQ >> 8;
+      const double2 cs = rcs[bi], CS = rcs[bj];
+      const double c = cs.x, s = cs.y, C = CS.x, S = CS.y;
+      const double apP = a[p * SJLD + P], apQ = a[p * SJLD + Q];
+      const double aqP = a[q * SJLD + P], aqQ = a[q * SJLD + Q];
+      // right: columns (P, Q); left: rows (p, q)
+      const double xpP = C * apP - S * apQ, xpQ = S * apP + C * apQ;
+      const double xqP = C * aqP - S * aqQ, xqQ = S * aqP + C * aqQ;
+      double ypP = c * xpP - s * xqP, yqP = s * xpP + c * xqP;
+      double ypQ = c * xpQ - s * xqQ, yqQ = s * xpQ + c * xqQ;
+      if (bi == bj && s != 0.0) {
+        ypQ = 0.0;
+        yqP = 0.0;
+      }
+      a[p * SJLD + P] = ypP;
+      a[p * SJLD + Q] = ypQ;
+      a[q * SJLD + P] = yqP;
+      a[q * SJLD + Q] = yqQ;
+    } else if (kind == 1) {
+      const int r = x0, j = x1;
+      const int PQ = rpq[j];
+      const int P = PQ & 255, Q = PQ >> 8;
+      const double2 CS = rcs[j];
+      const double vP = v[r * SJLD + P], vQ = v[r * SJLD + Q];
+      v[r * SJLD + P] = CS.x * vP - CS.y * vQ;
+      v[r * SJLD + Q] = CS.y * vP + CS.x * vQ;
+    }
+  };
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     double off = 0.0, dg = 0.0;
-    for (int e = tid; e < n * n; e += nt) {
+    for (int e = tid; e < n * n; e += SNT) {
       const int i = e % n, j = e / n;
       const double x = a[i * SJLD + j];
       if (i == j) dg += x * x;
@@ -41,57 +168,33 @@ __device__ int jacobi_sweeps(double* a, double* v, int n, int np, int max_sweeps
     }
     off = wave_sum(off);
     dg = wave_sum(dg);
-    __syncthreads();
-    if (lane == 0) { red[wv] = off; red[8 + wv] = dg; }
+    if (lane == 0) {
+      red[0][wv] = off;
+      red[1][wv] = dg;
+    }
     __syncthreads();
     off = 0.0;
     dg = 0.0;
-    for (int w = 0; w < nt / 64; ++w) { off += red[w]; dg += red[8 + w]; }
+#pragma unroll
+    for (int w = 0; w < SW; ++w) {
+      off += red[0][w];
+      dg += red[1][w];
+    }
     __syncthreads();
     if (off <= tol * tol * dg || off == 0.0) return 1;
     for (int step = 0; step < np - 1; ++step) {
       if (tid < half) {
-        int p, q;
-        if (tid == 0) { p = np - 1; q = step; }
-        else {
-          p = (step + tid) % (np - 1);
-          q = (step - tid + (np - 1)) % (np - 1);
-        }
-        if (p > q) { const int t = p; p = q; q = t; }
-        const double app = a[p * SJLD + p], aqq = a[q * SJLD + q], apq = a[p * SJLD + q];
-        double c = 1.0, s = 0.0;
-        if (apq != 0.0 && fabs(apq) > 1e-300) {
-          const double tau = (aqq - app) / (2.0 * apq);
-          const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-          c = 1.0 / sqrt(1.0 + t * t);
-          s = t * c;
-        }
-        pp[tid] = p; qq[tid] = q; cs[tid] = c; sn[tid] = s;
+        const int pq = sched[step * half + tid];
+        const int p = pq & 255, q = pq >> 8;
+        double c, s;
+        jacobi_rotation(a[p * SJLD + p], a[q * SJLD + q], a[p * SJLD + q], c, s);
+        rpq[tid] = pq;
+        rcs[tid] = make_double2(c, s);
       }
       __syncthreads();
-      for (int e = tid; e < half * half; e += nt) {
-        const int bi = e % half, bj = e / half;
-        const int p = pp[bi], q = qq[bi], P = pp[bj], Q = qq[bj];
-        const double c = cs[bi], s = sn[bi], C = cs[bj], S = sn[bj];
-        const double apP = a[p * SJLD + P], apQ = a[p * SJLD + Q];
-        const double aqP = a[q * SJLD + P], aqQ = a[q * SJLD + Q];
-        // right: columns (P, Q); left: rows (p, q)
-        const double xpP = C * apP - S * apQ, xpQ = S * apP + C * apQ;
-        const double xqP = C * aqP - S * aqQ, xqQ = S * aqP + C * aqQ;
-        double ypP = c * xpP - s * xqP, yqP = s * xpP + c * xqP;
-        double ypQ = c * xpQ - s * xqQ, yqQ = s * xpQ + c * xqQ;
-        if (bi == bj && s != 0.0) { ypQ = 0.0; yqP = 0.0; }
-        a[p * SJLD + P] = ypP; a[p * SJLD + Q] = ypQ;
-        a[q * SJLD + P] = yqP; a[q * SJLD + Q] = yqQ;
-      }
-      for (int e = tid; e < np * half; e += nt) {
-        const int r = e % np, j = e / np;
-        const int P = pp[j], Q = qq[j];
-        const double C = cs[j], S = sn[j];
-        const double vP = v[r * SJLD + P], vQ = v[r * SJLD + Q];
-        v[r * SJLD + P] = C * vP - S * vQ;
-        v[r * SJLD + Q] = S * vP + C * vQ;
-      }
+      apply(k0, a0, b0);
+      apply(k1, a1, b1);
+      apply(k2, a2, b2);
       __syncthreads();
     }
   }
@@ -100,8 +203,8 @@ __device__ int jacobi_sweeps(double* a, double* v, int n, int np, int max_sweeps
 
 // Load an n x n matrix into a (padded to even np with a decoupled, large negative
 // diagonal entry that sorts last), set v = I.
-__device__ void jacobi_load(double* a, double* v, const double* A, int lda, int n, int np, bool symmetrise) {
-  __shared__ double red[4];
+__device__ __forceinline__ void jacobi_load(double* a, double* v, const double* A, int lda, int n, int np, bool symmetrise) {
+  __shared__ double red[SW];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double amax = 0.0;
   for (int e = tid; e < np * np; e += blockDim.x) {
@@ -126,36 +229,43 @@ __device__ void jacobi_load(double* a, double* v, const double* A, int lda, int 
   __syncthreads();
 }
 
-// Descending order of the diagonal of a (one thread).
-__device__ void jacobi_order(const double* a, int np, int* order) {
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < np; ++i) order[i] = i;
-    for (int i = 0; i < np; ++i) {
-      int best = i;
-      for (int j = i + 1; j < np; ++j)
-        if (a[order[j] * SJLD + order[j]] > a[order[best] * SJLD + order[best]]) best = j;
-      const int t = order[i]; order[i] = order[best]; order[best] = t;
+// Descending order of the diagonal of a: thread i places index i at its rank (ties by
+// index), so order is a permutation.
+__device__ __forceinline__ void jacobi_order(const double* a, int np, int* order) {
+  const int i = threadIdx.x;
+  if (i < np) {
+    const double di = a[i * SJLD + i];
+    int rank = 0;
+    for (int j = 0; j < np; ++j) {
+      const double dj = a[j * SJLD + j];
+      rank += (dj > di || (dj == di && j < i)) ? 1 : 0;
     }
+    order[rank] = i;
   }
   __syncthreads();
 }
 
-// Full eigen decomposition of n x n (n <= 64) symmetric matrices, batched.
-__global__ __launch_bounds__(256) void jacobi_full_kernel(const double* __restrict__ A, long sA, int n, double* __restrict__ w,
+// Full eigen decomposition of n x n (n <= 64) symmetric matrices, batched (one block of
+// 1024 threads each); columns of V in descending eigenvalue order with canonical signs
+// (largest-magnitude entry positive).
+__global__ __launch_bounds__(1024) void jacobi_full_kernel(const double* __restrict__ A, long sA, int n, double* __restrict__ w,
                                                           long sw, double* __restrict__ V, long sV, int* __restrict__ info,
                                                           int max_sweeps, double tol) {
-  __shared__ double a[SJ * SJLD];
+  __shared__ double a0[SJ * SJLD];
   __shared__ double v[SJ * SJLD];
+  __shared__ unsigned short sched[(SJ - 1) * (SJ / 2)];
   __shared__ int order[SJ];
   const int bidx = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   A += bidx * sA;
   w += bidx * sw;
   V += bidx * sV;
   const int np = (n + 1) & ~1;
+  jacobi_schedule(sched, np);
+  double* a = a0;
   jacobi_load(a, v, A, n, n, np, false);
-  const int conv = jacobi_sweeps(a, v, n, np, max_sweeps, tol);
+  const int conv = jacobi_sweeps(a, v, n, np, max_sweeps, tol, sched);
   jacobi_order(a, np, order);
-  for (int j = wv; j < n; j += 4) {
+  for (int j = wv; j < n; j += SW) {
     const int src = order[j];
     double best = -1.0;
     int bi = 0;
@@ -175,121 +285,352 @@ __global__ __launch_bounds__(256) void jacobi_full_kernel(const double* __restri
   if (tid == 0 && info) info[bidx] = conv ? 0 : 1;
 }
 
-// Y = G^q W for a batch of problems (n <= 256): two columns per work-group of 1024
-// threads; thread (row r, part p) sums the p-th quarter of the row's dot products, the
-// quarters are added through LDS, and each power is followed by a column normalisation.
-constexpr int PW_COLS = 2;
+// Y = f(G) W for a batch of problems (n <= 256, row-major blocks, ld b): PW_COLS columns
+// per work-group of 1024 threads; thread (row r, part p) sums the p-th quarter of the
+// row's dot products and the quarters are added through LDS.
+//   mode 0: q powers, each followed by a column normalisation (the span is unchanged);
+//   mode 1: one product, unnormalised;
+//   mode 2: the degree-q Chebyshev filter on [0, e], e = cut[problem]:
+//           T_0 = W, T_1 = L W, T_{j+1} = 2 L T_j - T_{j-1}, L = (2 G - e I) / e, with
+//           (T_j, T_{j-1}) rescaled together by the column norm of T_j at every step.
+//           Falls back to mode 0 when e <= 0.
+constexpr int PW_COLS = 4;
 __global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ G, long sG, const double* __restrict__ W,
-                                                     long sW, double* __restrict__ Y, long sY, int n, int b, int q,
-                                                     int normalise) {
+                                                     double* __restrict__ Y, long sWY, int n, int b, int q, int mode,
+                                                     const double* __restrict__ cut) {
   __shared__ double cur[256 * PW_COLS];
+  __shared__ double prv[256 * PW_COLS];
   __shared__ double part[4][256 * PW_COLS];
-  __shared__ double red[16][PW_COLS];
+  __shared__ double red[SW][PW_COLS];
   __shared__ double scl[PW_COLS];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int r = tid & 255, p = tid >> 8;
   const int c0 = blockIdx.x * PW_COLS, bidx = blockIdx.y;
   const int nc = min(PW_COLS, b - c0);
   G += bidx * sG;
-  W += bidx * sW;
-  Y += bidx * sY;
-  for (int e = tid; e < 256 * PW_COLS; e += 1024) {
-    const int rr = e / PW_COLS, c = e % PW_COLS;
-    cur[e] = (rr < n && c < nc) ? W[rr + (long)n * (c0 + c)] : 0.0;
+  W += bidx * sWY;
+  Y += bidx * sWY;
+  double e = 0.0;
+  if (mode == 2) {
+    e = cut[bidx];
+    if (!(e > 0.0)) mode = 0;
+  }
+  for (int x = tid; x < 256 * PW_COLS; x += 1024) {
+    const int rr = x / PW_COLS, c = x % PW_COLS;
+    cur[x] = (rr < n && c < nc) ? W[(long)rr * b + c0 + c] : 0.0;
+    prv[x] = 0.0;
   }
   __syncthreads();
   const int l0 = (p * n) / 4, l1 = ((p + 1) * n) / 4;
+  const bool own = tid < 256 * PW_COLS;  // tid = row * PW_COLS + column
   for (int it = 0; it < q; ++it) {
-    double a0 = 0.0, a1 = 0.0;
+    double acc[PW_COLS];
+#pragma unroll
+    for (int c = 0; c < PW_COLS; ++c) acc[c] = 0.0;
     if (r < n) {
       const double* gp = G + r;
-#pragma unroll 16
+#pragma unroll 8
       for (int l = l0; l < l1; ++l) {
         const double g = gp[(long)n * l];
-        a0 += g * cur[l * PW_COLS + 0];
-        a1 += g * cur[l * PW_COLS + 1];
+#pragma unroll
+        for (int c = 0; c < PW_COLS; ++c) acc[c] += g * cur[l * PW_COLS + c];
       }
     }
-    part[p][r * PW_COLS + 0] = a0;
-    part[p][r * PW_COLS + 1] = a1;
+#pragma unroll
+    for (int c = 0; c < PW_COLS; ++c) part[p][r * PW_COLS + c] = acc[c];
     __syncthreads();
-    double y = 0.0;
-    if (tid < 256 * PW_COLS) {
+    double y = 0.0, x = 0.0;
+    if (own) {
       y = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+      x = cur[tid];
+      if (mode == 2) {
+        const double ly = (2.0 * y - e * x) / e;
+        y = (it == 0) ? ly : 2.0 * ly - prv[tid];
+      }
     }
-    if (normalise) {
-      // tid < 512: row tid / 2, column tid % 2; reduce squares per column
-      const double y2 = (tid < 256 * PW_COLS) ? y * y : 0.0;
-      double s0 = (tid % 2 == 0) ? y2 : 0.0, s1 = (tid % 2 == 1) ? y2 : 0.0;
-      s0 = wave_sum(s0);
-      s1 = wave_sum(s1);
-      if (lane == 0) { red[wv][0] = s0; red[wv][1] = s1; }
+    if (mode != 1) {
+      double s[PW_COLS];
+#pragma unroll
+      for (int c = 0; c < PW_COLS; ++c) s[c] = wave_sum((own && (tid % PW_COLS) == c) ? y * y : 0.0);
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < PW_COLS; ++c) red[wv][c] = s[c];
+      }
       __syncthreads();
       if (tid < PW_COLS) {
-        double s = 0.0;
-        for (int g = 0; g < 16; ++g) s += red[g][tid];
-        scl[tid] = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+        double t = 0.0;
+        for (int w = 0; w < SW; ++w) t += red[w][tid];
+        scl[tid] = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
       }
       __syncthreads();
-      if (tid < 256 * PW_COLS) y *= scl[tid % PW_COLS];
+      if (own) {
+        const double f = scl[tid % PW_COLS];
+        y *= f;
+        x *= f;
+      }
     }
-    if (tid < 256 * PW_COLS) cur[tid] = y;
+    if (own) {
+      prv[tid] = x;
+      cur[tid] = y;
+    }
     __syncthreads();
   }
-  for (int e = tid; e < n * nc; e += 1024) {
-    const int rr = e % n, c = e / n;
-    Y[rr + (long)n * (c0 + c)] = cur[rr * PW_COLS + c];
+  for (int x = tid; x < n * nc; x += 1024) {
+    const int rr = x / nc, c = x % nc;
+    Y[(long)rr * b + c0 + c] = cur[rr * PW_COLS + c];
   }
 }
 
-// Q = orth(Y) (n x b, b <= n, n * b <= HH_MAX) by Householder QR. The reflectors and the
-// explicitly formed Q both stay in LDS (row stride b).
-constexpr int HH_MAX = 9400;
-__global__ __launch_bounds__(1024) void hhqr_kernel(const double* __restrict__ Y, long sY, double* __restrict__ Q,
-                                                    long sQ, int n, int b) {
-  __shared__ double smem[2 * HH_MAX];
-  __shared__ double red[16 * TQ_B];
-  __shared__ double tau[TQ_B];
-  const int tid = threadIdx.x, bidx = blockIdx.x;
-  Y += bidx * sY;
-  Q += bidx * sQ;
-  double* A = smem;
-  double* M = smem + HH_MAX;
-  for (int e = tid; e < n * b; e += 1024) {
-    const int r = e % n, c = e / n;
-    A[r * b + c] = Y[r + (long)n * c];
+// Lane packing of the row-distributed kernels: a wave holds BP columns (lane c < BP) of
+// RS = 64 / BP row sets (h = lane / BP); thread (wave g, c, h) holds rows
+// r = (i * SW + g) * RS + h, i < SR, so one block covers n <= SW * RS * SR rows.
+template <int BP>
+__device__ __forceinline__ double col_bcast(double x, int j, int h) {
+  double r = lane_bcast(x, j);
+#pragma unroll
+  for (int q = 1; q < 64 / BP; ++q) {
+    const double y = lane_bcast(x, j + q * BP);
+    r = (h == q) ? y : r;
   }
-  __syncthreads();
-  block_householder(A, tau, red, n, b, b);
-  for (int e = tid; e < n * b; e += 1024) {
-    const int r = e / b, c = e % b;
-    M[e] = (r == c) ? 1.0 : 0.0;
+  return r;
+}
+// x[L % 32] + x[32 + L % 32] in every lane L (v_permlane32_swap exchanges the upper half
+// of one register with the lower half of the other).
+__device__ __forceinline__ double halves_sum(double x) {
+  const int lo = __double2loint(x), hi = __double2hiint(x);
+  const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double(rh[0], rl[0]) + __hiloint2double(rh[1], rl[1]);
+}
+template <int BP>
+__device__ __forceinline__ double rowset_sum(double s) {
+#pragma unroll
+  for (int o = BP; o < 32; o <<= 1) s += __shfl_xor(s, o, 64);
+  if constexpr (BP <= 32) s = halves_sum(s);
+  return s;
+}
+
+// Q = orth(Y) (row-major n x b, b <= n, b <= BP, n <= SW * (64/BP) * SR), rows in
+// registers (thread (g, c, h) holds column c of its rows).
+//   First choice, CholeskyQR2: B = X^T X, B = R^T R (right-looking Cholesky in LDS),
+//   X <- X R^{-1} by forward substitution across the lanes of each row, twice. It is
+//   taken only when every first-pass pivot exceeds 1e-12 * max diag(B) (cond(Y) below
+//   ~1e6), where two passes give orthogonality to rounding.
+//   Otherwise Householder QR: the reflector of column j lives in the lanes of column j
+//   (rows > j, unit diagonal implied) and reaches the other lanes by readlane. Per column:
+//   thread 0 forms (tau, 1/(alpha - beta), beta) from the partial norms, every thread adds
+//   its rows' share of w = v^T A, and the owners of columns > j apply A -= v (tau w). Q is
+//   then formed explicitly as H_0 ... H_{b-1} I.
+template <int BP, int SR>
+__global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, double* __restrict__ Q, long sYQ, int n,
+                                                  int b) {
+  constexpr int RS = 64 / BP;
+  __shared__ double B[SJ * SJLD];
+  __shared__ double nrm[SW];
+  __shared__ double red[2][SW][BP];
+  __shared__ double taus[64];
+  __shared__ double scal[3];
+  __shared__ double alpha_s;
+  const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
+  const int c = lane % BP, h = lane / BP;
+  Y += blockIdx.x * sYQ;
+  Q += blockIdx.x * sYQ;
+  double a[SR];
+#pragma unroll
+  for (int i = 0; i < SR; ++i) {
+    const int r = (i * SW + g) * RS + h;
+    a[i] = (r < n && c < b) ? Y[(long)r * b + c] : 0.0;
   }
+  // ---- CholeskyQR2
+  bool chol = true;
+  for (int pass = 0; pass < 2 && chol; ++pass) {
+    for (int e = tid; e < b * b; e += SNT) B[(e / b) * SJLD + e % b] = 0.0;
+    __syncthreads();
+    for (int l = 0; l < b; ++l) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < SR; ++i) s += col_bcast<BP>(a[i], l, h) * a[i];
+      s = rowset_sum<BP>(s);
+      if (h == 0 && c < b && c >= l) atomicAdd(&B[l * SJLD + c], s);
+    }
+    __syncthreads();
+    double dmax = 0.0;
+    for (int k = 0; k < b; ++k) dmax = fmax(dmax, B[k * SJLD + k]);
+    const double piv_tol = (pass == 0 ? 1e-12 : 1e-3) * dmax;
+    for (int k = 0; k < b; ++k) {
+      const double d = B[k * SJLD + k];
+      if (!(d > piv_tol)) {  // uniform: every thread read the same value
+        chol = false;
+        break;
+      }
+      const double inv = 1.0 / sqrt(d);
+      const int m = b - k - 1;
+      for (int e = tid; e < m * m; e += SNT) {
+        const int i = k + 1 + e % m, j = k + 1 + e / m;
+        if (i <= j) B[i * SJLD + j] -= (B[k * SJLD + i] * inv) * (B[k * SJLD + j] * inv);
+      }
+      __syncthreads();
+      // row k of R (read by the substitution only; the next step reads row k + 1)
+      if (tid > k && tid < b) B[k * SJLD + tid] *= inv;
+      if (tid == k) B[k * SJLD + k] = d * inv;
+    }
+    __syncthreads();
+    if (!chol) break;
+    // X <- X R^{-1}: column j is final after dividing by R_jj; later columns subtract it.
+    for (int j = 0; j < b; ++j) {
+      const double rinv = 1.0 / B[j * SJLD + j];
+      const double rjc = (c > j && c < b) ? B[j * SJLD + c] : 0.0;
+#pragma unroll
+      for (int i = 0; i < SR; ++i) {
+        if (c == j) a[i] *= rinv;
+        a[i] -= col_bcast<BP>(a[i], j, h) * rjc;
+      }
+    }
+    __syncthreads();
+  }
+  if (chol) {
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      if (r < n && c < b) Q[(long)r * b + c] = a[i];
+    }
+    return;
+  }
+  // ---- Householder QR (from the original block)
+#pragma unroll
+  for (int i = 0; i < SR; ++i) {
+    const int r = (i * SW + g) * RS + h;
+    a[i] = (r < n && c < b) ? Y[(long)r * b + c] : 0.0;
+  }
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      s += (c == 0 && r > 0) ? a[i] * a[i] : 0.0;
+      if (c == 0 && r == 0) alpha_s = a[i];
+    }
+    s = rowset_sum<BP>(s);
+    if (lane == 0) nrm[g] = s;
+  }
+  for (int j = 0; j < b; ++j) {
+    __syncthreads();
+    if (tid == 0) {
+      double sub = 0.0;
+#pragma unroll
+      for (int w = 0; w < SW; ++w) sub += nrm[w];
+      const double alpha = alpha_s;
+      double tau = 0.0, scale = 0.0, beta = alpha;
+      if (sub > 0.0) {
+        const double nr = sqrt(alpha * alpha + sub);
+        beta = alpha >= 0.0 ? -nr : nr;
+        tau = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
+      scal[0] = tau;
+      scal[1] = scale;
+      scal[2] = beta;
+      taus[j] = tau;
+    }
+    __syncthreads();
+    const double tau = scal[0], scale = scal[1], beta = scal[2];
+    double vv[SR];
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      const double xj = col_bcast<BP>(a[i], j, h);
+      vv[i] = r > j ? xj * scale : (r == j ? 1.0 : 0.0);
+      s += vv[i] * a[i];
+    }
+    s = rowset_sum<BP>(s);
+    const int par = j & 1;
+    if (h == 0) red[par][g][c] = s;
+    __syncthreads();
+    if (c > j && c < b && tau != 0.0) {
+      double wsum = 0.0;
+#pragma unroll
+      for (int w = 0; w < SW; ++w) wsum += red[par][w][c];
+      const double wc = tau * wsum;
+#pragma unroll
+      for (int i = 0; i < SR; ++i) a[i] -= vv[i] * wc;
+    }
+    if (c == j) {
+#pragma unroll
+      for (int i = 0; i < SR; ++i) {
+        const int r = (i * SW + g) * RS + h;
+        if (r > j) a[i] = vv[i];
+        else if (r == j) a[i] = beta;
+      }
+    }
+    // partial norm of column j + 1 below the diagonal, and its diagonal entry
+    double s2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      s2 += (c == j + 1 && r > j + 1) ? a[i] * a[i] : 0.0;
+      if (c == j + 1 && r == j + 1) alpha_s = a[i];
+    }
+    s2 = rowset_sum<BP>(s2);
+    if (lane == ((j + 1) % BP) && j + 1 < BP) nrm[g] = s2;
+  }
+  double m[SR];
+#pragma unroll
+  for (int i = 0; i < SR; ++i) m[i] = ((i * SW + g) * RS + h == c && c < b) ? 1.0 : 0.0;
   __syncthreads();
-  apply_q(VLds{A, b}, tau, M, red, n, b, b);
-  for (int e = tid; e < n * b; e += 1024) {
-    const int r = e % n, c = e / n;
-    Q[r + (long)n * c] = M[r * b + c];
+  for (int j = b - 1; j >= 0; --j) {
+    const double tau = taus[j];
+    double vv[SR];
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      const double xj = col_bcast<BP>(a[i], j, h);
+      vv[i] = r > j ? xj : (r == j ? 1.0 : 0.0);
+      s += vv[i] * m[i];
+    }
+    s = rowset_sum<BP>(s);
+    const int par = j & 1;
+    if (h == 0) red[par][g][c] = s;
+    __syncthreads();
+    if (c < b && tau != 0.0) {
+      double wsum = 0.0;
+#pragma unroll
+      for (int w = 0; w < SW; ++w) wsum += red[par][w][c];
+      const double wc = tau * wsum;
+#pragma unroll
+      for (int i = 0; i < SR; ++i) m[i] -= vv[i] * wc;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < SR; ++i) {
+    const int r = (i * SW + g) * RS + h;
+    if (r < n && c < b) Q[(long)r * b + c] = m[i];
   }
 }
 
 // Rayleigh-Ritz: T = sym(Q^T Z), Jacobi(T) = S Th S^T; Uf = Q S (all b Ritz vectors,
-// descending), U = first k (canonical signs), w = Th[:k], converged[bidx] = 1 when every
-// top-k Ritz residual ||Z s_i - th_i Q s_i|| <= tol * |th_1|. Q and Z are streamed
-// through LDS in chunks of RR_ROWS rows.
-constexpr int RR_ROWS = 32;
-__global__ __launch_bounds__(256) void rr_kernel(const double* __restrict__ Q, const double* __restrict__ Z, long sQ,
-                                                 int n, int b, int k, double* __restrict__ Uf, double* __restrict__ U,
-                                                 long sU, double* __restrict__ w, long sw, double* __restrict__ wall,
-                                                 double tol, int* __restrict__ converged, int max_sweeps) {
-  __shared__ double a[SJ * SJLD];
+// descending, row-major), U = first k (column-major, canonical signs), w = Th[:k],
+// wall = Th, cut = Th[b-1], converged = 1 when every top-k Ritz residual
+// ||Z s_i - th_i Q s_i|| <= tol * |th_1|. Same lane packing as qr_kernel.
+template <int BP, int SR>
+__global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, const double* __restrict__ Z, long sQ,
+                                                  int n, int b, int k, double* __restrict__ Uf, double* __restrict__ U,
+                                                  long sU, double* __restrict__ w, long sw, double* __restrict__ wall,
+                                                  double* __restrict__ cut, double tol, int* __restrict__ converged,
+                                                  int max_sweeps) {
+  constexpr int RS = 64 / BP;
+  __shared__ double a0[SJ * SJLD];
   __shared__ double v[SJ * SJLD];
-  __shared__ double qc[RR_ROWS * SJ];
-  __shared__ double zc[RR_ROWS * SJ];
+  __shared__ unsigned short sched[(SJ - 1) * (SJ / 2)];
   __shared__ int order[SJ];
   __shared__ double res[SJ];
+  __shared__ double bestv[SW][BP];
+  __shared__ double bests[SW][BP];
+  __shared__ int besti[SW][BP];
   __shared__ double sgn[SJ];
-  const int tid = threadIdx.x, bidx = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6, bidx = blockIdx.x;
+  const int c = lane % BP, h = lane / BP;
   Q += bidx * sQ;
   Z += bidx * sQ;
   Uf += bidx * sQ;
@@ -297,56 +638,40 @@ __global__ __launch_bounds__(256) void rr_kernel(const double* __restrict__ Q, c
   w += bidx * sw;
   wall += (long)bidx * SJ;
   const int np = (b + 1) & ~1;
-  const int npairs = b * (b + 1) / 2;
-  // per-thread accumulators for up to 9 (i <= j) pairs (b <= 64: 2080 pairs / 256)
-  double t1[9], t2[9];
-#pragma unroll
-  for (int u = 0; u < 9; ++u) { t1[u] = 0.0; t2[u] = 0.0; }
-  for (int r0 = 0; r0 < n; r0 += RR_ROWS) {
-    const int nr = min(RR_ROWS, n - r0);
-    __syncthreads();
-    for (int e = tid; e < RR_ROWS * b; e += 256) {
-      const int r = e % RR_ROWS, c = e / RR_ROWS;
-      qc[r * SJ + c] = (r < nr) ? Q[r0 + r + (long)n * c] : 0.0;
-      zc[r * SJ + c] = (r < nr) ? Z[r0 + r + (long)n * c] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 9; ++u) {
-      const int pidx = tid + 256 * u;
-      if (pidx < npairs) {
-        // pidx -> (i <= j): column-major upper triangle enumeration
-        int j = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
-        while (j * (j + 1) / 2 > pidx) --j;
-        while ((j + 1) * (j + 2) / 2 <= pidx) ++j;
-        const int i = pidx - j * (j + 1) / 2;
-        double s1 = 0.0, s2 = 0.0;
-        for (int r = 0; r < nr; ++r) {
-          s1 += qc[r * SJ + i] * zc[r * SJ + j];
-          s2 += qc[r * SJ + j] * zc[r * SJ + i];
-        }
-        t1[u] += s1;
-        t2[u] += s2;
-      }
-    }
-  }
-  for (int e = tid; e < np * np; e += 256) {
+  double* a = a0;
+  for (int e = tid; e < np * np; e += SNT) {
     const int i = e % np, j = e / np;
     a[i * SJLD + j] = 0.0;
     v[i * SJLD + j] = (i == j) ? 1.0 : 0.0;
   }
-  __syncthreads();
+  if (tid < SJ) res[tid] = 0.0;
+  jacobi_schedule(sched, np);
+  {
+    double qr_[SR], zr[SR];
 #pragma unroll
-  for (int u = 0; u < 9; ++u) {
-    const int pidx = tid + 256 * u;
-    if (pidx < npairs) {
-      int j = (int)((sqrt(8.0 * pidx + 1.0) - 1.0) * 0.5);
-      while (j * (j + 1) / 2 > pidx) --j;
-      while ((j + 1) * (j + 2) / 2 <= pidx) ++j;
-      const int i = pidx - j * (j + 1) / 2;
-      const double t = 0.5 * (t1[u] + t2[u]);
-      a[i * SJLD + j] = t;
-      a[j * SJLD + i] = t;
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      const bool ok = r < n && c < b;
+      qr_[i] = ok ? Q[(long)r * b + c] : 0.0;
+      zr[i] = ok ? Z[(long)r * b + c] : 0.0;
+    }
+    __syncthreads();
+    // T[l][c] += sum over this wave's rows of Q[r][l] Z[r][c]
+    for (int l = 0; l < b; ++l) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < SR; ++i) s += col_bcast<BP>(qr_[i], l, h) * zr[i];
+      s = rowset_sum<BP>(s);
+      if (h == 0 && c < b) atomicAdd(&a[l * SJLD + c], s);
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < b * b; e += SNT) {
+    const int i = e % b, j = e / b;
+    if (i < j) {
+      const double x = 0.5 * (a[i * SJLD + j] + a[j * SJLD + i]);
+      a[i * SJLD + j] = x;
+      a[j * SJLD + i] = x;
     }
   }
   __syncthreads();
@@ -356,97 +681,154 @@ __global__ __launch_bounds__(256) void rr_kernel(const double* __restrict__ Q, c
     a[(np - 1) * SJLD + (np - 1)] = -(m * (double)b + 1.0) * 4.0;
   }
   __syncthreads();
-  jacobi_sweeps(a, v, b, np, max_sweeps, 1e-14);
+  jacobi_sweeps(a, v, b, np, max_sweeps, 1e-14, sched);
   jacobi_order(a, np, order);
-  if (tid < SJ) res[tid] = 0.0;
-  __syncthreads();
-  // Uf = Q S (descending order) and G u = Z S, chunk by chunk; residuals and the
-  // largest-magnitude entry of each of the first k columns on the way.
-  const int lane = tid & 63, wv = tid >> 6;
-  for (int r0 = 0; r0 < n; r0 += RR_ROWS) {
-    const int nr = min(RR_ROWS, n - r0);
-    __syncthreads();
-    for (int e = tid; e < RR_ROWS * b; e += 256) {
-      const int r = e % RR_ROWS, c = e / RR_ROWS;
-      qc[r * SJ + c] = (r < nr) ? Q[r0 + r + (long)n * c] : 0.0;
-      zc[r * SJ + c] = (r < nr) ? Z[r0 + r + (long)n * c] : 0.0;
+  // Ritz vectors u = Q S[:, order[c]] and G u = Z S[:, order[c]] on this thread's rows.
+  const int src = c < b ? order[c] : 0;
+  const double th = c < b ? a[src * SJLD + src] : 0.0;
+  double u[SR];
+  {
+    double qr_[SR];
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      qr_[i] = (r < n && c < b) ? Q[(long)r * b + c] : 0.0;
+      u[i] = 0.0;
     }
-    __syncthreads();
-    for (int e = tid; e < RR_ROWS * b; e += 256) {
-      const int r = e % RR_ROWS, j = e / RR_ROWS;
-      if (r >= nr) continue;
-      const int src = order[j];
-      double u = 0.0, gu = 0.0;
-      for (int l = 0; l < b; ++l) {
-        const double sv = v[l * SJLD + src];
-        u += qc[r * SJ + l] * sv;
-        gu += zc[r * SJ + l] * sv;
-      }
-      Uf[r0 + r + (long)n * j] = u;
-      if (j < k) {
-        const double d = gu - a[src * SJLD + src] * u;
-        atomicAdd(&res[j], d * d);
-      }
+    for (int l = 0; l < b; ++l) {
+      const double sl = v[l * SJLD + src];
+#pragma unroll
+      for (int i = 0; i < SR; ++i) u[i] += col_bcast<BP>(qr_[i], l, h) * sl;
     }
   }
-  __syncthreads();
-  // canonical signs of the first k columns (read back from Uf)
-  for (int j = wv; j < k; j += 4) {
-    double best = -1.0;
-    int bi = 0;
-    for (int r = lane; r < n; r += 64) {
-      const double x = fabs(Uf[r + (long)n * j]);
-      if (x > best) { best = x; bi = r; }
+  double rsum = 0.0;
+  {
+    double zr[SR], gu[SR];
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      zr[i] = (r < n && c < b) ? Z[(long)r * b + c] : 0.0;
+      gu[i] = 0.0;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ob = __shfl_xor(best, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    for (int l = 0; l < b; ++l) {
+      const double sl = v[l * SJLD + src];
+#pragma unroll
+      for (int i = 0; i < SR; ++i) gu[i] += col_bcast<BP>(zr[i], l, h) * sl;
     }
-    if (lane == 0) sgn[j] = Uf[bi + (long)n * j] < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const double d = gu[i] - th * u[i];
+      rsum += d * d;
+    }
+  }
+  rsum = rowset_sum<BP>(rsum);
+  // largest-magnitude entry of this thread's rows (first occurrence), then Uf
+  double bv = -1.0, bs = 0.0;
+  int bi = 0;
+#pragma unroll
+  for (int i = 0; i < SR; ++i) {
+    const int r = (i * SW + g) * RS + h;
+    if (r < n) {
+      const double x = fabs(u[i]);
+      if (x > bv) { bv = x; bs = u[i]; bi = r; }
+      if (c < b) Uf[(long)r * b + c] = u[i];
+    }
+  }
+#pragma unroll
+  for (int o = BP; o < 64; o <<= 1) {
+    const double ov = __shfl_xor(bv, o, 64);
+    const double os = __shfl_xor(bs, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bs = os; bi = oi; }
+  }
+  if (h == 0) {
+    bestv[g][c] = bv;
+    bests[g][c] = bs;
+    besti[g][c] = bi;
+    if (c < b) atomicAdd(&res[c], rsum);
   }
   __syncthreads();
-  for (int e = tid; e < n * k; e += 256) {
-    const int r = e % n, j = e / n;
-    U[r + (long)n * j] = sgn[j] * Uf[r + (long)n * j];
+  if (tid < b) {
+    double best = -1.0, sb = 0.0;
+    int ib = 0;
+    for (int ww = 0; ww < SW; ++ww) {
+      const double x = bestv[ww][tid];
+      const int ix = besti[ww][tid];
+      if (x > best || (x == best && ix < ib)) { best = x; ib = ix; sb = bests[ww][tid]; }
+    }
+    sgn[tid] = sb < 0.0 ? -1.0 : 1.0;
   }
-  if (tid < b) wall[tid] = a[order[tid] * SJLD + order[tid]];
-  if (tid < k) w[tid] = a[order[tid] * SJLD + order[tid]];
+  __syncthreads();
+  if (c < k) {
+    const double sg = sgn[c];
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      if (r < n) U[r + (long)n * c] = sg * u[i];
+    }
+  }
+  if (g == 0 && h == 0 && c < b) {
+    wall[c] = th;
+    if (c < k) w[c] = th;
+  }
   if (tid == 0) {
     const double th1 = fabs(a[order[0] * SJLD + order[0]]);
     double rmax = 0.0;
     for (int j = 0; j < k; ++j) rmax = fmax(rmax, sqrt(res[j]));
     converged[bidx] = (rmax <= tol * th1 || th1 == 0.0) ? 1 : 0;
+    cut[bidx] = a[order[b - 1] * SJLD + order[b - 1]];
   }
 }
 
 int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* w, long sw, double* V, long sV,
                        int batch, int* info, int max_sweeps, double tol) {
   if (n < 1 || n > SJ || batch < 1) return 8;
-  hipLaunchKernelGGL(jacobi_full_kernel, dim3(batch), dim3(256), 0, st, A, sA, n, w, sw, V, sV, info, max_sweeps, tol);
+  hipLaunchKernelGGL(jacobi_full_kernel, dim3(batch), dim3(SNT), 0, st, A, sA, n, w, sw, V, sV, info, max_sweeps, tol);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
-int power_launch(hipStream_t st, const double* G, long sG, const double* W, long sW, double* Y, long sY, int n, int b,
-                 int q, int normalise, int batch) {
-  if (n < 1 || n > 256 || b < 1 || b > 64 || batch < 1) return 8;
-  hipLaunchKernelGGL(power_kernel, dim3((b + PW_COLS - 1) / PW_COLS, batch), dim3(1024), 0, st, G, sG, W, sW, Y, sY, n, b,
-                     q, normalise);
+int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
+                 int mode, const double* cut, int batch) {
+  if (n < 1 || n > 256 || b < 1 || b > 64 || q < 1 || mode < 0 || mode > 2 || batch < 1) return 8;
+  if (mode == 2 && !cut) return 8;
+  hipLaunchKernelGGL(power_kernel, dim3((b + PW_COLS - 1) / PW_COLS, batch), dim3(1024), 0, st, G, sG, W, Y, sWY, n,
+                     b, q, mode, cut);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
-int hhqr_launch(hipStream_t st, const double* Y, long sY, double* Vscratch, double* Q, long sQ, int n, int b, int batch) {
-  (void)Vscratch;
-  if (n < b || (long)n * b > HH_MAX || b < 1 || b > TQ_B || batch < 1) return 8;
-  hipLaunchKernelGGL(hhqr_kernel, dim3(batch), dim3(1024), 0, st, Y, sY, Q, sQ, n, b);
+// Launches pick the smallest lane packing (BP, SR) with b <= BP and
+// n <= 16 * (64 / BP) * SR: <16,2>, <32,4>, <32,8>, <64,8>, <64,16>.
+int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int batch) {
+  if (n < 1 || n > SW * SR || b < 1 || b > 64 || b > n || batch < 1) return 8;
+  const dim3 grid(batch), block(SNT);
+  if (b <= 16 && n <= 128) hipLaunchKernelGGL((qr_kernel<16, 2>), grid, block, 0, st, Y, Q, sYQ, n, b);
+  else if (b <= 32 && n <= 128) hipLaunchKernelGGL((qr_kernel<32, 4>), grid, block, 0, st, Y, Q, sYQ, n, b);
+  else if (b <= 32) hipLaunchKernelGGL((qr_kernel<32, 8>), grid, block, 0, st, Y, Q, sYQ, n, b);
+  else if (n <= 128) hipLaunchKernelGGL((qr_kernel<64, 8>), grid, block, 0, st, Y, Q, sYQ, n, b);
+  else hipLaunchKernelGGL((qr_kernel<64, 16>), grid, block, 0, st, Y, Q, sYQ, n, b);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
 int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, int n, int b, int k, double* Uf, double* U,
-              long sU, double* w, long sw, double* wall, double tol, int* converged, int batch) {
-  if (n < 1 || b < 1 || b > SJ || k < 1 || k > b || batch < 1) return 8;
-  hipLaunchKernelGGL(rr_kernel, dim3(batch), dim3(256), 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, tol, converged,
-                     20);
+              long sU, double* w, long sw, double* wall, double* cut, double tol, int* converged, int batch) {
+  if (n < 1 || n > SW * SR || b < 1 || b > SJ || b > n || k < 1 || k > b || batch < 1) return 8;
+  const dim3 grid(batch), block(SNT);
+  const int sweeps = 20;
+  if (b <= 16 && n <= 128)
+    hipLaunchKernelGGL((rr_kernel<16, 2>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
+                       converged, sweeps);
+  else if (b <= 32 && n <= 128)
+    hipLaunchKernelGGL((rr_kernel<32, 4>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
+                       converged, sweeps);
+  else if (b <= 32)
+    hipLaunchKernelGGL((rr_kernel<32, 8>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
+                       converged, sweeps);
+  else if (n <= 128)
+    hipLaunchKernelGGL((rr_kernel<64, 8>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
+                       converged, sweeps);
+  else
+    hipLaunchKernelGGL((rr_kernel<64, 16>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
+                       converged, sweeps);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
