@@ -393,111 +393,86 @@ __global__ __launch_bounds__(256) void lowrank_kernel(const float* __restrict__ 
   }
 }
 
+template <int KP>
 __global__ void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M, long N, int k,
                                 const float* __restrict__ X, double* __restrict__ acc_out);
 
 int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, long M, long N, int k,
                    const float* X, float* out, double* acc, int* n_blocks_out) {
   if (M < 1 || N < 1 || k < 1 || k > 4096) return 8;
+  if (mode == 1 && M > (1L << 24)) return 8;  // 32-bit column offsets of the residual pass
   dim3 grid((unsigned)((M + LR_T - 1) / LR_T), (unsigned)((N + LR_T - 1) / LR_T));
   if (n_blocks_out) *n_blocks_out = (int)(grid.x * grid.y);
   if (mode == 0)
     hipLaunchKernelGGL(lowrank_kernel<0>, grid, dim3(256), 0, st, L, R, M, N, k, X, out, acc);
+  else if (k <= 16)
+    hipLaunchKernelGGL(residual_kernel<16>, grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+  else if (k <= 32)
+    hipLaunchKernelGGL(residual_kernel<32>, grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+  else if (k <= 64)
+    hipLaunchKernelGGL(residual_kernel<64>, grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
   else
-    hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+    return 8;
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
 // Residual pass: acc[2*block] = sum (X - L R^T)^2, acc[2*block+1] = sum X^2 over one
-// 128 x 128 tile of X (ld M). The X tile is requested (float4, coalesced along m) before
-// the low-rank tile is formed with MFMA, then compared against it through LDS.
+// 128 (m) x 128 (n) region of X (ld M). No LDS: wave w takes the m strip
+// [m0 + 32 w, +32) and four 32-column subtiles; the MFMA operands (R rows as A, L rows
+// as B, k <= KP) come straight from global memory (L2-resident), and X is read in the
+// accumulator's own layout (element e of lane l is (m0 + l % 32, n0 + acc32_row(e, l)):
+// each half-wave reads one 128-byte column segment), so the comparison needs no
+// transposition. The X loads of a subtile are issued before its products.
+template <int KP>
 __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M,
                                                        long N, int k, const float* __restrict__ X,
                                                        double* __restrict__ acc_out) {
-  __shared__ float Ls[LR_K * LR_T];
-  __shared__ float Rs[LR_K * LR_T];
-  __shared__ float Cs[LR_T * (LR_T + 4)];
   __shared__ double red[8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int wr = w >> 1, wc = w & 1;
-  const long m0 = (long)blockIdx.x * LR_T, n0 = (long)blockIdx.y * LR_T;
-  const bool vec = (M % 4 == 0) && (((uintptr_t)X & 15) == 0);
-  // X tile prefetch: element (m0 + 4*mq .. +3, n0 + col), 16 float4 per thread.
-  float4 xv[16];
-  const int mq = tid & 31;
+  const int li = lane & 31, kh = lane >> 5;
+  const long mw = (long)blockIdx.x * 128 + 32 * w;  // wave's first row
+  const long m = mw + li;
+  const long nbase = (long)blockIdx.y * 128;
+  const bool mok = m < M;
+  float bl[KP / 2];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int col = (tid >> 5) + 8 * i;
-    const long n = n0 + col, m = m0 + 4 * mq;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (n < N) {
-      const float* p = X + m + M * n;
-      if (vec && m + 3 < M) v = *reinterpret_cast<const float4*>(p);
-      else {
-        if (m < M) v.x = p[0];
-        if (m + 1 < M) v.y = p[1];
-        if (m + 2 < M) v.z = p[2];
-        if (m + 3 < M) v.w = p[3];
-      }
-    }
-    xv[i] = v;
+  for (int j = 0; j < KP / 2; ++j) {
+    const int kk = 2 * j + kh;
+    bl[j] = (kk < k && mok) ? L[m + M * (long)kk] : 0.f;
   }
-  f32x16 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = f32x16{};
-  for (int k0 = 0; k0 < k; k0 += LR_K) {
-    __syncthreads();
-    for (int e = tid; e < LR_K * LR_T; e += 256) {
-      const int kk = e / LR_T, row = e % LR_T;
-      const long m = m0 + row, n = n0 + row;
-      Ls[e] = (k0 + kk < k && m < M) ? L[m + M * (long)(k0 + kk)] : 0.f;
-      Rs[e] = (k0 + kk < k && n < N) ? R[n + N * (long)(k0 + kk)] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < LR_K / 2; ++kk) {
-      float af[2], bf[2];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) af[a] = Rs[(2 * kk + (lane >> 5)) * LR_T + 64 * wr + 32 * a + (lane & 31)];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) bf[b] = Ls[(2 * kk + (lane >> 5)) * LR_T + 64 * wc + 32 * b + (lane & 31)];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[a], bf[b], acc[a][b], 0, 0, 0);
-    }
-  }
-  // C^T tile -> LDS: Cs[n_local][m_local]
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int nl = 64 * wr + 32 * a + acc32_row(e, lane);
-        const int ml = 64 * wc + 32 * b + (lane & 31);
-        Cs[nl * (LR_T + 4) + ml] = acc[a][b][e];
-      }
-  __syncthreads();
+  // X element e of this lane in subtile nb: (m, nb + 8 (e / 4) + 4 kh + e % 4); the column
+  // part of the offset is a 32-bit multiple of M from the wave-uniform base X + mw + M nb.
+  const unsigned mu = (unsigned)M;
   double se = 0.0, sx = 0.0;
+#pragma unroll 1
+  for (int t = 0; t < 4; ++t) {
+    const long nb = nbase + 32 * t;
+    const float* base = X + mw + M * nb;
+    float xv[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int col = (tid >> 5) + 8 * i;
-    if (n0 + col >= N) continue;
-    const float4 c4 = *reinterpret_cast<const float4*>(&Cs[col * (LR_T + 4) + 4 * mq]);
-    const float xs[4] = {xv[i].x, xv[i].y, xv[i].z, xv[i].w};
-    const float cc[4] = {c4.x, c4.y, c4.z, c4.w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (m0 + 4 * mq + j < M) {
-        const float d = xs[j] - cc[j];
-        se += (double)(d * d);
-        sx += (double)(xs[j] * xs[j]);
-      }
+    for (int e = 0; e < 16; ++e) {
+      const int row = 8 * (e >> 2) + 4 * kh + (e & 3);
+      xv[e] = (mok && nb + row < N) ? base[(unsigned)li + mu * (unsigned)row] : 0.f;
     }
+    float ar[KP / 2];
+#pragma unroll
+    for (int j = 0; j < KP / 2; ++j) {
+      const int kk = 2 * j + kh;
+      const long n = nb + li;
+      ar[j] = (kk < k && n < N) ? R[n + N * (long)kk] : 0.f;
+    }
+    f32x16 acc = f32x16{};
+#pragma unroll
+    for (int j = 0; j < KP / 2; ++j) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[j], bl[j], acc, 0, 0, 0);
+    float e2 = 0.f, x2 = 0.f;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const float d = xv[e] - acc[e];  // out-of-range elements: X = 0 and acc = 0
+      e2 += d * d;
+      x2 += xv[e] * xv[e];
+    }
+    se += (double)e2;
+    sx += (double)x2;
   }
   se = wave_sum(se);
   sx = wave_sum(sx);

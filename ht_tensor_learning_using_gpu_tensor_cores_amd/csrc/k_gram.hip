@@ -398,104 +398,120 @@ template <int PREC>
 __global__ __launch_bounds__(512) void slab_gram_kernel(const float* __restrict__ x, int n1, int n2, long S,
                                                         int n_blocks, float* __restrict__ part1,
                                                         float* __restrict__ part2, double* __restrict__ norms) {
-  constexpr int LD = 128 + 8;
-  __shared__ __attribute__((aligned(16))) uint16_t Srm[128 * LD];  // [i][j]
-  __shared__ __attribute__((aligned(16))) uint16_t Scm[128 * LD];  // [j][i]
+  // Scm[j][i] = S[i][j] (the slab's own column-major order, padded rows); Srm holds the
+  // transpose, Srm[i][j ^ swz(i)] with swz(i) = 8 ((i / 4) mod 16): the XOR keeps each
+  // aligned group of 8 columns contiguous (so MFMA operands stay single 16-byte reads)
+  // and spreads the 4-element row stores of 16 consecutive row groups over all banks.
+  // Two copies of (Srm, Scm): the slab being multiplied and the next one being stored.
+  constexpr int LDC = 128 + 8;
+  constexpr int BUF = 128 * 128 + 128 * LDC;
+  __shared__ __attribute__((aligned(16))) uint16_t lds[2 * BUF];
   __shared__ double red[8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int g = w >> 2, q = w & 3, wr = q >> 1, wc = q & 1;
-  for (int i = tid; i < 128 * LD; i += 512) { Srm[i] = 0; Scm[i] = 0; }
+  for (int i = tid; i < 2 * BUF; i += 512) lds[i] = 0;
 
   const long s_begin = (long)blockIdx.x * S / n_blocks;
   const long s_end = (long)(blockIdx.x + 1) * S / n_blocks;
   const long slab = (long)n1 * n2;
-  const bool vec = ((n1 & 3) == 0) && (((uintptr_t)x & 15) == 0);
-  const int n4 = (int)(slab / 4);
+
+  // this thread's 4 x 4 blocks: b = tid + 512 u, rows i4 = 4 (b % 32), columns j4 = 4 (b / 32)
+  float4 r[2][4];
+  double sumsq = 0.0;
+  auto load = [&](long s) {
+    const float* base = x + s * slab;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int bb = tid + 512 * u, i4 = 4 * (bb & 31), j4 = 4 * (bb >> 5);
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        float4 v4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i4 < n1 && j4 + d < n2) v4 = *reinterpret_cast<const float4*>(base + (long)(j4 + d) * n1 + i4);
+        r[u][d] = v4;
+      }
+    }
+  };
+  auto pack = [](uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
+    uint2 wv;
+    wv.x = (uint32_t)a | ((uint32_t)b << 16);
+    wv.y = (uint32_t)c | ((uint32_t)d << 16);
+    return wv;
+  };
+  auto store = [&](uint16_t* Srm, uint16_t* Scm) {
+    float ss = 0.f;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int bb = tid + 512 * u, i4 = 4 * (bb & 31), j4 = 4 * (bb >> 5);
+      uint16_t h[4][4];  // [d][e]: column j4 + d, row i4 + e
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        h[d][0] = to16<PREC>(r[u][d].x);
+        h[d][1] = to16<PREC>(r[u][d].y);
+        h[d][2] = to16<PREC>(r[u][d].z);
+        h[d][3] = to16<PREC>(r[u][d].w);
+        ss += r[u][d].x * r[u][d].x + r[u][d].y * r[u][d].y + r[u][d].z * r[u][d].z + r[u][d].w * r[u][d].w;
+        *reinterpret_cast<uint2*>(&Scm[(j4 + d) * LDC + i4]) = pack(h[d][0], h[d][1], h[d][2], h[d][3]);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i4 + e;
+        *reinterpret_cast<uint2*>(&Srm[i * 128 + (j4 ^ (8 * ((i >> 2) & 15)))]) =
+            pack(h[0][e], h[1][e], h[2][e], h[3][e]);
+      }
+    }
+    sumsq += (double)ss;
+  };
 
   f32x16 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f32x16{};
-  float r[32];
-  double sumsq = 0.0;
 
-  auto load = [&](long s) {
-    const float* base = x + s * slab;
-    if (vec) {
-#pragma unroll
-      for (int it = 0; it < 8; ++it) {
-        const int f = tid + 512 * it;
-        float4 v4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (f < n4) v4 = reinterpret_cast<const float4*>(base)[f];
-        r[4 * it] = v4.x; r[4 * it + 1] = v4.y; r[4 * it + 2] = v4.z; r[4 * it + 3] = v4.w;
-      }
-    } else {
-#pragma unroll 4
-      for (int it = 0; it < 32; ++it) {
-        const int e = tid + 512 * it;
-        r[it] = (e < slab) ? base[e] : 0.f;
-      }
-    }
-  };
-  auto store = [&]() {
-    if (vec) {
-#pragma unroll
-      for (int it = 0; it < 8; ++it) {
-        const int f = tid + 512 * it;
-        if (f < n4) {
-          const int e = 4 * f;
-          const int i = e % n1, j = e / n1;
-          uint16_t h0 = to16<PREC>(r[4 * it]), h1 = to16<PREC>(r[4 * it + 1]);
-          uint16_t h2 = to16<PREC>(r[4 * it + 2]), h3 = to16<PREC>(r[4 * it + 3]);
-          uint2 wv;
-          wv.x = (uint32_t)h0 | ((uint32_t)h1 << 16);
-          wv.y = (uint32_t)h2 | ((uint32_t)h3 << 16);
-          *reinterpret_cast<uint2*>(&Scm[j * LD + i]) = wv;
-          Srm[(i + 0) * LD + j] = h0;
-          Srm[(i + 1) * LD + j] = h1;
-          Srm[(i + 2) * LD + j] = h2;
-          Srm[(i + 3) * LD + j] = h3;
-          sumsq += (double)(r[4 * it] * r[4 * it] + r[4 * it + 1] * r[4 * it + 1] +
-                            r[4 * it + 2] * r[4 * it + 2] + r[4 * it + 3] * r[4 * it + 3]);
-        }
-      }
-    } else {
-#pragma unroll 4
-      for (int it = 0; it < 32; ++it) {
-        const int e = tid + 512 * it;
-        if (e < slab) {
-          const int i = e % n1, j = e / n1;
-          const uint16_t h = to16<PREC>(r[it]);
-          Scm[j * LD + i] = h;
-          Srm[i * LD + j] = h;
-          sumsq += (double)(r[it] * r[it]);
-        }
-      }
-    }
-  };
-
-  if (s_begin < s_end) load(s_begin);
+  if (s_begin < s_end) {
+    load(s_begin);
+    __syncthreads();  // zero fill done
+    store(lds, lds + 128 * 128);
+    if (s_begin + 1 < s_end) load(s_begin + 1);
+    __syncthreads();
+  }
   for (long s = s_begin; s < s_end; ++s) {
-    __syncthreads();
-    store();
-    __syncthreads();
-    if (s + 1 < s_end) load(s + 1);
-    const uint16_t* T = g == 0 ? Srm : Scm;
+    const uint16_t* Srm = lds + ((s - s_begin) & 1) * BUF;
+    const uint16_t* Scm = Srm + 128 * 128;
     const int kmax = g == 0 ? n2 : n1;
     for (int kk = 0; kk < (kmax + 15) / 16; ++kk) {
+      const int kc = 16 * kk + 8 * (lane >> 5);
       u16x8 af[2], bf[2];
+      if (g == 0) {
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
-        af[a] = *reinterpret_cast<const u16x8*>(&T[(64 * wr + 32 * a + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+        for (int a = 0; a < 2; ++a) {
+          const int row = 64 * wr + 32 * a + (lane & 31);
+          af[a] = *reinterpret_cast<const u16x8*>(&Srm[row * 128 + (kc ^ (8 * ((row >> 2) & 15)))]);
+        }
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
-        bf[b] = *reinterpret_cast<const u16x8*>(&T[(64 * wc + 32 * b + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+        for (int b = 0; b < 2; ++b) {
+          const int row = 64 * wc + 32 * b + (lane & 31);
+          bf[b] = *reinterpret_cast<const u16x8*>(&Srm[row * 128 + (kc ^ (8 * ((row >> 2) & 15)))]);
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+          af[a] = *reinterpret_cast<const u16x8*>(&Scm[(64 * wr + 32 * a + (lane & 31)) * LDC + kc]);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          bf[b] = *reinterpret_cast<const u16x8*>(&Scm[(64 * wc + 32 * b + (lane & 31)) * LDC + kc]);
+      }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = mfma32x32x16<PREC>(af[a], bf[b], acc[a][b]);
     }
+    if (s + 1 < s_end) {
+      uint16_t* nx = lds + ((s + 1 - s_begin) & 1) * BUF;
+      store(nx, nx + 128 * 128);
+      if (s + 2 < s_end) load(s + 2);
+    }
+    __syncthreads();
   }
   float* out = (g == 0 ? part1 : part2) + (long)blockIdx.x * (128 * 128);
 #pragma unroll
@@ -520,7 +536,8 @@ __global__ __launch_bounds__(512) void slab_gram_kernel(const float* __restrict_
 
 int slab_gram_launch(hipStream_t st, const float* x, int n1, int n2, long S, int prec, int n_blocks,
                      float* part1, float* part2, double* norms) {
-  if (n1 < 1 || n2 < 1 || n1 > 128 || n2 > 128 || S < 1 || n_blocks < 1) return 8;
+  if (n1 < 1 || n2 < 1 || n1 > 128 || n2 > 128 || n1 % 4 || n2 % 4 || S < 1 || n_blocks < 1) return 8;
+  if (((uintptr_t)x & 15) != 0) return 8;
   if (prec == PREC_F16)
     hipLaunchKernelGGL(slab_gram_kernel<PREC_F16>, dim3(n_blocks), dim3(512), 0, st, x, n1, n2, S, n_blocks,
                        part1, part2, norms);
