@@ -177,10 +177,11 @@ int scale_cols_inv_launch(hipStream_t st, double* A, int m, int n, const double*
 int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* w, long sw, double* V, long sV,
                        int batch, int* info, int max_sweeps, double tol);
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
-                 int mode, const double* cut, int batch);
-int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int batch);
-int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, int n, int b, int k, double* Uf, double* U,
-              long sU, double* w, long sw, double* wall, double* cut, double tol, int* converged, int batch);
+                 int mode, const double* cut, double* T, int batch);
+int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int try_chol, int batch);
+int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, const double* T, int n, int b, int k,
+              double* Uf, double* U, long sU, double* w, long sw, double* wall, double* cut, double tol,
+              int* converged, int batch);
 // Casts and sign canonicalisation.
 int cast_f64_f32_launch(hipStream_t st, const double* a, float* b, long n);
 int cast_f32_f64_launch(hipStream_t st, const float* a, double* b, long n);

@@ -269,28 +269,34 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     return 0;
   }
   if (n > 256) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream);
-  const int bmax = std::min(64, n);
+  // The QR and Rayleigh-Ritz kernels stage n x b blocks in LDS: n * b <= 8192.
+  const int bmax = std::min(std::min(64, n), 8192 / n);
   if (b <= 0) b = k + 8;
   b = std::min(std::max(b, k), bmax);
+  if (k > b) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream);
   const size_t nbmax = (size_t)n * bmax;
   double* W = alloc<double>(batch * nbmax);
   double* Y = alloc<double>(batch * nbmax);
   double* Q = alloc<double>(batch * nbmax);
   double* Z = alloc<double>(batch * nbmax);
   double* Uf = alloc<double>(batch * nbmax);
+  double* T = alloc<double>((size_t)batch * 64 * 64);
   double* wall = alloc<double>((size_t)batch * 64);
   double* cut = alloc<double>(batch);
   int* conv = alloc<int>(batch);
-  if (!W || !Y || !Q || !Z || !Uf || !wall || !cut || !conv) return HT_ERR_ALLOC;
+  if (!W || !Y || !Q || !Z || !Uf || !T || !wall || !cut || !conv) return HT_ERR_ALLOC;
   long nb = (long)n * b;
   CK(fill_normals_f64_launch(st, W, batch * nb, opt.seed, stream));
   std::vector<int> hconv(batch);
   int q = std::max(1, q0), mode = 0, iters = 0;
   for (int round = 0;; ++round) {
-    CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, batch));
-    CK(qr_launch(st, Y, Q, nb, n, b, batch));
-    CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, batch));
-    CK(rr_launch(st, Q, Z, nb, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, conv, batch));
+    // CholeskyQR2 first, except on the first round after very few powers (blocks from
+    // a fast-decaying spectrum are then numerically rank deficient)
+    const int try_chol = (round > 0 || q >= 4) ? 1 : 0;
+    CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, nullptr, batch));
+    CK(qr_launch(st, Y, Q, nb, n, b, try_chol, batch));
+    CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, T, batch));
+    CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, conv, batch));
     iters += q + 1;
     CK(d2h(hconv.data(), conv, sizeof(int) * batch));
     bool all = true;

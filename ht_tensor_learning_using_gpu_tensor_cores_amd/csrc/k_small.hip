@@ -294,10 +294,13 @@ __global__ __launch_bounds__(1024) void jacobi_full_kernel(const double* __restr
 //           T_0 = W, T_1 = L W, T_{j+1} = 2 L T_j - T_{j-1}, L = (2 G - e I) / e, with
 //           (T_j, T_{j-1}) rescaled together by the column norm of T_j at every step.
 //           Falls back to mode 0 when e <= 0.
+// With T != nullptr (mode 1), the block also forms T[l][c] = sum_r W[r][l] Y[r][c] for its
+// columns (row-major b x b per problem, stride 64 * 64): the Rayleigh quotient Q^T G Q
+// when W = Q.
 constexpr int PW_COLS = 4;
 __global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ G, long sG, const double* __restrict__ W,
                                                      double* __restrict__ Y, long sWY, int n, int b, int q, int mode,
-                                                     const double* __restrict__ cut) {
+                                                     const double* __restrict__ cut, double* __restrict__ T) {
   __shared__ double cur[256 * PW_COLS];
   __shared__ double prv[256 * PW_COLS];
   __shared__ double part[4][256 * PW_COLS];
@@ -379,6 +382,21 @@ __global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ 
     const int rr = x / nc, c = x % nc;
     Y[(long)rr * b + c0 + c] = cur[rr * PW_COLS + c];
   }
+  if (T) {
+    const int l = tid & 63, c = (tid >> 6) & 3, pp = tid >> 8;
+    double t = 0.0;
+    if (l < b && c < nc) {
+      const int r0 = (pp * n) / 4, r1 = ((pp + 1) * n) / 4;
+      for (int rr = r0; rr < r1; ++rr) t += W[(long)rr * b + l] * cur[rr * PW_COLS + c];
+    }
+    part[pp][l * PW_COLS + c] = t;
+    __syncthreads();
+    if (tid < 64 * PW_COLS) {
+      const int l2 = tid / PW_COLS, c2 = tid % PW_COLS;
+      if (l2 < b && c2 < nc)
+        T[(long)bidx * 64 * 64 + l2 * b + c0 + c2] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    }
+  }
 }
 
 // Lane packing of the row-distributed kernels: a wave holds BP columns (lane c < BP) of
@@ -410,27 +428,41 @@ __device__ __forceinline__ double rowset_sum(double s) {
   return s;
 }
 
-// Q = orth(Y) (row-major n x b, b <= n, b <= BP, n <= SW * (64/BP) * SR), rows in
-// registers (thread (g, c, h) holds column c of its rows).
-//   First choice, CholeskyQR2: B = X^T X, B = R^T R (right-looking Cholesky in LDS),
-//   X <- X R^{-1} by forward substitution across the lanes of each row, twice. It is
-//   taken only when every first-pass pivot exceeds 1e-12 * max diag(B) (cond(Y) below
-//   ~1e6), where two passes give orthogonality to rounding.
-//   Otherwise Householder QR: the reflector of column j lives in the lanes of column j
-//   (rows > j, unit diagonal implied) and reaches the other lanes by readlane. Per column:
-//   thread 0 forms (tau, 1/(alpha - beta), beta) from the partial norms, every thread adds
-//   its rows' share of w = v^T A, and the owners of columns > j apply A -= v (tau w). Q is
-//   then formed explicitly as H_0 ... H_{b-1} I.
+// Orders the LDS accesses of one wave (lanes read what other lanes of the same wave
+// wrote before): LDS operations of a wave complete in order; the fences keep the
+// compiler from moving loads or stores across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int QR_LDS = 8192;  // n * b limit of the staged blocks (doubles)
+
+// Q = orth(Y) (row-major n x b, b <= n, b <= BP, n <= SW * (64/BP) * SR, n * b <= QR_LDS),
+// rows in registers (thread (g, c, h) holds column c of its rows), the block staged in LDS.
+//   First choice (when try_chol), CholeskyQR2: B = X^T X (one (l, c) dot product per
+//   thread over the staged rows), B = R^T R and R^{-1} by wave 0 alone (right-looking
+//   Cholesky, then back substitution with one column of R^{-1} per lane), X <- X R^{-1};
+//   twice. It is taken only when every first-pass pivot exceeds 1e-12 * max diag(B)
+//   (cond(Y) below ~1e6), where two passes give orthogonality to rounding.
+//   Otherwise Householder QR: per column j the lanes of column j form (tau, beta) and the
+//   reflector, store the reflector in the staging block (which then keeps all of them),
+//   every thread adds its rows' share of w = v^T A, and the owners of columns > j apply
+//   A -= v (tau w). Q is then formed explicitly as H_0 ... H_{b-1} I.
 template <int BP, int SR>
 __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, double* __restrict__ Q, long sYQ, int n,
-                                                  int b) {
+                                                  int b, int try_chol) {
   constexpr int RS = 64 / BP;
+  __shared__ double Ys[QR_LDS];
   __shared__ double B[SJ * SJLD];
+  __shared__ double Ri[SJ * SJLD];
   __shared__ double nrm[SW];
   __shared__ double red[2][SW][BP];
   __shared__ double taus[64];
-  __shared__ double scal[3];
+  __shared__ double scal[2];
   __shared__ double alpha_s;
+  __shared__ int bad;
   const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
   const int c = lane % BP, h = lane / BP;
   Y += blockIdx.x * sYQ;
@@ -442,49 +474,66 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
     a[i] = (r < n && c < b) ? Y[(long)r * b + c] : 0.0;
   }
   // ---- CholeskyQR2
-  bool chol = true;
+  bool chol = try_chol != 0;
   for (int pass = 0; pass < 2 && chol; ++pass) {
-    for (int e = tid; e < b * b; e += SNT) B[(e / b) * SJLD + e % b] = 0.0;
-    __syncthreads();
-    for (int l = 0; l < b; ++l) {
-      double s = 0.0;
 #pragma unroll
-      for (int i = 0; i < SR; ++i) s += col_bcast<BP>(a[i], l, h) * a[i];
-      s = rowset_sum<BP>(s);
-      if (h == 0 && c < b && c >= l) atomicAdd(&B[l * SJLD + c], s);
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      if (r < n && c < b) Ys[r * b + c] = a[i];
     }
     __syncthreads();
-    double dmax = 0.0;
-    for (int k = 0; k < b; ++k) dmax = fmax(dmax, B[k * SJLD + k]);
-    const double piv_tol = (pass == 0 ? 1e-12 : 1e-3) * dmax;
-    for (int k = 0; k < b; ++k) {
-      const double d = B[k * SJLD + k];
-      if (!(d > piv_tol)) {  // uniform: every thread read the same value
-        chol = false;
-        break;
-      }
-      const double inv = 1.0 / sqrt(d);
-      const int m = b - k - 1;
-      for (int e = tid; e < m * m; e += SNT) {
-        const int i = k + 1 + e % m, j = k + 1 + e / m;
-        if (i <= j) B[i * SJLD + j] -= (B[k * SJLD + i] * inv) * (B[k * SJLD + j] * inv);
-      }
-      __syncthreads();
-      // row k of R (read by the substitution only; the next step reads row k + 1)
-      if (tid > k && tid < b) B[k * SJLD + tid] *= inv;
-      if (tid == k) B[k * SJLD + k] = d * inv;
+    for (int e = tid; e < b * b; e += SNT) {
+      const int l = e / b, cc = e % b;
+      double t = 0.0;
+      for (int r = 0; r < n; ++r) t += Ys[r * b + l] * Ys[r * b + cc];
+      B[l * SJLD + cc] = t;
     }
     __syncthreads();
-    if (!chol) break;
-    // X <- X R^{-1}: column j is final after dividing by R_jj; later columns subtract it.
-    for (int j = 0; j < b; ++j) {
-      const double rinv = 1.0 / B[j * SJLD + j];
-      const double rjc = (c > j && c < b) ? B[j * SJLD + c] : 0.0;
-#pragma unroll
-      for (int i = 0; i < SR; ++i) {
-        if (c == j) a[i] *= rinv;
-        a[i] -= col_bcast<BP>(a[i], j, h) * rjc;
+    if (g == 0) {
+      int ok = 1;
+      double dmax = 0.0;
+      for (int k = 0; k < b; ++k) dmax = fmax(dmax, B[k * SJLD + k]);
+      const double piv_tol = (pass == 0 ? 1e-12 : 1e-3) * dmax;
+      for (int k = 0; k < b; ++k) {
+        const double d = B[k * SJLD + k];
+        if (!(d > piv_tol)) {
+          ok = 0;
+          break;
+        }
+        const double inv = 1.0 / sqrt(d);
+        const int m = b - k - 1;
+        for (int e = lane; e < m * m; e += 64) {
+          const int i = k + 1 + e % m, j = k + 1 + e / m;
+          if (i <= j) B[i * SJLD + j] -= (B[k * SJLD + i] * inv) * (B[k * SJLD + j] * inv);
+        }
+        wave_sync();
+        for (int j = k + 1 + lane; j < b; j += 64) B[k * SJLD + j] *= inv;
+        if (lane == 0) B[k * SJLD + k] = d * inv;
+        wave_sync();
       }
+      if (ok && lane < b) {
+        // column `lane` of R^{-1} (upper): back substitution
+        for (int i = b - 1; i >= 0; --i) {
+          double t = (i == lane) ? 1.0 : 0.0;
+          for (int k = i + 1; k <= lane; ++k) t -= B[i * SJLD + k] * Ri[k * SJLD + lane];
+          Ri[i * SJLD + lane] = (i <= lane) ? t / B[i * SJLD + i] : 0.0;
+        }
+      }
+      if (lane == 0) bad = !ok;
+    }
+    __syncthreads();
+    if (bad) {
+      chol = false;
+      break;
+    }
+    // X <- X R^{-1}
+#pragma unroll
+    for (int i = 0; i < SR; ++i) {
+      const int r = (i * SW + g) * RS + h;
+      double t = 0.0;
+      if (r < n && c < b)
+        for (int l = 0; l <= c; ++l) t += Ys[r * b + l] * Ri[l * SJLD + c];
+      a[i] = t;
     }
     __syncthreads();
   }
@@ -496,26 +545,26 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
     }
     return;
   }
-  // ---- Householder QR (from the original block)
+  // ---- Householder QR
 #pragma unroll
   for (int i = 0; i < SR; ++i) {
     const int r = (i * SW + g) * RS + h;
     a[i] = (r < n && c < b) ? Y[(long)r * b + c] : 0.0;
   }
   {
-    double s = 0.0;
+    double t = 0.0;
 #pragma unroll
     for (int i = 0; i < SR; ++i) {
       const int r = (i * SW + g) * RS + h;
-      s += (c == 0 && r > 0) ? a[i] * a[i] : 0.0;
+      t += (c == 0 && r > 0) ? a[i] * a[i] : 0.0;
       if (c == 0 && r == 0) alpha_s = a[i];
     }
-    s = rowset_sum<BP>(s);
-    if (lane == 0) nrm[g] = s;
+    t = rowset_sum<BP>(t);
+    if (lane == 0) nrm[g] = t;
   }
   for (int j = 0; j < b; ++j) {
     __syncthreads();
-    if (tid == 0) {
+    if (c == j) {
       double sub = 0.0;
 #pragma unroll
       for (int w = 0; w < SW; ++w) sub += nrm[w];
@@ -527,25 +576,30 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
         tau = (beta - alpha) / beta;
         scale = 1.0 / (alpha - beta);
       }
-      scal[0] = tau;
-      scal[1] = scale;
-      scal[2] = beta;
-      taus[j] = tau;
+#pragma unroll
+      for (int i = 0; i < SR; ++i) {
+        const int r = (i * SW + g) * RS + h;
+        if (r < n) Ys[r * b + j] = r > j ? a[i] * scale : (r == j ? 1.0 : 0.0);
+      }
+      if (g == 0 && h == 0) {
+        taus[j] = tau;
+        scal[0] = tau;
+        scal[1] = beta;
+      }
     }
     __syncthreads();
-    const double tau = scal[0], scale = scal[1], beta = scal[2];
+    const double tau = scal[0], beta = scal[1];
     double vv[SR];
-    double s = 0.0;
+    double t = 0.0;
 #pragma unroll
     for (int i = 0; i < SR; ++i) {
       const int r = (i * SW + g) * RS + h;
-      const double xj = col_bcast<BP>(a[i], j, h);
-      vv[i] = r > j ? xj * scale : (r == j ? 1.0 : 0.0);
-      s += vv[i] * a[i];
+      vv[i] = r < n ? Ys[r * b + j] : 0.0;
+      t += vv[i] * a[i];
     }
-    s = rowset_sum<BP>(s);
+    t = rowset_sum<BP>(t);
     const int par = j & 1;
-    if (h == 0) red[par][g][c] = s;
+    if (h == 0) red[par][g][c] = t;
     __syncthreads();
     if (c > j && c < b && tau != 0.0) {
       double wsum = 0.0;
@@ -559,20 +613,19 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
 #pragma unroll
       for (int i = 0; i < SR; ++i) {
         const int r = (i * SW + g) * RS + h;
-        if (r > j) a[i] = vv[i];
-        else if (r == j) a[i] = beta;
+        if (r == j) a[i] = beta;
       }
     }
     // partial norm of column j + 1 below the diagonal, and its diagonal entry
-    double s2 = 0.0;
+    double t2 = 0.0;
 #pragma unroll
     for (int i = 0; i < SR; ++i) {
       const int r = (i * SW + g) * RS + h;
-      s2 += (c == j + 1 && r > j + 1) ? a[i] * a[i] : 0.0;
+      t2 += (c == j + 1 && r > j + 1) ? a[i] * a[i] : 0.0;
       if (c == j + 1 && r == j + 1) alpha_s = a[i];
     }
-    s2 = rowset_sum<BP>(s2);
-    if (lane == ((j + 1) % BP) && j + 1 < BP) nrm[g] = s2;
+    t2 = rowset_sum<BP>(t2);
+    if (lane == ((j + 1) % BP) && j + 1 < BP) nrm[g] = t2;
   }
   double m[SR];
 #pragma unroll
@@ -581,17 +634,16 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
   for (int j = b - 1; j >= 0; --j) {
     const double tau = taus[j];
     double vv[SR];
-    double s = 0.0;
+    double t = 0.0;
 #pragma unroll
     for (int i = 0; i < SR; ++i) {
       const int r = (i * SW + g) * RS + h;
-      const double xj = col_bcast<BP>(a[i], j, h);
-      vv[i] = r > j ? xj : (r == j ? 1.0 : 0.0);
-      s += vv[i] * m[i];
+      vv[i] = r < n ? Ys[r * b + j] : 0.0;
+      t += vv[i] * m[i];
     }
-    s = rowset_sum<BP>(s);
+    t = rowset_sum<BP>(t);
     const int par = j & 1;
-    if (h == 0) red[par][g][c] = s;
+    if (h == 0) red[par][g][c] = t;
     __syncthreads();
     if (c < b && tau != 0.0) {
       double wsum = 0.0;
@@ -609,23 +661,25 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
   }
 }
 
-// Rayleigh-Ritz: T = sym(Q^T Z), Jacobi(T) = S Th S^T; Uf = Q S (all b Ritz vectors,
-// descending, row-major), U = first k (column-major, canonical signs), w = Th[:k],
-// wall = Th, cut = Th[b-1], converged = 1 when every top-k Ritz residual
-// ||Z s_i - th_i Q s_i|| <= tol * |th_1|. Same lane packing as qr_kernel.
+// Rayleigh-Ritz from T = Q^T G Q (row-major b x b, formed with Z = G Q): Jacobi on
+// sym(T) = S Th S^T; Uf = Q S (all b Ritz vectors, descending, row-major), U = first k
+// (column-major, canonical signs), w = Th[:k], wall = Th, cut = Th[b-1], converged = 1
+// when every top-k Ritz residual ||Z s_i - th_i Q s_i|| <= tol * |th_1|. Q and then Z are
+// staged in LDS; thread (g, c, h) forms column c of the Ritz vectors on its rows.
 template <int BP, int SR>
 __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, const double* __restrict__ Z, long sQ,
-                                                  int n, int b, int k, double* __restrict__ Uf, double* __restrict__ U,
-                                                  long sU, double* __restrict__ w, long sw, double* __restrict__ wall,
+                                                  const double* __restrict__ T, int n, int b, int k,
+                                                  double* __restrict__ Uf, double* __restrict__ U, long sU,
+                                                  double* __restrict__ w, long sw, double* __restrict__ wall,
                                                   double* __restrict__ cut, double tol, int* __restrict__ converged,
                                                   int max_sweeps) {
   constexpr int RS = 64 / BP;
-  __shared__ double a0[SJ * SJLD];
+  __shared__ double a[SJ * SJLD];
   __shared__ double v[SJ * SJLD];
+  __shared__ double Ms[QR_LDS];
   __shared__ unsigned short sched[(SJ - 1) * (SJ / 2)];
   __shared__ int order[SJ];
   __shared__ double res[SJ];
-  __shared__ double bestv[SW][BP];
   __shared__ double bests[SW][BP];
   __shared__ int besti[SW][BP];
   __shared__ double sgn[SJ];
@@ -634,46 +688,19 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
   Q += bidx * sQ;
   Z += bidx * sQ;
   Uf += bidx * sQ;
+  T += (long)bidx * 64 * 64;
   U += bidx * sU;
   w += bidx * sw;
   wall += (long)bidx * SJ;
   const int np = (b + 1) & ~1;
-  double* a = a0;
   for (int e = tid; e < np * np; e += SNT) {
     const int i = e % np, j = e / np;
-    a[i * SJLD + j] = 0.0;
+    a[i * SJLD + j] = (i < b && j < b) ? 0.5 * (T[i * b + j] + T[j * b + i]) : 0.0;
     v[i * SJLD + j] = (i == j) ? 1.0 : 0.0;
   }
   if (tid < SJ) res[tid] = 0.0;
   jacobi_schedule(sched, np);
-  {
-    double qr_[SR], zr[SR];
-#pragma unroll
-    for (int i = 0; i < SR; ++i) {
-      const int r = (i * SW + g) * RS + h;
-      const bool ok = r < n && c < b;
-      qr_[i] = ok ? Q[(long)r * b + c] : 0.0;
-      zr[i] = ok ? Z[(long)r * b + c] : 0.0;
-    }
-    __syncthreads();
-    // T[l][c] += sum over this wave's rows of Q[r][l] Z[r][c]
-    for (int l = 0; l < b; ++l) {
-      double s = 0.0;
-#pragma unroll
-      for (int i = 0; i < SR; ++i) s += col_bcast<BP>(qr_[i], l, h) * zr[i];
-      s = rowset_sum<BP>(s);
-      if (h == 0 && c < b) atomicAdd(&a[l * SJLD + c], s);
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < b * b; e += SNT) {
-    const int i = e % b, j = e / b;
-    if (i < j) {
-      const double x = 0.5 * (a[i * SJLD + j] + a[j * SJLD + i]);
-      a[i * SJLD + j] = x;
-      a[j * SJLD + i] = x;
-    }
-  }
+  for (int e = tid; e < n * b; e += SNT) Ms[e] = Q[e];
   __syncthreads();
   if (np != b && tid == 0) {
     double m = 0.0;
@@ -687,39 +714,26 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
   const int src = c < b ? order[c] : 0;
   const double th = c < b ? a[src * SJLD + src] : 0.0;
   double u[SR];
-  {
-    double qr_[SR];
 #pragma unroll
-    for (int i = 0; i < SR; ++i) {
-      const int r = (i * SW + g) * RS + h;
-      qr_[i] = (r < n && c < b) ? Q[(long)r * b + c] : 0.0;
-      u[i] = 0.0;
-    }
-    for (int l = 0; l < b; ++l) {
-      const double sl = v[l * SJLD + src];
-#pragma unroll
-      for (int i = 0; i < SR; ++i) u[i] += col_bcast<BP>(qr_[i], l, h) * sl;
-    }
+  for (int i = 0; i < SR; ++i) {
+    const int r = (i * SW + g) * RS + h;
+    double t = 0.0;
+    if (r < n && c < b)
+      for (int l = 0; l < b; ++l) t += Ms[r * b + l] * v[l * SJLD + src];
+    u[i] = t;
   }
+  __syncthreads();
+  for (int e = tid; e < n * b; e += SNT) Ms[e] = Z[e];
+  __syncthreads();
   double rsum = 0.0;
-  {
-    double zr[SR], gu[SR];
 #pragma unroll
-    for (int i = 0; i < SR; ++i) {
-      const int r = (i * SW + g) * RS + h;
-      zr[i] = (r < n && c < b) ? Z[(long)r * b + c] : 0.0;
-      gu[i] = 0.0;
-    }
-    for (int l = 0; l < b; ++l) {
-      const double sl = v[l * SJLD + src];
-#pragma unroll
-      for (int i = 0; i < SR; ++i) gu[i] += col_bcast<BP>(zr[i], l, h) * sl;
-    }
-#pragma unroll
-    for (int i = 0; i < SR; ++i) {
-      const double d = gu[i] - th * u[i];
-      rsum += d * d;
-    }
+  for (int i = 0; i < SR; ++i) {
+    const int r = (i * SW + g) * RS + h;
+    double t = 0.0;
+    if (r < n && c < b)
+      for (int l = 0; l < b; ++l) t += Ms[r * b + l] * v[l * SJLD + src];
+    const double d = t - th * u[i];
+    rsum += d * d;
   }
   rsum = rowset_sum<BP>(rsum);
   // largest-magnitude entry of this thread's rows (first occurrence), then Uf
@@ -742,7 +756,6 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
     if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bs = os; bi = oi; }
   }
   if (h == 0) {
-    bestv[g][c] = bv;
     bests[g][c] = bs;
     besti[g][c] = bi;
     if (c < b) atomicAdd(&res[c], rsum);
@@ -752,7 +765,7 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
     double best = -1.0, sb = 0.0;
     int ib = 0;
     for (int ww = 0; ww < SW; ++ww) {
-      const double x = bestv[ww][tid];
+      const double x = fabs(bests[ww][tid]);  // -1 sentinel rows only when n < 1
       const int ix = besti[ww][tid];
       if (x > best || (x == best && ix < ib)) { best = x; ib = ix; sb = bests[ww][tid]; }
     }
@@ -788,47 +801,43 @@ int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* 
 }
 
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
-                 int mode, const double* cut, int batch) {
+                 int mode, const double* cut, double* T, int batch) {
   if (n < 1 || n > 256 || b < 1 || b > 64 || q < 1 || mode < 0 || mode > 2 || batch < 1) return 8;
   if (mode == 2 && !cut) return 8;
+  if (T && (mode != 1 || q != 1)) return 8;
   hipLaunchKernelGGL(power_kernel, dim3((b + PW_COLS - 1) / PW_COLS, batch), dim3(1024), 0, st, G, sG, W, Y, sWY, n,
-                     b, q, mode, cut);
+                     b, q, mode, cut, T);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
 // Launches pick the smallest lane packing (BP, SR) with b <= BP and
 // n <= 16 * (64 / BP) * SR: <16,2>, <32,4>, <32,8>, <64,8>, <64,16>.
-int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int batch) {
-  if (n < 1 || n > SW * SR || b < 1 || b > 64 || b > n || batch < 1) return 8;
+int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int try_chol, int batch) {
+  if (n < 1 || n > SW * SR || b < 1 || b > 64 || b > n || n * b > QR_LDS || batch < 1) return 8;
   const dim3 grid(batch), block(SNT);
-  if (b <= 16 && n <= 128) hipLaunchKernelGGL((qr_kernel<16, 2>), grid, block, 0, st, Y, Q, sYQ, n, b);
-  else if (b <= 32 && n <= 128) hipLaunchKernelGGL((qr_kernel<32, 4>), grid, block, 0, st, Y, Q, sYQ, n, b);
-  else if (b <= 32) hipLaunchKernelGGL((qr_kernel<32, 8>), grid, block, 0, st, Y, Q, sYQ, n, b);
-  else if (n <= 128) hipLaunchKernelGGL((qr_kernel<64, 8>), grid, block, 0, st, Y, Q, sYQ, n, b);
-  else hipLaunchKernelGGL((qr_kernel<64, 16>), grid, block, 0, st, Y, Q, sYQ, n, b);
+  if (b <= 16 && n <= 128) hipLaunchKernelGGL((qr_kernel<16, 2>), grid, block, 0, st, Y, Q, sYQ, n, b, try_chol);
+  else if (b <= 32 && n <= 128) hipLaunchKernelGGL((qr_kernel<32, 4>), grid, block, 0, st, Y, Q, sYQ, n, b, try_chol);
+  else if (b <= 32) hipLaunchKernelGGL((qr_kernel<32, 8>), grid, block, 0, st, Y, Q, sYQ, n, b, try_chol);
+  else if (n <= 128) hipLaunchKernelGGL((qr_kernel<64, 8>), grid, block, 0, st, Y, Q, sYQ, n, b, try_chol);
+  else hipLaunchKernelGGL((qr_kernel<64, 16>), grid, block, 0, st, Y, Q, sYQ, n, b, try_chol);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
-int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, int n, int b, int k, double* Uf, double* U,
-              long sU, double* w, long sw, double* wall, double* cut, double tol, int* converged, int batch) {
-  if (n < 1 || n > SW * SR || b < 1 || b > SJ || b > n || k < 1 || k > b || batch < 1) return 8;
+int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, const double* T, int n, int b, int k,
+              double* Uf, double* U, long sU, double* w, long sw, double* wall, double* cut, double tol,
+              int* converged, int batch) {
+  if (n < 1 || n > SW * SR || b < 1 || b > SJ || b > n || n * b > QR_LDS || k < 1 || k > b || batch < 1) return 8;
   const dim3 grid(batch), block(SNT);
   const int sweeps = 20;
-  if (b <= 16 && n <= 128)
-    hipLaunchKernelGGL((rr_kernel<16, 2>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
-                       converged, sweeps);
-  else if (b <= 32 && n <= 128)
-    hipLaunchKernelGGL((rr_kernel<32, 4>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
-                       converged, sweeps);
-  else if (b <= 32)
-    hipLaunchKernelGGL((rr_kernel<32, 8>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
-                       converged, sweeps);
-  else if (n <= 128)
-    hipLaunchKernelGGL((rr_kernel<64, 8>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
-                       converged, sweeps);
-  else
-    hipLaunchKernelGGL((rr_kernel<64, 16>), grid, block, 0, st, Q, Z, sQ, n, b, k, Uf, U, sU, w, sw, wall, cut, tol,
-                       converged, sweeps);
+#define HT_RR(BPv, SRv)                                                                                          \
+  hipLaunchKernelGGL((rr_kernel<BPv, SRv>), grid, block, 0, st, Q, Z, sQ, T, n, b, k, Uf, U, sU, w, sw, wall, cut, \
+                     tol, converged, sweeps)
+  if (b <= 16 && n <= 128) HT_RR(16, 2);
+  else if (b <= 32 && n <= 128) HT_RR(32, 4);
+  else if (b <= 32) HT_RR(32, 8);
+  else if (n <= 128) HT_RR(64, 8);
+  else HT_RR(64, 16);
+#undef HT_RR
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
