@@ -40,14 +40,17 @@ def parse():
 
 def read_traffic(path, kernel):
     """HBM bytes per launch of `kernel` from a committed rocprofv3 PMC summary (lines
-    'kernel FETCH_KiB WRITE_KiB scale'), or None."""
+    'kernel FETCH_KiB WRITE_KiB fetch_scale write_scale', the scales calibrated against a
+    copy kernel of known size), or None."""
     try:
         with open(path) as f:
             for line in f:
+                if line.startswith("#"):
+                    continue
                 p = line.split()
-                if len(p) >= 4 and p[0] == kernel:
-                    return (float(p[1]) + float(p[2])) * 1024.0 * float(p[3])
-    except OSError:
+                if len(p) >= 5 and p[0] == kernel:
+                    return round(float(p[1]) * 1024.0 * float(p[3]) + float(p[2]) * 1024.0 * float(p[4]))
+    except (OSError, ValueError):
         return None
     return None
 

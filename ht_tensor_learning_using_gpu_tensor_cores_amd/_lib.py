@@ -82,6 +82,7 @@ SIGNATURES = {
     "ht_philox_normals_host": (None, [U64, U32, I64, PD]),
     "ht_profile_enable": (I32, [H, I32]),
     "ht_profile_read": (I32, [H, PD, PI64, I32, I32]),
+    "ht_debug_copy": (I32, [H, P, P, I64]),
 }
 
 PROFILE_KERNELS = ["slab_gram", "slab_proj", "mode_gram", "residual", "skinny_gemm", "tsqr", "small_eig",

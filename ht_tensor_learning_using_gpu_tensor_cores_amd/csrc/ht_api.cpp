@@ -513,6 +513,12 @@ ht_status ht_profile_read(ht_handle_t h, double* ms, int64_t* launches, int32_t 
 }
 
 // ------------------------------------------------------------------ component entry points
+ht_status ht_debug_copy(ht_handle_t h, const float* src, float* dst, int64_t n) {
+  if (!h || n < 0 || (n > 0 && (!src || !dst))) return HT_ERR_INVALID_ARG;
+  CKS(copy_f32_launch(h->stream, src, dst, n));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
 ht_status ht_debug_mfma_probe(ht_handle_t h, const float* a, const float* b, float* o_bf16_32, float* o_f32_32,
                               float* o_f32_16, double* o_f64_16, float* o_bf16_16) {
   if (!h) return HT_ERR_INVALID_ARG;

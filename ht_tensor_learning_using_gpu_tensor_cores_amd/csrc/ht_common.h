@@ -200,6 +200,7 @@ int tsqr_launch(hipStream_t st, float* W, long m, int b, float* Q, double* R, vo
 // Generators.
 int gen_noise_sumsq_launch(hipStream_t st, long n, uint64_t seed, uint32_t stream, double* acc, int* nb);
 int gen_add_noise_launch(hipStream_t st, float* x, long n, uint64_t seed, uint32_t stream, double scale);
+int copy_f32_launch(hipStream_t st, const float* src, float* dst, long n);
 int gen_bumps_launch(hipStream_t st, float* x, int order, long n, const float* factors, const float* amp,
                      int n_bumps);
 

@@ -67,4 +67,28 @@ int gen_bumps_launch(hipStream_t st, float* x, int order, long n, const float* f
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
+// Plain streaming copy (float4 when both pointers are 16-byte aligned): the reference
+// kernel that calibrates hardware byte counters against a known traffic of 2 n floats.
+__global__ void copy_f32_kernel(const float* __restrict__ src, float* __restrict__ dst, long n, int vec) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (vec) {
+    const long n4 = n / 4;
+    for (long i = t0; i < n4; i += stride)
+      reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(src)[i];
+    for (long i = 4 * n4 + t0; i < n; i += stride) dst[i] = src[i];
+  } else {
+    for (long i = t0; i < n; i += stride) dst[i] = src[i];
+  }
+}
+int copy_f32_launch(hipStream_t st, const float* src, float* dst, long n) {
+  if (n <= 0) return 0;
+  const int vec = ((((uintptr_t)src) | ((uintptr_t)dst)) & 15) == 0;
+  const long work = vec ? (n + 3) / 4 : n;
+  long blocks = (work + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(copy_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, n, vec);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
 }  // namespace ht

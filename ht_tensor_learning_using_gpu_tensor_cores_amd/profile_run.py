@@ -1,0 +1,72 @@
+"""Profiling driver: a fixed number of decompositions of one named workload, optionally
+followed by one calibration copy of the tensor, so that rocprofv3 can attribute kernel
+time (--kernel-trace --stats) and HBM counters (--pmc FETCH_SIZE / WRITE_SIZE) to the
+engine's kernels. The copy kernel (copy_f32_kernel) moves exactly 8 |X| / 4 bytes, which
+calibrates the counters.
+
+  python3 -m ht_tensor_learning_using_gpu_tensor_cores_amd.profile_run --workload c2 --reps 3 --copy
+
+Workloads (generated on the device):
+  c1  3rd order 128^3, Tucker core 8^3 + 1e-3 noise, seed 0, tree {1},{2,3}, ranks 8, plain FP32
+  c2  4th order 128^4, Tucker core 16^4 + 1e-4 noise, seed 1, tree {1,2},{3,4}, ranks 16,
+      leaves-to-root BF16 (the benchmark workload)
+  c3  3rd order 1024^3, core 32^3 with mode spectra k^-1.5 + 1e-5 noise, seed 2, tree
+      {1},{2,3}, ranks 32, randomised (TSQR) BF16
+"""
+import argparse
+import time
+
+
+def workload(name):
+    from . import tree as T
+    if name == "c1":
+        return dict(dims=[128] * 3, gen=("tucker", [8] * 3, 0, 1e-3), tree=T.first_vs_rest_3d(), ranks=[1] + [8] * 4,
+                     method="plain", precision="fp32")
+    if name == "c2":
+        return dict(dims=[128] * 4, gen=("tucker", [16] * 4, 1, 1e-4), tree=T.balanced_4d(), ranks=[1] + [16] * 6,
+                    method="ltr", precision="bf16")
+    if name == "c3":
+        return dict(dims=[1024] * 3, gen=("cp_decay", 32, 2, 1e-5), tree=T.first_vs_rest_3d(), ranks=[1] + [32] * 4,
+                    method="rsvd", precision="bf16")
+    raise ValueError(f"unknown workload {name}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--method", default=None)
+    ap.add_argument("--precision", default=None)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--copy", action="store_true", help="one calibration copy of the tensor after the runs")
+    args = ap.parse_args()
+    import torch
+    from . import engine as E
+    from . import _lib
+    w = workload(args.workload)
+    method = args.method or w["method"]
+    prec = args.precision or w["precision"]
+    h = E.default_handle(0)
+    kind = w["gen"][0]
+    if kind == "tucker":
+        _, core, seed, noise = w["gen"]
+        x = E.generate_tucker(w["dims"], core, seed=seed, noise_rel=noise, handle=h)
+    else:
+        _, core, seed, noise = w["gen"]
+        x = E.generate_cp_decay(w["dims"], core, seed=seed, noise_rel=noise, handle=h)
+    torch.cuda.synchronize()
+    for i in range(args.reps + 1):
+        t0 = time.perf_counter()
+        r = E.decompose(x, w["dims"], w["tree"], w["ranks"], method=method, precision=prec, handle=h)
+        torch.cuda.synchronize()
+        print(f"{args.workload} {method}/{prec} rep {i}: {(time.perf_counter() - t0) * 1e3:.3f} ms, "
+              f"rel_err {r.rel_err:.10f}", flush=True)
+    if args.copy:
+        lib = _lib.load()
+        y = torch.empty_like(x)
+        st = lib.ht_debug_copy(h.ptr, x.data_ptr(), y.data_ptr(), x.numel())
+        torch.cuda.synchronize()
+        print(f"calibration copy of {x.numel() * 4} bytes: status {st}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

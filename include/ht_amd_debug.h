@@ -64,6 +64,11 @@ ht_status ht_skinny_gemm(ht_handle_t h, const float* A, int64_t M, int64_t K, in
 ht_status ht_philox_normals(ht_handle_t h, uint64_t seed, uint32_t stream, int64_t n, double* out_dev);
 void ht_philox_normals_host(uint64_t seed, uint32_t stream, int64_t n, double* out);
 
+/* dst[0..n) = src[0..n) by a plain streaming kernel (copy_f32_kernel): the known-traffic
+ * reference (8 n bytes) against which rocprofv3 FETCH_SIZE / WRITE_SIZE counts of the
+ * engine's kernels are scaled. No counterpart in the reference. */
+ht_status ht_debug_copy(ht_handle_t h, const float* src, float* dst, int64_t n);
+
 /* Per-kernel device timing. When enabled, every decomposition records HIP events on
  * the handle's stream around its main kernels and accumulates their durations. Kernel
  * ids: 0 slab Gram, 1 slab projection, 2 mode Gram, 3 residual, 4 skinny GEMM, 5 TSQR,
