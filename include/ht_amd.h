@@ -82,7 +82,7 @@ typedef struct {
   int32_t precision;       /* ht_precision of the large GEMMs (plain always FP32)        */
   int32_t oversample;      /* p for HT_METHOD_RSVD (default 8)                           */
   int32_t power_iters;     /* q for HT_METHOD_RSVD (default 1)                           */
-  int32_t err_mode;        /* ht_err_mode (default HT_ERR_MODE_IDENTITY)                 */
+  int32_t err_mode;        /* ht_err_mode (default HT_ERR_MODE_RESIDUAL)                 */
   int32_t max_eig_iters;   /* cap on subspace iterations of the exact solvers (200)      */
   double eig_tol;          /* relative Ritz-value tolerance of the exact solvers (1e-10) */
   uint64_t seed;           /* seed of the Philox test matrices (1234)                    */
