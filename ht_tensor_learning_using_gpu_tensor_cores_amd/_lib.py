@@ -64,6 +64,10 @@ SIGNATURES = {
     "ht_residual": (I32, [H, P, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, I32, P, PD]),
     "ht_reconstruct": (I32, [H, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, I32, P, P]),
     "ht_generate_tucker": (I32, [H, I32, PI64, PI32, U64, D, P]),
+    "ht_batch_output_layout": (I32, [I32, PI64, PI32, PI64, PI64]),
+    "ht_decompose_batched": (I32, [H, P, I32, PI64, PI32, PI64, P, PI64, PD, ctypes.POINTER(Stats)]),
+    "ht_batch_shapes": (I32, [I32, U64, I32, I32, I32, I32, PI64, PI32]),
+    "ht_generate_batch": (I32, [H, I32, PI64, PI32, U64, D, P, PI64]),
     "ht_generate_cp_decay": (I32, [H, I32, PI64, I32, D, U64, D, P]),
     "ht_generate_bumps": (I32, [H, I32, I64, I32, U64, I64, P]),
     "htd": (ctypes.c_float, [P, ctypes.c_long, ctypes.c_long, ctypes.c_long, PI32, ctypes.c_float, ctypes.c_float]),

@@ -18,6 +18,111 @@ using namespace ht;
     if (_s) return (ht_status)_s; \
   } while (0)
 
+// ------------------------------------------------------------------ generators
+// Q of the QR factorisation of the n x r Philox Gaussian (seed, stream), diag(R) > 0,
+// by Gram-Schmidt with reorthogonalisation in FP64 (the Q factor with positive diag(R)
+// is unique, so it matches any stable QR up to rounding).
+std::vector<double> ht::orthonormal_factor(uint64_t seed, uint32_t stream, long n, int r) {
+  std::vector<double> A((size_t)n * r);
+  for (long i = 0; i < n * r; ++i) A[i] = philox_normal(seed, stream, (uint64_t)i);
+  for (int j = 0; j < r; ++j) {
+    double* a = &A[(size_t)n * j];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = 0; i < j; ++i) {
+        const double* q = &A[(size_t)n * i];
+        double d = 0.0;
+        for (long t = 0; t < n; ++t) d += q[t] * a[t];
+        for (long t = 0; t < n; ++t) a[t] -= d * q[t];
+      }
+    double nr = 0.0;
+    for (long t = 0; t < n; ++t) nr += a[t] * a[t];
+    nr = std::sqrt(nr);
+    for (long t = 0; t < n; ++t) a[t] /= nr;
+  }
+  return A;
+}
+
+// x = core x_n U_n (+ noise) with the modes split into [0, m) and [m, d): X_(left) = L R^T,
+// L = U_{m-1} (x) ... (x) U_0, R = (core x_{n>=m} U_n)_(left)^T.
+ht_status ht::generate_from_core(ht_handle_t h, int order, const int64_t* dims, const int32_t* core_dims,
+                             std::vector<double> core, std::vector<std::vector<double>> U, uint64_t seed,
+                             double noise_rel, float* x_dev, uint32_t noise_stream) {
+  const int m = (order + 1) / 2;
+  long nl = 1, kl = 1, nr = 1;
+  for (int k = 0; k < m; ++k) { nl *= dims[k]; kl *= core_dims[k]; }
+  for (int k = m; k < order; ++k) nr *= dims[k];
+  if (kl > 4096) return HT_ERR_UNSUPPORTED;
+  // L (nl x kl)
+  std::vector<float> L((size_t)nl * kl);
+  for (long col = 0; col < kl; ++col)
+    for (long row = 0; row < nl; ++row) {
+      long rr = row, cc = col;
+      double v = 1.0;
+      for (int k = 0; k < m; ++k) {
+        const long i = rr % dims[k], j = cc % core_dims[k];
+        rr /= dims[k];
+        cc /= core_dims[k];
+        v *= U[k][i + dims[k] * j];
+      }
+      L[row + nl * col] = (float)v;
+    }
+  // T = core x_{k>=m} U_k, modes (core_dims[0..m), dims[m..d)).
+  std::vector<long> td(order);
+  for (int k = 0; k < order; ++k) td[k] = core_dims[k];
+  std::vector<double> T = std::move(core);
+  for (int k = m; k < order; ++k) {
+    long P = 1, S = 1;
+    for (int q = 0; q < k; ++q) P *= td[q];
+    for (int q = k + 1; q < order; ++q) S *= td[q];
+    const long rk = td[k], nk = dims[k];
+    std::vector<double> T2((size_t)P * nk * S, 0.0);
+    for (long s = 0; s < S; ++s)
+      for (long j = 0; j < rk; ++j)
+        for (long i = 0; i < nk; ++i) {
+          const double u = U[k][i + nk * j];
+          const double* src = &T[(size_t)P * (j + rk * s)];
+          double* dst = &T2[(size_t)P * (i + nk * s)];
+          for (long p = 0; p < P; ++p) dst[p] += u * src[p];
+        }
+    T.swap(T2);
+    td[k] = nk;
+  }
+  // R[i_right, j_left] = T[j_left + kl * i_right]
+  std::vector<float> R((size_t)nr * kl);
+  for (long i = 0; i < nr; ++i)
+    for (long j = 0; j < kl; ++j) R[i + nr * j] = (float)T[j + kl * i];
+  float *Ld = nullptr, *Rd = nullptr;
+  double* acc = nullptr;
+  const size_t accn = 2048;
+  if (hipMalloc((void**)&Ld, L.size() * 4) != hipSuccess) return HT_ERR_ALLOC;
+  if (hipMalloc((void**)&Rd, R.size() * 4) != hipSuccess) { (void)hipFree(Ld); return HT_ERR_ALLOC; }
+  if (hipMalloc((void**)&acc, (accn + 8192) * 8) != hipSuccess) { (void)hipFree(Ld); (void)hipFree(Rd); return HT_ERR_ALLOC; }
+  ht_status st = HT_OK;
+  do {
+    if (hipMemcpy(Ld, L.data(), L.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { st = HT_ERR_HIP; break; }
+    if (hipMemcpy(Rd, R.data(), R.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { st = HT_ERR_HIP; break; }
+    if (lowrank_launch(h->stream, 0, Ld, Rd, nl, nr, (int)kl, nullptr, x_dev, nullptr, nullptr)) { st = HT_ERR_HIP; break; }
+    if (noise_rel > 0.0) {
+      if (sumsq_f32_launch(h->stream, x_dev, nl * nr, acc)) { st = HT_ERR_HIP; break; }
+      int nb = 0;
+      if (gen_noise_sumsq_launch(h->stream, nl * nr, seed, noise_stream, acc + 2048, &nb)) { st = HT_ERR_HIP; break; }
+      if (gram_reduce_norms_launch(h->stream, acc + 2048, nb, acc + 2047)) { st = HT_ERR_HIP; break; }
+      double hv[2];
+      if (hipMemcpyAsync(&hv[0], acc, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
+      if (hipMemcpyAsync(&hv[1], acc + 2047, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
+      if (hipStreamSynchronize(h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
+      const double scale = noise_rel * std::sqrt(hv[0]) / std::sqrt(hv[1]);
+      if (gen_add_noise_launch(h->stream, x_dev, nl * nr, seed, noise_stream, scale)) { st = HT_ERR_HIP; break; }
+    }
+    if (hipStreamSynchronize(h->stream) != hipSuccess) st = HT_ERR_HIP;
+  } while (0);
+  (void)hipFree(Ld);
+  (void)hipFree(Rd);
+  (void)hipFree(acc);
+  return st;
+}
+
+
 extern "C" {
 
 const char* ht_version(void) { return "ht_amd 0.1 (gfx950)"; }
@@ -63,6 +168,8 @@ ht_status ht_destroy(ht_handle_t h) {
   if (!h) return HT_ERR_INVALID_ARG;
   (void)hipSetDevice(h->device);
   (void)hipStreamSynchronize(h->stream);
+  if (h->bexec) (void)hipGraphExecDestroy(h->bexec);
+  if (h->bgraph) (void)hipGraphDestroy(h->bgraph);
   h->arena.release();
   h->prof.destroy();
   if (h->xbuf) (void)hipFree(h->xbuf);
@@ -244,110 +351,6 @@ ht_status ht_reconstruct(ht_handle_t h, int32_t order, const int64_t* dims, int3
   return HT_OK;
 }
 
-// ------------------------------------------------------------------ generators
-// Q of the QR factorisation of the n x r Philox Gaussian (seed, stream), diag(R) > 0,
-// by Gram-Schmidt with reorthogonalisation in FP64 (the Q factor with positive diag(R)
-// is unique, so it matches any stable QR up to rounding).
-static std::vector<double> orthonormal_factor(uint64_t seed, uint32_t stream, long n, int r) {
-  std::vector<double> A((size_t)n * r);
-  for (long i = 0; i < n * r; ++i) A[i] = philox_normal(seed, stream, (uint64_t)i);
-  for (int j = 0; j < r; ++j) {
-    double* a = &A[(size_t)n * j];
-    for (int pass = 0; pass < 2; ++pass)
-      for (int i = 0; i < j; ++i) {
-        const double* q = &A[(size_t)n * i];
-        double d = 0.0;
-        for (long t = 0; t < n; ++t) d += q[t] * a[t];
-        for (long t = 0; t < n; ++t) a[t] -= d * q[t];
-      }
-    double nr = 0.0;
-    for (long t = 0; t < n; ++t) nr += a[t] * a[t];
-    nr = std::sqrt(nr);
-    for (long t = 0; t < n; ++t) a[t] /= nr;
-  }
-  return A;
-}
-
-// x = core x_n U_n (+ noise) with the modes split into [0, m) and [m, d): X_(left) = L R^T,
-// L = U_{m-1} (x) ... (x) U_0, R = (core x_{n>=m} U_n)_(left)^T.
-static ht_status generate_from_core(ht_handle_t h, int order, const int64_t* dims, const int32_t* core_dims,
-                                    std::vector<double> core, std::vector<std::vector<double>> U, uint64_t seed,
-                                    double noise_rel, float* x_dev) {
-  const int m = (order + 1) / 2;
-  long nl = 1, kl = 1, nr = 1;
-  for (int k = 0; k < m; ++k) { nl *= dims[k]; kl *= core_dims[k]; }
-  for (int k = m; k < order; ++k) nr *= dims[k];
-  if (kl > 4096) return HT_ERR_UNSUPPORTED;
-  // L (nl x kl)
-  std::vector<float> L((size_t)nl * kl);
-  for (long col = 0; col < kl; ++col)
-    for (long row = 0; row < nl; ++row) {
-      long rr = row, cc = col;
-      double v = 1.0;
-      for (int k = 0; k < m; ++k) {
-        const long i = rr % dims[k], j = cc % core_dims[k];
-        rr /= dims[k];
-        cc /= core_dims[k];
-        v *= U[k][i + dims[k] * j];
-      }
-      L[row + nl * col] = (float)v;
-    }
-  // T = core x_{k>=m} U_k, modes (core_dims[0..m), dims[m..d)).
-  std::vector<long> td(order);
-  for (int k = 0; k < order; ++k) td[k] = core_dims[k];
-  std::vector<double> T = std::move(core);
-  for (int k = m; k < order; ++k) {
-    long P = 1, S = 1;
-    for (int q = 0; q < k; ++q) P *= td[q];
-    for (int q = k + 1; q < order; ++q) S *= td[q];
-    const long rk = td[k], nk = dims[k];
-    std::vector<double> T2((size_t)P * nk * S, 0.0);
-    for (long s = 0; s < S; ++s)
-      for (long j = 0; j < rk; ++j)
-        for (long i = 0; i < nk; ++i) {
-          const double u = U[k][i + nk * j];
-          const double* src = &T[(size_t)P * (j + rk * s)];
-          double* dst = &T2[(size_t)P * (i + nk * s)];
-          for (long p = 0; p < P; ++p) dst[p] += u * src[p];
-        }
-    T.swap(T2);
-    td[k] = nk;
-  }
-  // R[i_right, j_left] = T[j_left + kl * i_right]
-  std::vector<float> R((size_t)nr * kl);
-  for (long i = 0; i < nr; ++i)
-    for (long j = 0; j < kl; ++j) R[i + nr * j] = (float)T[j + kl * i];
-  float *Ld = nullptr, *Rd = nullptr;
-  double* acc = nullptr;
-  const size_t accn = 2048;
-  if (hipMalloc((void**)&Ld, L.size() * 4) != hipSuccess) return HT_ERR_ALLOC;
-  if (hipMalloc((void**)&Rd, R.size() * 4) != hipSuccess) { (void)hipFree(Ld); return HT_ERR_ALLOC; }
-  if (hipMalloc((void**)&acc, (accn + 8192) * 8) != hipSuccess) { (void)hipFree(Ld); (void)hipFree(Rd); return HT_ERR_ALLOC; }
-  ht_status st = HT_OK;
-  do {
-    if (hipMemcpy(Ld, L.data(), L.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { st = HT_ERR_HIP; break; }
-    if (hipMemcpy(Rd, R.data(), R.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { st = HT_ERR_HIP; break; }
-    if (lowrank_launch(h->stream, 0, Ld, Rd, nl, nr, (int)kl, nullptr, x_dev, nullptr, nullptr)) { st = HT_ERR_HIP; break; }
-    if (noise_rel > 0.0) {
-      if (sumsq_f32_launch(h->stream, x_dev, nl * nr, acc)) { st = HT_ERR_HIP; break; }
-      int nb = 0;
-      if (gen_noise_sumsq_launch(h->stream, nl * nr, seed, 64, acc + 2048, &nb)) { st = HT_ERR_HIP; break; }
-      if (gram_reduce_norms_launch(h->stream, acc + 2048, nb, acc + 2047)) { st = HT_ERR_HIP; break; }
-      double hv[2];
-      if (hipMemcpyAsync(&hv[0], acc, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
-      if (hipMemcpyAsync(&hv[1], acc + 2047, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
-      if (hipStreamSynchronize(h->stream) != hipSuccess) { st = HT_ERR_HIP; break; }
-      const double scale = noise_rel * std::sqrt(hv[0]) / std::sqrt(hv[1]);
-      if (gen_add_noise_launch(h->stream, x_dev, nl * nr, seed, 64, scale)) { st = HT_ERR_HIP; break; }
-    }
-    if (hipStreamSynchronize(h->stream) != hipSuccess) st = HT_ERR_HIP;
-  } while (0);
-  (void)hipFree(Ld);
-  (void)hipFree(Rd);
-  (void)hipFree(acc);
-  return st;
-}
-
 ht_status ht_generate_tucker(ht_handle_t h, int32_t order, const int64_t* dims, const int32_t* core_dims,
                              uint64_t seed, double noise_rel, float* x_dev) {
   if (!h || !dims || !core_dims || !x_dev || order < 2 || order > HT_MAX_ORDER || noise_rel < 0) return HT_ERR_INVALID_ARG;
@@ -361,7 +364,7 @@ ht_status ht_generate_tucker(ht_handle_t h, int32_t order, const int64_t* dims, 
   for (long i = 0; i < nc; ++i) core[i] = philox_normal(seed, 1, (uint64_t)i);
   std::vector<std::vector<double>> U(order);
   for (int k = 0; k < order; ++k) U[k] = orthonormal_factor(seed, 16 + k, dims[k], core_dims[k]);
-  return generate_from_core(h, order, dims, core_dims, std::move(core), std::move(U), seed, noise_rel, x_dev);
+  return generate_from_core(h, order, dims, core_dims, std::move(core), std::move(U), seed, noise_rel, x_dev, 64);
 }
 
 ht_status ht_generate_cp_decay(ht_handle_t h, int32_t order, const int64_t* dims, int32_t rank, double exponent,
@@ -380,7 +383,7 @@ ht_status ht_generate_cp_decay(ht_handle_t h, int32_t order, const int64_t* dims
   for (int k = 0; k < rank; ++k) core[(size_t)k * stride] = std::pow(k + 1.0, -exponent);
   std::vector<std::vector<double>> U(order);
   for (int k = 0; k < order; ++k) U[k] = orthonormal_factor(seed, 16 + k, dims[k], rank);
-  return generate_from_core(h, order, dims, cd.data(), std::move(core), std::move(U), seed, noise_rel, x_dev);
+  return generate_from_core(h, order, dims, cd.data(), std::move(core), std::move(U), seed, noise_rel, x_dev, 64);
 }
 
 ht_status ht_generate_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t n_bumps, uint64_t seed, int64_t sample,

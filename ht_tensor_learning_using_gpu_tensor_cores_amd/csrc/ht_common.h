@@ -201,6 +201,19 @@ int tsqr_launch(hipStream_t st, float* W, long m, int b, float* Q, double* R, vo
 int gen_noise_sumsq_launch(hipStream_t st, long n, uint64_t seed, uint32_t stream, double* acc, int* nb);
 int gen_add_noise_launch(hipStream_t st, float* x, long n, uint64_t seed, uint32_t stream, double scale);
 int copy_f32_launch(hipStream_t st, const float* src, float* dst, long n);
+// Batched HT (k_batch.hip): one descriptor per tensor, prepared on the host.
+struct BatchItem {
+  long x_off;     // element offset of the tensor in x
+  long out_off;   // float offset of its packed output block
+  int n[3];       // mode sizes
+  int k[5];       // heap ranks (k[0] = 1)
+  int u_off[5];   // offsets within the block (floats) of U_t (leaves 1, 3, 4), else -1
+  int b_off[5];   // offsets of B_t (interior 0, 2), else -1
+  int s_off[5];   // offsets of the singular values of nodes 1..4
+};
+size_t batch_scratch_bytes(int blocks);
+int batch_ht_launch(hipStream_t st, const float* x, const void* items, const int* order, int count, float* out,
+                    double* rel_err, double* scratch, int blocks, int kc);
 int gen_bumps_launch(hipStream_t st, float* x, int order, long n, const float* factors, const float* amp,
                      int n_bumps);
 

@@ -1,5 +1,6 @@
 // Internal host-side types of the HT engine.
 #pragma once
+#include <map>
 #include <utility>
 #include <vector>
 
@@ -113,4 +114,18 @@ struct ht_handle_s {
   float* obuf = nullptr;   // device output staging for host outputs
   size_t obuf_bytes = 0;
   ht::Profiler prof;
+  // batched entry point: autotuned launch shape per (count, largest mode size) and the
+  // graph of the last launch with the arguments it was captured for
+  std::map<std::pair<int, int>, std::pair<int, int>> batch_tune;
+  hipGraph_t bgraph = nullptr;
+  hipGraphExec_t bexec = nullptr;
+  std::vector<const void*> bkey;
 };
+
+namespace ht {
+// Generators shared by the entry points (ht_api.cpp).
+std::vector<double> orthonormal_factor(uint64_t seed, uint32_t stream, long n, int r);
+ht_status generate_from_core(ht_handle_t h, int order, const int64_t* dims, const int32_t* core_dims,
+                             std::vector<double> core, std::vector<std::vector<double>> U, uint64_t seed,
+                             double noise_rel, float* x_dev, uint32_t noise_stream);
+}  // namespace ht

@@ -1,0 +1,237 @@
+// Jacobi eigen solver device routines (FP64, n <= 64) shared by the small solvers and
+// the batched kernel.
+#pragma once
+#include "ht_common.h"
+
+namespace ht {
+
+constexpr int SJ = 64;        // largest Jacobi problem
+constexpr int SJLD = SJ + 1;  // LDS row stride of the Jacobi matrices
+constexpr int SW = 16;        // waves of the 1024-thread kernels
+constexpr int SR = 16;        // rows per thread there: n <= SW * SR
+constexpr int SNT = SW * 64;  // threads of the 1024-thread kernels
+
+__device__ __forceinline__ double lane_bcast(double x, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+
+// Round-robin pairing for np (even) indices: step s pairs (np-1, s) and
+// ((s+t) mod (np-1), (s-t) mod (np-1)), t = 1 .. np/2-1; every pair meets once per sweep.
+// Entry (s, t) = p | q << 8 with p < q.
+__device__ __forceinline__ void jacobi_schedule(unsigned short* sched, int np) {
+  const int half = np / 2;
+  for (int e = threadIdx.x; e < (np - 1) * half; e += blockDim.x) {
+    const int s = e / half, t = e % half;
+    int p, q;
+    if (t == 0) {
+      p = np - 1;
+      q = s;
+    } else {
+      p = (s + t) % (np - 1);
+      q = (s - t + (np - 1)) % (np - 1);
+    }
+    if (p > q) {
+      const int x = p;
+      p = q;
+      q = x;
+    }
+    sched[e] = (unsigned short)(p | (q << 8));
+  }
+}
+
+// 1/x and 1/sqrt(x) in FP64 from the hardware approximations (v_rcp_f64, v_rsq_f64) and
+// two Newton steps each (quadratic convergence from ~2^-22 leaves ~1 ulp).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  r = fma(0.5 * r, fma(-x * r, r, 1.0), r);
+  r = fma(0.5 * r, fma(-x * r, r, 1.0), r);
+  return r;
+}
+
+// Jacobi rotation (c, s) that annihilates a_pq (Rutishauser's form: t = tan(phi) of the
+// smaller angle, c = 1/sqrt(1 + t^2), s = t c; c^2 + s^2 = 1 to rounding).
+__device__ __forceinline__ void jacobi_rotation(double app, double aqq, double apq, double& c, double& s) {
+  c = 1.0;
+  s = 0.0;
+  if (apq == 0.0) return;
+  const double tau = (aqq - app) * rcp_nr(2.0 * apq);
+  const double at = fabs(tau);
+  double t;
+  if (at > 1e150) {
+    t = 0.5 * rcp_nr(at);
+  } else {
+    const double h = 1.0 + tau * tau;
+    t = rcp_nr(at + h * rsq_nr(h));
+  }
+  if (tau < 0.0) t = -t;
+  c = rsq_nr(1.0 + t * t);
+  s = t * c;
+}
+
+// Cyclic parallel Jacobi on the np x np symmetric matrix a (LDS, ld SJLD), accumulating
+// the rotations into v, for a block of NT threads (256 or 1024). Each step rotates np/2
+// disjoint pairs (phase A, one thread per pair) and then updates every 2x2 block (rows of
+// pair i, columns of pair j) as R_i^T A_ij R_j and the two columns of v per pair (phase
+// B, over one combined item space). Returns 1 when the off-diagonal mass fell below
+// tol^2 * diagonal mass. The schedule must be in sched (jacobi_schedule + barrier).
+// With TAB the per-thread items are kept in `tab` (an LDS array of 3072 entries) instead
+// of in registers, which small blocks need to stay within their register budget.
+template <int NT = SNT, bool TAB = false>
+__device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np, int max_sweeps, double tol,
+                                             const unsigned short* sched, unsigned short* tab = nullptr) {
+  constexpr int NW = NT / 64;
+  constexpr int NI = (3072 + NT - 1) / NT;  // items per thread: (np/2)^2 + np * np/2 <= 3072
+  static_assert(NI <= 12, "block too small");
+  __shared__ double2 rcs[SJ / 2];
+  __shared__ int rpq[SJ / 2];
+  __shared__ double red[2][NW];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int half = np / 2, nblk = half * half, nitem = nblk + np * half;
+  // items of this thread (e = tid + NT u), packed as kind | x0 << 2 | x1 << 8:
+  // kind 1 = 2x2 block (x0, x1) = (bi, bj); kind 2 = row x0 of v for pair x1; 0 = none
+  auto item = [&](int e) {
+    if (e < nblk) return 1 | ((e % half) << 2) | ((e / half) << 8);
+    if (e < nitem) return 2 | (((e - nblk) % np) << 2) | (((e - nblk) / np) << 8);
+    return 0;
+  };
+  int it[TAB ? 1 : NI];
+  if constexpr (TAB) {
+    for (int e = tid; e < nitem; e += NT) tab[e] = (unsigned short)item(e);
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int u = 0; u < NI; ++u) it[u] = item(tid + NT * u);
+  }
+  auto apply = [&](int packed) {
+    const int kind = packed & 3, x0 = (packed >> 2) & 63, x1 = packed >> 8;
+    if (kind == 1) {
+      const int bi = x0, bj = x1;
+      const int pq = rpq[bi], PQ = rpq[bj];
+      const int p = pq & 255, q = pq >> 8, P = PQ & 255, Q = PQ >> 8;
+      const double2 cs = rcs[bi], CS = rcs[bj];
+      const double c = cs.x, s = cs.y, C = CS.x, S = CS.y;
+      const double apP = a[p * SJLD + P], apQ = a[p * SJLD + Q];
+      const double aqP = a[q * SJLD + P], aqQ = a[q * SJLD + Q];
+      // right: columns (P, Q); left: rows (p, q)
+      const double xpP = C * apP - S * apQ, xpQ = S * apP + C * apQ;
+      const double xqP = C * aqP - S * aqQ, xqQ = S * aqP + C * aqQ;
+      double ypP = c * xpP - s * xqP, yqP = s * xpP + c * xqP;
+      double ypQ = c * xpQ - s * xqQ, yqQ = s * xpQ + c * xqQ;
+      if (bi == bj && s != 0.0) {
+        ypQ = 0.0;
+        yqP = 0.0;
+      }
+      a[p * SJLD + P] = ypP;
+      a[p * SJLD + Q] = ypQ;
+      a[q * SJLD + P] = yqP;
+      a[q * SJLD + Q] = yqQ;
+    } else if (kind == 2) {
+      const int r = x0, j = x1;
+      const int PQ = rpq[j];
+      const int P = PQ & 255, Q = PQ >> 8;
+      const double2 CS = rcs[j];
+      const double vP = v[r * SJLD + P], vQ = v[r * SJLD + Q];
+      v[r * SJLD + P] = CS.x * vP - CS.y * vQ;
+      v[r * SJLD + Q] = CS.y * vP + CS.x * vQ;
+    }
+  };
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    double off = 0.0, dg = 0.0;
+    for (int e = tid; e < n * n; e += NT) {
+      const int i = e % n, j = e / n;
+      const double x = a[i * SJLD + j];
+      if (i == j) dg += x * x;
+      else off += x * x;
+    }
+    off = wave_sum(off);
+    dg = wave_sum(dg);
+    if (lane == 0) {
+      red[0][wv] = off;
+      red[1][wv] = dg;
+    }
+    __syncthreads();
+    off = 0.0;
+    dg = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      off += red[0][w];
+      dg += red[1][w];
+    }
+    __syncthreads();
+    if (off <= tol * tol * dg || off == 0.0) return 1;
+    for (int step = 0; step < np - 1; ++step) {
+      if (tid < half) {
+        const int pq = sched[step * half + tid];
+        const int p = pq & 255, q = pq >> 8;
+        double c, s;
+        jacobi_rotation(a[p * SJLD + p], a[q * SJLD + q], a[p * SJLD + q], c, s);
+        rpq[tid] = pq;
+        rcs[tid] = make_double2(c, s);
+      }
+      __syncthreads();
+      if constexpr (TAB) {
+#pragma unroll 1
+        for (int e = tid; e < nitem; e += NT) apply(tab[e]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < NI; ++u) apply(it[u]);
+      }
+      __syncthreads();
+    }
+  }
+  return 0;
+}
+
+// Load an n x n matrix into a (padded to even np with a decoupled, large negative
+// diagonal entry that sorts last), set v = I.
+__device__ __forceinline__ void jacobi_load(double* a, double* v, const double* A, int lda, int n, int np, bool symmetrise) {
+  __shared__ double red[SW];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double amax = 0.0;
+  for (int e = tid; e < np * np; e += blockDim.x) {
+    const int i = e % np, j = e / np;
+    double x = 0.0;
+    if (i < n && j < n) {
+      x = A[i + (long)lda * j];
+      if (symmetrise) x = 0.5 * (x + A[j + (long)lda * i]);
+    }
+    a[i * SJLD + j] = x;
+    v[i * SJLD + j] = (i == j) ? 1.0 : 0.0;
+    amax = fmax(amax, fabs(x));
+  }
+  for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o, 64));
+  if (lane == 0) red[wv] = amax;
+  __syncthreads();
+  if (np != n && tid == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)blockDim.x / 64; ++w) m = fmax(m, red[w]);
+    a[(np - 1) * SJLD + (np - 1)] = -(m * (double)n + 1.0) * 4.0;
+  }
+  __syncthreads();
+}
+
+// Descending order of the diagonal of a: thread i places index i at its rank (ties by
+// index), so order is a permutation.
+__device__ __forceinline__ void jacobi_order(const double* a, int np, int* order) {
+  const int i = threadIdx.x;
+  if (i < np) {
+    const double di = a[i * SJLD + i];
+    int rank = 0;
+    for (int j = 0; j < np; ++j) {
+      const double dj = a[j * SJLD + j];
+      rank += (dj > di || (dj == di && j < i)) ? 1 : 0;
+    }
+    order[rank] = i;
+  }
+  __syncthreads();
+}
+
+}  // namespace ht

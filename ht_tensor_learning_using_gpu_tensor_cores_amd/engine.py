@@ -242,6 +242,101 @@ def generate_bumps(n, order, seed, sample, n_bumps=6, device=0, handle=None):
     return x
 
 
+class Batch:
+    """A batch of small 3rd-order tensors packed in one device buffer (tree {1},{2,3})."""
+
+    def __init__(self, dims, x, x_offsets):
+        self.dims = np.asarray(dims, dtype=np.int64).reshape(-1, 3)
+        self.x = x
+        self.x_offsets = np.asarray(x_offsets, dtype=np.int64)
+
+    @property
+    def count(self):
+        return self.dims.shape[0]
+
+    def member(self, b):
+        n = int(np.prod(self.dims[b]))
+        o = int(self.x_offsets[b])
+        return self.x[o:o + n]
+
+
+def batch_shapes(count, seed, lo=16, hi=64, rlo=4, rhi=8):
+    """Mode sizes and Tucker ranks of the batched workload (count x 3 each)."""
+    lib = _lib.load()
+    dims = np.zeros(3 * count, dtype=np.int64)
+    core = np.zeros(3 * count, dtype=np.int32)
+    _lib.check(lib.ht_batch_shapes(int(count), int(seed), lo, hi, rlo, rhi,
+                                   dims.ctypes.data_as(_lib.PI64), core.ctypes.data_as(_lib.PI32)), "ht_batch_shapes")
+    return dims.reshape(-1, 3), core.reshape(-1, 3)
+
+
+def generate_batch(dims, core, seed, noise_rel, device=0, handle=None):
+    """Seeded Tucker members (Philox streams 4096 + 16 b ...) packed back to back on the device."""
+    torch = _torch()
+    lib = _lib.load()
+    h = handle or default_handle(device)
+    dims = np.ascontiguousarray(np.asarray(dims, dtype=np.int64).reshape(-1, 3))
+    core = np.ascontiguousarray(np.asarray(core, dtype=np.int32).reshape(-1, 3))
+    sizes = np.prod(dims, axis=1)
+    # 16-byte aligned members
+    sizes_al = (sizes + 3) // 4 * 4
+    offs = np.concatenate([[0], np.cumsum(sizes_al)[:-1]]).astype(np.int64)
+    x = torch.zeros(int(sizes_al.sum()), dtype=torch.float32, device=f"cuda:{device}")
+    torch.cuda.synchronize(x.device)
+    _lib.check(lib.ht_generate_batch(h.ptr, dims.shape[0], dims.ctypes.data_as(_lib.PI64),
+                                     core.ctypes.data_as(_lib.PI32), int(seed), float(noise_rel),
+                                     ctypes.c_void_p(x.data_ptr()), offs.ctypes.data_as(_lib.PI64)),
+               "ht_generate_batch")
+    return Batch(dims, x, offs)
+
+
+def batch_ranks(core):
+    """Heap ranks [1, k1, k23 = k1, k2, k3] of the tree {1},{2,3} from Tucker ranks."""
+    core = np.asarray(core, dtype=np.int32).reshape(-1, 3)
+    r = np.empty((core.shape[0], 5), dtype=np.int32)
+    r[:, 0] = 1
+    r[:, 1] = core[:, 0]
+    r[:, 2] = core[:, 0]
+    r[:, 3] = core[:, 1]
+    r[:, 4] = core[:, 2]
+    return r
+
+
+def decompose_batched(batch: Batch, ranks, handle=None):
+    """ht_decompose_batched: returns (list of HTensor views, rel_err array, stats)."""
+    torch = _torch()
+    lib = _lib.load()
+    h = handle or default_handle()
+    ranks = np.ascontiguousarray(np.asarray(ranks, dtype=np.int32).reshape(-1, 5))
+    count = batch.count
+    dims = np.ascontiguousarray(batch.dims)
+    offs = np.zeros(count, dtype=np.int64)
+    total = ctypes.c_int64()
+    _lib.check(lib.ht_batch_output_layout(count, dims.ctypes.data_as(_lib.PI64), ranks.ctypes.data_as(_lib.PI32),
+                                          offs.ctypes.data_as(_lib.PI64), ctypes.byref(total)), "ht_batch_output_layout")
+    out = torch.empty(int(total.value), dtype=torch.float32, device=batch.x.device)
+    err = np.zeros(count, dtype=np.float64)
+    st = _lib.Stats()
+    torch.cuda.synchronize(batch.x.device)
+    _lib.check(lib.ht_decompose_batched(h.ptr, ctypes.c_void_p(batch.x.data_ptr()), count,
+                                        dims.ctypes.data_as(_lib.PI64), ranks.ctypes.data_as(_lib.PI32),
+                                        batch.x_offsets.ctypes.data_as(_lib.PI64), ctypes.c_void_p(out.data_ptr()),
+                                        offs.ctypes.data_as(_lib.PI64), err.ctypes.data_as(_lib.PD),
+                                        ctypes.byref(st)), "ht_decompose_batched")
+    from . import tree as _tree
+    tr = _tree.first_vs_rest_3d()
+    results = []
+    for b in range(count):
+        d = [int(v) for v in dims[b]]
+        rk = [int(v) for v in ranks[b]]
+        layout = output_layout(d, tr, rk, False)
+        o = int(offs[b])
+        res = HTensor(dims=d, tree=tr, ranks=rk, packed=out[o:o + layout[3]], rel_err=float(err[b]))
+        _unpack(res, layout)
+        results.append(res)
+    return results, err, dict(total_ms=st.total_ms)
+
+
 def _host_f32(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1))
 

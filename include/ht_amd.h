@@ -171,6 +171,34 @@ ht_status ht_generate_cp_decay(ht_handle_t handle, int32_t order, const int64_t*
 ht_status ht_generate_bumps(ht_handle_t handle, int32_t order, int64_t n, int32_t n_bumps,
                             uint64_t seed, int64_t sample, float* x_dev);
 
+/* ---- batched HT ------------------------------------------------------------------
+ * Many small, unevenly sized 3rd-order tensors decomposed in one persistent launch
+ * (captured in a hipGraph and replayed while the arguments stay the same), with the
+ * tree {1},{2,3} (heap nodes: 0 = {1,2,3}, 1 = {1}, 2 = {2,3}, 3 = {2}, 4 = {3}) and
+ * the plain semantics. No counterpart in the reference: its third_order_tensor_decompsition/
+ * CUDA/Optimized/htd_batch.cu is a single decomposition and batch_tensorcore.cu:46-95 a
+ * standalone batched-GEMM test. Limits: mode sizes <= 64, ranks <= 8.
+ *   dims       count x 3 mode sizes (host);  ranks  count x 5 heap ranks, k[0] = 1 (host)
+ *   x          device; tensor b (column-major) at x + x_offsets[b] (host offsets)
+ *   out        device; tensor b's block, in ht_output_layout's order for that tree, at
+ *              out + out_offsets[b] (ht_batch_output_layout gives a packed choice)
+ *   rel_err    host, count relative errors (may be NULL)
+ * The launch shape (fibre-chunk length, blocks per CU) is timed once per (count,
+ * largest mode size) on the first call and cached in the handle. */
+ht_status ht_batch_output_layout(int32_t count, const int64_t* dims, const int32_t* ranks,
+                                 int64_t* out_offsets, int64_t* total);
+ht_status ht_decompose_batched(ht_handle_t handle, const float* x, int32_t count, const int64_t* dims,
+                               const int32_t* ranks, const int64_t* x_offsets, float* out,
+                               const int64_t* out_offsets, double* rel_err, ht_stats* stats);
+/* Shapes of the batched workload: mode sizes uniform in [lo, hi] and Tucker ranks
+ * uniform in [rlo, rhi] (Philox uniforms of stream 4095, six per tensor; host). */
+ht_status ht_batch_shapes(int32_t count, uint64_t seed, int32_t lo, int32_t hi, int32_t rlo, int32_t rhi,
+                          int64_t* dims, int32_t* core);
+/* Tensor b = Tucker(core[b] i.i.d. N(0,1), orthonormal factors) + noise, Philox streams
+ * 4096 + 16 b (core), + 1 + m (factor of mode m), + 15 (noise); written at x_dev + x_offsets[b]. */
+ht_status ht_generate_batch(ht_handle_t handle, int32_t count, const int64_t* dims, const int32_t* core,
+                            uint64_t seed, double noise_rel, float* x_dev, const int64_t* x_offsets);
+
 /* ---- compatibility wrappers keeping the reference names --------------------------- */
 
 /* third_order_tensor_decompsition/CUDA/Baseline/function.h:6: plain FP32 HT of the
