@@ -12,6 +12,9 @@ Workloads (generated on the device):
       leaves-to-root BF16 (the benchmark workload)
   c3  3rd order 1024^3, core 32^3 with mode spectra k^-1.5 + 1e-5 noise, seed 2, tree
       {1},{2,3}, ranks 32, randomised (TSQR) BF16
+  c4  4096 3rd-order tensors, mode sizes U{16..64}, Tucker ranks U{4..8}, 1e-3 noise, seed 3,
+      tree {1},{2,3}: the batched entry point (one persistent launch); reports aggregate
+      decompositions/s and the p50/p99 relative errors
 """
 import argparse
 import time
@@ -33,7 +36,8 @@ def workload(name):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--count", type=int, default=4096, help="c4: number of tensors")
     ap.add_argument("--method", default=None)
     ap.add_argument("--precision", default=None)
     ap.add_argument("--reps", type=int, default=3)
@@ -42,6 +46,8 @@ def main():
     import torch
     from . import engine as E
     from . import _lib
+    if args.workload == "c4":
+        return batched(args, torch, E)
     w = workload(args.workload)
     method = args.method or w["method"]
     prec = args.precision or w["precision"]
@@ -66,6 +72,24 @@ def main():
         st = lib.ht_debug_copy(h.ptr, x.data_ptr(), y.data_ptr(), x.numel())
         torch.cuda.synchronize()
         print(f"calibration copy of {x.numel() * 4} bytes: status {st}", flush=True)
+
+
+def batched(args, torch, E):
+    import numpy as np
+    dims, core = E.batch_shapes(args.count, 3)
+    t0 = time.perf_counter()
+    batch = E.generate_batch(dims, core, 3, 1e-3)
+    torch.cuda.synchronize()
+    gb = sum(int(np.prod(d)) for d in dims) * 4 / 1e9
+    print(f"c4: generated {args.count} tensors ({gb:.3f} GB) in {time.perf_counter() - t0:.1f} s", flush=True)
+    ranks = E.batch_ranks(core)
+    for i in range(args.reps + 1):
+        t0 = time.perf_counter()
+        _, err, st = E.decompose_batched(batch, ranks)
+        wall = time.perf_counter() - t0
+        print(f"c4 rep {i}: device {st['total_ms']:.3f} ms ({args.count / st['total_ms'] * 1e3:.0f} decomps/s, "
+              f"{gb / st['total_ms'] * 1e3:.1f} GB/s), wall {wall * 1e3:.1f} ms, rel_err p50 {np.percentile(err, 50):.3e} "
+              f"p99 {np.percentile(err, 99):.3e} max {err.max():.3e}", flush=True)
 
 
 if __name__ == "__main__":
