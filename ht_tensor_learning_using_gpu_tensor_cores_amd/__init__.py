@@ -3,14 +3,14 @@
     import ht_tensor_learning_using_gpu_tensor_cores_amd as ht
     res = ht.decompose(x, dims, ht.tree.balanced_4d(), ranks, method="ltr")
 """
-from . import tree
+from . import sketch, tree
 from ._lib import HTError, LIB_PATH, load as load_library
 from .engine import (Batch, Handle, HTensor, batch_ranks, batch_shapes, decompose, decompose_batched,
                      default_handle, generate_batch, generate_bumps, generate_cp_decay, generate_tucker, htd, htd4,
                      htd4_tc, htd_tc, make_options, output_layout, reconstruct, residual)
 
 __all__ = [
-    "tree", "HTError", "LIB_PATH", "load_library", "Handle", "HTensor", "decompose", "default_handle",
+    "sketch", "tree", "HTError", "LIB_PATH", "load_library", "Handle", "HTensor", "decompose", "default_handle",
     "generate_bumps", "generate_cp_decay", "generate_tucker", "htd", "htd4", "htd4_tc", "htd_tc",
     "make_options", "output_layout", "reconstruct", "residual", "Batch", "batch_ranks", "batch_shapes",
     "decompose_batched", "generate_batch",

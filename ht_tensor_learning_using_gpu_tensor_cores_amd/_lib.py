@@ -15,6 +15,7 @@ STATUS_NAMES = {
 METHOD = {"plain": 0, "ltr": 1, "rsvd": 2}
 PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2}
 ERR_MODE = {"none": 0, "identity": 1, "residual": 2}
+SKETCH = {"gaussian": 0, "learned": 1}
 
 
 class HTError(RuntimeError):
@@ -28,7 +29,8 @@ class Options(ctypes.Structure):
         ("method", ctypes.c_int32), ("precision", ctypes.c_int32), ("oversample", ctypes.c_int32),
         ("power_iters", ctypes.c_int32), ("err_mode", ctypes.c_int32), ("max_eig_iters", ctypes.c_int32),
         ("eig_tol", ctypes.c_double), ("seed", ctypes.c_uint64), ("want_interior_u", ctypes.c_int32),
-        ("check_finite", ctypes.c_int32),
+        ("check_finite", ctypes.c_int32), ("sketch", ctypes.c_int32), ("sketch_width", ctypes.c_int32),
+        ("sketch_weights", ctypes.c_void_p), ("sketch_tol", ctypes.c_double),
     ]
 
 
@@ -36,6 +38,14 @@ class Stats(ctypes.Structure):
     _fields_ = [
         ("total_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double), ("error_ms", ctypes.c_double),
         ("bytes_x_passes", ctypes.c_double), ("eig_iters_max", ctypes.c_int32), ("eig_calls", ctypes.c_int32),
+        ("sketch_fallbacks", ctypes.c_int32), ("sketch_nodes", ctypes.c_int32), ("sketch_resid", ctypes.c_double),
+    ]
+
+
+class SketchTrainOptions(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32), ("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
+        ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
     ]
 
 
@@ -87,6 +97,13 @@ SIGNATURES = {
     "ht_profile_enable": (I32, [H, I32]),
     "ht_profile_read": (I32, [H, PD, PI64, I32, I32]),
     "ht_debug_copy": (I32, [H, P, P, I64]),
+    "ht_sketch_default_train_options": (None, [ctypes.POINTER(SketchTrainOptions)]),
+    "ht_sketch_layout": (I32, [I32, PI64, I32, PI64, PI64]),
+    "ht_sketch_init_gaussian": (I32, [H, I32, PI64, I32, U64, P]),
+    "ht_sketch_loss_grad": (I32, [H, P, I32, PI64, I32, PI32, PI32, PI32, PI32, I32, P, P, PD]),
+    "ht_sketch_train_bumps": (I32, [H, I32, I64, I32, U64, I64, I64, I32, PI32, PI32, PI32, PI32,
+                                    ctypes.POINTER(SketchTrainOptions), P, PD]),
+    "ht_sketch_eval_bumps": (I32, [H, I32, I64, I32, U64, I64, I64, I32, PI32, PI32, PI32, PI32, I32, P, PD]),
 }
 
 PROFILE_KERNELS = ["slab_gram", "slab_proj", "mode_gram", "residual", "skinny_gemm", "tsqr", "small_eig",

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "libht_amd.so")
 SOURCES = ["k_gram.hip", "k_gemm.hip", "k_eig.hip", "k_tsqr.hip", "k_gen.hip", "k_small.hip", "k_batch.hip",
-           "ht_engine.cpp", "ht_api.cpp", "ht_batch.cpp"]
+           "k_sketch.hip", "ht_engine.cpp", "ht_api.cpp", "ht_batch.cpp", "ht_sketch.cpp"]
 ARCH = os.environ.get("HT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -140,6 +140,10 @@ void ht_default_options(ht_options* o) {
   o->seed = 1234;
   o->want_interior_u = 0;
   o->check_finite = 1;
+  o->sketch = HT_SKETCH_GAUSSIAN;
+  o->sketch_width = 20;
+  o->sketch_weights = nullptr;
+  o->sketch_tol = 1e-3;
 }
 
 ht_status ht_create(ht_handle_t* handle, int32_t device) {
@@ -222,7 +226,11 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
   if (opt_in) opt = *opt_in;
   else ht_default_options(&opt);
   if (opt.method < 0 || opt.method > 2 || opt.precision < 0 || opt.precision > 2 || opt.oversample < 0 ||
-      opt.power_iters < 0 || opt.max_eig_iters < 1 || !(opt.eig_tol > 0))
+      opt.power_iters < 0 || opt.max_eig_iters < 1 || !(opt.eig_tol > 0) || opt.sketch < 0 || opt.sketch > 1)
+    return HT_ERR_INVALID_ARG;
+  if (opt.sketch == HT_SKETCH_LEARNED &&
+      (opt.method != HT_METHOD_RSVD || !opt.sketch_weights || opt.sketch_width < 1 || opt.sketch_width > 64 ||
+       !(opt.sketch_tol >= 0)))
     return HT_ERR_INVALID_ARG;
   Problem p;
   CKS(p.setup(order, dims, n_nodes, lo, hi, left, right, ranks, opt.want_interior_u));
@@ -271,6 +279,9 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
     stats->bytes_x_passes = e.passes;
     stats->eig_iters_max = e.eig_iters_max;
     stats->eig_calls = e.eig_calls;
+    stats->sketch_fallbacks = e.sketch_fallbacks;
+    stats->sketch_nodes = e.sketch_nodes;
+    stats->sketch_resid = e.sketch_resid;
   }
   if (!std::isfinite(e.x_sq)) return HT_ERR_NONFINITE;
   return (ht_status)s;
@@ -390,20 +401,8 @@ ht_status ht_generate_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t n_b
                             float* x_dev) {
   if (!h || !x_dev || order < 1 || order > HT_MAX_ORDER || n < 2 || n_bumps < 1 || sample < 0) return HT_ERR_INVALID_ARG;
   if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
-  const int per = n_bumps * (2 + order);
   std::vector<float> amp(n_bumps), fac((size_t)n_bumps * order * n);
-  for (int j = 0; j < n_bumps; ++j) {
-    auto u = [&](int c) { return philox_uniform(seed, 96, (uint64_t)sample * per + (uint64_t)j * (2 + order) + c); };
-    const double a = 0.5 + 1.5 * u(0), w = 0.1 + 0.3 * u(1);
-    amp[j] = (float)a;
-    for (int m = 0; m < order; ++m) {
-      const double c = u(2 + m);
-      for (long i = 0; i < n; ++i) {
-        const double g = (double)i / (double)(n - 1);
-        fac[((size_t)j * order + m) * n + i] = (float)std::exp(-(g - c) * (g - c) / (2.0 * w * w));
-      }
-    }
-  }
+  bump_factors_host(order, n, n_bumps, seed, sample, amp.data(), fac.data());
   float *fd = nullptr, *ad = nullptr;
   if (hipMalloc((void**)&fd, fac.size() * 4) != hipSuccess) return HT_ERR_ALLOC;
   if (hipMalloc((void**)&ad, amp.size() * 4) != hipSuccess) { (void)hipFree(fd); return HT_ERR_ALLOC; }

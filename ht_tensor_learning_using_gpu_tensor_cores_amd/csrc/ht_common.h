@@ -217,6 +217,33 @@ int batch_ht_launch(hipStream_t st, const float* x, const void* items, const int
 int gen_bumps_launch(hipStream_t st, float* x, int order, long n, const float* factors, const float* amp,
                      int n_bumps);
 
+// Learned per-mode sketch (k_sketch.hip). S_m is w x n_m row-major at off[m] of the
+// flat sketch array; a node's test matrix is the Khatri-Rao product over its column
+// modes m0..m1-1 (cols = prod n, column-major over those modes).
+#define HT_KR_MAX 7
+int kr_omega_launch(hipStream_t st, const float* S, const int64_t* n, const int64_t* off, int m0, int m1, int w,
+                    float* om);
+int kr_grad_launch(hipStream_t st, const float* S, const float* dom, const int64_t* n, const int64_t* off, int m0,
+                   int m1, int w, float scale, float* grad);
+// out = A^T B (wa x wb, FP64) of two tall FP32 matrices (rows x wa, rows x wb); part
+// holds tall_gram_blocks(rows) * wa * wb doubles.
+int tall_gram_blocks(long rows);
+int tall_gram_launch(hipStream_t st, const float* A, int wa, const float* B, int wb, long rows, double* part,
+                     double* out);
+int sketch_solve_launch(hipStream_t st, const double* W, const double* E, const double* xsq, int w, double ridge,
+                        double* C1, double* C2s, double* coef, double* loss_acc, int* info);
+int tall_small_launch(hipStream_t st, const float* A, long rows, int w, const double* C, const float* B,
+                      const double* beta, float* out);
+int adam_launch(hipStream_t st, float* S, float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+                float scale, float bc1, float bc2);
+int fill_f32_launch(hipStream_t st, float* a, long n, float val);
+int fill_f64_launch(hipStream_t st, double* a, long n, double val);
+// out[0] = sum_i ||P_i - Q QtP_i||^2 (Q: rows x b, P: rows x np, QtP: b x np FP64);
+// part holds probe_resid_blocks(rows) doubles.
+int probe_resid_blocks(long rows);
+int probe_resid_launch(hipStream_t st, const float* Q, const float* P, const double* QtP, long rows, int b, int np,
+                       double* part, double* out);
+
 // MFMA lane-layout probe (diagnostic): runs each MFMA shape once on given inputs.
 int mfma_probe_launch(hipStream_t st, const float* a, const float* b, float* out_bf16_32, float* out_f32_32,
                       float* out_f32_16, double* out_f64_16, float* out_bf16_16);

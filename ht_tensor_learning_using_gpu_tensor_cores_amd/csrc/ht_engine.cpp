@@ -224,6 +224,9 @@ template <typename T>
 T* Engine::alloc(size_t n) {
   return (T*)arena->get(n * sizeof(T) + 16);
 }
+template float* Engine::alloc<float>(size_t);
+template double* Engine::alloc<double>(size_t);
+template int* Engine::alloc<int>(size_t);
 
 // Gram of an (P,M,S) view of FP32 data, reduced to FP64 (M x M). Returns ||x||^2 of the
 // data in *sumsq (device scalar) when sumsq != nullptr.
@@ -945,9 +948,99 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
 
 // Randomised root-to-leaves method (4O semantics with oversampling, power iterations,
 // TSQR and an FP64 projected SVD).
+// Range finder of A = X_(t) (view v) from the test matrix Om (cols x l): Y = A Om, Q R = Y
+// (TSQR), then q power iterations Q <- orth(A orth(A^T Q)).
+int Engine::range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z,
+                         float* Qz, double* R) {
+  const long rows = v.M, cols = v.P * v.S;
+  CK(view_gemm(x, v, 0, Om, l, prec, Y));
+  CK(tsqr(Y, rows, l, Q, R));
+  for (int q = 0; q < opt.power_iters; ++q) {
+    CK(view_gemm(x, v, 1, Q, l, prec, Z));
+    CK(tsqr(Z, cols, l, Qz, R));
+    CK(view_gemm(x, v, 0, Qz, l, prec, Y));
+    CK(tsqr(Y, rows, l, Q, R));
+  }
+  return 0;
+}
+
+// A-posteriori estimate of ||(I - Q Q^T) A||_F / ||X||_F from four Gaussian probe vectors
+// w_i (Philox stream 224 + t): E ||(I - QQ^T) A w||^2 = ||(I - QQ^T) A||_F^2. The probe
+// product runs in FP32 so that 16-bit rounding of X does not set a floor under the
+// estimate. One device-to-host read.
+int Engine::range_residual(const float* x, PMS v, int t, const float* Q, int l, const double* xsq, double* est) {
+  const long rows = v.M, cols = v.P * v.S;
+  constexpr int NP = 4;
+  float* Op = alloc<float>((size_t)cols * NP);
+  float* Pp = alloc<float>((size_t)rows * NP);
+  double* QtP = alloc<double>((size_t)l * NP);
+  const size_t np_part = std::max((size_t)tall_gram_blocks(rows) * l * NP, (size_t)probe_resid_blocks(rows));
+  double* part = alloc<double>(np_part);
+  double* res = alloc<double>(2);
+  if (!Op || !Pp || !QtP || !part || !res) return HT_ERR_ALLOC;
+  CK(fill_normals_f32_launch(st, Op, cols * NP, opt.seed, 224 + t));
+  CK(view_gemm(x, v, 0, Op, NP, PREC_F32, Pp));
+  CK(tall_gram_launch(st, Q, l, Pp, NP, rows, part, QtP));
+  CK(probe_resid_launch(st, Q, Pp, QtP, rows, l, NP, part, res));
+  HT_HIP_CHECK(hipMemcpyAsync(res + 1, xsq, sizeof(double), hipMemcpyDeviceToDevice, st));
+  double hv[2];
+  CK(d2h(hv, res, sizeof(hv)));
+  *est = hv[1] > 0 ? std::sqrt(std::max(hv[0], 0.0) / (NP * hv[1])) : 0.0;
+  return 0;
+}
+
+// Loss of the learned sketch on one node (A = X_(t), Omega = Khatri-Rao of the column
+// modes' sketches, w columns) and its gradient:
+//   Y = A Omega, Z = A^T Y, W = Y^T Y (+ ridge), E = Z^T Z,
+//   loss = 1 - tr(W^-1 E) / ||X||^2,
+//   dloss/dOmega = -(2/||X||^2) (A^T A Z W^-1 - Z C2 - (ridge/w) tr(C2) Z),  C2 = W^-1 E W^-1
+//   (the last term because the ridge is relative to tr(Y^T Y)),
+// the A^T A Z W^-1 term by two more passes (T = A (Z W^-1), A^T T). All large products
+// are FP32 MFMA; W^-1 and the small products are FP64.
+static constexpr double SKETCH_RIDGE = 1e-6;
+int Engine::sketch_node(const float* x, int t, const float* S, const int64_t* off, int w, float* grad, float scale,
+                        double* loss_acc, const double* xsq, int* info) {
+  const Problem& p = *prob;
+  const PMS v = view_of(p.dims, p.order, p.lo[t], p.hi[t]);
+  const long rows = v.M, cols = v.P * v.S;
+  int m0 = 0, m1 = 0;
+  CK(node_col_modes(p, t, &m0, &m1));
+  if (w < 1 || w > 64 || w > rows || w > cols) return HT_ERR_RANK;
+  float* Om = alloc<float>((size_t)cols * w);
+  float* Y = alloc<float>((size_t)rows * w);
+  float* Z = alloc<float>((size_t)cols * w);
+  double* Wg = alloc<double>((size_t)w * w);
+  double* Eg = alloc<double>((size_t)w * w);
+  double* C1 = alloc<double>((size_t)w * w);
+  double* C2s = alloc<double>((size_t)w * w);
+  double* coef = alloc<double>(1);
+  double* part = alloc<double>((size_t)tall_gram_blocks(std::max(rows, cols)) * w * w);
+  if (!Om || !Y || !Z || !Wg || !Eg || !C1 || !C2s || !coef || !part) return HT_ERR_ALLOC;
+  CK(kr_omega_launch(st, S, p.dims, off, m0, m1, w, Om));
+  CK(view_gemm(x, v, 0, Om, w, PREC_F32, Y));
+  CK(view_gemm(x, v, 1, Y, w, PREC_F32, Z));
+  CK(tall_gram_launch(st, Y, w, Y, w, rows, part, Wg));
+  CK(tall_gram_launch(st, Z, w, Z, w, cols, part, Eg));
+  CK(sketch_solve_launch(st, Wg, Eg, xsq, w, SKETCH_RIDGE, C1, C2s, coef, loss_acc, info));
+  if (!grad) return 0;
+  float* V = alloc<float>((size_t)cols * w);
+  float* T = alloc<float>((size_t)rows * w);
+  float* G1 = alloc<float>((size_t)cols * w);
+  if (!V || !T || !G1) return HT_ERR_ALLOC;
+  CK(tall_small_launch(st, Z, cols, w, C1, nullptr, nullptr, V));  // Z W^-1
+  CK(view_gemm(x, v, 0, V, w, PREC_F32, T));                        // A Z W^-1
+  CK(view_gemm(x, v, 1, T, w, PREC_F32, G1));                       // A^T A Z W^-1
+  CK(tall_small_launch(st, Z, cols, w, C2s, G1, coef, V));          // dloss/dOmega
+  CK(kr_grad_launch(st, S, V, p.dims, off, m0, m1, w, scale, grad));
+  return 0;
+}
+
 int Engine::run_rsvd(const float* x) {
   const Problem& p = *prob;
   const int prec = opt.precision == HT_PREC_FP32 ? PREC_F32 : (opt.precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
+  const bool learned = opt.sketch == HT_SKETCH_LEARNED;
+  int64_t soff[HT_MAX_ORDER], stotal = 0;
+  if (learned) CK(sketch_offsets(p.order, p.dims, opt.sketch_width, soff, &stotal));
   double* xsq = alloc<double>(1);
   CK(leaves_from_x(x, prec, xsq));
   std::vector<int> done(p.n_nodes, 0);
@@ -961,22 +1054,39 @@ int Engine::run_rsvd(const float* x) {
     const int k = p.rank[t];
     const int kW = (sib >= 0 && p.left[sib] >= 0) ? p.rank[sib] : 0;
     const int kk = std::max(k, kW);
-    const int l = (int)std::min<long>(std::min<long>(kk + opt.oversample, 64), std::min(rows, cols));
-    float* Om = alloc<float>((size_t)cols * l);
-    float* Y = alloc<float>((size_t)rows * l);
-    float* Q = alloc<float>((size_t)rows * l);
-    float* Z = alloc<float>((size_t)cols * l);
-    float* Qz = alloc<float>((size_t)cols * l);
-    double* R = alloc<double>((size_t)l * l);
+    const int lg = (int)std::min<long>(std::min<long>(kk + opt.oversample, 64), std::min(rows, cols));
+    int l = lg;
+    if (learned) {
+      l = opt.sketch_width;
+      if (l < kk || l > rows || l > cols) return HT_ERR_RANK;
+    }
+    const int lmax = std::max(l, lg);
+    float* Om = alloc<float>((size_t)cols * lmax);
+    float* Y = alloc<float>((size_t)rows * lmax);
+    float* Q = alloc<float>((size_t)rows * lmax);
+    float* Z = alloc<float>((size_t)cols * lmax);
+    float* Qz = alloc<float>((size_t)cols * lmax);
+    double* R = alloc<double>((size_t)lmax * lmax);
     if (!Om || !Y || !Q || !Z || !Qz || !R) return HT_ERR_ALLOC;
-    CK(fill_normals_f32_launch(st, Om, cols * l, opt.seed, 256 + t));
-    CK(view_gemm(x, v, 0, Om, l, prec, Y));
-    CK(tsqr(Y, rows, l, Q, R));
-    for (int q = 0; q < opt.power_iters; ++q) {
-      CK(view_gemm(x, v, 1, Q, l, prec, Z));
-      CK(tsqr(Z, cols, l, Qz, R));
-      CK(view_gemm(x, v, 0, Qz, l, prec, Y));
-      CK(tsqr(Y, rows, l, Q, R));
+    bool gaussian = !learned;
+    if (learned) {
+      int m0 = 0, m1 = 0;
+      CK(node_col_modes(p, t, &m0, &m1));
+      CK(kr_omega_launch(st, opt.sketch_weights, p.dims, soff, m0, m1, l, Om));
+      CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz, R));
+      double est = 0.0;
+      CK(range_residual(x, v, t, Q, l, xsq, &est));
+      ++sketch_nodes;
+      sketch_resid = std::max(sketch_resid, est);
+      if (!(est <= opt.sketch_tol)) {
+        gaussian = true;
+        l = lg;
+        ++sketch_fallbacks;
+      }
+    }
+    if (gaussian) {
+      CK(fill_normals_f32_launch(st, Om, cols * l, opt.seed, 256 + t));
+      CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz, R));
     }
     // A^T Q = Qz Rz  =>  Q^T A = Rz^T Qz^T; SVD of Rz^T via eig(Rz^T Rz).
     CK(view_gemm(x, v, 1, Q, l, prec, Z));
@@ -1049,6 +1159,9 @@ int Engine::run(const float* x, float* out) {
   eig_iters_max = 0;
   eig_calls = 0;
   noconv = 0;
+  sketch_fallbacks = 0;
+  sketch_nodes = 0;
+  sketch_resid = 0.0;
   int s = 0;
   if (opt.method == HT_METHOD_PLAIN) s = run_plain(x);
   else if (opt.method == HT_METHOD_LTR) s = run_ltr(x);

@@ -17,6 +17,12 @@ struct Arena {
   void* get(size_t bytes);
   void reset();
   void release();
+  // Rewind to an earlier position: what was handed out after `mark()` is reused.
+  std::pair<size_t, size_t> mark() const { return {cur, off}; }
+  void rewind(std::pair<size_t, size_t> m) {
+    cur = m.first;
+    off = m.second;
+  }
 };
 
 int validate_tree(int order, int n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
@@ -69,6 +75,8 @@ struct Engine {
   double* SVd[HT_MAX_NODES];
   double passes = 0.0;
   int eig_iters_max = 0, eig_calls = 0, noconv = 0;
+  int sketch_fallbacks = 0, sketch_nodes = 0;
+  double sketch_resid = 0.0;
   double rel_err = 0.0, x_sq = 0.0;
 
   template <typename T>
@@ -98,6 +106,13 @@ struct Engine {
   int ltr_interior(double* C, int64_t* cd, const double* xsq, const float* x);
   int small_side_svd(const double* C, PMS v, int k, double* U, double* w, int* done);
   int run_rsvd(const float* x);
+  int range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z, float* Qz,
+                   double* R);
+  int range_residual(const float* x, PMS v, int t, const float* Q, int l, const double* xsq, double* est);
+  // One sketched node's loss (accumulated into loss_acc[0]) and, when grad != nullptr,
+  // scale times its gradient (accumulated into grad).
+  int sketch_node(const float* x, int t, const float* S, const int64_t* off, int w, float* grad, float scale,
+                  double* loss_acc, const double* xsq, int* info);
   int write_outputs(float* out);
   int run(const float* x, float* out);
 };
@@ -123,6 +138,14 @@ struct ht_handle_s {
 };
 
 namespace ht {
+// Learned-sketch helpers (ht_sketch.cpp).
+int sketch_offsets(int order, const int64_t* dims, int w, int64_t* off, int64_t* total);
+// Nodes the randomised method runs a range finder on, in its order.
+int sketched_nodes(const Problem& p, std::vector<int>& nodes);
+// Column modes [m0, m1) of node t's matricisation.
+int node_col_modes(const Problem& p, int t, int* m0, int* m1);
+// Amplitudes (n_bumps) and per-mode factors (n_bumps x order x n) of one bump sample.
+void bump_factors_host(int order, long n, int n_bumps, uint64_t seed, int64_t sample, float* amp, float* fac);
 // Generators shared by the entry points (ht_api.cpp).
 std::vector<double> orthonormal_factor(uint64_t seed, uint32_t stream, long n, int r);
 ht_status generate_from_core(ht_handle_t h, int order, const int64_t* dims, const int32_t* core_dims,

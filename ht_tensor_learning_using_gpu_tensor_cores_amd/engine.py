@@ -65,10 +65,18 @@ def default_handle(device=0):
 
 
 def make_options(method="ltr", precision="bf16", err="residual", oversample=8, power_iters=1, seed=1234,
-                 eig_tol=1e-10, max_eig_iters=200, want_interior_u=False):
+                 eig_tol=1e-10, max_eig_iters=200, want_interior_u=False, sketch=None, sketch_tol=1e-3):
+    """ht_options. `sketch` (a sketch.Sketch on the device) selects the learned Khatri-Rao
+    test matrix of the randomised method, with the Gaussian one as its fallback above
+    `sketch_tol`."""
     lib = _lib.load()
     o = _lib.Options()
     lib.ht_default_options(ctypes.byref(o))
+    if sketch is not None:
+        o.sketch = _lib.SKETCH["learned"]
+        o.sketch_width = int(sketch.width)
+        o.sketch_weights = ctypes.c_void_p(sketch.weights.data_ptr())
+        o.sketch_tol = float(sketch_tol)
     o.method = _lib.METHOD[method]
     o.precision = _lib.PRECISION[precision]
     o.err_mode = _lib.ERR_MODE[err]
@@ -173,7 +181,9 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
     _lib.check(status, "ht_decompose")
     res = HTensor(dims=dims, tree=tree, ranks=ranks, packed=out, rel_err=rel.value, want_interior_u=want_interior_u,
                   stats=dict(total_ms=st.total_ms, h2d_ms=st.h2d_ms, passes=st.bytes_x_passes,
-                             eig_iters_max=st.eig_iters_max, eig_calls=st.eig_calls))
+                             eig_iters_max=st.eig_iters_max, eig_calls=st.eig_calls,
+                             sketch_fallbacks=st.sketch_fallbacks, sketch_nodes=st.sketch_nodes,
+                             sketch_resid=st.sketch_resid))
     _unpack(res, layout)
     return res
 

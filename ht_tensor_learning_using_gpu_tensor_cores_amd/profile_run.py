@@ -15,6 +15,13 @@ Workloads (generated on the device):
   c4  4096 3rd-order tensors, mode sizes U{16..64}, Tucker ranks U{4..8}, 1e-3 noise, seed 3,
       tree {1},{2,3}: the batched entry point (one persistent launch); reports aggregate
       decompositions/s and the p50/p99 relative errors
+  c5  learned per-mode sketch on 128^4 Gaussian-bump tensors, tree {1,2},{3,4}: trains the
+      sketch (width --width, default 20) on --count tensors of seed 4 (generated on the
+      device one at a time), evaluates the sketch loss on 200 held-out tensors of seed 5
+      against the untrained Gaussian sketch, then decomposes --reps held-out tensors at
+      ranks min(12, width) with the randomised method (BF16, no power iteration) using the
+      learned sketch (fallback above --tol), the Gaussian test matrix, and the plain method
+      (the exact judge); --save writes the trained sketch (BF16) to the given .npz
 """
 import argparse
 import time
@@ -36,8 +43,12 @@ def workload(name):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
-    ap.add_argument("--count", type=int, default=4096, help="c4: number of tensors")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--count", type=int, default=None, help="c4: number of tensors (4096); c5: training tensors (2000)")
+    ap.add_argument("--width", type=int, default=20, help="c5: sketch width")
+    ap.add_argument("--epochs", type=int, default=1, help="c5: training epochs")
+    ap.add_argument("--save", default=None, help="c5: .npz for the trained sketch")
+    ap.add_argument("--tol", type=float, default=1e-3, help="c5: residual tolerance of the Gaussian fallback")
     ap.add_argument("--method", default=None)
     ap.add_argument("--precision", default=None)
     ap.add_argument("--reps", type=int, default=3)
@@ -47,7 +58,11 @@ def main():
     from . import engine as E
     from . import _lib
     if args.workload == "c4":
+        args.count = args.count or 4096
         return batched(args, torch, E)
+    if args.workload == "c5":
+        args.count = args.count or 2000
+        return learned(args, torch, E)
     w = workload(args.workload)
     method = args.method or w["method"]
     prec = args.precision or w["precision"]
@@ -90,6 +105,50 @@ def batched(args, torch, E):
         print(f"c4 rep {i}: device {st['total_ms']:.3f} ms ({args.count / st['total_ms'] * 1e3:.0f} decomps/s, "
               f"{gb / st['total_ms'] * 1e3:.1f} GB/s), wall {wall * 1e3:.1f} ms, rel_err p50 {np.percentile(err, 50):.3e} "
               f"p99 {np.percentile(err, 99):.3e} max {err.max():.3e}", flush=True)
+
+
+def learned(args, torch, E):
+    import numpy as np
+    from . import sketch as SK
+    from . import tree as T
+    n, w = 128, args.width
+    tr = T.balanced_4d()
+    init = SK.gaussian([n] * 4, w, seed=7)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sk, curve = SK.train_bumps(n, 4, tr, w, count=args.count, seed=4, epochs=args.epochs, batch=8, lr=0.05,
+                               init=init)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"c5 w={w}: trained on {args.count} tensors x {args.epochs} epochs in {dt:.1f} s "
+          f"({dt / (args.count * args.epochs) * 1e3:.2f} ms per tensor step); epoch losses {curve}", flush=True)
+    skb = sk.bf16()
+    if args.save:
+        skb.save(args.save)
+    t0 = time.perf_counter()
+    lg = SK.eval_bumps(init, n, 4, tr, 200, seed=5)
+    ll = SK.eval_bumps(skb, n, 4, tr, 200, seed=5)
+    print(f"c5 w={w}: held-out sketch loss (200 tensors): gaussian mean {lg.mean():.4e} p50 {np.median(lg):.4e}; "
+          f"learned(bf16) mean {ll.mean():.4e} p50 {np.median(ll):.4e}; ratio {ll.mean() / lg.mean():.3f} "
+          f"({time.perf_counter() - t0:.1f} s)", flush=True)
+    k = min(12, w)
+    ranks = [1] + [k] * 6
+    for i in range(args.reps):
+        x = E.generate_bumps(n, 4, 5, i)
+        torch.cuda.synchronize()
+        out = {}
+        for name, kw in (("learned", dict(method="rsvd", precision="bf16", power_iters=0, sketch=skb,
+                                          sketch_tol=args.tol)),
+                         ("gaussian", dict(method="rsvd", precision="bf16", power_iters=0, oversample=w - k)),
+                         ("plain", dict(method="plain", precision="fp32"))):
+            r = E.decompose(x, [n] * 4, tr, ranks, **kw)
+            out[name] = r
+        print(f"c5 tensor {i} ranks {k}: rel_err learned {out['learned'].rel_err:.6e} "
+              f"({out['learned'].stats['total_ms']:.2f} ms, resid est {out['learned'].stats['sketch_resid']:.2e}, "
+              f"fallbacks {out['learned'].stats['sketch_fallbacks']}), gaussian {out['gaussian'].rel_err:.6e} "
+              f"({out['gaussian'].stats['total_ms']:.2f} ms), plain {out['plain'].rel_err:.6e} "
+              f"({out['plain'].stats['total_ms']:.2f} ms)", flush=True)
+        del x
 
 
 if __name__ == "__main__":

@@ -88,7 +88,22 @@ typedef struct {
   uint64_t seed;           /* seed of the Philox test matrices (1234)                    */
   int32_t want_interior_u; /* also return U_t of interior non-root nodes                 */
   int32_t check_finite;    /* scan the input for NaN/Inf first (default 1)               */
+  /* Test matrix of HT_METHOD_RSVD (ht_sketch_kind). With HT_SKETCH_LEARNED the range
+   * finder of node t uses Omega_t = Khatri-Rao product of the per-mode sketches of t's
+   * column modes (width sketch_width, which replaces k + oversample), and an FP32
+   * a-posteriori estimate of the range residual ||(I - QQ^T) X_(t)||_F / ||X||_F from four
+   * Gaussian probe vectors decides: above sketch_tol the node is redone with the dense
+   * Gaussian test matrix. */
+  int32_t sketch;          /* ht_sketch_kind (default HT_SKETCH_GAUSSIAN)                */
+  int32_t sketch_width;    /* width w of the learned sketch                              */
+  const float* sketch_weights; /* device: per-mode sketches at ht_sketch_layout offsets  */
+  double sketch_tol;       /* residual estimate above which the Gaussian is used (1e-3)  */
 } ht_options;
+
+typedef enum {
+  HT_SKETCH_GAUSSIAN = 0,  /* dense Philox Gaussian test matrix (the reference's choice)  */
+  HT_SKETCH_LEARNED = 1    /* learned per-mode Khatri-Rao sketch (see ht_sketch_*)        */
+} ht_sketch_kind;
 
 /* Timing and accounting of the last call (filled when stats != NULL). */
 typedef struct {
@@ -98,6 +113,9 @@ typedef struct {
   double bytes_x_passes;   /* number of full passes over X made by the decomposition     */
   int32_t eig_iters_max;   /* largest subspace-iteration count of any node               */
   int32_t eig_calls;       /* number of node eigen problems solved                       */
+  int32_t sketch_fallbacks;/* nodes redone with the Gaussian test matrix                 */
+  int32_t sketch_nodes;    /* nodes whose range finder used the learned sketch first     */
+  double sketch_resid;     /* largest range-residual estimate of the learned sketch      */
 } ht_stats;
 
 typedef struct ht_handle_s* ht_handle_t;
@@ -198,6 +216,52 @@ ht_status ht_batch_shapes(int32_t count, uint64_t seed, int32_t lo, int32_t hi, 
  * 4096 + 16 b (core), + 1 + m (factor of mode m), + 15 (noise); written at x_dev + x_offsets[b]. */
 ht_status ht_generate_batch(ht_handle_t handle, int32_t count, const int64_t* dims, const int32_t* core,
                             uint64_t seed, double noise_rel, float* x_dev, const int64_t* x_offsets);
+
+/* ---- learned per-mode sketch ------------------------------------------------------
+ * No counterpart in the reference, whose range finder draws a dense Gaussian test
+ * matrix (fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/kernel.cu:419-444).
+ * Mode m owns a sketch S_m (width x dims[m], row-major) at offsets[m] of one flat FP32
+ * array; the nodes sketched are those HT_METHOD_RSVD runs a range finder on (a non-root
+ * interior node whose sibling is not handled through it), and node t's test matrix is
+ *   Omega_t[c, j] = prod_{m in cols(t)} S_m[j, i_m(c)],
+ * cols(t) being the modes of t's matricisation columns (column-major index c).
+ * The loss of one tensor is the mean over the sketched nodes of the energy fraction the
+ * sketched range misses, 1 - ||P_Y X_(t)||_F^2 / ||X||_F^2 with Y = X_(t) Omega_t, computed
+ * with FP32 MFMA products and an FP64 solve (relative ridge 1e-6 on Y^T Y). */
+typedef struct {
+  int32_t width;   /* w, 1..64 (default 20)                                   */
+  int32_t epochs;  /* passes over the training tensors (default 2)            */
+  int32_t batch;   /* tensors per Adam step (default 8)                       */
+  float lr;        /* Adam step size (default 0.05)                           */
+  float beta1;     /* Adam moments (0.9, 0.999) and epsilon (1e-8)            */
+  float beta2;
+  float eps;
+} ht_sketch_train_options;
+void ht_sketch_default_train_options(ht_sketch_train_options* opt);
+/* offsets[order] and *total (floats) of the flat sketch array. */
+ht_status ht_sketch_layout(int32_t order, const int64_t* dims, int32_t width, int64_t* offsets, int64_t* total);
+/* i.i.d. N(0,1) sketches (Philox stream 200 of `seed`), the untrained starting point. */
+ht_status ht_sketch_init_gaussian(ht_handle_t handle, int32_t order, const int64_t* dims, int32_t width,
+                                  uint64_t seed, float* sketch_dev);
+/* Loss of one device tensor and, when grad_dev != NULL, its gradient (written, same
+ * layout as the sketch; modes outside every sketched node get zeros). loss: host. */
+ht_status ht_sketch_loss_grad(ht_handle_t handle, const float* x_dev, int32_t order, const int64_t* dims,
+                              int32_t n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                              const int32_t* right, int32_t width, const float* sketch_dev, float* grad_dev,
+                              double* loss);
+/* Train on `count` Gaussian-bump tensors (ht_generate_bumps samples first_sample.. of
+ * `seed`, each generated on the device when it is used) with Adam; sketch_dev holds the
+ * starting point and receives the result; epoch_loss (host, opt->epochs) the mean loss
+ * of each epoch, taken before each step's update. */
+ht_status ht_sketch_train_bumps(ht_handle_t handle, int32_t order, int64_t n, int32_t n_bumps, uint64_t seed,
+                                int64_t first_sample, int64_t count, int32_t n_nodes, const int32_t* lo,
+                                const int32_t* hi, const int32_t* left, const int32_t* right,
+                                const ht_sketch_train_options* opt, float* sketch_dev, double* epoch_loss);
+/* Per-tensor loss (host, count) of a sketch on held-out bump tensors. */
+ht_status ht_sketch_eval_bumps(ht_handle_t handle, int32_t order, int64_t n, int32_t n_bumps, uint64_t seed,
+                               int64_t first_sample, int64_t count, int32_t n_nodes, const int32_t* lo,
+                               const int32_t* hi, const int32_t* left, const int32_t* right, int32_t width,
+                               const float* sketch_dev, double* loss);
 
 /* ---- compatibility wrappers keeping the reference names --------------------------- */
 
