@@ -45,7 +45,7 @@ class Stats(ctypes.Structure):
 class SketchTrainOptions(ctypes.Structure):
     _fields_ = [
         ("width", ctypes.c_int32), ("epochs", ctypes.c_int32), ("batch", ctypes.c_int32), ("lr", ctypes.c_float),
-        ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+        ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("min_loss", ctypes.c_float),
     ]
 
 

@@ -596,7 +596,7 @@ ht_status ht_sym_topk(ht_handle_t h, const double* G, int32_t n, int32_t k, doub
   ht_default_options(&e.opt);
   e.eig_iters_max = 0;
   e.noconv = 0;
-  CKS(e.sym_topk(G, n, k, U, w, tol, 500, 77));
+  CKS(e.sym_topk(G, n, k, U, w, tol, 500, 77, 0.0));
   if (iters) *iters = e.eig_iters_max;
   if (hipStreamSynchronize(h->stream) != hipSuccess) return HT_ERR_HIP;
   return e.noconv ? HT_ERR_NOCONV : HT_OK;

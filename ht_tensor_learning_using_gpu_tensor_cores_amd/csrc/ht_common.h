@@ -181,7 +181,7 @@ int power_launch(hipStream_t st, const double* G, long sG, const double* W, doub
 int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int try_chol, int batch);
 int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, const double* T, int n, int b, int k,
               double* Uf, double* U, long sU, double* w, long sw, double* wall, double* cut, double tol,
-              int* converged, int batch);
+              double floor, int* converged, double* rres, int batch);
 // Casts and sign canonicalisation.
 int cast_f64_f32_launch(hipStream_t st, const double* a, float* b, long n);
 int cast_f32_f64_launch(hipStream_t st, const float* a, double* b, long n);
@@ -235,7 +235,8 @@ int sketch_solve_launch(hipStream_t st, const double* W, const double* E, const 
 int tall_small_launch(hipStream_t st, const float* A, long rows, int w, const double* C, const float* B,
                       const double* beta, float* out);
 int adam_launch(hipStream_t st, float* S, float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
-                float scale, float bc1, float bc2);
+                float scale, float bc1, float bc2, const double* step_loss, double min_loss_sum);
+int fold_loss_launch(hipStream_t st, double* step, double* acc);
 int fill_f32_launch(hipStream_t st, float* a, long n, float val);
 int fill_f64_launch(hipStream_t st, double* a, long n, double val);
 // out[0] = sum_i ||P_i - Q QtP_i||^2 (Q: rows x b, P: rows x np, QtP: b x np FP64);

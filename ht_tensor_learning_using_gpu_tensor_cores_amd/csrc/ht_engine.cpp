@@ -220,6 +220,10 @@ int Engine::d2h(void* dst, const void* src, size_t bytes) {
   return 0;
 }
 
+// Eigenvalues of a Gram accumulated in FP32 (or from 16-bit inputs) below this fraction
+// of the largest are not resolved by it: their Ritz vectors are not refined further.
+static constexpr double GRAM_FLOOR = 1e-6;
+
 template <typename T>
 T* Engine::alloc(size_t n) {
   return (T*)arena->get(n * sizeof(T) + 16);
@@ -256,7 +260,7 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
 // Chebyshev filter (degree 8, 16, ...) that damps [0, smallest Ritz value]. A missed
 // round also widens a block narrower than 2k by 16 columns (up to min(64, n)).
 int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
-                           uint32_t stream) {
+                           uint32_t stream, double floor) {
   ProfScope ps(prof, st, PK_EIG);
   if (n <= 64) {
     double* V = alloc<double>((size_t)batch * n * n);
@@ -271,12 +275,12 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     eig_calls += batch;
     return 0;
   }
-  if (n > 256) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream);
+  if (n > 256) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream, floor);
   // The QR and Rayleigh-Ritz kernels stage n x b blocks in LDS: n * b <= 8192.
   const int bmax = std::min(std::min(64, n), 8192 / n);
   if (b <= 0) b = k + 8;
   b = std::min(std::max(b, k), bmax);
-  if (k > b) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream);
+  if (k > b) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream, floor);
   const size_t nbmax = (size_t)n * bmax;
   double* W = alloc<double>(batch * nbmax);
   double* Y = alloc<double>(batch * nbmax);
@@ -287,10 +291,12 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   double* wall = alloc<double>((size_t)batch * 64);
   double* cut = alloc<double>(batch);
   int* conv = alloc<int>(batch);
-  if (!W || !Y || !Q || !Z || !Uf || !T || !wall || !cut || !conv) return HT_ERR_ALLOC;
+  double* rres = alloc<double>(batch);
+  if (!W || !Y || !Q || !Z || !Uf || !T || !wall || !cut || !conv || !rres) return HT_ERR_ALLOC;
   long nb = (long)n * b;
   CK(fill_normals_f64_launch(st, W, batch * nb, opt.seed, stream));
   std::vector<int> hconv(batch);
+  std::vector<double> hwall((size_t)batch * 64);
   int q = std::max(1, q0), mode = 0, iters = 0;
   for (int round = 0;; ++round) {
     // CholeskyQR2 first, except on the first round after very few powers (blocks from
@@ -299,14 +305,31 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, nullptr, batch));
     CK(qr_launch(st, Y, Q, nb, n, b, try_chol, batch));
     CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, T, batch));
-    CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, conv, batch));
+    CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, floor, conv, rres, batch));
     iters += q + 1;
-    CK(d2h(hconv.data(), conv, sizeof(int) * batch));
+    HT_HIP_CHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * batch, hipMemcpyDeviceToHost, st));
+    HT_HIP_CHECK(hipMemcpyAsync(hwall.data(), wall, sizeof(double) * batch * 64, hipMemcpyDeviceToHost, st));
+    HT_HIP_CHECK(hipStreamSynchronize(st));
+    const int b_rr = b;  // block width of this round's Ritz values
     bool all = true;
     for (int i = 0; i < batch; ++i) all = all && hconv[i];
     if (all) break;
     if (iters >= opt.max_eig_iters) {
       noconv = 1;
+      if (getenv("HT_DEBUG")) {
+        std::vector<double> hw((size_t)batch * 64), hr(batch);
+        CK(d2h(hw.data(), wall, sizeof(double) * batch * 64));
+        CK(d2h(hr.data(), rres, sizeof(double) * batch));
+        for (int pb = 0; pb < batch; ++pb) {
+          if (hconv[pb]) continue;
+          const double* th = &hw[(size_t)pb * 64];
+          fprintf(stderr, "ht: sym_topk_batch n=%d k=%d b=%d tol=%g floor=%g: problem %d of %d not converged after %d"
+                  " iterations, residual/th1 %.3e; Ritz values (relative):", n, k, b, tol, floor, pb, batch, iters,
+                  hr[pb]);
+          for (int i = 0; i < b && i < 64; ++i) fprintf(stderr, " %.3e", th[i] / std::max(std::fabs(th[0]), 1e-300));
+          fprintf(stderr, "\n");
+        }
+      }
       break;
     }
     if (b < std::min(bmax, 2 * k)) {
@@ -321,7 +344,25 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     } else {
       std::swap(W, Uf);
     }
+    // Degree of the next filter. Between th_1 and the last wanted Ritz value th_k above
+    // the floor the Chebyshev filter on [0, cut] amplifies by T_d(x_1) / T_d(x_k) ~
+    // exp(d (acosh x_1 - acosh x_k)), x = 2 th / cut - 1; keeping that below 1e12 keeps
+    // every wanted direction in the FP64 block (a spectrum spanning decades otherwise
+    // collapses onto its top vectors).
+    double spread = 0.0;
+    for (int pb = 0; pb < batch; ++pb) {
+      if (hconv[pb]) continue;
+      const double* th = &hwall[(size_t)pb * 64];
+      const double e = th[b_rr - 1];
+      if (!(e > 0.0)) continue;
+      int kk = 0;
+      for (int i = 0; i < k; ++i)
+        if (std::fabs(th[i]) > floor * std::fabs(th[0])) kk = i;
+      const double x1 = std::max(1.0, 2.0 * th[0] / e - 1.0), xk = std::max(1.0, 2.0 * th[kk] / e - 1.0);
+      spread = std::max(spread, std::acosh(x1) - std::acosh(xk));
+    }
     q = round == 0 ? 8 : std::min(2 * q, 64);
+    if (spread > 0.0) q = std::max(1, std::min(q, (int)(std::log(1e12) / spread)));
     mode = 2;
   }
   eig_iters_max = std::max(eig_iters_max, iters);
@@ -329,13 +370,15 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   return 0;
 }
 
-int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream) {
+int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
+                     double floor) {
   (void)maxit;
-  return sym_topk_batch(G, n, k, 1, U, w, tol, 2, 0, stream);
+  return sym_topk_batch(G, n, k, 1, U, w, tol, 2, 0, stream, floor);
 }
 
 // Subspace iteration for n > 256 (multi-kernel, host-checked convergence).
-int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream) {
+int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
+                           double floor) {
   ProfScope ps(prof, st, PK_EIG);
   int* info = alloc<int>(4);
   int b = std::min(64, std::max(2 * k, k + 16));
@@ -372,15 +415,21 @@ int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, 
     CK(ritz_resid_launch(st, YS, QS, th, n, k, res));
     CK(d2h(hres.data(), res, sizeof(double) * k));
     CK(d2h(hth.data(), th, sizeof(double) * b));
-    double rmax = 0.0;
-    for (int i = 0; i < k; ++i) rmax = std::max(rmax, hres[i]);
     const double scale = std::max(std::fabs(hth[0]), 1e-300);
+    double rmax = 0.0;
+    for (int i = 0; i < k; ++i)
+      if (std::fabs(hth[i]) > floor * scale) rmax = std::max(rmax, hres[i]);
     done = (rmax <= tol * scale) || (it >= maxit) || (hth[0] == 0.0);
     std::swap(Q, QS);  // continue from the Ritz basis
     if (done) {
       HT_HIP_CHECK(hipMemcpyAsync(U, Q, sizeof(double) * n * k, hipMemcpyDeviceToDevice, st));
       HT_HIP_CHECK(hipMemcpyAsync(w, th, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
-      if (rmax > tol * scale && hth[0] != 0.0) noconv = 1;
+      if (rmax > tol * scale && hth[0] != 0.0) {
+        noconv = 1;
+        if (getenv("HT_DEBUG"))
+          fprintf(stderr, "ht: sym_topk_large n=%d k=%d: residual %.3e > %.3e after %d iterations\n", n, k, rmax,
+                  tol * scale, it);
+      }
     }
     steps = std::min(8, steps * 2);
   }
@@ -469,7 +518,15 @@ int Engine::exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U,
     }
     const bool conv = (it >= 2 && diff <= tol * std::max(cur[0], 1e-300)) || cur[0] == 0.0;
     if (conv || it >= opt.max_eig_iters) {
-      if (!conv) noconv = 1;
+      if (!conv) {
+        noconv = 1;
+        if (getenv("HT_DEBUG")) {
+          fprintf(stderr, "ht: exact_svd_subspace rows=%ld cols=%ld k=%d: change %.3e > %.3e after %d iterations;"
+                  " sigma^2 (relative):", rows, cols, kk, diff, tol * cur[0], it);
+          for (int i = 0; i < b; ++i) fprintf(stderr, " %.3e", cur[i] / std::max(cur[0], 1e-300));
+          fprintf(stderr, "\n");
+        }
+      }
       break;
     }
   }
@@ -613,7 +670,7 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
       CK(gram(x, view_of(p.dims, p.order, m, m + 1), prec, G + (size_t)i * n * n, have_norm ? nullptr : xsq_dev));
       have_norm = true;
     }
-    CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16));
+    CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
     for (int i = 0; i < nb; ++i) {
       Ufull[grp[i]] = U + (size_t)i * n * k;
       CK(eig_to_sv(w + (size_t)i * k, k, SVd[grp[i]]));
@@ -644,7 +701,7 @@ int Engine::run_plain(const float* x) {
       double* G = alloc<double>((size_t)rows * rows);
       double* w = alloc<double>(k);
       CK(gram(x, v, PREC_F32, G, nullptr));
-      CK(sym_topk(G, (int)rows, k, Ufull[t], w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16));
+      CK(sym_topk(G, (int)rows, k, Ufull[t], w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16, GRAM_FLOOR));
       CK(eig_to_sv(w, k, SVd[t]));
     } else if (cols <= 1024 && (p.lo[t] == 0 || p.hi[t] == p.order)) {
       // Gram of the complementary (contiguous) mode range, then U = X_(t) V / sigma via TSQR.
@@ -653,7 +710,7 @@ int Engine::run_plain(const float* x) {
       double* Vc = alloc<double>((size_t)cols * k);
       double* w = alloc<double>(k);
       CK(gram(x, vc, PREC_F32, G, nullptr));
-      CK(sym_topk(G, (int)cols, k, Vc, w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16));
+      CK(sym_topk(G, (int)cols, k, Vc, w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16, GRAM_FLOOR));
       CK(eig_to_sv(w, k, SVd[t]));
       float* Vf = alloc<float>((size_t)cols * k);
       float* Y = alloc<float>((size_t)rows * k);
@@ -777,10 +834,10 @@ int Engine::run_ltr(const float* x) {
   double* w0 = alloc<double>(r0 + r1);
   double* w1 = w0 + r0;
   if (n0 == n1 && r0 == r1 && G1 == G0 + (size_t)n0 * n0) {
-    CK(sym_topk_batch(G0, (int)n0, r0, 2, Ufull[t0], w0, 1e-9, 2, 0, 1000 + t0 * 16));
+    CK(sym_topk_batch(G0, (int)n0, r0, 2, Ufull[t0], w0, 1e-9, 2, 0, 1000 + t0 * 16, GRAM_FLOOR));
   } else {
-    CK(sym_topk_batch(G0, (int)n0, r0, 1, Ufull[t0], w0, 1e-9, 2, 0, 1000 + t0 * 16));
-    CK(sym_topk_batch(G1, (int)n1, r1, 1, Ufull[t1], w1, 1e-9, 2, 0, 1000 + t1 * 16));
+    CK(sym_topk_batch(G0, (int)n0, r0, 1, Ufull[t0], w0, 1e-9, 2, 0, 1000 + t0 * 16, GRAM_FLOOR));
+    CK(sym_topk_batch(G1, (int)n1, r1, 1, Ufull[t1], w1, 1e-9, 2, 0, 1000 + t1 * 16, GRAM_FLOOR));
   }
   CK(eig_to_sv(w0, r0, SVd[t0]));
   CK(eig_to_sv(w1, r1, SVd[t1]));
@@ -825,7 +882,7 @@ int Engine::run_ltr(const float* x) {
         CK(gram(Z, view_of(zd, d, grp[i], grp[i] + 1), PREC_F32, G + (size_t)i * n * n, nullptr));
         passes -= 1.0;  // Z is small; not a pass over X
       }
-      CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16));
+      CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
       for (int i = 0; i < nb; ++i) {
         const int t = p.leaf_of_mode(grp[i]);
         Ufull[t] = U + (size_t)i * n * k;
@@ -929,7 +986,7 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
       double* G = alloc<double>((size_t)M * M);
       if (!G) return HT_ERR_ALLOC;
       CK(gram_pms_f64_launch(st, C, v, G));
-      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-5, 4, 32, 4000 + t * 16));
+      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-5, 4, 32, 4000 + t * 16, 1e-12));
     }
     CK(eig_to_sv(w, k, SVd[t]));
     Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself

@@ -84,10 +84,15 @@ struct Engine {
   int sync();
   int d2h(void* dst, const void* src, size_t bytes);
   int gram(const float* x, PMS v, int prec, double* G, double* sumsq);
-  int sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream);
+  // Top-k eigenpairs of symmetric G (FP64). A Ritz pair counts as converged when its
+  // residual is <= tol * th_1 or its Ritz value is <= floor * th_1 (below what the Gram's
+  // own accumulation error resolves, where any basis of the cluster serves).
+  int sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
+               double floor);
   int sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
-                     uint32_t stream);
-  int sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream);
+                     uint32_t stream, double floor);
+  int sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
+                     double floor);
   int tsqr(float* Y, long m, int b, float* Q, double* R);
   int view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y);
   int exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U, double* Wr, double* sv, uint32_t stream);

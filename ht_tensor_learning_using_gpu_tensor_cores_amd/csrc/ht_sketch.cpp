@@ -88,6 +88,7 @@ void ht_sketch_default_train_options(ht_sketch_train_options* o) {
   o->beta1 = 0.9f;
   o->beta2 = 0.999f;
   o->eps = 1e-8f;
+  o->min_loss = 1e-5f;
 }
 
 ht_status ht_sketch_layout(int32_t order, const int64_t* dims, int32_t width, int64_t* offsets, int64_t* total) {
@@ -196,7 +197,7 @@ ht_status ht_sketch_train_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t
   else ht_sketch_default_train_options(&opt);
   if (!sketch_dev || !epoch_loss || !bump_args_ok(order, n, n_bumps, first_sample, count) || opt.epochs < 1 ||
       opt.batch < 1 || !(opt.lr > 0) || !(opt.beta1 >= 0 && opt.beta1 < 1) || !(opt.beta2 >= 0 && opt.beta2 < 1) ||
-      !(opt.eps > 0))
+      !(opt.eps > 0) || !(opt.min_loss >= 0))
     return HT_ERR_INVALID_ARG;
   std::vector<int64_t> dims(order, n);
   SketchCtx c;
@@ -209,9 +210,10 @@ ht_status ht_sketch_train_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t
   float* m2 = e.alloc<float>(c.total);
   double* xsq = e.alloc<double>(1 + 1024);
   double* lacc = e.alloc<double>(1);
+  double* sacc = e.alloc<double>(1);
   int* info = e.alloc<int>(1);
   float *amp_d = nullptr, *fac_d = nullptr;
-  if (!X || !grad || !m1 || !m2 || !xsq || !lacc || !info) return HT_ERR_ALLOC;
+  if (!X || !grad || !m1 || !m2 || !xsq || !lacc || !sacc || !info) return HT_ERR_ALLOC;
   CKS(upload_bumps(e, order, n, n_bumps, seed, first_sample, count, &amp_d, &fac_d));
   CKS(fill_f32_launch(e.st, grad, c.total, 0.f));
   CKS(fill_f32_launch(e.st, m1, c.total, 0.f));
@@ -220,6 +222,7 @@ ht_status ht_sketch_train_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t
   const size_t per_f = (size_t)n_bumps * order * n;
   const float node_scale = 1.0f / (float)c.nodes.size();
   long step = 0;
+  CKS(fill_f64_launch(e.st, sacc, 1, 0.0));
   for (int ep = 0; ep < opt.epochs; ++ep) {
     CKS(fill_f64_launch(e.st, lacc, 1, 0.0));
     int in_step = 0;
@@ -227,14 +230,16 @@ ht_status ht_sketch_train_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t
       CKS(gen_bumps_launch(e.st, X, order, n, fac_d + (size_t)s * per_f, amp_d + (size_t)s * n_bumps, n_bumps));
       CKS(sumsq_f32_launch(e.st, X, c.p.N, xsq));
       const auto mk = h->arena.mark();
-      for (int t : c.nodes) CKS(e.sketch_node(X, t, sketch_dev, c.off, w, grad, node_scale, lacc, xsq, info));
+      for (int t : c.nodes) CKS(e.sketch_node(X, t, sketch_dev, c.off, w, grad, node_scale, sacc, xsq, info));
       h->arena.rewind(mk);
       ++in_step;
       if (in_step == opt.batch || s == count - 1) {
         ++step;
         const float bc1 = 1.0f - std::pow(opt.beta1, (float)step), bc2 = 1.0f - std::pow(opt.beta2, (float)step);
         CKS(adam_launch(e.st, sketch_dev, grad, m1, m2, c.total, opt.lr, opt.beta1, opt.beta2, opt.eps,
-                        1.0f / (float)in_step, bc1, bc2));
+                        1.0f / (float)in_step, bc1, bc2, sacc,
+                        (double)opt.min_loss * in_step * (double)c.nodes.size()));
+        CKS(fold_loss_launch(e.st, sacc, lacc));
         in_step = 0;
       }
     }

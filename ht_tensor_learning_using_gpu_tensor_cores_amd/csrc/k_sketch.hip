@@ -252,11 +252,19 @@ __global__ __launch_bounds__(256) void tall_small_kernel(const float* __restrict
   }
 }
 
-// Adam on the sketch: g is scaled by `scale` (1 / tensors per step) and cleared.
+// Adam on the sketch: g is scaled by `scale` (1 / tensors per step) and cleared. The step
+// is skipped (g only cleared) when the step's summed loss is below min_loss_sum: at that
+// level the gradient is rounding noise of the FP32 products.
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ S, float* __restrict__ g, float* __restrict__ m,
                                                    float* __restrict__ v, long n, float lr, float b1, float b2,
-                                                   float eps, float scale, float bc1, float bc2) {
+                                                   float eps, float scale, float bc1, float bc2,
+                                                   const double* __restrict__ step_loss, double min_loss_sum) {
+  const bool skip = step_loss[0] < min_loss_sum;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    if (skip) {
+      g[i] = 0.f;
+      continue;
+    }
     const float gi = g[i] * scale;
     const float mi = b1 * m[i] + (1.f - b1) * gi;
     const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
@@ -367,9 +375,21 @@ int tall_small_launch(hipStream_t st, const float* A, long rows, int w, const do
 }
 
 int adam_launch(hipStream_t st, float* S, float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
-                float scale, float bc1, float bc2) {
+                float scale, float bc1, float bc2, const double* step_loss, double min_loss_sum) {
   hipLaunchKernelGGL(adam_kernel, dim3(grid_for(n)), dim3(256), 0, st, S, g, m, v, n, lr, b1, b2, eps, scale, bc1,
-                     bc2);
+                     bc2, step_loss, min_loss_sum);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// acc[0] += step[0]; step[0] = 0 (one thread).
+__global__ void fold_loss_kernel(double* __restrict__ step, double* __restrict__ acc) {
+  if (threadIdx.x == 0) {
+    acc[0] += step[0];
+    step[0] = 0.0;
+  }
+}
+int fold_loss_launch(hipStream_t st, double* step, double* acc) {
+  hipLaunchKernelGGL(fold_loss_kernel, dim3(1), dim3(64), 0, st, step, acc);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 

@@ -123,8 +123,9 @@ def loss_grad(x, dims, tree: Tree, sk: Sketch, want_grad=True, handle=None):
 
 
 def train_bumps(n, order, tree: Tree, width, count, seed=4, first=0, epochs=2, batch=8, lr=0.05, n_bumps=6,
-                init=None, init_seed=7, handle=None):
-    """Train per-mode sketches on `count` bump tensors (samples first.. of `seed`).
+                init=None, init_seed=7, min_loss=1e-5, handle=None):
+    """Train per-mode sketches on `count` bump tensors (samples first.. of `seed`); steps
+    whose mean loss is below `min_loss` are skipped (their gradient is rounding noise).
     Returns (trained Sketch in FP32, per-epoch mean training loss)."""
     lib = _lib.load()
     h = handle or default_handle()
@@ -135,6 +136,7 @@ def train_bumps(n, order, tree: Tree, width, count, seed=4, first=0, epochs=2, b
     o = _lib.SketchTrainOptions()
     lib.ht_sketch_default_train_options(ctypes.byref(o))
     o.width, o.epochs, o.batch, o.lr = int(width), int(epochs), int(batch), float(lr)
+    o.min_loss = float(min_loss)
     curve = (ctypes.c_double * int(epochs))()
     _lib.check(lib.ht_sketch_train_bumps(h.ptr, int(order), int(n), int(n_bumps), int(seed), int(first), int(count),
                                          *_tree_args(tree), ctypes.byref(o), ctypes.c_void_p(sk.weights.data_ptr()),

@@ -236,6 +236,8 @@ typedef struct {
   float beta1;     /* Adam moments (0.9, 0.999) and epsilon (1e-8)            */
   float beta2;
   float eps;
+  float min_loss;  /* a step whose mean loss is below this is skipped: there the
+                      gradient is rounding noise of the FP32 products (1e-5)   */
 } ht_sketch_train_options;
 void ht_sketch_default_train_options(ht_sketch_train_options* opt);
 /* offsets[order] and *total (floats) of the flat sketch array. */
