@@ -409,7 +409,8 @@ ht_status ht_generate_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t n_b
   ht_status st = HT_OK;
   if (hipMemcpy(fd, fac.data(), fac.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(ad, amp.data(), amp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      gen_bumps_launch(h->stream, x_dev, order, n, fd, ad, n_bumps) || hipStreamSynchronize(h->stream) != hipSuccess)
+      gen_bumps_launch(h->stream, x_dev, order, n, fd, ad, n_bumps, nullptr, nullptr) ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
     st = HT_ERR_HIP;
   (void)hipFree(fd);
   (void)hipFree(ad);

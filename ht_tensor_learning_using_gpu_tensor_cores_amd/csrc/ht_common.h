@@ -214,8 +214,12 @@ struct BatchItem {
 size_t batch_scratch_bytes(int blocks);
 int batch_ht_launch(hipStream_t st, const float* x, const void* items, const int* order, int count, float* out,
                     double* rel_err, double* scratch, int blocks, int kc);
+// Bump tensor from its factors (n_bumps x order x n) and amplitudes. With sumsq != null
+// also sumsq[0] = ||x||^2 of the FP32 values, part holding gen_bumps_planes(order, n)
+// doubles (order >= 2, n <= 1024, n_bumps <= 8, x 16-byte aligned; else an error).
+long gen_bumps_planes(int order, long n);
 int gen_bumps_launch(hipStream_t st, float* x, int order, long n, const float* factors, const float* amp,
-                     int n_bumps);
+                     int n_bumps, double* sumsq, double* part);
 
 // Learned per-mode sketch (k_sketch.hip). S_m is w x n_m row-major at off[m] of the
 // flat sketch array; a node's test matrix is the Khatri-Rao product over its column

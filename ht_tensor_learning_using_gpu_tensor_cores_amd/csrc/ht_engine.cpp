@@ -275,12 +275,18 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     eig_calls += batch;
     return 0;
   }
-  if (n > 256) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream, floor);
+  if (n > 256 || k > std::min(std::min(64, n), 8192 / n)) {
+    // one problem at a time (each its own Philox stream)
+    for (int i = 0; i < batch; ++i)
+      CK(sym_topk_large(G + (size_t)i * n * n, n, k, U + (size_t)i * n * k, w + (size_t)i * k, tol,
+                        opt.max_eig_iters, stream + 8 * i, floor));
+    return 0;
+  }
   // The QR and Rayleigh-Ritz kernels stage n x b blocks in LDS: n * b <= 8192.
   const int bmax = std::min(std::min(64, n), 8192 / n);
   if (b <= 0) b = k + 8;
   b = std::min(std::max(b, k), bmax);
-  if (k > b) return sym_topk_large(G, n, k, U, w, tol, opt.max_eig_iters, stream, floor);
+
   const size_t nbmax = (size_t)n * bmax;
   double* W = alloc<double>(batch * nbmax);
   double* Y = alloc<double>(batch * nbmax);

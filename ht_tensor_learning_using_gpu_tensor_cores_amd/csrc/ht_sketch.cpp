@@ -156,8 +156,10 @@ int upload_bumps(Engine& e, int order, long n, int n_bumps, uint64_t seed, int64
   return e.sync();  // the host vectors go out of scope
 }
 
+// The training and evaluation loops use the plane generator with its fused ||X||^2.
 int bump_args_ok(int order, long n, int n_bumps, int64_t first, int64_t count) {
-  return order >= 2 && order <= HT_MAX_ORDER && n >= 2 && n_bumps >= 1 && first >= 0 && count >= 1;
+  return order >= 2 && order <= HT_MAX_ORDER && n >= 2 && n <= 1024 && n_bumps >= 1 && n_bumps <= 8 && first >= 0 &&
+         count >= 1;
 }
 
 }  // namespace
@@ -208,12 +210,13 @@ ht_status ht_sketch_train_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t
   float* grad = e.alloc<float>(c.total);
   float* m1 = e.alloc<float>(c.total);
   float* m2 = e.alloc<float>(c.total);
-  double* xsq = e.alloc<double>(1 + 1024);
+  double* xsq = e.alloc<double>(1);
+  double* part = e.alloc<double>((size_t)gen_bumps_planes(order, n));
   double* lacc = e.alloc<double>(1);
   double* sacc = e.alloc<double>(1);
   int* info = e.alloc<int>(1);
   float *amp_d = nullptr, *fac_d = nullptr;
-  if (!X || !grad || !m1 || !m2 || !xsq || !lacc || !sacc || !info) return HT_ERR_ALLOC;
+  if (!X || !grad || !m1 || !m2 || !xsq || !part || !lacc || !sacc || !info) return HT_ERR_ALLOC;
   CKS(upload_bumps(e, order, n, n_bumps, seed, first_sample, count, &amp_d, &fac_d));
   CKS(fill_f32_launch(e.st, grad, c.total, 0.f));
   CKS(fill_f32_launch(e.st, m1, c.total, 0.f));
@@ -227,8 +230,8 @@ ht_status ht_sketch_train_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t
     CKS(fill_f64_launch(e.st, lacc, 1, 0.0));
     int in_step = 0;
     for (int64_t s = 0; s < count; ++s) {
-      CKS(gen_bumps_launch(e.st, X, order, n, fac_d + (size_t)s * per_f, amp_d + (size_t)s * n_bumps, n_bumps));
-      CKS(sumsq_f32_launch(e.st, X, c.p.N, xsq));
+      CKS(gen_bumps_launch(e.st, X, order, n, fac_d + (size_t)s * per_f, amp_d + (size_t)s * n_bumps, n_bumps, xsq,
+                           part));
       const auto mk = h->arena.mark();
       for (int t : c.nodes) CKS(e.sketch_node(X, t, sketch_dev, c.off, w, grad, node_scale, sacc, xsq, info));
       h->arena.rewind(mk);
@@ -264,18 +267,19 @@ ht_status ht_sketch_eval_bumps(ht_handle_t h, int32_t order, int64_t n, int32_t 
   CKS(c.init(h, order, dims.data(), n_nodes, lo, hi, left, right, width));
   Engine& e = c.e;
   float* X = e.alloc<float>((size_t)c.p.N);
-  double* xsq = e.alloc<double>(1 + 1024);
+  double* xsq = e.alloc<double>(1);
+  double* part = e.alloc<double>((size_t)gen_bumps_planes(order, n));
   double* lacc = e.alloc<double>(count);
   int* info = e.alloc<int>(1);
   float *amp_d = nullptr, *fac_d = nullptr;
-  if (!X || !xsq || !lacc || !info) return HT_ERR_ALLOC;
+  if (!X || !xsq || !part || !lacc || !info) return HT_ERR_ALLOC;
   CKS(upload_bumps(e, order, n, n_bumps, seed, first_sample, count, &amp_d, &fac_d));
   CKS(fill_f64_launch(e.st, lacc, count, 0.0));
   if (hipMemsetAsync(info, 0, sizeof(int), e.st) != hipSuccess) return HT_ERR_HIP;
   const size_t per_f = (size_t)n_bumps * order * n;
   for (int64_t s = 0; s < count; ++s) {
-    CKS(gen_bumps_launch(e.st, X, order, n, fac_d + (size_t)s * per_f, amp_d + (size_t)s * n_bumps, n_bumps));
-    CKS(sumsq_f32_launch(e.st, X, c.p.N, xsq));
+    CKS(gen_bumps_launch(e.st, X, order, n, fac_d + (size_t)s * per_f, amp_d + (size_t)s * n_bumps, n_bumps, xsq,
+                         part));
     const auto mk = h->arena.mark();
     for (int t : c.nodes) CKS(e.sketch_node(X, t, sketch_dev, c.off, width, nullptr, 0.f, lacc + s, xsq, info));
     h->arena.rewind(mk);

@@ -204,21 +204,42 @@ int dgemm_small_launch(hipStream_t st, int ta, int tb, int m, int n, int k, doub
 
 // ------------------------------------------------------------------ cholqr
 // One work-group. Z (m x b, FP64, ld m) is orthonormalised in place by two Cholesky-QR
-// passes; a column whose Cholesky pivot falls below 1e-13 of the largest diagonal is
-// replaced by a Philox Gaussian vector and the pass restarts. W (b x b) is scratch;
-// R (b x b) receives the product of the two triangular factors.
+// passes after scaling its columns to unit norm; a column whose Cholesky pivot falls
+// below 1e-13 of the largest diagonal is replaced by a Philox Gaussian vector and the
+// passes restart. W (b x b) is scratch; R (b x b) receives the product of the two
+// triangular factors and the column scaling (Z_in = Z_out R).
 __global__ __launch_bounds__(256) void cholqr_kernel(double* __restrict__ Z, int m, int b, double* __restrict__ W,
                                                      double* __restrict__ Rout, uint64_t seed, uint32_t stream,
                                                      int* __restrict__ info) {
   __shared__ double R[JMAX * JLD];
   __shared__ double Racc[JMAX * JLD];
+  __shared__ double cn[JMAX];
   __shared__ int bad;
   __shared__ double dmax;
   const int tid = threadIdx.x;
-  for (int e = tid; e < b * b; e += 256) Racc[(e % b) * JLD + e / b] = (e % b == e / b) ? 1.0 : 0.0;
   int passes = 0, restarts = 0;
   while (passes < 2 && restarts < 4 * JMAX) {
     __syncthreads();
+    if (passes == 0) {
+      // Equilibrate: unit columns (the span is unchanged). A block G Q whose columns carry
+      // a spread spectrum (norms th_1 .. th_b) is then well conditioned, which keeps the
+      // Cholesky factor, and the small directions, accurate. Racc starts as diag(norms).
+      if (tid < b) {
+        double s = 0.0;
+        for (int r = 0; r < m; ++r) s += Z[r + (long)m * tid] * Z[r + (long)m * tid];
+        cn[tid] = sqrt(s);
+      }
+      __syncthreads();
+      for (int e = tid; e < m * b; e += 256) {
+        const int j = e / m;
+        if (cn[j] > 0.0) Z[(long)e] /= cn[j];
+      }
+      for (int e = tid; e < b * b; e += 256) {
+        const int i = e % b, j = e / b;
+        Racc[i * JLD + j] = (i == j) ? (cn[i] > 0.0 ? cn[i] : 1.0) : 0.0;
+      }
+      __syncthreads();
+    }
     // W = Z^T Z (upper), one thread per (i <= j).
     for (int e = tid; e < b * b; e += 256) {
       const int i = e % b, j = e / b;
@@ -260,7 +281,6 @@ __global__ __launch_bounds__(256) void cholqr_kernel(double* __restrict__ Z, int
         Z[r + (long)m * j] = philox_normal(seed, stream, (uint64_t)(restarts + 1) * (uint64_t)m * JMAX + (uint64_t)j * m + r);
       ++restarts;
       passes = 0;
-      for (int e = tid; e < b * b; e += 256) Racc[(e % b) * JLD + e / b] = (e % b == e / b) ? 1.0 : 0.0;
       __syncthreads();
       continue;
     }

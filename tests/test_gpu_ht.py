@@ -47,6 +47,8 @@ CASES_3D = [
     ((32, 24, 20), (4, 5, 3), "ref", [1, 6, 4, 4, 5]),
     ((48, 32, 16), (8, 8, 8), "first", [1, 8, 8, 8, 8]),
     ((128, 64, 40), (8, 8, 8), "first", [1, 8, 8, 8, 8]),
+    # two equal leaves above 256 share one batched call of the large eigensolver
+    ((264, 264, 12), (6, 6, 6), "first", [1, 6, 6, 6, 6]),
 ]
 
 # Tolerance on |err_engine - err_oracle| for the same algorithm: FP32 products for

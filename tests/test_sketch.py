@@ -294,3 +294,19 @@ def test_gpu_sketch_save_load_roundtrip(trained, tmp_path):
     sk2 = ht.sketch.Sketch.load(p)
     assert sk2.dims == trained["sk"].dims and sk2.width == trained["sk"].width
     assert torch.equal(sk2.weights, trained["sk"].bf16().weights)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,precision,tol", [("rsvd", "bf16", 2e-3), ("ltr", "bf16", 2e-3), ("plain", "fp32", 1e-4)])
+def test_gpu_ranks_above_numerical_rank(method, precision, tol):
+    # Bump tensors have CP rank 6 with mode spectra spanning five decades; ranks 12 ask the
+    # eigensolvers for vectors of a cluster at the Gram's rounding level. Every method
+    # must finish (no HT_ERR_NOCONV) with an error at that rounding level.
+    import ht_tensor_learning_using_gpu_tensor_cores_amd as ht
+    n = 64
+    t = ht.tree.balanced_4d()
+    for s in range(3):
+        print(f"ranks 12 on bump sample {s}, n={n}, {method}/{precision}", flush=True)
+        x = ht.generate_bumps(n, 4, 5, s)
+        r = ht.decompose(x, [n] * 4, t, [1] + [12] * 6, method=method, precision=precision, power_iters=0)
+        assert r.rel_err < tol, (s, r.rel_err)
