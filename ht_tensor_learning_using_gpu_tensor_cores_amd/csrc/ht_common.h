@@ -163,9 +163,18 @@ int jacobi_eig_batched_launch(hipStream_t st, const double* A, int n, long strid
 // C(m x n) = op(A) op(B) with op = transpose flags; multi-block; all FP64, column-major.
 int dgemm_small_launch(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double* A,
                        int lda, const double* B, int ldb, double beta, double* C, int ldc);
+// The same with the k range split over work-groups (small C, long k): ws holds
+// dgemm_splitk_slices(m, n, k) * m * n doubles; the slices are summed in a fixed order.
+int dgemm_splitk_slices(int m, int n, int k);
+int dgemm_splitk_launch(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double* A, int lda,
+                        const double* B, int ldb, double beta, double* C, int ldc, double* ws);
 // Cholesky QR with column replacement (Z: m x b FP64, in place); R out (b x b).
 int cholqr_launch(hipStream_t st, double* Z, int m, int b, double* W, double* R, uint64_t seed,
                   uint32_t stream, int* info);
+// One pass of shifted CholeskyQR from W = Z^T Z (first pass: column equilibration and
+// shift): Rinv (b x b) with Z Rinv orthonormal; Z is m x b.
+int chol_rinv_launch(hipStream_t st, const double* W, int b, long m, int first, double* Rinv, int* info);
+int refresh_zero_cols_launch(hipStream_t st, double* Z, long m, int b, double* W, uint64_t seed, uint32_t stream);
 int ritz_resid_launch(hipStream_t st, const double* GU, const double* U, const double* w, int n, int k,
                       double* out);
 int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G);

@@ -382,6 +382,24 @@ int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double
   return sym_topk_batch(G, n, k, 1, U, w, tol, 2, 0, stream, floor);
 }
 
+// Shifted CholeskyQR3 of Z (m x b, FP64, ld m) in place: three passes of W = Z^T Z (tiled
+// FP64 GEMM), the triangular inverse (one work-group) and Z <- Z R^-1; the first pass
+// equilibrates the columns and shifts W (chol_rinv_kernel). Zt is an m x b scratch block.
+static int cholqr3(hipStream_t st, double* Z, double* Zt, long m, int b, double* W, double* Rinv, int* info,
+                   uint64_t seed, uint32_t stream, double* ws) {
+  double* src = Z;
+  double* dst = Zt;
+  for (int pass = 0; pass < 3; ++pass) {
+    CK(dgemm_splitk_launch(st, 1, 0, b, b, (int)m, 1.0, src, (int)m, src, (int)m, 0.0, W, b, ws));
+    if (pass == 0) CK(refresh_zero_cols_launch(st, src, m, b, W, seed, stream));
+    CK(chol_rinv_launch(st, W, b, m, pass == 0, Rinv, info));
+    CK(dgemm_small_launch(st, 0, 0, (int)m, b, b, 1.0, src, (int)m, Rinv, b, 0.0, dst, (int)m));
+    std::swap(src, dst);
+  }
+  if (src != Z) HT_HIP_CHECK(hipMemcpyAsync(Z, src, sizeof(double) * m * b, hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
 // Subspace iteration for n > 256 (multi-kernel, host-checked convergence).
 int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
                            double floor) {
@@ -396,22 +414,25 @@ int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, 
   double* th = alloc<double>(b);
   double* W = alloc<double>((size_t)b * b);
   double* res = alloc<double>(k);
-  if (!Q || !Y || !T || !S || !th || !W || !res) return HT_ERR_ALLOC;
+  double* Zt = alloc<double>((size_t)n * b);
+  double* Ri = alloc<double>((size_t)b * b);
+  double* ws = alloc<double>((size_t)dgemm_splitk_slices(b, b, n) * b * b);
+  if (!Q || !Y || !T || !S || !th || !W || !res || !Zt || !Ri || !ws) return HT_ERR_ALLOC;
   CK(fill_normals_f64_launch(st, Q, (long)n * b, opt.seed, stream));
-  CK(cholqr_launch(st, Q, n, b, W, nullptr, opt.seed, stream + 1, info));
+  CK(cholqr3(st, Q, Zt, n, b, W, Ri, info, opt.seed, stream + 1, ws));
   int it = 0, steps = 2;
   std::vector<double> hres(k), hth(b);
   bool done = false;
   while (!done) {
     for (int s = 0; s < steps; ++s) {
       CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, G, n, Q, n, 0.0, Y, n));
-      CK(cholqr_launch(st, Y, n, b, W, nullptr, opt.seed, stream + 2 + it, info));
+      CK(cholqr3(st, Y, Zt, n, b, W, Ri, info, opt.seed, stream + 2 + it, ws));
       std::swap(Q, Y);
       ++it;
     }
     // Rayleigh-Ritz: Y = G Q, T = Q^T Y, T = S diag(th) S^T, U_full = Q S, G U_full = Y S.
     CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, G, n, Q, n, 0.0, Y, n));
-    CK(dgemm_small_launch(st, 1, 0, b, b, n, 1.0, Q, n, Y, n, 0.0, T, b));
+    CK(dgemm_splitk_launch(st, 1, 0, b, b, n, 1.0, Q, n, Y, n, 0.0, T, b, ws));
     CK(jacobi_eig_launch(st, T, b, b, th, S, b, info, 60, 1e-15));
     double* QS = alloc<double>((size_t)n * b);
     double* YS = alloc<double>((size_t)n * b);
@@ -774,7 +795,9 @@ int Engine::root_transfer(const float* x) {
   const PMS v{1, nl, nr};  // X as (nl x nr), A^T U_l
   CK(view_gemm(x, v, 1, Ulf, kl, PREC_F32, Y));
   CK(cast_f32_f64_launch(st, Y, Yd, nr * kl));
-  CK(dgemm_small_launch(st, 1, 0, kl, kr, (int)nr, 1.0, Yd, (int)nr, Ufull[r], (int)nr, 0.0, Bd[0], kl));
+  double* ws = alloc<double>((size_t)dgemm_splitk_slices(kl, kr, (int)nr) * kl * kr);
+  if (!ws) return HT_ERR_ALLOC;
+  CK(dgemm_splitk_launch(st, 1, 0, kl, kr, (int)nr, 1.0, Yd, (int)nr, Ufull[r], (int)nr, 0.0, Bd[0], kl, ws));
   return 0;
 }
 

@@ -178,27 +178,256 @@ int jacobi_eig_batched_launch(hipStream_t st, const double* A, int n, long strid
 }
 
 // ------------------------------------------------------------------ dgemm_small
-__global__ void dgemm_small_kernel(int ta, int tb, int m, int n, int k, double alpha, const double* __restrict__ A,
-                                   int lda, const double* __restrict__ B, int ldb, double beta,
-                                   double* __restrict__ C, int ldc) {
+// C (m x n) = alpha op(A) op(B) + beta C, FP64, column-major. 64 x 64 output tile per
+// work-group through LDS in steps of 16 along k; thread (tx, ty) of a 16 x 16 grid owns the
+// 4 x 4 outputs (tx + 16 a, ty + 16 b), so C is written with consecutive rows per lane.
+// FP64 FMA (same peak as FP64 MFMA on this GPU). Loads follow each operand's contiguous
+// dimension: A by rows (TA = 0) or by k (TA = 1), B by k (TB = 0) or by columns (TB = 1).
+constexpr int DG_T = 64, DG_K = 16;
+// Split-K: slice z = blockIdx.z covers k in [z kchunk, (z+1) kchunk) and writes its tile
+// to C + z cstride (kchunk = k, cstride = 0 without a split).
+template <int TA, int TB>
+__global__ __launch_bounds__(256) void dgemm_tiled_kernel(int m, int n, int k, double alpha,
+                                                          const double* __restrict__ A, int lda,
+                                                          const double* __restrict__ B, int ldb, double beta,
+                                                          double* __restrict__ C, int ldc, int kchunk, long cstride) {
+  __shared__ double As[DG_K][DG_T + 1];
+  __shared__ double Bs[DG_K][DG_T + 1];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const long i0 = (long)blockIdx.x * DG_T, j0 = (long)blockIdx.y * DG_T;
+  const int kb = blockIdx.z * kchunk;
+  const int ke = min(k, kb + kchunk);
+  C += (long)blockIdx.z * cstride;
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int k0 = kb; k0 < ke; k0 += DG_K) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int ii, kk;
+      if (TA == 0) { ii = e % DG_T; kk = e / DG_T; } else { kk = e % DG_K; ii = e / DG_K; }
+      const long gi = i0 + ii;
+      const int gk = k0 + kk;
+      double v = 0.0;
+      if (gi < m && gk < ke) v = TA == 0 ? A[gi + (long)lda * gk] : A[gk + (long)lda * gi];
+      As[kk][ii] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int jj, kk;
+      if (TB == 0) { kk = e % DG_K; jj = e / DG_K; } else { jj = e % DG_T; kk = e / DG_T; }
+      const long gj = j0 + jj;
+      const int gk = k0 + kk;
+      double v = 0.0;
+      if (gj < n && gk < ke) v = TB == 0 ? B[gk + (long)ldb * gj] : B[gj + (long)ldb * gk];
+      Bs[kk][jj] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < DG_K; ++kk) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[kk][tx + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][ty + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const long i = i0 + tx + 16 * a, j = j0 + ty + 16 * b;
+      if (i < m && j < n) {
+        double* c = C + i + (long)ldc * j;
+        *c = alpha * acc[a][b] + (beta == 0.0 ? 0.0 : beta * (*c));
+      }
+    }
+}
+
+// C[i + ldc j] = alpha sum_z P[z][i + m j] + beta C (deterministic order over z).
+__global__ void dgemm_splitk_reduce_kernel(const double* __restrict__ P, int m, int n, int nz, double alpha,
+                                           double beta, double* __restrict__ C, int ldc) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)m * n) return;
   const int i = (int)(e % m), j = (int)(e / m);
   double s = 0.0;
-  if (!ta && !tb) for (int l = 0; l < k; ++l) s += A[i + (long)lda * l] * B[l + (long)ldb * j];
-  else if (ta && !tb) for (int l = 0; l < k; ++l) s += A[l + (long)lda * i] * B[l + (long)ldb * j];
-  else if (!ta && tb) for (int l = 0; l < k; ++l) s += A[i + (long)lda * l] * B[j + (long)ldb * l];
-  else for (int l = 0; l < k; ++l) s += A[l + (long)lda * i] * B[j + (long)ldb * l];
+  for (int z = 0; z < nz; ++z) s += P[(long)z * m * n + e];
   double* c = C + i + (long)ldc * j;
   *c = alpha * s + (beta == 0.0 ? 0.0 : beta * (*c));
+}
+
+template <int TA, int TB>
+static void dgemm_dispatch(hipStream_t st, dim3 grid, int m, int n, int k, double alpha, const double* A, int lda,
+                           const double* B, int ldb, double beta, double* C, int ldc, int kchunk, long cstride) {
+  hipLaunchKernelGGL((dgemm_tiled_kernel<TA, TB>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                     ldc, kchunk, cstride);
+}
+
+static void dgemm_any(hipStream_t st, int ta, int tb, dim3 grid, int m, int n, int k, double alpha, const double* A,
+                      int lda, const double* B, int ldb, double beta, double* C, int ldc, int kchunk, long cstride) {
+  if (!ta && !tb) dgemm_dispatch<0, 0>(st, grid, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk, cstride);
+  else if (ta && !tb) dgemm_dispatch<1, 0>(st, grid, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk, cstride);
+  else if (!ta && tb) dgemm_dispatch<0, 1>(st, grid, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk, cstride);
+  else dgemm_dispatch<1, 1>(st, grid, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, kchunk, cstride);
 }
 
 int dgemm_small_launch(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double* A, int lda,
                        const double* B, int ldb, double beta, double* C, int ldc) {
   if (m < 1 || n < 1) return 0;
+  const dim3 grid((unsigned)((m + DG_T - 1) / DG_T), (unsigned)((n + DG_T - 1) / DG_T), 1);
+  if (grid.y > 65535) return 8;
+  dgemm_any(st, ta, tb, grid, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, k < 1 ? 1 : k, 0);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+int dgemm_splitk_slices(int m, int n, int k) {
+  const long tiles = (long)((m + DG_T - 1) / DG_T) * ((n + DG_T - 1) / DG_T);
+  long z = 512 / tiles;  // about two work-groups per CU
+  const long zk = k / 1024;  // at least 1024 of k per slice
+  if (z > zk) z = zk;
+  if (z > 1024) z = 1024;
+  return z < 1 ? 1 : (int)z;
+}
+
+int dgemm_splitk_launch(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double* A, int lda,
+                        const double* B, int ldb, double beta, double* C, int ldc, double* ws) {
+  if (m < 1 || n < 1) return 0;
+  const int nz = dgemm_splitk_slices(m, n, k);
+  if (nz == 1) return dgemm_small_launch(st, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  int kchunk = (k + nz - 1) / nz;
+  kchunk = (kchunk + DG_K - 1) / DG_K * DG_K;
+  const int z = (k + kchunk - 1) / kchunk;
+  const dim3 grid((unsigned)((m + DG_T - 1) / DG_T), (unsigned)((n + DG_T - 1) / DG_T), (unsigned)z);
+  dgemm_any(st, ta, tb, grid, m, n, k, 1.0, A, lda, B, ldb, 0.0, ws, m, kchunk, (long)m * n);
   const long tot = (long)m * n;
-  hipLaunchKernelGGL(dgemm_small_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ta, tb, m, n, k,
-                     alpha, A, lda, B, ldb, beta, C, ldc);
+  hipLaunchKernelGGL(dgemm_splitk_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ws, m, n, z,
+                     alpha, beta, C, ldc);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ chol_rinv
+// One pass of shifted CholeskyQR (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa,
+// SIAM J. Sci. Comput. 2020). From W = Z^T Z (b x b, b <= 64): on the first pass the
+// columns are equilibrated (S = diag(W)^-1/2) and W_s = S W S + sigma I with
+// sigma = 11 (m b + b (b + 1)) u tr(W_s), which keeps the Cholesky factor of a numerically
+// rank-deficient block defined; later passes use S = I, sigma = 0. Writes
+// Rinv = S (chol(W_s)^T)^-1 (upper triangular, column-major) so that Z Rinv has
+// orthonormal columns up to the pass's accuracy. info[0] |= 1 on a non-positive pivot.
+__global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict__ W, int b, long m, int first,
+                                                        double* __restrict__ Rinv, int* __restrict__ info) {
+  __shared__ double L[JMAX * JLD];
+  __shared__ double T[JMAX * JLD];
+  __shared__ double sc[JMAX];
+  __shared__ double shift;
+  const int tid = threadIdx.x;
+  if (tid < b) {
+    const double d = W[tid + (long)b * tid];
+    sc[tid] = (first && d > 0.0) ? 1.0 / sqrt(d) : 1.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < b * b; e += 256) {
+    const int i = e % b, j = e / b;
+    L[i * JLD + j] = sc[i] * W[e] * sc[j];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double tr = 0.0;
+    for (int i = 0; i < b; ++i) tr += L[i * JLD + i];
+    shift = first ? 11.0 * ((double)m * b + (double)b * (b + 1)) * 1.1102230246251565e-16 * tr : 0.0;
+  }
+  __syncthreads();
+  if (tid < b) L[tid * JLD + tid] += shift;
+  __syncthreads();
+  for (int k = 0; k < b; ++k) {
+    if (tid == 0) {
+      double d = L[k * JLD + k];
+      if (!(d > 0.0)) {
+        if (info) info[0] |= 1;
+        d = 1e-300;
+      }
+      L[k * JLD + k] = sqrt(d);
+    }
+    __syncthreads();
+    for (int i = k + 1 + tid; i < b; i += 256) L[i * JLD + k] /= L[k * JLD + k];
+    __syncthreads();
+    const int r = b - k - 1;
+    for (int e = tid; e < r * r; e += 256) {
+      const int i = k + 1 + e % r, j = k + 1 + e / r;
+      if (j <= i) L[i * JLD + j] -= L[i * JLD + k] * L[j * JLD + k];
+    }
+    __syncthreads();
+  }
+  // T = L^-1 (lower), column j by thread j
+  if (tid < b) {
+    const int j = tid;
+    for (int i = 0; i < b; ++i) {
+      double s = (i == j) ? 1.0 : 0.0;
+      for (int q = j; q < i; ++q) s -= L[i * JLD + q] * T[q * JLD + j];
+      T[i * JLD + j] = i < j ? 0.0 : s / L[i * JLD + i];
+    }
+  }
+  __syncthreads();
+  // Rinv[i, j] = sc[i] (L^-T)[i, j] = sc[i] T[j, i]
+  for (int e = tid; e < b * b; e += 256) {
+    const int i = e % b, j = e / b;
+    Rinv[e] = sc[i] * T[j * JLD + i];
+  }
+}
+
+// Columns of Z (m x b) whose squared norm W[j][j] is at most 1e-28 of the largest are
+// replaced by Philox Gaussian vectors and their row and column of W = Z^T Z recomputed
+// (one work-group; a block from an exactly singular operator would otherwise keep a zero
+// column through every pass).
+__global__ __launch_bounds__(256) void refresh_zero_cols_kernel(double* __restrict__ Z, long m, int b,
+                                                                double* __restrict__ W, uint64_t seed,
+                                                                uint32_t stream) {
+  __shared__ double dmax;
+  __shared__ double red[4];
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    double d = 0.0;
+    for (int i = 0; i < b; ++i) d = fmax(d, W[i + (long)b * i]);
+    dmax = d;
+  }
+  __syncthreads();
+  for (int j = 0; j < b; ++j) {
+    if (W[j + (long)b * j] > 1e-28 * dmax && dmax > 0.0) continue;  // uniform across the block
+    for (long r = tid; r < m; r += 256) Z[r + m * j] = philox_normal(seed, stream, (uint64_t)j * m + r);
+    __syncthreads();
+    for (int i = 0; i < b; ++i) {
+      double s = 0.0;
+      for (long r = tid; r < m; r += 256) s += Z[r + m * i] * Z[r + m * j];
+      s = wave_sum(s);
+      if ((tid & 63) == 0) red[tid >> 6] = s;
+      __syncthreads();
+      if (tid == 0) {
+        const double v = red[0] + red[1] + red[2] + red[3];
+        W[i + (long)b * j] = v;
+        W[j + (long)b * i] = v;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+int refresh_zero_cols_launch(hipStream_t st, double* Z, long m, int b, double* W, uint64_t seed, uint32_t stream) {
+  if (b < 1 || b > JMAX) return 8;
+  hipLaunchKernelGGL(refresh_zero_cols_kernel, dim3(1), dim3(256), 0, st, Z, m, b, W, seed, stream);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+int chol_rinv_launch(hipStream_t st, const double* W, int b, long m, int first, double* Rinv, int* info) {
+  if (b < 1 || b > JMAX || m < b) return 8;
+  hipLaunchKernelGGL(chol_rinv_kernel, dim3(1), dim3(256), 0, st, W, b, m, first, Rinv, info);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
