@@ -26,7 +26,8 @@ constexpr int SK_K = 64;     // reduction chunk
 // Y(K x b) = A^T B. Block: 128 columns of A (128 output rows) x 64; 4 waves, wave w owns
 // output rows 32w..32w+31 and all 64 columns (two 32x32 accumulators). LDS tiles are
 // [column][m] for A and [j][m] for B: both are column-major in memory along m.
-template <int PREC>
+// NB = number of 32-column accumulators (1 when b <= 32: half the MFMA work).
+template <int PREC, int NB>
 __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__ A, long M, long K,
                                                        const float* __restrict__ B, int b, int n_split,
                                                        float* __restrict__ part) {
@@ -41,7 +42,9 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
   const bool vecA = (M % 4 == 0) && (((uintptr_t)A & 15) == 0);
   const bool vecB = (M % 4 == 0) && (((uintptr_t)B & 15) == 0);
 
-  f32x16 acc[2] = {f32x16{}, f32x16{}};
+  f32x16 acc[NB];
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb) acc[bb] = f32x16{};
   for (long ch = k_begin; ch < k_end; ++ch) {
     const long m0 = ch * SK_K;
     __syncthreads();
@@ -65,7 +68,7 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
       *reinterpret_cast<float4*>(&As[col * LD + 4 * mq]) = v;
     }
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
+    for (int it = 0; it < 2 * NB; ++it) {
       const int f = tid + 256 * it;
       const int j = f >> 4, mq = f & 15;
       const long m = m0 + 4 * mq;
@@ -88,7 +91,7 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
       for (int kk = 0; kk < SK_K / 2; ++kk) {
         const float af = As[(32 * w + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
 #pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
+        for (int bb = 0; bb < NB; ++bb) {
           const float bf = Bs[(32 * bb + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
           acc[bb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af, bf, acc[bb], 0, 0, 0);
         }
@@ -101,7 +104,7 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
 #pragma unroll
         for (int i = 0; i < 8; ++i) af[i] = to16<PREC>(ap[i]);
 #pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
+        for (int bb = 0; bb < NB; ++bb) {
           u16x8 bf;
           const float* bp = &Bs[(32 * bb + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)];
 #pragma unroll
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
   // part[split][row c][64]
   float* out = part + (long)split * K * SK_B;
 #pragma unroll
-  for (int bb = 0; bb < 2; ++bb)
+  for (int bb = 0; bb < NB; ++bb)
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const long c = c0 + 32 * w + acc32_row(e, lane);
@@ -126,7 +129,7 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
 // ------------------------------------------------------------------ gemm_ab
 // Y(M x b) = A B. Block: 128 rows of A x 64; reduction over columns of A in chunks of
 // 64. A tile stored transposed in LDS ([m][c]), B tile as [j][c].
-template <int PREC>
+template <int PREC, int NB>
 __global__ __launch_bounds__(256) void gemm_ab_kernel(const float* __restrict__ A, long M, long K,
                                                       const float* __restrict__ B, int b, int n_split,
                                                       float* __restrict__ part) {
@@ -140,7 +143,9 @@ __global__ __launch_bounds__(256) void gemm_ab_kernel(const float* __restrict__ 
   const long k_begin = (long)split * n_chunks / n_split, k_end = (long)(split + 1) * n_chunks / n_split;
   const bool vecA = (M % 4 == 0) && (((uintptr_t)A & 15) == 0);
 
-  f32x16 acc[2] = {f32x16{}, f32x16{}};
+  f32x16 acc[NB];
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb) acc[bb] = f32x16{};
   for (long ch = k_begin; ch < k_end; ++ch) {
     const long cc0 = ch * SK_K;
     __syncthreads();
@@ -168,7 +173,7 @@ __global__ __launch_bounds__(256) void gemm_ab_kernel(const float* __restrict__ 
     }
     // B tile: 64 (j) x 64 (c); B col-major (K x b): B[c + K*j].
 #pragma unroll
-    for (int it = 0; it < 16; ++it) {
+    for (int it = 0; it < 8 * NB; ++it) {
       const int f = tid + 256 * it;
       const int j = f >> 6, k = f & 63;
       const long c = cc0 + k;
@@ -180,7 +185,7 @@ __global__ __launch_bounds__(256) void gemm_ab_kernel(const float* __restrict__ 
       for (int kk = 0; kk < SK_K / 2; ++kk) {
         const float af = As[(32 * w + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
 #pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
+        for (int bb = 0; bb < NB; ++bb) {
           const float bf = Bs[(32 * bb + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
           acc[bb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af, bf, acc[bb], 0, 0, 0);
         }
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(256) void gemm_ab_kernel(const float* __restrict__ 
 #pragma unroll
         for (int i = 0; i < 8; ++i) af[i] = to16<PREC>(ap[i]);
 #pragma unroll
-        for (int bb = 0; bb < 2; ++bb) {
+        for (int bb = 0; bb < NB; ++bb) {
           u16x8 bf;
           const float* bp = &Bs[(32 * bb + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)];
 #pragma unroll
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(256) void gemm_ab_kernel(const float* __restrict__ 
   }
   float* out = part + (long)split * M * SK_B;
 #pragma unroll
-  for (int bb = 0; bb < 2; ++bb)
+  for (int bb = 0; bb < NB; ++bb)
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const long m = m0 + 32 * w + acc32_row(e, lane);
@@ -251,12 +256,18 @@ int gemm_atb_launch(hipStream_t st, const float* A, long M, long K, const float*
   const int ns = pick_split(tiles, chunks);
   if ((size_t)ns * K * SK_B > ws_floats) return 4;
   dim3 grid((unsigned)tiles, ns);
-  if (prec == PREC_BF16)
-    hipLaunchKernelGGL(gemm_atb_kernel<PREC_BF16>, grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-  else if (prec == PREC_F16)
-    hipLaunchKernelGGL(gemm_atb_kernel<PREC_F16>, grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-  else
-    hipLaunchKernelGGL(gemm_atb_kernel<PREC_F32>, grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  if (prec == PREC_BF16) {
+    if (b <= 32) hipLaunchKernelGGL((gemm_atb_kernel<PREC_BF16, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+    else hipLaunchKernelGGL((gemm_atb_kernel<PREC_BF16, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  }
+  else if (prec == PREC_F16) {
+    if (b <= 32) hipLaunchKernelGGL((gemm_atb_kernel<PREC_F16, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+    else hipLaunchKernelGGL((gemm_atb_kernel<PREC_F16, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  }
+  else {
+    if (b <= 32) hipLaunchKernelGGL((gemm_atb_kernel<PREC_F32, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+    else hipLaunchKernelGGL((gemm_atb_kernel<PREC_F32, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  }
   const long tot = K * SK_B;
   hipLaunchKernelGGL(skinny_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ws, K, b, ns, Y);
   return hipGetLastError() == hipSuccess ? 0 : 5;
@@ -269,12 +280,18 @@ int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* 
   const int ns = pick_split(tiles, chunks);
   if ((size_t)ns * M * SK_B > ws_floats) return 4;
   dim3 grid((unsigned)tiles, ns);
-  if (prec == PREC_BF16)
-    hipLaunchKernelGGL(gemm_ab_kernel<PREC_BF16>, grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-  else if (prec == PREC_F16)
-    hipLaunchKernelGGL(gemm_ab_kernel<PREC_F16>, grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-  else
-    hipLaunchKernelGGL(gemm_ab_kernel<PREC_F32>, grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  if (prec == PREC_BF16) {
+    if (b <= 32) hipLaunchKernelGGL((gemm_ab_kernel<PREC_BF16, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+    else hipLaunchKernelGGL((gemm_ab_kernel<PREC_BF16, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  }
+  else if (prec == PREC_F16) {
+    if (b <= 32) hipLaunchKernelGGL((gemm_ab_kernel<PREC_F16, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+    else hipLaunchKernelGGL((gemm_ab_kernel<PREC_F16, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  }
+  else {
+    if (b <= 32) hipLaunchKernelGGL((gemm_ab_kernel<PREC_F32, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+    else hipLaunchKernelGGL((gemm_ab_kernel<PREC_F32, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  }
   const long tot = M * SK_B;
   hipLaunchKernelGGL(skinny_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ws, M, b, ns, Y);
   return hipGetLastError() == hipSuccess ? 0 : 5;
