@@ -21,6 +21,7 @@ trained sketch is stored rounded to BF16.
     res = decompose(x, dims, tree, ranks, method="rsvd", power_iters=0, sketch=sk)
 """
 import ctypes
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -84,6 +85,15 @@ class Sketch:
             bits = f["weights_bf16"].astype(np.uint32) << 16
         w = torch.from_numpy(bits.view(np.float32).copy()).to(f"cuda:{device}")
         return Sketch(dims, width, w)
+
+
+_BUNDLED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "sketches")
+
+
+def bundled(name, device=0):
+    """A sketch trained by `profile_run --workload c5` and shipped with the package:
+    "bumps128_w20" (width 20) or "bumps128_w4" (width 4), both for 128^4 bump tensors."""
+    return Sketch.load(os.path.join(_BUNDLED, f"sketch_{name}.npz"), device=device)
 
 
 def gaussian(dims, width, seed=7, device=0, handle=None):

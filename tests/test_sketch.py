@@ -310,3 +310,20 @@ def test_gpu_ranks_above_numerical_rank(method, precision, tol):
         x = ht.generate_bumps(n, 4, 5, s)
         r = ht.decompose(x, [n] * 4, t, [1] + [12] * 6, method=method, precision=precision, power_iters=0)
         assert r.rel_err < tol, (s, r.rel_err)
+
+
+@pytest.mark.gpu
+def test_gpu_bundled_sketch_beats_gaussian_at_128():
+    # the width-4 sketch shipped with the package (trained on 2000 tensors of seed 4) on
+    # held-out tensors of seed 5 it has not seen in training
+    import ht_tensor_learning_using_gpu_tensor_cores_amd as ht
+    sk = ht.sketch.bundled("bumps128_w4")
+    assert sk.dims == [128] * 4 and sk.width == 4
+    t = ht.tree.balanced_4d()
+    print("bundled width-4 sketch on 16 held-out 128^4 tensors", flush=True)
+    ll = ht.sketch.eval_bumps(sk, 128, 4, t, 16, seed=5, first=300)
+    lg = ht.sketch.eval_bumps(ht.sketch.gaussian([128] * 4, 4, seed=7), 128, 4, t, 16, seed=5, first=300)
+    exact = np.array([SO.bump_exact_miss(*SO.bump_factors(128, 5, 300 + s), 4) for s in range(16)])
+    print(f"  learned {ll.mean():.4e} gaussian {lg.mean():.4e} exact {exact.mean():.4e}", flush=True)
+    assert np.all(ll >= exact - 1e-6)
+    assert ll.mean() < 0.7 * lg.mean()
