@@ -97,6 +97,7 @@ SIGNATURES = {
     "ht_profile_enable": (I32, [H, I32]),
     "ht_profile_read": (I32, [H, PD, PI64, I32, I32]),
     "ht_debug_copy": (I32, [H, P, P, I64]),
+    "ht_debug_qr_fallbacks": (I32, [H, PI64]),
     "ht_sketch_default_train_options": (None, [ctypes.POINTER(SketchTrainOptions)]),
     "ht_sketch_layout": (I32, [I32, PI64, I32, PI64, PI64]),
     "ht_sketch_init_gaussian": (I32, [H, I32, PI64, I32, U64, P]),

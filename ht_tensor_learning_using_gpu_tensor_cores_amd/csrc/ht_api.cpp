@@ -164,6 +164,8 @@ ht_status ht_create(ht_handle_t* handle, int32_t device) {
       delete h;
       return HT_ERR_HIP;
     }
+  if (hipHostMalloc((void**)&h->pinned, 64 << 10, hipHostMallocDefault) == hipSuccess) h->pinned_bytes = 64 << 10;
+  else h->pinned = nullptr;
   *handle = h;
   return HT_OK;
 }
@@ -176,6 +178,7 @@ ht_status ht_destroy(ht_handle_t h) {
   if (h->bgraph) (void)hipGraphDestroy(h->bgraph);
   h->arena.release();
   h->prof.destroy();
+  if (h->pinned) (void)hipHostFree(h->pinned);
   if (h->xbuf) (void)hipFree(h->xbuf);
   if (h->obuf) (void)hipFree(h->obuf);
   for (auto& e : h->ev) (void)hipEventDestroy(e);
@@ -257,6 +260,8 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
   e.prob = &p;
   e.opt = opt;
   e.prof = &h->prof;
+  e.pin = h->pinned;
+  e.pin_bytes = h->pinned_bytes;
   if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
   int s = e.run(xd, od);
   if (s && s != HT_ERR_NOCONV) {
@@ -516,6 +521,15 @@ ht_status ht_profile_read(ht_handle_t h, double* ms, int64_t* launches, int32_t 
 }
 
 // ------------------------------------------------------------------ component entry points
+ht_status ht_debug_qr_fallbacks(ht_handle_t h, int64_t* count) {
+  if (!h || !count) return HT_ERR_INVALID_ARG;
+  if (hipSetDevice(h->device) != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) return HT_ERR_HIP;
+  unsigned long long v = 0;
+  if (qr_fallback_count(&v)) return HT_ERR_HIP;
+  *count = (int64_t)v;
+  return HT_OK;
+}
+
 ht_status ht_debug_copy(ht_handle_t h, const float* src, float* dst, int64_t n) {
   if (!h || n < 0 || (n > 0 && (!src || !dst))) return HT_ERR_INVALID_ARG;
   CKS(copy_f32_launch(h->stream, src, dst, n));

@@ -188,6 +188,8 @@ int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* 
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
                  int mode, const double* cut, double* T, int batch);
 int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int try_chol, int batch);
+// Running count of qr_kernel calls that took the Householder fallback (process-wide).
+int qr_fallback_count(unsigned long long* out);
 int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, const double* T, int n, int b, int k,
               double* Uf, double* U, long sU, double* w, long sw, double* wall, double* cut, double tol,
               double floor, int* converged, double* rres, int batch);

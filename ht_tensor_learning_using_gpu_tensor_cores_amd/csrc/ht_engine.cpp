@@ -215,6 +215,12 @@ int Engine::sync() {
 }
 
 int Engine::d2h(void* dst, const void* src, size_t bytes) {
+  if (pin && bytes <= pin_bytes) {
+    HT_HIP_CHECK(hipMemcpyAsync(pin, src, bytes, hipMemcpyDeviceToHost, st));
+    HT_HIP_CHECK(hipStreamSynchronize(st));
+    std::memcpy(dst, pin, bytes);
+    return 0;
+  }
   HT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
   HT_HIP_CHECK(hipStreamSynchronize(st));
   return 0;
@@ -313,9 +319,21 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, T, batch));
     CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, floor, conv, rres, batch));
     iters += q + 1;
-    HT_HIP_CHECK(hipMemcpyAsync(hconv.data(), conv, sizeof(int) * batch, hipMemcpyDeviceToHost, st));
-    HT_HIP_CHECK(hipMemcpyAsync(hwall.data(), wall, sizeof(double) * batch * 64, hipMemcpyDeviceToHost, st));
-    HT_HIP_CHECK(hipStreamSynchronize(st));
+    {
+      // both reads queue behind one synchronisation when they fit the pinned staging
+      const size_t cb = sizeof(int) * batch, wo = (cb + 255) & ~(size_t)255, wb = sizeof(double) * batch * 64;
+      if (pin && wo + wb <= pin_bytes) {
+        HT_HIP_CHECK(hipMemcpyAsync(pin, conv, cb, hipMemcpyDeviceToHost, st));
+        HT_HIP_CHECK(hipMemcpyAsync(pin + wo, wall, wb, hipMemcpyDeviceToHost, st));
+        HT_HIP_CHECK(hipStreamSynchronize(st));
+        std::memcpy(hconv.data(), pin, cb);
+        std::memcpy(hwall.data(), pin + wo, wb);
+      } else {
+        HT_HIP_CHECK(hipMemcpyAsync(hconv.data(), conv, cb, hipMemcpyDeviceToHost, st));
+        HT_HIP_CHECK(hipMemcpyAsync(hwall.data(), wall, wb, hipMemcpyDeviceToHost, st));
+        HT_HIP_CHECK(hipStreamSynchronize(st));
+      }
+    }
     const int b_rr = b;  // block width of this round's Ritz values
     bool all = true;
     for (int i = 0; i < batch; ++i) all = all && hconv[i];

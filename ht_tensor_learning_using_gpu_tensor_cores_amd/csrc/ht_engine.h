@@ -78,6 +78,10 @@ struct Engine {
   int sketch_fallbacks = 0, sketch_nodes = 0;
   double sketch_resid = 0.0;
   double rel_err = 0.0, x_sq = 0.0;
+  // pinned host staging for the small device-to-host reads (flags, Ritz values, scalars):
+  // copies into pageable memory complete synchronously one by one
+  char* pin = nullptr;
+  size_t pin_bytes = 0;
 
   template <typename T>
   T* alloc(size_t n);
@@ -140,6 +144,9 @@ struct ht_handle_s {
   hipGraph_t bgraph = nullptr;
   hipGraphExec_t bexec = nullptr;
   std::vector<const void*> bkey;
+  // pinned staging for small device-to-host reads
+  char* pinned = nullptr;
+  size_t pinned_bytes = 0;
 };
 
 namespace ht {

@@ -226,6 +226,13 @@ constexpr int QR_LDS = 8192;  // n * b limit of the staged blocks (doubles)
 //   reflector, store the reflector in the staging block (which then keeps all of them),
 //   every thread adds its rows' share of w = v^T A, and the owners of columns > j apply
 //   A -= v (tau w). Q is then formed explicitly as H_0 ... H_{b-1} I.
+// Number of qr_kernel calls that fell back from Cholesky QR to Householder (diagnostics).
+__device__ unsigned long long g_qr_householder = 0;
+
+int qr_fallback_count(unsigned long long* out) {
+  return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_qr_householder), sizeof(*out)) == hipSuccess ? 0 : 5;
+}
+
 template <int BP, int SR>
 __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, double* __restrict__ Q, long sYQ, int n,
                                                   int b, int try_chol) {
@@ -322,6 +329,7 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
     return;
   }
   // ---- Householder QR
+  if (tid == 0) atomicAdd(&g_qr_householder, 1ull);
 #pragma unroll
   for (int i = 0; i < SR; ++i) {
     const int r = (i * SW + g) * RS + h;

@@ -68,6 +68,9 @@ void ht_philox_normals_host(uint64_t seed, uint32_t stream, int64_t n, double* o
  * reference (8 n bytes) against which rocprofv3 FETCH_SIZE / WRITE_SIZE counts of the
  * engine's kernels are scaled. No counterpart in the reference. */
 ht_status ht_debug_copy(ht_handle_t h, const float* src, float* dst, int64_t n);
+/* Diagnostics: process-wide count of small-solver QR calls that fell back from Cholesky
+ * QR to Householder (the first round after few powers uses Householder directly). */
+ht_status ht_debug_qr_fallbacks(ht_handle_t h, int64_t* count);
 
 /* Per-kernel device timing. When enabled, every decomposition records HIP events on
  * the handle's stream around its main kernels and accumulates their durations. Kernel
