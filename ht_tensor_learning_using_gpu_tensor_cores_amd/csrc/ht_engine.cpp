@@ -820,11 +820,20 @@ int Engine::root_transfer(const float* x) {
 }
 
 int Engine::finish_error(const float* x, const double* xsq_dev) {
+  rel_err = NAN;
+  if (opt.err_mode == HT_ERR_MODE_RESIDUAL) {
+    // the residual pass returns ||X||^2 with ||X - X_HT||^2: one host read
+    double esq = 0.0, xs2 = 0.0;
+    CK(residual(x, &esq, &xs2));
+    x_sq = xs2;
+    if (!std::isfinite(xs2)) return HT_ERR_NONFINITE;
+    rel_err = xs2 > 0 ? std::sqrt(esq / xs2) : 0.0;
+    return 0;
+  }
   double xsq = 0.0;
   CK(d2h(&xsq, xsq_dev, sizeof(double)));
   x_sq = xsq;
   if (!std::isfinite(xsq)) return HT_ERR_NONFINITE;
-  rel_err = NAN;
   if (opt.err_mode == HT_ERR_MODE_IDENTITY) {
     const Problem& p = *prob;
     const int n = p.rank[p.left[0]] * p.rank[p.right[0]];
@@ -833,10 +842,6 @@ int Engine::finish_error(const float* x, const double* xsq_dev) {
     double bsq = 0.0;
     CK(d2h(&bsq, out, sizeof(double)));
     rel_err = xsq > 0 ? std::sqrt(std::max(xsq - bsq, 0.0) / xsq) : 0.0;
-  } else if (opt.err_mode == HT_ERR_MODE_RESIDUAL) {
-    double esq = 0.0, xs2 = 0.0;
-    CK(residual(x, &esq, &xs2));
-    rel_err = xs2 > 0 ? std::sqrt(esq / xs2) : 0.0;
   }
   return 0;
 }
