@@ -308,7 +308,7 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   long nb = (long)n * b;
   CK(fill_normals_f64_launch(st, W, batch * nb, opt.seed, stream));
   std::vector<int> hconv(batch);
-  std::vector<double> hwall((size_t)batch * 64);
+  std::vector<double> hwall((size_t)batch * 64), hres(batch);
   int q = std::max(1, q0), mode = 0, iters = 0;
   for (int round = 0;; ++round) {
     // CholeskyQR2 first, except on the first round after very few powers (blocks from
@@ -322,19 +322,26 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     {
       // both reads queue behind one synchronisation when they fit the pinned staging
       const size_t cb = sizeof(int) * batch, wo = (cb + 255) & ~(size_t)255, wb = sizeof(double) * batch * 64;
-      if (pin && wo + wb <= pin_bytes) {
+      const size_t ro = wo + ((wb + 255) & ~(size_t)255), rb = sizeof(double) * batch;
+      if (pin && ro + rb <= pin_bytes) {
         HT_HIP_CHECK(hipMemcpyAsync(pin, conv, cb, hipMemcpyDeviceToHost, st));
         HT_HIP_CHECK(hipMemcpyAsync(pin + wo, wall, wb, hipMemcpyDeviceToHost, st));
+        HT_HIP_CHECK(hipMemcpyAsync(pin + ro, rres, rb, hipMemcpyDeviceToHost, st));
         HT_HIP_CHECK(hipStreamSynchronize(st));
         std::memcpy(hconv.data(), pin, cb);
         std::memcpy(hwall.data(), pin + wo, wb);
+        std::memcpy(hres.data(), pin + ro, rb);
       } else {
         HT_HIP_CHECK(hipMemcpyAsync(hconv.data(), conv, cb, hipMemcpyDeviceToHost, st));
         HT_HIP_CHECK(hipMemcpyAsync(hwall.data(), wall, wb, hipMemcpyDeviceToHost, st));
+        HT_HIP_CHECK(hipMemcpyAsync(hres.data(), rres, rb, hipMemcpyDeviceToHost, st));
         HT_HIP_CHECK(hipStreamSynchronize(st));
       }
     }
     const int b_rr = b;  // block width of this round's Ritz values
+    if (getenv("HT_DEBUG") && getenv("HT_DEBUG")[0] == '2')
+      fprintf(stderr, "ht: sym_topk_batch n=%d k=%d b=%d round %d (q=%d mode %d): residual/th1 %.3e, th_k/th_1 %.3e, cut/th_1 %.3e\n",
+              n, k, b, round, q, mode, hres[0], hwall[k - 1] / hwall[0], hwall[b - 1] / hwall[0]);
     bool all = true;
     for (int i = 0; i < batch; ++i) all = all && hconv[i];
     if (all) break;
@@ -373,7 +380,10 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     // exp(d (acosh x_1 - acosh x_k)), x = 2 th / cut - 1; keeping that below 1e12 keeps
     // every wanted direction in the FP64 block (a spectrum spanning decades otherwise
     // collapses onto its top vectors).
-    double spread = 0.0;
+    // A second cap keeps the block's lower columns: a Ritz vector with error eps along the
+    // top eigenvector comes out of the filter with that component grown by ~T_d(x_1); with
+    // eps T_d(x_1) <= 1e8 the block stays far from collapsing onto its top directions.
+    double spread = 0.0, top = 0.0, eps = 0.0;
     for (int pb = 0; pb < batch; ++pb) {
       if (hconv[pb]) continue;
       const double* th = &hwall[(size_t)pb * 64];
@@ -384,9 +394,13 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
         if (std::fabs(th[i]) > floor * std::fabs(th[0])) kk = i;
       const double x1 = std::max(1.0, 2.0 * th[0] / e - 1.0), xk = std::max(1.0, 2.0 * th[kk] / e - 1.0);
       spread = std::max(spread, std::acosh(x1) - std::acosh(xk));
+      top = std::max(top, std::acosh(x1));
+      eps = std::max(eps, hres[pb]);
     }
-    q = round == 0 ? 8 : std::min(2 * q, 64);
+    q = round == 0 ? 16 : std::min(2 * q, 64);
     if (spread > 0.0) q = std::max(1, std::min(q, (int)(std::log(1e12) / spread)));
+    if (top > 0.0 && eps > 0.0)
+      q = std::max(1, std::min(q, (int)((std::log(1e8 / std::max(eps, 1e-16)) + std::log(2.0)) / top)));
     mode = 2;
   }
   eig_iters_max = std::max(eig_iters_max, iters);
@@ -1038,7 +1052,7 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
       double* G = alloc<double>((size_t)M * M);
       if (!G) return HT_ERR_ALLOC;
       CK(gram_pms_f64_launch(st, C, v, G));
-      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-5, 4, 32, 4000 + t * 16, 1e-12));
+      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-4, 8, 32, 4000 + t * 16, 1e-12));
     }
     CK(eig_to_sv(w, k, SVd[t]));
     Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself
