@@ -290,7 +290,10 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   }
   // The QR and Rayleigh-Ritz kernels stage n x b blocks in LDS: n * b <= 8192.
   const int bmax = std::min(std::min(64, n), 8192 / n);
-  if (b <= 0) b = k + 8;
+  // default block: exactly k columns. A spectrum with a clear gap after k (a Tucker rank
+  // met exactly) converges in the first round with the cheapest kernels and a
+  // well-conditioned block; a flat one misses the first round and is widened below.
+  if (b <= 0) b = k;
   b = std::min(std::max(b, k), bmax);
 
   const size_t nbmax = (size_t)n * bmax;
@@ -311,9 +314,10 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   std::vector<double> hwall((size_t)batch * 64), hres(batch);
   int q = std::max(1, q0), mode = 0, iters = 0;
   for (int round = 0;; ++round) {
-    // CholeskyQR2 first, except on the first round after very few powers (blocks from
-    // a fast-decaying spectrum are then numerically rank deficient)
-    const int try_chol = (round > 0 || q >= 4) ? 1 : 0;
+    // CholeskyQR2 first, except on the first round of an oversampled block after very
+    // few powers (its extra columns are then numerically rank deficient when the spectrum
+    // falls off after k)
+    const int try_chol = (round > 0 || q >= 4 || b == k) ? 1 : 0;
     CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, nullptr, batch));
     CK(qr_launch(st, Y, Q, nb, n, b, try_chol, batch));
     CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, T, batch));
@@ -339,9 +343,15 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
       }
     }
     const int b_rr = b;  // block width of this round's Ritz values
-    if (getenv("HT_DEBUG") && getenv("HT_DEBUG")[0] == '2')
-      fprintf(stderr, "ht: sym_topk_batch n=%d k=%d b=%d round %d (q=%d mode %d): residual/th1 %.3e, th_k/th_1 %.3e, cut/th_1 %.3e\n",
-              n, k, b, round, q, mode, hres[0], hwall[k - 1] / hwall[0], hwall[b - 1] / hwall[0]);
+    if (getenv("HT_DEBUG") && getenv("HT_DEBUG")[0] == '2') {
+      static unsigned long long fb_prev = 0;
+      unsigned long long fb = 0;
+      CK(qr_fallback_count(&fb));
+      fprintf(stderr, "ht: sym_topk_batch n=%d k=%d b=%d round %d (q=%d mode %d, chol %d): residual/th1 %.3e, th_k/th_1 "
+              "%.3e, cut/th_1 %.3e, householder %llu\n", n, k, b, round, q, mode, try_chol, hres[0],
+              hwall[k - 1] / hwall[0], hwall[b - 1] / hwall[0], fb - fb_prev);
+      fb_prev = fb;
+    }
     bool all = true;
     for (int i = 0; i < batch; ++i) all = all && hconv[i];
     if (all) break;
