@@ -39,6 +39,24 @@ __device__ __forceinline__ uint16_t to16(float f) {
   else return f2bf_bits(f);
 }
 
+// Two FP32 values to one packed pair of 16-bit values (low half = a), round to nearest
+// even: one v_cvt_pk_bf16_f32 on gfx950 for BF16.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <int PREC>
+__device__ __forceinline__ uint32_t cvt2(float a, float b) {
+  const f32x2 v = {a, b};
+  if constexpr (PREC == PREC_F16) return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, f16x2));
+  else return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+// Streaming 16-byte load that the caches need not keep (X is read once per pass).
+__device__ __forceinline__ float4 load_stream(const float* p) {
+  const f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
+  return make_float4(t[0], t[1], t[2], t[3]);
+}
+
 // 32x32x16 MFMA on 16-bit operands held as raw bits.
 template <int PREC>
 __device__ __forceinline__ f32x16 mfma32x32x16(u16x8 a, u16x8 b, f32x16 c) {

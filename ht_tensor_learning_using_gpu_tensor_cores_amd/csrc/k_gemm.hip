@@ -410,7 +410,7 @@ __global__ __launch_bounds__(256) void lowrank_kernel(const float* __restrict__ 
   }
 }
 
-template <int KP>
+template <int KP, bool FULL>
 __global__ void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M, long N, int k,
                                 const float* __restrict__ X, double* __restrict__ acc_out);
 
@@ -422,12 +422,21 @@ int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, lon
   if (n_blocks_out) *n_blocks_out = (int)(grid.x * grid.y);
   if (mode == 0)
     hipLaunchKernelGGL(lowrank_kernel<0>, grid, dim3(256), 0, st, L, R, M, N, k, X, out, acc);
-  else if (k <= 16)
-    hipLaunchKernelGGL(residual_kernel<16>, grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+  else if (M % LR_T == 0 && N % LR_T == 0) {
+    if (k <= 16)
+      hipLaunchKernelGGL((residual_kernel<16, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+    else if (k <= 32)
+      hipLaunchKernelGGL((residual_kernel<32, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+    else if (k <= 64)
+      hipLaunchKernelGGL((residual_kernel<64, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+    else
+      return 8;
+  } else if (k <= 16)
+    hipLaunchKernelGGL((residual_kernel<16, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
   else if (k <= 32)
-    hipLaunchKernelGGL(residual_kernel<32>, grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+    hipLaunchKernelGGL((residual_kernel<32, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
   else if (k <= 64)
-    hipLaunchKernelGGL(residual_kernel<64>, grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+    hipLaunchKernelGGL((residual_kernel<64, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
   else
     return 8;
   return hipGetLastError() == hipSuccess ? 0 : 5;
@@ -439,8 +448,11 @@ int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, lon
 // as B, k <= KP) come straight from global memory (L2-resident), and X is read in the
 // accumulator's own layout (element e of lane l is (m0 + l % 32, n0 + acc32_row(e, l)):
 // each half-wave reads one 128-byte column segment), so the comparison needs no
-// transposition. The X loads of a subtile are issued before its products.
-template <int KP>
+// transposition. FULL (M and N multiples of 128): no bounds checks, and (k <= 16) the
+// next subtile's X is loaded while the current one is multiplied. Loads are never behind a
+// per-lane branch (indices past k are clamped and the values zeroed at their use): a
+// branch join would make the compiler wait for them on the spot.
+template <int KP, bool FULL>
 __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M,
                                                        long N, int k, const float* __restrict__ X,
                                                        double* __restrict__ acc_out) {
@@ -450,33 +462,44 @@ __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__
   const long mw = (long)blockIdx.x * 128 + 32 * w;  // wave's first row
   const long m = mw + li;
   const long nbase = (long)blockIdx.y * 128;
-  const bool mok = m < M;
+  const bool mok = FULL || m < M;
+  const long mc = mok ? m : M - 1;
   float bl[KP / 2];
 #pragma unroll
   for (int j = 0; j < KP / 2; ++j) {
     const int kk = 2 * j + kh;
-    bl[j] = (kk < k && mok) ? L[m + M * (long)kk] : 0.f;
+    const float v = L[mc + M * (long)min(kk, k - 1)];
+    bl[j] = (kk < k && mok) ? v : 0.f;
   }
   // X element e of this lane in subtile nb: (m, nb + 8 (e / 4) + 4 kh + e % 4); the column
   // part of the offset is a 32-bit multiple of M from the wave-uniform base X + mw + M nb.
   const unsigned mu = (unsigned)M;
-  double se = 0.0, sx = 0.0;
-#pragma unroll 1
-  for (int t = 0; t < 4; ++t) {
+  auto load_x = [&](float* xv, int t) {
     const long nb = nbase + 32 * t;
     const float* base = X + mw + M * nb;
-    float xv[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const int row = 8 * (e >> 2) + 4 * kh + (e & 3);
-      xv[e] = (mok && nb + row < N) ? base[(unsigned)li + mu * (unsigned)row] : 0.f;
+      if (FULL) {
+        xv[e] = base[(unsigned)li + mu * (unsigned)row];
+      } else {
+        const bool ok = mok && nb + row < N;
+        const float v = X[ok ? mw + li + M * (nb + row) : 0];
+        xv[e] = ok ? v : 0.f;
+      }
     }
+  };
+  double se = 0.0, sx = 0.0;
+  auto tile = [&](const float* xv, int t) {
+    const long nb = nbase + 32 * t;
+    const long n = nb + li;
+    const long nc = (FULL || n < N) ? n : N - 1;
     float ar[KP / 2];
 #pragma unroll
     for (int j = 0; j < KP / 2; ++j) {
       const int kk = 2 * j + kh;
-      const long n = nb + li;
-      ar[j] = (kk < k && n < N) ? R[n + N * (long)kk] : 0.f;
+      const float v = R[nc + N * (long)min(kk, k - 1)];
+      ar[j] = (kk < k && (FULL || n < N)) ? v : 0.f;
     }
     f32x16 acc = f32x16{};
 #pragma unroll
@@ -490,6 +513,27 @@ __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__
     }
     se += (double)e2;
     sx += (double)x2;
+  };
+  if constexpr (FULL && KP <= 16) {
+    float xa[16], xb[16];
+    load_x(xa, 0);
+    load_x(xb, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    tile(xa, 0);
+    load_x(xa, 2);
+    __builtin_amdgcn_sched_barrier(0);
+    tile(xb, 1);
+    load_x(xb, 3);
+    __builtin_amdgcn_sched_barrier(0);
+    tile(xa, 2);
+    tile(xb, 3);
+  } else {
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+      float xv[16];
+      load_x(xv, t);
+      tile(xv, t);
+    }
   }
   se = wave_sum(se);
   sx = wave_sum(sx);

@@ -9,7 +9,7 @@
 //
 // slab_gram: the Grams of the two leading modes from one read of each n1 x n2 slab.
 // slab_proj: z_s = U1^T S_s U2 per slab (the leaf-core projection of the two leading
-// modes), one wavefront per slab, FP32 MFMA, no LDS.
+// modes), one wavefront per slab, FP32 MFMA, U1 and U2 staged in LDS.
 #include "ht_common.h"
 
 namespace ht {
@@ -394,14 +394,20 @@ int gram_reduce_norms_launch(hipStream_t st, const double* norms, int n, double*
 // 512 threads = 8 waves. Waves 0-3: quadrants of G1 = sum_s S S^T (A operand rows i,
 // k = j, from the row-major copy); waves 4-7: quadrants of G2 = sum_s S^T S (rows j,
 // k = i, from the column-major copy). Slabs are padded to 128 x 128 with zeros.
+// The next slab's loads are issued unconditionally (clamped to valid addresses, the
+// padding applied when the values are stored to LDS): a load behind a divergent branch
+// makes the compiler wait for it at the branch join, which would serialise every slab's
+// HBM latency with the multiply.
 template <int PREC>
 __global__ __launch_bounds__(512) void slab_gram_kernel(const float* __restrict__ x, int n1, int n2, long S,
                                                         int n_blocks, float* __restrict__ part1,
                                                         float* __restrict__ part2, double* __restrict__ norms) {
   // Scm[j][i] = S[i][j] (the slab's own column-major order, padded rows); Srm holds the
-  // transpose, Srm[i][j ^ swz(i)] with swz(i) = 8 ((i / 4) mod 16): the XOR keeps each
-  // aligned group of 8 columns contiguous (so MFMA operands stay single 16-byte reads)
-  // and spreads the 4-element row stores of 16 consecutive row groups over all banks.
+  // transpose, Srm[i][j ^ swz(i)] with swz(i) = 8 (((i / 4) ^ 4 i) mod 16): the XOR keeps
+  // each aligned group of 8 columns contiguous (so MFMA operands stay single 16-byte
+  // reads); it is a bijection both over 16 consecutive rows (the operand reads, lane =
+  // row) and over rows 4m + e, m = 0..15 (the 4-element row stores), so neither access
+  // has bank conflicts.
   // Two copies of (Srm, Scm): the slab being multiplied and the next one being stored.
   constexpr int LDC = 128 + 8;
   constexpr int BUF = 128 * 128 + 128 * LDC;
@@ -425,38 +431,38 @@ __global__ __launch_bounds__(512) void slab_gram_kernel(const float* __restrict_
       const int bb = tid + 512 * u, i4 = 4 * (bb & 31), j4 = 4 * (bb >> 5);
 #pragma unroll
       for (int d = 0; d < 4; ++d) {
-        float4 v4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i4 < n1 && j4 + d < n2) v4 = *reinterpret_cast<const float4*>(base + (long)(j4 + d) * n1 + i4);
-        r[u][d] = v4;
+        const bool ok = i4 < n1 && j4 + d < n2;
+        r[u][d] = load_stream(base + (ok ? (long)(j4 + d) * n1 + i4 : 0));
       }
     }
   };
-  auto pack = [](uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
-    uint2 wv;
-    wv.x = (uint32_t)a | ((uint32_t)b << 16);
-    wv.y = (uint32_t)c | ((uint32_t)d << 16);
-    return wv;
-  };
+  auto swz = [](int i) { return 8 * (((i >> 2) ^ (i << 2)) & 15); };
   auto store = [&](uint16_t* Srm, uint16_t* Scm) {
     float ss = 0.f;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int bb = tid + 512 * u, i4 = 4 * (bb & 31), j4 = 4 * (bb >> 5);
-      uint16_t h[4][4];  // [d][e]: column j4 + d, row i4 + e
+      float f[4][4];  // [d][e]: column j4 + d, row i4 + e (zero outside the slab)
 #pragma unroll
       for (int d = 0; d < 4; ++d) {
-        h[d][0] = to16<PREC>(r[u][d].x);
-        h[d][1] = to16<PREC>(r[u][d].y);
-        h[d][2] = to16<PREC>(r[u][d].z);
-        h[d][3] = to16<PREC>(r[u][d].w);
-        ss += r[u][d].x * r[u][d].x + r[u][d].y * r[u][d].y + r[u][d].z * r[u][d].z + r[u][d].w * r[u][d].w;
-        *reinterpret_cast<uint2*>(&Scm[(j4 + d) * LDC + i4]) = pack(h[d][0], h[d][1], h[d][2], h[d][3]);
+        const bool ok = i4 < n1 && j4 + d < n2;
+        f[d][0] = ok ? r[u][d].x : 0.f;
+        f[d][1] = ok ? r[u][d].y : 0.f;
+        f[d][2] = ok ? r[u][d].z : 0.f;
+        f[d][3] = ok ? r[u][d].w : 0.f;
+        ss += f[d][0] * f[d][0] + f[d][1] * f[d][1] + f[d][2] * f[d][2] + f[d][3] * f[d][3];
+        uint2 wv;
+        wv.x = cvt2<PREC>(f[d][0], f[d][1]);
+        wv.y = cvt2<PREC>(f[d][2], f[d][3]);
+        *reinterpret_cast<uint2*>(&Scm[(j4 + d) * LDC + i4]) = wv;
       }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int i = i4 + e;
-        *reinterpret_cast<uint2*>(&Srm[i * 128 + (j4 ^ (8 * ((i >> 2) & 15)))]) =
-            pack(h[0][e], h[1][e], h[2][e], h[3][e]);
+        uint2 wv;
+        wv.x = cvt2<PREC>(f[0][e], f[1][e]);
+        wv.y = cvt2<PREC>(f[2][e], f[3][e]);
+        *reinterpret_cast<uint2*>(&Srm[i * 128 + (j4 ^ swz(i))]) = wv;
       }
     }
     sumsq += (double)ss;
@@ -472,7 +478,7 @@ __global__ __launch_bounds__(512) void slab_gram_kernel(const float* __restrict_
     load(s_begin);
     __syncthreads();  // zero fill done
     store(lds, lds + 128 * 128);
-    if (s_begin + 1 < s_end) load(s_begin + 1);
+    load(min(s_begin + 1, s_end - 1));
     __syncthreads();
   }
   for (long s = s_begin; s < s_end; ++s) {
@@ -486,12 +492,12 @@ __global__ __launch_bounds__(512) void slab_gram_kernel(const float* __restrict_
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
           const int row = 64 * wr + 32 * a + (lane & 31);
-          af[a] = *reinterpret_cast<const u16x8*>(&Srm[row * 128 + (kc ^ (8 * ((row >> 2) & 15)))]);
+          af[a] = *reinterpret_cast<const u16x8*>(&Srm[row * 128 + (kc ^ swz(row))]);
         }
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
           const int row = 64 * wc + 32 * b + (lane & 31);
-          bf[b] = *reinterpret_cast<const u16x8*>(&Srm[row * 128 + (kc ^ (8 * ((row >> 2) & 15)))]);
+          bf[b] = *reinterpret_cast<const u16x8*>(&Srm[row * 128 + (kc ^ swz(row))]);
         }
       } else {
 #pragma unroll
@@ -509,7 +515,7 @@ __global__ __launch_bounds__(512) void slab_gram_kernel(const float* __restrict_
     if (s + 1 < s_end) {
       uint16_t* nx = lds + ((s + 1 - s_begin) & 1) * BUF;
       store(nx, nx + 128 * 128);
-      if (s + 2 < s_end) load(s + 2);
+      load(min(s + 2, s_end - 1));
     }
     __syncthreads();
   }
@@ -555,94 +561,135 @@ int slab_gram_launch(hipStream_t st, const float* x, int n1, int n2, long S, int
 //                               i = i0 + 4g + s, i.e. S[i0+4g .. i0+4g+3][j] (one float4);
 //   Z^T (r2 x r1) = U2^T T^T -- the T^T accumulators are already in B-operand layout
 //                               (their element e is row 4g + e = the slot of step e).
+// U1 and U2 sit in LDS in the B/A-operand order (one float4 = the four steps of a lane),
+// which keeps the wave's registers for two 16-column blocks of the slab: the next block
+// (of this slab or of the wave's next one) is loaded while the current one is
+// multiplied. Those loads are unconditional -- rows past n1 are clamped (their U1 rows
+// are zero in LDS) and past the last block the current block is loaded again -- so the
+// number of loads in flight is the same on every path and the waits stay one block
+// behind; a scheduling barrier keeps the compiler from sinking them below the MFMAs.
 template <int RB1, int RB2>
 __global__ __launch_bounds__(256) void slab_proj_kernel(const float* __restrict__ x, int n1, int n2, long S,
                                                         const float* __restrict__ U1, int r1,
                                                         const float* __restrict__ U2, int r2,
                                                         float* __restrict__ z) {
+  // lu1[((a * 8 + ig) * 4 + g) * 16 + col] = U1[16 ig + 4 g + 0..3][16 a + col], lu2 alike
+  __shared__ float4 lu1[RB1 * 512], lu2[RB2 * 512];
+  const int n1g = n1 / 16;  // row groups of 16 (<= 8)
+  const int nb2 = n2 / 16;  // column blocks of 16 (<= 8)
+  for (int e = threadIdx.x; e < RB1 * 512; e += blockDim.x) {
+    const int cl = e & 15, gg = (e >> 4) & 3, ig = (e >> 6) & 7, a = e >> 9;
+    const int c = 16 * a + cl, i = 16 * ig + 4 * gg;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ig < n1g && c < r1) {
+      const float* p = U1 + i + (long)n1 * c;
+      v = make_float4(p[0], p[1], p[2], p[3]);
+    }
+    lu1[e] = v;
+  }
+  for (int e = threadIdx.x; e < RB2 * 512; e += blockDim.x) {
+    const int cl = e & 15, gg = (e >> 4) & 3, ig = (e >> 6) & 7, b = e >> 9;
+    const int c = 16 * b + cl, i = 16 * ig + 4 * gg;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ig < nb2 && c < r2) {
+      const float* p = U2 + i + (long)n2 * c;
+      v = make_float4(p[0], p[1], p[2], p[3]);
+    }
+    lu2[e] = v;
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, g = lane >> 4;
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long n_waves = ((long)gridDim.x * blockDim.x) >> 6;
-  const int n1g = n1 / 16;  // row groups of 16 (<= 8)
-  const int nb2 = n2 / 16;  // column blocks of 16 (<= 8)
+  if (wave >= S) return;
+  const long slab = (long)n1 * n2;
 
-  // u1r[ig][st][a] = U1[16 ig + 4g + st][16 a + col]; u2r[bb][st][b] = U2[16 bb + 4g + st][16 b + col].
-  float u1r[8][4][RB1], u2r[8][4][RB2];
+  auto load = [&](float4* sv, long s, int bb) {
+    const float* colp = x + s * slab + (long)(16 * bb + col) * n1 + 4 * g;
 #pragma unroll
-  for (int ig = 0; ig < 8; ++ig)
+    for (int ig = 0; ig < 8; ++ig) sv[ig] = *reinterpret_cast<const float4*>(colp + 16 * min(ig, n1g - 1));
+  };
+  f32x4 zt[RB2][RB1];
 #pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      const int i = 16 * ig + 4 * g + st;
+  for (int b = 0; b < RB2; ++b)
+#pragma unroll
+    for (int a = 0; a < RB1; ++a) zt[b][a] = f32x4{};
+  auto compute = [&](const float4* sv, long s, int bb) {
+    f32x4 tt[RB1];
+#pragma unroll
+    for (int a = 0; a < RB1; ++a) tt[a] = f32x4{};
+#pragma unroll
+    for (int ig = 0; ig < 8; ++ig) {
+      const float svs[4] = {sv[ig].x, sv[ig].y, sv[ig].z, sv[ig].w};
 #pragma unroll
       for (int a = 0; a < RB1; ++a) {
-        const int ac = 16 * a + col;
-        u1r[ig][st][a] = (ig < n1g && ac < r1) ? U1[i + (long)n1 * ac] : 0.f;
-      }
+        const float4 u = lu1[((a * 8 + ig) * 4 + g) * 16 + col];
+        const float us[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
-      for (int b = 0; b < RB2; ++b) {
-        const int bc = 16 * b + col;
-        u2r[ig][st][b] = (ig < nb2 && bc < r2) ? U2[i + (long)n2 * bc] : 0.f;
+        for (int st = 0; st < 4; ++st) tt[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(svs[st], us[st], tt[a], 0, 0, 0);
       }
     }
-
-  for (long s = wave; s < S; s += n_waves) {
-    const float* Sb = x + s * (long)n1 * n2;
-    f32x4 zt[RB2][RB1];
+    // tt[a] element e: T^T[row 16bb + 4g + e][col 16a + col].
 #pragma unroll
-    for (int b = 0; b < RB2; ++b)
-#pragma unroll
-      for (int a = 0; a < RB1; ++a) zt[b][a] = f32x4{};
-#pragma unroll
-    for (int bb = 0; bb < 8; ++bb) {
-      if (bb >= nb2) break;
-      const float* colp = Sb + (long)(16 * bb + col) * n1 + 4 * g;
-      float4 sv[8];
-#pragma unroll
-      for (int ig = 0; ig < 8; ++ig)
-        sv[ig] = (ig < n1g) ? *reinterpret_cast<const float4*>(colp + 16 * ig) : make_float4(0.f, 0.f, 0.f, 0.f);
-      f32x4 tt[RB1];
-#pragma unroll
-      for (int a = 0; a < RB1; ++a) tt[a] = f32x4{};
-#pragma unroll
-      for (int ig = 0; ig < 8; ++ig) {
-        const float svs[4] = {sv[ig].x, sv[ig].y, sv[ig].z, sv[ig].w};
-#pragma unroll
-        for (int st = 0; st < 4; ++st)
-#pragma unroll
-          for (int a = 0; a < RB1; ++a)
-            tt[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(svs[st], u1r[ig][st][a], tt[a], 0, 0, 0);
-      }
-      // tt[a] element e: T^T[row 16bb + 4g + e][col 16a + col].
+    for (int b = 0; b < RB2; ++b) {
+      const float4 u = lu2[((b * 8 + bb) * 4 + g) * 16 + col];
+      const float us[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
       for (int st = 0; st < 4; ++st)
 #pragma unroll
-        for (int b = 0; b < RB2; ++b)
-#pragma unroll
-          for (int a = 0; a < RB1; ++a)
-            zt[b][a] = __builtin_amdgcn_mfma_f32_16x16x4f32(u2r[bb][st][b], tt[a][st], zt[b][a], 0, 0, 0);
+        for (int a = 0; a < RB1; ++a)
+          zt[b][a] = __builtin_amdgcn_mfma_f32_16x16x4f32(us[st], tt[a][st], zt[b][a], 0, 0, 0);
     }
-    // zt[b][a] element e: Z^T[row 16b + 4g + e][col 16a + col] = Z[a'][b'].
-    float* zs = z + s * (long)r1 * r2;
+    if (bb == nb2 - 1) {
+      // zt[b][a] element e: Z^T[row 16b + 4g + e][col 16a + col] = Z[a'][b'].
+      float* zs = z + s * (long)r1 * r2;
 #pragma unroll
-    for (int b = 0; b < RB2; ++b)
+      for (int b = 0; b < RB2; ++b)
 #pragma unroll
-      for (int a = 0; a < RB1; ++a)
+        for (int a = 0; a < RB1; ++a) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int bp = 16 * b + 4 * g + e, ap = 16 * a + col;
-          if (ap < r1 && bp < r2) zs[ap + (long)r1 * bp] = zt[b][a][e];
+          for (int e = 0; e < 4; ++e) {
+            const int bp = 16 * b + 4 * g + e, ap = 16 * a + col;
+            if (ap < r1 && bp < r2) zs[ap + (long)r1 * bp] = zt[b][a][e];
+          }
+          zt[b][a] = f32x4{};
         }
+    }
+  };
+  float4 A[8], B[8];
+  long s = wave;
+  int bb = 0;
+  load(A, s, bb);
+  while (true) {
+    long s2 = s;
+    int bb2 = bb + 1;
+    if (bb2 == nb2) { bb2 = 0; s2 += n_waves; }
+    const bool more = s2 < S;
+    load(B, more ? s2 : s, more ? bb2 : bb);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(A, s, bb);
+    if (!more) break;
+    s = s2; bb = bb2;
+    s2 = s; bb2 = bb + 1;
+    if (bb2 == nb2) { bb2 = 0; s2 += n_waves; }
+    const bool more2 = s2 < S;
+    load(A, more2 ? s2 : s, more2 ? bb2 : bb);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(B, s, bb);
+    if (!more2) break;
+    s = s2; bb = bb2;
   }
 }
 
 int slab_proj_launch(hipStream_t st, const float* x, int n1, int n2, long S, const float* U1, int r1,
                      const float* U2, int r2, float* z) {
-  if (n1 % 16 || n2 % 16 || n1 > 128 || n2 > 128 || r1 < 1 || r2 < 1 || r1 > 32 || r2 > 32) return 8;
+  if (n1 < 16 || n2 < 16 || n1 % 16 || n2 % 16 || n1 > 128 || n2 > 128 || r1 < 1 || r2 < 1 || r1 > 32 || r2 > 32)
+    return 8;
   if (((uintptr_t)x & 15) != 0) return 8;
   long waves = S;
   long blocks = (waves + 3) / 4;
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > 1024) blocks = 1024;
   dim3 grid((unsigned)blocks);
   if (r1 <= 16 && r2 <= 16)
     hipLaunchKernelGGL((slab_proj_kernel<1, 1>), grid, dim3(256), 0, st, x, n1, n2, S, U1, r1, U2, r2, z);
