@@ -412,56 +412,67 @@ __global__ __launch_bounds__(256) void lowrank_kernel(const float* __restrict__ 
 
 template <int KP, bool FULL>
 __global__ void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M, long N, int k,
-                                const float* __restrict__ X, double* __restrict__ acc_out);
+                                const float* __restrict__ X, int nts, double* __restrict__ acc_out);
+__global__ void residual_stream_kernel(const float* __restrict__ L, const float* __restrict__ R, long M, long N,
+                                       int k, const float* __restrict__ X, int nts, double* __restrict__ acc_out);
 
 int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, long M, long N, int k,
                    const float* X, float* out, double* acc, int* n_blocks_out) {
   if (M < 1 || N < 1 || k < 1 || k > 4096) return 8;
   if (mode == 1 && M > (1L << 24)) return 8;  // 32-bit column offsets of the residual pass
   dim3 grid((unsigned)((M + LR_T - 1) / LR_T), (unsigned)((N + LR_T - 1) / LR_T));
-  if (n_blocks_out) *n_blocks_out = (int)(grid.x * grid.y);
-  if (mode == 0)
+  if (mode == 0) {
+    if (n_blocks_out) *n_blocks_out = (int)(grid.x * grid.y);
     hipLaunchKernelGGL(lowrank_kernel<0>, grid, dim3(256), 0, st, L, R, M, N, k, X, out, acc);
-  else if (M % LR_T == 0 && N % LR_T == 0) {
+    return hipGetLastError() == hipSuccess ? 0 : 5;
+  }
+  if (k > 64) return 8;
+  const bool full = M % LR_T == 0 && N % LR_T == 0;
+  // Residual blocks: 128 rows x 32 nts columns. Waves that stream more subtiles keep
+  // their prefetch busy for longer; nts shrinks while the grid would fall below 2048
+  // blocks (8 per CU).
+  int nts = 4;
+  if (full && k <= 16) {
+    nts = 16;
+    while (nts > 4 && (N % (32L * nts) != 0 || grid.x * (N / (32L * nts)) < 2048)) nts /= 2;
+    grid.y = (unsigned)(N / (32L * nts));
+  }
+  if (n_blocks_out) *n_blocks_out = (int)(grid.x * grid.y);
+  if (full) {
     if (k <= 16)
-      hipLaunchKernelGGL((residual_kernel<16, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+      hipLaunchKernelGGL(residual_stream_kernel, grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
     else if (k <= 32)
-      hipLaunchKernelGGL((residual_kernel<32, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
-    else if (k <= 64)
-      hipLaunchKernelGGL((residual_kernel<64, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+      hipLaunchKernelGGL((residual_kernel<32, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
     else
-      return 8;
+      hipLaunchKernelGGL((residual_kernel<64, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
   } else if (k <= 16)
-    hipLaunchKernelGGL((residual_kernel<16, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+    hipLaunchKernelGGL((residual_kernel<16, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
   else if (k <= 32)
-    hipLaunchKernelGGL((residual_kernel<32, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
-  else if (k <= 64)
-    hipLaunchKernelGGL((residual_kernel<64, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, acc);
+    hipLaunchKernelGGL((residual_kernel<32, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
   else
-    return 8;
+    hipLaunchKernelGGL((residual_kernel<64, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
 // Residual pass: acc[2*block] = sum (X - L R^T)^2, acc[2*block+1] = sum X^2 over one
-// 128 (m) x 128 (n) region of X (ld M). No LDS: wave w takes the m strip
-// [m0 + 32 w, +32) and four 32-column subtiles; the MFMA operands (R rows as A, L rows
+// 128 (m) x 32 nts (n) region of X (ld M). No LDS: wave w takes the m strip
+// [m0 + 32 w, +32) and nts 32-column subtiles; the MFMA operands (R rows as A, L rows
 // as B, k <= KP) come straight from global memory (L2-resident), and X is read in the
 // accumulator's own layout (element e of lane l is (m0 + l % 32, n0 + acc32_row(e, l)):
 // each half-wave reads one 128-byte column segment), so the comparison needs no
-// transposition. FULL (M and N multiples of 128): no bounds checks, and (k <= 16) the
-// next subtile's X is loaded while the current one is multiplied. Loads are never behind a
-// per-lane branch (indices past k are clamped and the values zeroed at their use): a
-// branch join would make the compiler wait for them on the spot.
+// transposition. FULL (M and N multiples of 128): no bounds checks. Loads are never
+// behind a per-lane branch (indices past k are clamped and the values zeroed at their
+// use): a branch join would make the compiler wait for them on the spot.
 template <int KP, bool FULL>
 __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M,
-                                                       long N, int k, const float* __restrict__ X,
+                                                       long N, int k, const float* __restrict__ X, int nts,
                                                        double* __restrict__ acc_out) {
   __shared__ double red[8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 31, kh = lane >> 5;
   const long mw = (long)blockIdx.x * 128 + 32 * w;  // wave's first row
   const long m = mw + li;
-  const long nbase = (long)blockIdx.y * 128;
+  const long nbase = (long)blockIdx.y * 32 * nts;
   const bool mok = FULL || m < M;
   const long mc = mok ? m : M - 1;
   float bl[KP / 2];
@@ -514,26 +525,84 @@ __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__
     se += (double)e2;
     sx += (double)x2;
   };
-  if constexpr (FULL && KP <= 16) {
-    float xa[16], xb[16];
-    load_x(xa, 0);
-    load_x(xb, 1);
-    __builtin_amdgcn_sched_barrier(0);
-    tile(xa, 0);
-    load_x(xa, 2);
-    __builtin_amdgcn_sched_barrier(0);
-    tile(xb, 1);
-    load_x(xb, 3);
-    __builtin_amdgcn_sched_barrier(0);
-    tile(xa, 2);
-    tile(xb, 3);
-  } else {
 #pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-      float xv[16];
-      load_x(xv, t);
-      tile(xv, t);
+  for (int t = 0; t < nts; ++t) {
+    float xv[16];
+    load_x(xv, t);
+    tile(xv, t);
+  }
+  se = wave_sum(se);
+  sx = wave_sum(sx);
+  if (lane == 0) { red[w] = se; red[4 + w] = sx; }
+  __syncthreads();
+  if (tid == 0) {
+    const long bid = (long)blockIdx.y * gridDim.x + blockIdx.x;
+    acc_out[2 * bid] = red[0] + red[1] + red[2] + red[3];
+    acc_out[2 * bid + 1] = red[4] + red[5] + red[6] + red[7];
+  }
+}
+
+// The residual pass for k <= 16 with M a multiple of 128 and N of 32 nts: the same
+// block, wave and lane layout as residual_kernel, no bounds checks, and the next
+// subtile's X is loaded while the current one is multiplied. The subtile after the last
+// is the last again (loaded, never used), so every path issues the same loads and each
+// wait covers only the subtile being multiplied; a scheduling barrier keeps the
+// compiler from sinking the loads below the products.
+__global__ __launch_bounds__(256) void residual_stream_kernel(const float* __restrict__ L, const float* __restrict__ R,
+                                                              long M, long N, int k, const float* __restrict__ X,
+                                                              int nts, double* __restrict__ acc_out) {
+  constexpr int KP = 16;
+  __shared__ double red[8];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 31, kh = lane >> 5;
+  const long mw = (long)blockIdx.x * 128 + 32 * w;
+  const long m = mw + li;
+  const long nbase = (long)blockIdx.y * 32 * nts;
+  float bl[KP / 2];
+#pragma unroll
+  for (int j = 0; j < KP / 2; ++j) {
+    const int kk = 2 * j + kh;
+    const float v = L[m + M * (long)min(kk, k - 1)];
+    bl[j] = kk < k ? v : 0.f;
+  }
+  const unsigned mu = (unsigned)M;
+  auto load_x = [&](float* xv, int t) {
+    const float* base = X + mw + M * (nbase + 32 * t);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) xv[e] = base[(unsigned)li + mu * (unsigned)acc32_row(e, lane)];
+  };
+  double se = 0.0, sx = 0.0;
+  auto tile = [&](const float* xv, int t) {
+    const long n = nbase + 32 * t + li;
+    float ar[KP / 2];
+#pragma unroll
+    for (int j = 0; j < KP / 2; ++j) {
+      const int kk = 2 * j + kh;
+      const float v = R[n + N * (long)min(kk, k - 1)];
+      ar[j] = kk < k ? v : 0.f;
     }
+    f32x16 acc = f32x16{};
+#pragma unroll
+    for (int j = 0; j < KP / 2; ++j) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[j], bl[j], acc, 0, 0, 0);
+    float e2 = 0.f, x2 = 0.f;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const float d = xv[e] - acc[e];
+      e2 += d * d;
+      x2 += xv[e] * xv[e];
+    }
+    se += (double)e2;
+    sx += (double)x2;
+  };
+  float xa[16], xb[16];
+  load_x(xa, 0);
+  for (int t = 0; t < nts; t += 2) {
+    load_x(xb, min(t + 1, nts - 1));
+    __builtin_amdgcn_sched_barrier(0);
+    tile(xa, t);
+    load_x(xa, min(t + 2, nts - 1));
+    __builtin_amdgcn_sched_barrier(0);
+    if (t + 1 < nts) tile(xb, t + 1);
   }
   se = wave_sum(se);
   sx = wave_sum(sx);
