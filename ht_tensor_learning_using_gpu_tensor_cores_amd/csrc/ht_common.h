@@ -71,8 +71,10 @@ __device__ __forceinline__ f32x16 mfma32x32x16(u16x8 a, u16x8 b, f32x16 c) {
 // Output position of element e (0..15) of a 32x32 MFMA accumulator in lane l:
 //   row = 8*(e/4) + 4*(l/32) + (e%4), col = l%32.
 __device__ __forceinline__ int acc32_row(int e, int lane) { return 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3); }
-// 16x16 accumulators (f32x4 / f64x4): row = 4*(l/16) + e, col = l%16.
+// 16x16 f32 accumulators (f32x4): row = 4*(l/16) + e, col = l%16.
 __device__ __forceinline__ int acc16_row(int e, int lane) { return 4 * (lane >> 4) + e; }
+// 16x16x4 f64 accumulators (f64x4) interleave the other way: row = 4*e + l/16, col = l%16.
+__device__ __forceinline__ int acc16_row_f64(int e, int lane) { return 4 * e + (lane >> 4); }
 
 __device__ __forceinline__ float wave_sum(float v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -50,13 +50,13 @@ __global__ void mfma_probe_kernel(const float* a, const float* b, float* o_bf16_
     acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
     for (int e = 0; e < 4; ++e) o_f32_16[acc16_row(e, l) * 16 + (l & 15)] = acc[e];
   }
-  // 16x16x4 f64: same operand layout.
+  // 16x16x4 f64: same operand layout, accumulator rows 4 e + l / 16.
   {
     double av = a[(l & 15) * 16 + (l >> 4)];
     double bv = b[(l & 15) * 16 + (l >> 4)];
     f64x4 acc = {};
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-    for (int e = 0; e < 4; ++e) o_f64_16[acc16_row(e, l) * 16 + (l & 15)] = acc[e];
+    for (int e = 0; e < 4; ++e) o_f64_16[acc16_row_f64(e, l) * 16 + (l & 15)] = acc[e];
   }
   // 16x16x32 bf16: A[row=l%16][k=8*(l/16)+i], B[k=8*(l/16)+i][col=l%16]; a, b as 16x32.
   {

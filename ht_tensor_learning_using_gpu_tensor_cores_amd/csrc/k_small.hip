@@ -233,6 +233,178 @@ int qr_fallback_count(unsigned long long* out) {
   return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_qr_householder), sizeof(*out)) == hipSuccess ? 0 : 5;
 }
 
+// Cholesky B = R^T R of the b x b SPD matrix B (LDS, ld SJLD, b <= 64) and X = R^{-1}, by
+// the whole block (SNT threads) with one barrier per step: a dependent step costs a few
+// hundred cycles whoever runs it, and spread over 16 waves its work is one or two items
+// per thread. Step k writes row k of R to R (its divisions read the unscaled row k of
+// B, which the trailing update of the same step only reads) and updates the trailing
+// square B[i][j] -= B[k][i] B[k][j] / B[k][k], i, j > k. The inverse then runs bottom-up,
+// row i of X from the rows below it, 16 threads per entry and a shuffle reduction. On
+// success R (upper, zeros below) is in R and X (upper) in B; returns 0 (uniformly) when
+// a pivot is not above piv_tol.
+__device__ int chol_rinv_block(double* B, double* R, int b, double piv_tol) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < b; ++k) {
+    const double d = B[k * SJLD + k];
+    if (!(d > piv_tol)) return 0;
+    const double rkk = sqrt(d), rinv = 1.0 / rkk, dinv = 1.0 / d;
+    const int m = b - k - 1;
+    for (int e = tid; e < b + m * m; e += SNT) {
+      if (e < b) {
+        R[k * SJLD + e] = e > k ? B[k * SJLD + e] * rinv : (e == k ? rkk : 0.0);
+      } else {
+        const int q = e - b, ii = q / m, jj = q - ii * m;
+        const int i = k + 1 + ii, j = k + 1 + jj;
+        B[i * SJLD + j] -= B[k * SJLD + i] * B[k * SJLD + j] * dinv;
+      }
+    }
+    __syncthreads();
+  }
+  // X[i][j] = (delta_ij - sum_{k = i+1..j} R[i][k] X[k][j]) / R[i][i]; thread (j, part)
+  // sums the terms k = i + 1 + part (mod 16), entry j < b, part < 16
+  const int j = tid >> 4, part = tid & 15;
+  for (int i = b - 1; i >= 0; --i) {
+    double t = 0.0;
+    if (j < b)
+      for (int k = i + 1 + part; k <= j; k += 16) t += R[i * SJLD + k] * B[k * SJLD + j];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (part == 0 && j < b) B[i * SJLD + j] = j > i ? -t / R[i * SJLD + i] : (j == i ? 1.0 / R[i * SJLD + i] : 0.0);
+    __syncthreads();
+  }
+  return 1;
+}
+
+// CholeskyQR2 of Y (row-major n x b, n % 64 == 0, b % 16 == 0, b <= BP, n * b <= QR_LDS)
+// for a block of SNT threads, with the products on FP64 MFMA (16x16x4):
+//   B = X^T X     waves own (16 x 16 tile, row range) pairs, partial tiles summed via LDS;
+//   B = R^T R, R^{-1}  the whole block (chol_rinv_block; R^{-1} lands in B);
+//   X <- X R^{-1} output tiles spread over the waves.
+// Twice; the first pass requires every pivot above 1e-12 * max diag(B), the second above
+// 1e-3 * max diag(B). The second pass starts from a nearly orthonormal X: when its Gram
+// is B = I + E with ||E||_F <= 1e-3 (the usual case) it uses W = B^{-1/2} from the
+// binomial series I - E/2 + 3/8 E^2 - 5/16 E^3 + 35/128 E^4 (next term below 3e-16)
+// instead, three b x b MFMA products and no dependent scalar steps; X W is orthonormal to
+// rounding and spans the same columns. Returns true with Q written, false (Q untouched)
+// when a pivot test failed. Ys holds n * b doubles, B and Ri SJ * SJLD each.
+template <int BP>
+__device__ bool cholqr2_mfma(const double* __restrict__ Y, double* __restrict__ Q, double* Ys, double* B, double* Ri,
+                             int n, int b) {
+  __shared__ double red[SW];
+  const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
+  const int i16 = lane & 15, kk = lane >> 4;
+  const int nt = b / 16, T = nt * nt, KQ = T >= 16 ? 1 : 16 / T;
+  for (int e = tid; e < n * b; e += SNT) Ys[e] = Y[e];
+  __syncthreads();
+  for (int pass = 0; pass < 2; ++pass) {
+    // B = Ys^T Ys: wave g < T * KQ takes tile t = g % T over rows [kq * n / KQ, +n / KQ);
+    // partial tiles go to Ri (KQ * T * 256 <= 4096 doubles), then are summed into B.
+    double* P = Ri;
+    if (g < T * KQ) {
+      const int t = g % T, kq = g / T, I = t / nt, J = t % nt;
+      const int rows = n / KQ;
+      f64x4 acc = {};
+      for (int st = 0; st < rows / 4; ++st) {
+        const int r = kq * rows + 4 * st + kk;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ys[r * b + 16 * I + i16], Ys[r * b + 16 * J + i16], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) P[(g * 16 + acc16_row_f64(e, lane)) * 16 + i16] = acc[e];
+    }
+    __syncthreads();
+    for (int e = tid; e < b * b; e += SNT) {
+      const int r = e / b, c = e % b;
+      const int t = (r / 16) * nt + c / 16;
+      double sum = 0.0;
+      for (int kq = 0; kq < KQ; ++kq) sum += P[((kq * T + t) * 16 + r % 16) * 16 + c % 16];
+      B[r * SJLD + c] = sum;
+    }
+    __syncthreads();
+    bool series = false;
+    if (pass == 1) {
+      double e2 = 0.0;
+      for (int e = tid; e < b * b; e += SNT) {
+        const int r = e / b, c = e % b;
+        const double x = B[r * SJLD + c] - (r == c ? 1.0 : 0.0);
+        e2 += x * x;
+      }
+      e2 = wave_sum(e2);
+      if (lane == 0) red[g] = e2;
+      __syncthreads();
+      double tot = 0.0;
+#pragma unroll
+      for (int w = 0; w < SW; ++w) tot += red[w];
+      series = tot <= 1e-6;  // uniform
+    }
+    const double* W = B;  // right factor of the update X <- X W
+    if (series) {
+      // E = B - I (in B); Horner: P = 35/128 E - 5/16 I, P = E P + 3/8 I, P = E P - 1/2 I,
+      // P = E P + I (in Ri)
+      for (int e = tid; e < b * b; e += SNT) {
+        const int r = e / b, c = e % b;
+        const double x = B[r * SJLD + c] - (r == c ? 1.0 : 0.0);
+        B[r * SJLD + c] = x;
+        Ri[r * SJLD + c] = (35.0 / 128.0) * x + (r == c ? -5.0 / 16.0 : 0.0);
+      }
+      __syncthreads();
+      const double cs[3] = {3.0 / 8.0, -0.5, 1.0};
+#pragma unroll
+      for (int h = 0; h < 3; ++h) {
+        f64x4 acc = {};
+        const int I = g / nt, J = g % nt;
+        if (g < T)
+          for (int st = 0; st < b / 4; ++st)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(B[(16 * I + i16) * SJLD + 4 * st + kk],
+                                                      Ri[(4 * st + kk) * SJLD + 16 * J + i16], acc, 0, 0, 0);
+        __syncthreads();
+        if (g < T) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int r = 16 * I + acc16_row_f64(e, lane), c = 16 * J + i16;
+            Ri[r * SJLD + c] = acc[e] + (r == c ? cs[h] : 0.0);
+          }
+        }
+        __syncthreads();
+      }
+      W = Ri;
+    } else {
+      double dmax = 0.0;
+      for (int k = 0; k < b; ++k) dmax = fmax(dmax, B[k * SJLD + k]);
+      const double piv_tol = (pass == 0 ? 1e-12 : 1e-3) * dmax;
+      if (!chol_rinv_block(B, Ri, b, piv_tol)) return false;  // uniform: every thread read the same pivot
+    }
+    // X <- X W: output tile (RT, CT); W = R^{-1} is upper triangular (k < 16 (CT + 1)),
+    // the series factor is full. n b <= 8192 makes at most 32 tiles, two per wave.
+    const int ntile = (n / 16) * nt;
+    f64x4 res[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int tile = g + 16 * u;
+      res[u] = f64x4{};
+      if (tile < ntile) {
+        const int RT = tile / nt, CT = tile % nt;
+        const int ksteps = series ? b / 4 : 4 * (CT + 1);
+        for (int st = 0; st < ksteps; ++st)
+          res[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ys[(16 * RT + i16) * b + 4 * st + kk],
+                                                        W[(4 * st + kk) * SJLD + 16 * CT + i16], res[u], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int tile = g + 16 * u;
+      if (tile < ntile) {
+        const int RT = tile / nt, CT = tile % nt;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Ys[(16 * RT + acc16_row_f64(e, lane)) * b + 16 * CT + i16] = res[u][e];
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * b; e += SNT) Q[e] = Ys[e];
+  return true;
+}
+
 template <int BP, int SR>
 __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, double* __restrict__ Q, long sYQ, int n,
                                                   int b, int try_chol) {
@@ -258,6 +430,14 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
   }
   // ---- CholeskyQR2
   bool chol = try_chol != 0;
+  // (n % 64 == 0 and n b <= 8192 leave n <= 128 for b > 32: the SR = 16 packing never
+  // takes this path)
+  if constexpr (SR <= 8) {
+    if (chol && n % 64 == 0 && b % 16 == 0) {
+      if (cholqr2_mfma<BP>(Y, Q, Ys, B, Ri, n, b)) return;
+      chol = false;
+    }
+  }
   for (int pass = 0; pass < 2 && chol; ++pass) {
 #pragma unroll
     for (int i = 0; i < SR; ++i) {
@@ -500,26 +680,42 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
   // Ritz vectors u = Q S[:, order[c]] and G u = Z S[:, order[c]] on this thread's rows.
   const int src = c < b ? order[c] : 0;
   const double th = c < b ? a[src * SJLD + src] : 0.0;
+  // (the row index is the inner loop: SR independent accumulators keep the LDS loads
+  // in flight; rows past n read row 0 and are not used)
   double u[SR];
 #pragma unroll
-  for (int i = 0; i < SR; ++i) {
-    const int r = (i * SW + g) * RS + h;
-    double t = 0.0;
-    if (r < n && c < b)
-      for (int l = 0; l < b; ++l) t += Ms[r * b + l] * v[l * SJLD + src];
-    u[i] = t;
+  for (int i = 0; i < SR; ++i) u[i] = 0.0;
+  if (c < b) {
+    for (int l = 0; l < b; ++l) {
+      const double sl = v[l * SJLD + src];
+#pragma unroll
+      for (int i = 0; i < SR; ++i) {
+        const int r = (i * SW + g) * RS + h;
+        u[i] += Ms[(r < n ? r : 0) * b + l] * sl;
+      }
+    }
   }
   __syncthreads();
   for (int e = tid; e < n * b; e += SNT) Ms[e] = Z[e];
   __syncthreads();
+  double gu[SR];
+#pragma unroll
+  for (int i = 0; i < SR; ++i) gu[i] = 0.0;
+  if (c < b) {
+    for (int l = 0; l < b; ++l) {
+      const double sl = v[l * SJLD + src];
+#pragma unroll
+      for (int i = 0; i < SR; ++i) {
+        const int r = (i * SW + g) * RS + h;
+        gu[i] += Ms[(r < n ? r : 0) * b + l] * sl;
+      }
+    }
+  }
   double rsum = 0.0;
 #pragma unroll
   for (int i = 0; i < SR; ++i) {
     const int r = (i * SW + g) * RS + h;
-    double t = 0.0;
-    if (r < n && c < b)
-      for (int l = 0; l < b; ++l) t += Ms[r * b + l] * v[l * SJLD + src];
-    const double d = t - th * u[i];
+    const double d = (r < n && c < b) ? gu[i] - th * u[i] : 0.0;
     rsum += d * d;
   }
   rsum = rowset_sum<BP>(rsum);
