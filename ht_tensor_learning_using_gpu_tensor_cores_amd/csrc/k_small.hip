@@ -62,13 +62,16 @@ __global__ __launch_bounds__(1024) void jacobi_full_kernel(const double* __restr
 }
 
 // Y = f(G) W for a batch of problems (n <= 256, row-major blocks, ld b): PW_COLS columns
-// per work-group of 1024 threads; thread (row r, part p) sums the p-th quarter of the
-// row's dot products and the quarters are added through LDS.
-//   mode 0: q powers, each followed by a column normalisation (the span is unchanged);
+// per work-group of 1024 threads; thread (row pair rp, part p) sums the p-th eighth of the
+// dot products of rows 2 rp and 2 rp + 1, and the eighths are added through LDS. Two rows
+// per thread let each value of the current block (one 16-byte LDS read per two columns)
+// serve two rows: the loop is bound by LDS instruction issue, not by G's L2 traffic.
+//   mode 0: q powers, every fourth one (and the last) followed by a column normalisation
+//           (the span is unchanged);
 //   mode 1: one product, unnormalised;
 //   mode 2: the degree-q Chebyshev filter on [0, e], e = cut[problem]:
 //           T_0 = W, T_1 = L W, T_{j+1} = 2 L T_j - T_{j-1}, L = (2 G - e I) / e, with
-//           (T_j, T_{j-1}) rescaled together by the column norm of T_j at every step.
+//           (T_j, T_{j-1}) rescaled together by the column norm of T_j every fourth step.
 //           Falls back to mode 0 when e <= 0.
 // With T != nullptr (mode 1), the block also forms T[l][c] = sum_r W[r][l] Y[r][c] for its
 // columns (row-major b x b per problem, stride 64 * 64): the Rayleigh quotient Q^T G Q
@@ -77,13 +80,12 @@ constexpr int PW_COLS = 4;
 __global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ G, long sG, const double* __restrict__ W,
                                                      double* __restrict__ Y, long sWY, int n, int b, int q, int mode,
                                                      const double* __restrict__ cut, double* __restrict__ T) {
-  __shared__ double cur[256 * PW_COLS];
+  __shared__ __attribute__((aligned(16))) double cur[256 * PW_COLS];
   __shared__ double prv[256 * PW_COLS];
-  __shared__ double part[4][256 * PW_COLS];
+  __shared__ double part[8][256 * PW_COLS];
   __shared__ double red[SW][PW_COLS];
   __shared__ double scl[PW_COLS];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int r = tid & 255, p = tid >> 8;
   const int c0 = blockIdx.x * PW_COLS, bidx = blockIdx.y;
   const int nc = min(PW_COLS, b - c0);
   G += bidx * sG;
@@ -100,34 +102,47 @@ __global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ 
     prv[x] = 0.0;
   }
   __syncthreads();
-  const int l0 = (p * n) / 4, l1 = ((p + 1) * n) / 4;
+  const int rp = tid & 127, p8 = tid >> 7;
+  const int ra = 2 * rp, rb = 2 * rp + 1;
+  const int l0 = (p8 * n) / 8, l1 = ((p8 + 1) * n) / 8;
   const bool own = tid < 256 * PW_COLS;  // tid = row * PW_COLS + column
+  static_assert(PW_COLS == 4, "two 16-byte reads per l");
   for (int it = 0; it < q; ++it) {
-    double acc[PW_COLS];
+    double acc0[PW_COLS], acc1[PW_COLS];
 #pragma unroll
-    for (int c = 0; c < PW_COLS; ++c) acc[c] = 0.0;
-    if (r < n) {
-      const double* gp = G + r;
-#pragma unroll 8
+    for (int c = 0; c < PW_COLS; ++c) acc0[c] = acc1[c] = 0.0;
+    if (ra < n) {
+      const double* ga = G + ra;
+      const double* gb = G + (rb < n ? rb : ra);
+#pragma unroll 4
       for (int l = l0; l < l1; ++l) {
-        const double g = gp[(long)n * l];
-#pragma unroll
-        for (int c = 0; c < PW_COLS; ++c) acc[c] += g * cur[l * PW_COLS + c];
+        const double g0 = ga[(long)n * l], g1 = gb[(long)n * l];
+        const double2 c01 = *reinterpret_cast<const double2*>(&cur[l * PW_COLS]);
+        const double2 c23 = *reinterpret_cast<const double2*>(&cur[l * PW_COLS + 2]);
+        acc0[0] += g0 * c01.x; acc0[1] += g0 * c01.y; acc0[2] += g0 * c23.x; acc0[3] += g0 * c23.y;
+        acc1[0] += g1 * c01.x; acc1[1] += g1 * c01.y; acc1[2] += g1 * c23.x; acc1[3] += g1 * c23.y;
       }
     }
 #pragma unroll
-    for (int c = 0; c < PW_COLS; ++c) part[p][r * PW_COLS + c] = acc[c];
+    for (int c = 0; c < PW_COLS; ++c) {
+      part[p8][ra * PW_COLS + c] = acc0[c];
+      part[p8][rb * PW_COLS + c] = rb < n ? acc1[c] : 0.0;  // rows past n stay zero
+    }
     __syncthreads();
     double y = 0.0, x = 0.0;
     if (own) {
-      y = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+#pragma unroll
+      for (int pp = 0; pp < 8; ++pp) y += part[pp][tid];
       x = cur[tid];
       if (mode == 2) {
         const double ly = (2.0 * y - e * x) / e;
         y = (it == 0) ? ly : 2.0 * ly - prv[tid];
       }
     }
-    if (mode != 1) {
+    // column normalisation every fourth step and on the last: it only keeps the values in
+    // range (the span is unchanged), and four steps grow them by at most T_4(x_1) (mode 2)
+    // or theta_1^4 (mode 0), far inside the FP64 range
+    if (mode != 1 && ((it & 3) == 3 || it == q - 1)) {
       double s[PW_COLS];
 #pragma unroll
       for (int c = 0; c < PW_COLS; ++c) s[c] = wave_sum((own && (tid % PW_COLS) == c) ? y * y : 0.0);
@@ -675,7 +690,9 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
     a[(np - 1) * SJLD + (np - 1)] = -(m * (double)b + 1.0) * 4.0;
   }
   __syncthreads();
-  jacobi_sweeps(a, v, b, np, max_sweeps, 1e-14, sched);
+  // the Ritz residuals are tested against tol * theta_1: an off-diagonal remainder 1e3
+  // times smaller than that is invisible in them (and never below 1e-14)
+  jacobi_sweeps(a, v, b, np, max_sweeps, fmax(1e-14, 1e-3 * tol), sched);
   jacobi_order(a, np, order);
   // Ritz vectors u = Q S[:, order[c]] and G u = Z S[:, order[c]] on this thread's rows.
   const int src = c < b ? order[c] : 0;
