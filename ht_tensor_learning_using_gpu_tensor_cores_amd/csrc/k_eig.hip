@@ -695,22 +695,55 @@ int ritz_resid_launch(hipStream_t st, const double* GU, const double* U, const d
 
 // ------------------------------------------------------------------ FP64 Gram of a view
 // G(i,j) = sum_{p,s} x[p + P i + P M s] x[p + P j + P M s] for small tensors (M <= 4096).
-__global__ void gram_pms_f64_kernel(const double* __restrict__ x, PMS v, double* __restrict__ G) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= v.M * v.M) return;
-  const long i = e % v.M, j = e / v.M;
-  if (i > j) return;
-  double s = 0.0;
-  for (long q = 0; q < v.S; ++q) {
-    const double* b = x + v.P * v.M * q;
-    for (long p = 0; p < v.P; ++p) s += b[p + v.P * i] * b[p + v.P * j];
+// One 16 x 16 tile (I <= J) per block of 4 waves on FP64 MFMA 16x16x4: wave w takes the
+// reduction steps k0 = 4 (w + 4 u), eight per batch with all sixteen loads issued before
+// the products (indices past K are clamped and their values zeroed); the four partial
+// tiles are added through LDS and the tile is written with its mirror.
+__global__ __launch_bounds__(256) void gram_pms_f64_kernel(const double* __restrict__ x, PMS v, double* __restrict__ G) {
+  const int I = blockIdx.x, J = blockIdx.y;
+  if (I > J) return;
+  __shared__ double part[4][256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i16 = lane & 15, kk = lane >> 4;
+  const long M = v.M, P = v.P, K = v.P * v.S;
+  const long i = 16L * I + i16, j = 16L * J + i16;
+  const bool iok = i < M, jok = j < M;
+  const long ic = iok ? i : M - 1, jc = jok ? j : M - 1;
+  auto at = [&](long row, long k) {  // x[p + P row + P M s], k = p + P s
+    const long s = k / P, q = k - s * P;
+    return x[q + P * row + P * M * s];
+  };
+  f64x4 acc = {};
+  constexpr int UB = 8;
+  for (long k0 = 4L * w; k0 < K; k0 += 16L * UB) {
+    double a[UB], b[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const long k = k0 + 16L * u + kk;
+      const long kc = k < K ? k : K - 1;
+      a[u] = at(ic, kc);
+      b[u] = at(jc, kc);
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const bool ok = k0 + 16L * u + kk < K;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ok && iok ? a[u] : 0.0, ok && jok ? b[u] : 0.0, acc, 0, 0, 0);
+    }
   }
-  G[i + v.M * j] = s;
-  G[j + v.M * i] = s;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) part[w][acc16_row_f64(e, lane) * 16 + i16] = acc[e];
+  __syncthreads();
+  const int r = tid >> 4, c = tid & 15;
+  const long gi = 16L * I + r, gj = 16L * J + c;
+  if (gi < M && gj < M) {
+    const double sum = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+    G[gi + M * gj] = sum;
+    G[gj + M * gi] = sum;
+  }
 }
 int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G) {
-  const long tot = v.M * v.M;
-  hipLaunchKernelGGL(gram_pms_f64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, x, v, G);
+  const unsigned nt = (unsigned)((v.M + 15) / 16);
+  hipLaunchKernelGGL(gram_pms_f64_kernel, dim3(nt, nt), dim3(256), 0, st, x, v, G);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 

@@ -316,10 +316,68 @@ __global__ void proj_pms_kernel(const TI* __restrict__ x, PMS v, const TU* __res
   }
 }
 
+// The projection as FP64 MFMA tiles: y[j, n] = sum_m U[m, j] X[m, n] with n = p + P s and
+// X[m, n] = x[p + P m + P M s]. One wave per 16 (j) x 16 (n) output tile, 16x16x4 steps
+// over m in batches of eight whose loads all issue before the products (indices past M,
+// r or N are clamped and their values zeroed). A lane's n (hence p, s and the base of
+// its X column) is fixed for the whole tile.
+template <typename TI, typename TU, typename TO>
+__global__ __launch_bounds__(256) void proj_mfma_kernel(const TI* __restrict__ x, PMS v, const TU* __restrict__ U,
+                                                        int r, TO* __restrict__ y, long ntn) {
+  const int lane = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long njt = (r + 15) / 16;
+  if (wave >= njt * ntn) return;
+  const long NT = wave / njt, JT = wave % njt;
+  const int i16 = lane & 15, kk = lane >> 4;
+  const long M = v.M, N = v.P * v.S;
+  const long n = NT * 16 + i16;
+  const bool nok = n < N;
+  const long nc = nok ? n : N - 1;
+  const long ss = nc / v.P, pp = nc - ss * v.P;
+  const TI* xb = x + pp + v.P * M * ss;
+  const long j = JT * 16 + i16;
+  const bool jok = j < r;
+  const TU* ub = U + M * (jok ? j : r - 1);
+  constexpr int UB = 8;
+  f64x4 acc = {};
+  for (long m0 = 0; m0 < M; m0 += 4 * UB) {
+    double a[UB], b[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const long m = m0 + 4 * u + kk;
+      const long mc = m < M ? m : M - 1;
+      a[u] = (double)ub[mc];
+      b[u] = (double)xb[v.P * mc];
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const bool ok = m0 + 4 * u + kk < M;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ok && jok ? a[u] : 0.0, ok && nok ? b[u] : 0.0, acc, 0, 0, 0);
+    }
+  }
+  if (nok) {
+    TO* yb = y + pp + v.P * (long)r * ss;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const long jj = JT * 16 + acc16_row_f64(e, lane);
+      if (jj < r) yb[v.P * jj] = (TO)acc[e];
+    }
+  }
+}
+
 template <typename TI, typename TU, typename TO>
 static int proj_launch(hipStream_t st, const TI* x, PMS v, const TU* U, int r, TO* y) {
   const long total = v.P * (long)r * v.S;
   if (total <= 0) return 0;
+  {
+    const long ntn = (v.P * v.S + 15) / 16, waves = ((r + 15) / 16) * ntn;
+    const long blocks = (waves + 3) / 4;
+    if (blocks <= (1L << 30)) {
+      hipLaunchKernelGGL((proj_mfma_kernel<TI, TU, TO>), dim3((unsigned)blocks), dim3(256), 0, st, x, v, U, r, y, ntn);
+      return hipGetLastError() == hipSuccess ? 0 : 5;
+    }
+  }
   long blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL((proj_pms_kernel<TI, TU, TO>), dim3((unsigned)blocks), dim3(256), 0, st, x, v, U, r, y);
