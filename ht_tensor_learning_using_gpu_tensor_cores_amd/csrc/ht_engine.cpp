@@ -303,11 +303,14 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   double* Z = alloc<double>(batch * nbmax);
   double* Uf = alloc<double>(batch * nbmax);
   double* T = alloc<double>((size_t)batch * 64 * 64);
-  double* wall = alloc<double>((size_t)batch * 64);
   double* cut = alloc<double>(batch);
-  int* conv = alloc<int>(batch);
-  double* rres = alloc<double>(batch);
-  if (!W || !Y || !Q || !Z || !Uf || !T || !wall || !cut || !conv || !rres) return HT_ERR_ALLOC;
+  // per-round results the host reads, in one block (one copy): Ritz values, residuals, flags
+  const size_t wb = sizeof(double) * batch * 64, rb = sizeof(double) * batch, cb = sizeof(int) * batch;
+  char* rd = alloc<char>(wb + rb + cb);
+  if (!W || !Y || !Q || !Z || !Uf || !T || !cut || !rd) return HT_ERR_ALLOC;
+  double* wall = reinterpret_cast<double*>(rd);
+  double* rres = reinterpret_cast<double*>(rd + wb);
+  int* conv = reinterpret_cast<int*>(rd + wb + rb);
   long nb = (long)n * b;
   CK(fill_normals_f64_launch(st, W, batch * nb, opt.seed, stream));
   std::vector<int> hconv(batch);
@@ -324,23 +327,19 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, floor, conv, rres, batch));
     iters += q + 1;
     {
-      // both reads queue behind one synchronisation when they fit the pinned staging
-      const size_t cb = sizeof(int) * batch, wo = (cb + 255) & ~(size_t)255, wb = sizeof(double) * batch * 64;
-      const size_t ro = wo + ((wb + 255) & ~(size_t)255), rb = sizeof(double) * batch;
-      if (pin && ro + rb <= pin_bytes) {
-        HT_HIP_CHECK(hipMemcpyAsync(pin, conv, cb, hipMemcpyDeviceToHost, st));
-        HT_HIP_CHECK(hipMemcpyAsync(pin + wo, wall, wb, hipMemcpyDeviceToHost, st));
-        HT_HIP_CHECK(hipMemcpyAsync(pin + ro, rres, rb, hipMemcpyDeviceToHost, st));
-        HT_HIP_CHECK(hipStreamSynchronize(st));
-        std::memcpy(hconv.data(), pin, cb);
-        std::memcpy(hwall.data(), pin + wo, wb);
-        std::memcpy(hres.data(), pin + ro, rb);
-      } else {
-        HT_HIP_CHECK(hipMemcpyAsync(hconv.data(), conv, cb, hipMemcpyDeviceToHost, st));
-        HT_HIP_CHECK(hipMemcpyAsync(hwall.data(), wall, wb, hipMemcpyDeviceToHost, st));
-        HT_HIP_CHECK(hipMemcpyAsync(hres.data(), rres, rb, hipMemcpyDeviceToHost, st));
-        HT_HIP_CHECK(hipStreamSynchronize(st));
+      // one copy of the result block, through the pinned staging when it fits
+      const size_t tot = wb + rb + cb;
+      char* hb = (pin && tot <= pin_bytes) ? pin : nullptr;
+      std::vector<char> tmp;
+      if (!hb) {
+        tmp.resize(tot);
+        hb = tmp.data();
       }
+      HT_HIP_CHECK(hipMemcpyAsync(hb, rd, tot, hipMemcpyDeviceToHost, st));
+      HT_HIP_CHECK(hipStreamSynchronize(st));
+      std::memcpy(hwall.data(), hb, wb);
+      std::memcpy(hres.data(), hb + wb, rb);
+      std::memcpy(hconv.data(), hb + wb + rb, cb);
     }
     const int b_rr = b;  // block width of this round's Ritz values
     if (getenv("HT_DEBUG") && getenv("HT_DEBUG")[0] == '2') {

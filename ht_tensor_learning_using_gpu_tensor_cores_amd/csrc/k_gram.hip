@@ -341,8 +341,14 @@ int gram_pms_launch(hipStream_t st, const float* x, PMS v, int prec, int n_split
 }
 
 // Sum the split partials of every tile pair into the full symmetric FP64 Gram.
-__global__ void gram_reduce_kernel(const float* __restrict__ part, int M, int nt, int n_split,
-                                   double* __restrict__ G) {
+// Sums the n_split FP32 partial tiles of each tile pair into the FP64 Gram. A block takes
+// 64 entries of one 128 x 128 tile (one per lane, consecutive in memory) and its 16 waves
+// split the partials (wave w: w, w + 16, ..., eight loads in flight per batch); the
+// waves' sums are added in a fixed order through LDS. Diagonal tiles produce every entry
+// themselves, off-diagonal ones also write the mirrored entry.
+__global__ __launch_bounds__(1024) void gram_reduce_kernel(const float* __restrict__ part, int M, int nt, int n_split,
+                                                           double* __restrict__ G) {
+  __shared__ double red[16][64];
   const int pair = blockIdx.y;
   int bi = 0, bj = 0;
   {
@@ -353,25 +359,43 @@ __global__ void gram_reduce_kernel(const float* __restrict__ part, int M, int nt
       p -= cnt;
     }
   }
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= GT_ROWS * GT_ROWS) return;
-  const int r = e / GT_ROWS, c = e % GT_ROWS;
-  const int i = GT_ROWS * bi + r, j = GT_ROWS * bj + c;
-  if (i >= M || j >= M) return;
-  if (bi == bj && r > c) return;
-  const float* p = part + (long)pair * n_split * (GT_ROWS * GT_ROWS) + e;
+  constexpr int T2 = GT_ROWS * GT_ROWS;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int e = blockIdx.x * 64 + lane;
+  const float* p = part + (long)pair * n_split * T2 + e;
   double s = 0.0;
-#pragma unroll 8
-  for (int k = 0; k < n_split; ++k) s += (double)p[(long)k * (GT_ROWS * GT_ROWS)];
-  G[i + (long)M * j] = s;
-  G[j + (long)M * i] = s;
+  constexpr int UB = 8;
+  for (int k0 = w; k0 < n_split; k0 += 16 * UB) {
+    float v[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int k = k0 + 16 * u;
+      v[u] = p[(long)(k < n_split ? k : n_split - 1) * T2];
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u)
+      if (k0 + 16 * u < n_split) s += (double)v[u];
+  }
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t += red[q][lane];
+    const int r = e / GT_ROWS, c = e % GT_ROWS;
+    const int i = GT_ROWS * bi + r, j = GT_ROWS * bj + c;
+    if (i < M && j < M) {
+      G[i + (long)M * j] = t;
+      if (bi != bj) G[j + (long)M * i] = t;
+    }
+  }
 }
 
 int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G) {
   const int nt = (M + GT_ROWS - 1) / GT_ROWS;
   const int pairs = nt * (nt + 1) / 2;
-  dim3 grid((GT_ROWS * GT_ROWS + 255) / 256, pairs);
-  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(256), 0, st, part, M, nt, n_split, G);
+  dim3 grid(GT_ROWS * GT_ROWS / 64, pairs);
+  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, part, M, nt, n_split, G);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
