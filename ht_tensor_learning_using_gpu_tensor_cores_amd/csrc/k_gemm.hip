@@ -486,12 +486,12 @@ int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, lon
   }
   if (k > 64) return 8;
   const bool full = M % LR_T == 0 && N % LR_T == 0;
-  // Residual blocks: 128 rows x 32 nts columns. Waves that stream more subtiles keep
-  // their prefetch busy for longer; nts shrinks while the grid would fall below 2048
-  // blocks (8 per CU).
+  // Residual blocks: 128 rows x 32 nts columns (nts <= 32). Waves that stream more
+  // subtiles keep their prefetch busy for longer; nts shrinks while the grid would fall
+  // below 2048 blocks (8 per CU).
   int nts = 4;
   if (full && k <= 16) {
-    nts = 16;
+    nts = 32;
     while (nts > 4 && (N % (32L * nts) != 0 || grid.x * (N / (32L * nts)) < 2048)) nts /= 2;
     grid.y = (unsigned)(N / (32L * nts));
   }
@@ -602,7 +602,8 @@ __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__
 
 // The residual pass for k <= 16 with M a multiple of 128 and N of 32 nts: the same
 // block, wave and lane layout as residual_kernel, no bounds checks, and the next
-// subtile's X is loaded while the current one is multiplied. The subtile after the last
+// subtile's X is loaded while the current one is multiplied. X is read once, with
+// streaming (nontemporal) loads; each half-wave load covers whole 128-byte lines. The subtile after the last
 // is the last again (loaded, never used), so every path issues the same loads and each
 // wait covers only the subtile being multiplied; a scheduling barrier keeps the
 // compiler from sinking the loads below the products.
@@ -627,7 +628,8 @@ __global__ __launch_bounds__(256) void residual_stream_kernel(const float* __res
   auto load_x = [&](float* xv, int t) {
     const float* base = X + mw + M * (nbase + 32 * t);
 #pragma unroll
-    for (int e = 0; e < 16; ++e) xv[e] = base[(unsigned)li + mu * (unsigned)acc32_row(e, lane)];
+    for (int e = 0; e < 16; ++e)
+      xv[e] = __builtin_nontemporal_load(base + ((unsigned)li + mu * (unsigned)acc32_row(e, lane)));
   };
   double se = 0.0, sx = 0.0;
   auto tile = [&](const float* xv, int t) {
