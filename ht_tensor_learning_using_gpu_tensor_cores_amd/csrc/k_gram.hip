@@ -577,6 +577,15 @@ int slab_gram_launch(hipStream_t st, const float* x, int n1, int n2, long S, int
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
+// Orders the LDS accesses of one wave (lanes read what other lanes of the same wave wrote
+// before): LDS operations of a wave complete in order; the fences keep the compiler from
+// moving loads or stores across this point.
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------------ slab_proj
 // One wavefront per slab S (n1 x n2, column-major, n1 % 16 == 0, n2 % 16 == 0,
 // r1, r2 <= 32). FP32 MFMA 16x16x4 with the reduction index permuted so that every
@@ -585,13 +594,14 @@ int slab_gram_launch(hipStream_t st, const float* x, int n1, int n2, long S, int
 //                               i = i0 + 4g + s, i.e. S[i0+4g .. i0+4g+3][j] (one float4);
 //   Z^T (r2 x r1) = U2^T T^T -- the T^T accumulators are already in B-operand layout
 //                               (their element e is row 4g + e = the slot of step e).
-// U1 and U2 sit in LDS in the B/A-operand order (one float4 = the four steps of a lane),
-// which keeps the wave's registers for two 16-column blocks of the slab: the next block
-// (of this slab or of the wave's next one) is loaded while the current one is
-// multiplied. Those loads are unconditional -- rows past n1 are clamped (their U1 rows
-// are zero in LDS) and past the last block the current block is loaded again -- so the
-// number of loads in flight is the same on every path and the waits stay one block
-// behind; a scheduling barrier keeps the compiler from sinking them below the MFMAs.
+// U1 and U2 sit in LDS in the B/A-operand order (one float4 = the four steps of a lane).
+// The slab's 16-column blocks (16 n1 contiguous floats) come from HBM as whole 1 KB wave
+// loads (lane l takes floats 4 l + 256 i), streaming (nontemporal), into registers one
+// block ahead; each block is then written to a per-wave LDS tile (column stride 132
+// floats: conflict-free float4 reads) and read back in the operand layout. A wave's
+// loads and LDS use are its own, so only wave-level ordering is needed. The loads are
+// unconditional (past the last block the current one is loaded again), so the waits stay
+// one block behind; a scheduling barrier keeps them above the MFMAs.
 template <int RB1, int RB2>
 __global__ __launch_bounds__(256) void slab_proj_kernel(const float* __restrict__ x, int n1, int n2, long S,
                                                         const float* __restrict__ U1, int r1,
@@ -599,6 +609,8 @@ __global__ __launch_bounds__(256) void slab_proj_kernel(const float* __restrict_
                                                         float* __restrict__ z) {
   // lu1[((a * 8 + ig) * 4 + g) * 16 + col] = U1[16 ig + 4 g + 0..3][16 a + col], lu2 alike
   __shared__ float4 lu1[RB1 * 512], lu2[RB2 * 512];
+  constexpr int SLD = 132;
+  __shared__ __attribute__((aligned(16))) float stage[4][16 * SLD];
   const int n1g = n1 / 16;  // row groups of 16 (<= 8)
   const int nb2 = n2 / 16;  // column blocks of 16 (<= 8)
   for (int e = threadIdx.x; e < RB1 * 512; e += blockDim.x) {
@@ -622,36 +634,50 @@ __global__ __launch_bounds__(256) void slab_proj_kernel(const float* __restrict_
     lu2[e] = v;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int col = lane & 15, g = lane >> 4;
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long n_waves = ((long)gridDim.x * blockDim.x) >> 6;
   if (wave >= S) return;
   const long slab = (long)n1 * n2;
+  float* stg = stage[wv];
 
-  auto load = [&](float4* sv, long s, int bb) {
-    const float* colp = x + s * slab + (long)(16 * bb + col) * n1 + 4 * g;
+  auto load = [&](f32x4* R, long s, int bb) {
+    const float* base = x + s * slab + (long)(16 * bb) * n1;
 #pragma unroll
-    for (int ig = 0; ig < 8; ++ig) sv[ig] = *reinterpret_cast<const float4*>(colp + 16 * min(ig, n1g - 1));
+    for (int i = 0; i < 8; ++i)
+      R[i] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(base + 4 * lane + 256 * min(i, n1g - 1)));
+  };
+  auto stage_store = [&](const f32x4* R) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i < n1g) {
+        const int off = 4 * lane + 256 * i, c = off / n1, r = off - c * n1;
+        *reinterpret_cast<f32x4*>(&stg[c * SLD + r]) = R[i];
+      }
+    }
   };
   f32x4 zt[RB2][RB1];
 #pragma unroll
   for (int b = 0; b < RB2; ++b)
 #pragma unroll
     for (int a = 0; a < RB1; ++a) zt[b][a] = f32x4{};
-  auto compute = [&](const float4* sv, long s, int bb) {
+  auto compute = [&](long s, int bb) {
+    f32x4 sv[8];
+#pragma unroll
+    for (int ig = 0; ig < 8; ++ig)
+      sv[ig] = ig < n1g ? *reinterpret_cast<const f32x4*>(&stg[col * SLD + 16 * ig + 4 * g]) : f32x4{};
     f32x4 tt[RB1];
 #pragma unroll
     for (int a = 0; a < RB1; ++a) tt[a] = f32x4{};
 #pragma unroll
     for (int ig = 0; ig < 8; ++ig) {
-      const float svs[4] = {sv[ig].x, sv[ig].y, sv[ig].z, sv[ig].w};
 #pragma unroll
       for (int a = 0; a < RB1; ++a) {
         const float4 u = lu1[((a * 8 + ig) * 4 + g) * 16 + col];
         const float us[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
-        for (int st = 0; st < 4; ++st) tt[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(svs[st], us[st], tt[a], 0, 0, 0);
+        for (int st = 0; st < 4; ++st) tt[a] = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[ig][st], us[st], tt[a], 0, 0, 0);
       }
     }
     // tt[a] element e: T^T[row 16bb + 4g + e][col 16a + col].
@@ -681,28 +707,24 @@ __global__ __launch_bounds__(256) void slab_proj_kernel(const float* __restrict_
         }
     }
   };
-  float4 A[8], B[8];
+  f32x4 R[8];
   long s = wave;
   int bb = 0;
-  load(A, s, bb);
+  load(R, s, bb);
   while (true) {
     long s2 = s;
     int bb2 = bb + 1;
     if (bb2 == nb2) { bb2 = 0; s2 += n_waves; }
     const bool more = s2 < S;
-    load(B, more ? s2 : s, more ? bb2 : bb);
+    wave_sync_lds();  // the previous block's operand reads are done
+    stage_store(R);
+    wave_sync_lds();
+    load(R, more ? s2 : s, more ? bb2 : bb);
     __builtin_amdgcn_sched_barrier(0);
-    compute(A, s, bb);
+    compute(s, bb);
     if (!more) break;
-    s = s2; bb = bb2;
-    s2 = s; bb2 = bb + 1;
-    if (bb2 == nb2) { bb2 = 0; s2 += n_waves; }
-    const bool more2 = s2 < S;
-    load(A, more2 ? s2 : s, more2 ? bb2 : bb);
-    __builtin_amdgcn_sched_barrier(0);
-    compute(B, s, bb);
-    if (!more2) break;
-    s = s2; bb = bb2;
+    s = s2;
+    bb = bb2;
   }
 }
 
@@ -711,9 +733,10 @@ int slab_proj_launch(hipStream_t st, const float* x, int n1, int n2, long S, con
   if (n1 < 16 || n2 < 16 || n1 % 16 || n2 % 16 || n1 > 128 || n2 > 128 || r1 < 1 || r2 < 1 || r1 > 32 || r2 > 32)
     return 8;
   if (((uintptr_t)x & 15) != 0) return 8;
+  // three blocks per CU fit the LDS (U tiles and four 8.4 KB stages each)
   long waves = S;
   long blocks = (waves + 3) / 4;
-  if (blocks > 1024) blocks = 1024;
+  if (blocks > 768) blocks = 768;
   dim3 grid((unsigned)blocks);
   if (r1 <= 16 && r2 <= 16)
     hipLaunchKernelGGL((slab_proj_kernel<1, 1>), grid, dim3(256), 0, st, x, n1, n2, S, U1, r1, U2, r2, z);
