@@ -206,7 +206,7 @@ int scale_cols_inv_launch(hipStream_t st, double* A, int m, int n, const double*
 int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* w, long sw, double* V, long sV,
                        int batch, int* info, int max_sweeps, double tol);
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
-                 int mode, const double* cut, double* T, int batch);
+                 int mode, const double* cut, double* T, int batch, const float* G32 = nullptr);
 int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int try_chol, int batch);
 // Running count of qr_kernel calls that took the Householder fallback (process-wide).
 int qr_fallback_count(unsigned long long* out);

@@ -313,6 +313,15 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   int* conv = reinterpret_cast<int*>(rd + wb + rb);
   long nb = (long)n * b;
   CK(fill_normals_f64_launch(st, W, batch * nb, opt.seed, stream));
+  // At tolerances of 1e-5 and above the filter steps read an FP32 copy of G: the subspace
+  // they converge to is that of G rounded to FP32, within ~6e-8 sqrt(n) theta_1 of G's (far
+  // below tol theta_1), and the Ritz values and residuals are still formed with G itself.
+  float* G32 = nullptr;
+  if (tol >= 1e-5) {
+    G32 = alloc<float>((size_t)batch * n * n);
+    if (!G32) return HT_ERR_ALLOC;
+    CK(cast_f64_f32_launch(st, G, G32, (long)batch * n * n));
+  }
   std::vector<int> hconv(batch);
   std::vector<double> hwall((size_t)batch * 64), hres(batch);
   int q = std::max(1, q0), mode = 0, iters = 0;
@@ -321,7 +330,7 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     // few powers (its extra columns are then numerically rank deficient when the spectrum
     // falls off after k)
     const int try_chol = (round > 0 || q >= 4 || b == k) ? 1 : 0;
-    CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, nullptr, batch));
+    CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, nullptr, batch, G32));
     CK(qr_launch(st, Y, Q, nb, n, b, try_chol, batch));
     CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, T, batch));
     CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, floor, conv, rres, batch));

@@ -18,6 +18,7 @@
 // after another. The row-distributed kernels (qr, rr) run 16 waves; thread (wave g,
 // lane c) holds column c of rows r = i * 16 + g, i < 16, in registers.
 #include "k_jacobi.h"
+#include <type_traits>
 
 namespace ht {
 
@@ -76,8 +77,8 @@ __global__ __launch_bounds__(1024) void jacobi_full_kernel(const double* __restr
 // With T != nullptr (mode 1), the block also forms T[l][c] = sum_r W[r][l] Y[r][c] for its
 // columns (row-major b x b per problem, stride 64 * 64): the Rayleigh quotient Q^T G Q
 // when W = Q.
-constexpr int PW_COLS = 4;
-__global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ G, long sG, const double* __restrict__ W,
+template <int PW_COLS, typename TG>
+__global__ __launch_bounds__(1024) void power_kernel(const TG* __restrict__ G, long sG, const double* __restrict__ W,
                                                      double* __restrict__ Y, long sWY, int n, int b, int q, int mode,
                                                      const double* __restrict__ cut, double* __restrict__ T) {
   __shared__ __attribute__((aligned(16))) double cur[256 * PW_COLS];
@@ -106,21 +107,52 @@ __global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ 
   const int ra = 2 * rp, rb = 2 * rp + 1;
   const int l0 = (p8 * n) / 8, l1 = ((p8 + 1) * n) / 8;
   const bool own = tid < 256 * PW_COLS;  // tid = row * PW_COLS + column
-  static_assert(PW_COLS == 4, "two 16-byte reads per l");
+  static_assert(PW_COLS == 2 || PW_COLS == 4, "one or two 16-byte reads per l");
   for (int it = 0; it < q; ++it) {
     double acc0[PW_COLS], acc1[PW_COLS];
 #pragma unroll
     for (int c = 0; c < PW_COLS; ++c) acc0[c] = acc1[c] = 0.0;
     if (ra < n) {
-      const double* ga = G + ra;
-      const double* gb = G + (rb < n ? rb : ra);
-#pragma unroll 4
-      for (int l = l0; l < l1; ++l) {
-        const double g0 = ga[(long)n * l], g1 = gb[(long)n * l];
-        const double2 c01 = *reinterpret_cast<const double2*>(&cur[l * PW_COLS]);
-        const double2 c23 = *reinterpret_cast<const double2*>(&cur[l * PW_COLS + 2]);
-        acc0[0] += g0 * c01.x; acc0[1] += g0 * c01.y; acc0[2] += g0 * c23.x; acc0[3] += g0 * c23.y;
-        acc1[0] += g1 * c01.x; acc1[1] += g1 * c01.y; acc1[2] += g1 * c23.x; acc1[3] += g1 * c23.y;
+      const TG* ga = G + ra;
+      const TG* gb = G + (rb < n ? rb : ra);
+      // eight steps of G per batch, all loads issued before their products (index clamped
+      // to the range, past it the value is not used). For even n the thread's two rows are
+      // adjacent in every column: one 2-element load, so a wave's load covers whole lines.
+      using TG2 = typename std::conditional<std::is_same<TG, float>::value, float2, double2>::type;
+      const bool pair = (n & 1) == 0;
+      for (int l = l0; l < l1; l += 8) {
+        TG g0v[8], g1v[8];
+        if (pair) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const long lc = min(l + u, l1 - 1);
+            const TG2 gg = *reinterpret_cast<const TG2*>(ga + (long)n * lc);
+            g0v[u] = gg.x;
+            g1v[u] = gg.y;
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const long lc = min(l + u, l1 - 1);
+            g0v[u] = ga[(long)n * lc];
+            g1v[u] = gb[(long)n * lc];
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (l + u < l1) {
+            const double g0 = (double)g0v[u], g1 = (double)g1v[u];
+            const double2 c01 = *reinterpret_cast<const double2*>(&cur[(l + u) * PW_COLS]);
+            acc0[0] += g0 * c01.x; acc0[1] += g0 * c01.y;
+            acc1[0] += g1 * c01.x; acc1[1] += g1 * c01.y;
+            if constexpr (PW_COLS == 4) {
+              const double2 c23 = *reinterpret_cast<const double2*>(&cur[(l + u) * PW_COLS + 2]);
+              acc0[2] += g0 * c23.x; acc0[3] += g0 * c23.y;
+              acc1[2] += g1 * c23.x; acc1[3] += g1 * c23.y;
+            }
+          }
+        }
       }
     }
 #pragma unroll
@@ -176,11 +208,11 @@ __global__ __launch_bounds__(1024) void power_kernel(const double* __restrict__ 
   if (T) {
     const int l = tid & 63, c = (tid >> 6) & 3, pp = tid >> 8;
     double t = 0.0;
-    if (l < b && c < nc) {
+    if (l < b && c < nc && c < PW_COLS) {
       const int r0 = (pp * n) / 4, r1 = ((pp + 1) * n) / 4;
       for (int rr = r0; rr < r1; ++rr) t += W[(long)rr * b + l] * cur[rr * PW_COLS + c];
     }
-    part[pp][l * PW_COLS + c] = t;
+    if (c < PW_COLS) part[pp][l * PW_COLS + c] = t;
     __syncthreads();
     if (tid < 64 * PW_COLS) {
       const int l2 = tid / PW_COLS, c2 = tid % PW_COLS;
@@ -802,13 +834,19 @@ int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* 
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
+// Two columns per work-group (twice the work-groups, half the FMAs each: a step is bound
+// by the work-group's stream of G and its FMAs, not by the grid). G32, when given, is an
+// FP32 copy of G that modes 0 and 2 read instead (half the bytes); mode 1 always reads G.
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
-                 int mode, const double* cut, double* T, int batch) {
+                 int mode, const double* cut, double* T, int batch, const float* G32) {
   if (n < 1 || n > 256 || b < 1 || b > 64 || q < 1 || mode < 0 || mode > 2 || batch < 1) return 8;
   if (mode == 2 && !cut) return 8;
   if (T && (mode != 1 || q != 1)) return 8;
-  hipLaunchKernelGGL(power_kernel, dim3((b + PW_COLS - 1) / PW_COLS, batch), dim3(1024), 0, st, G, sG, W, Y, sWY, n,
-                     b, q, mode, cut, T);
+  const dim3 grid((b + 1) / 2, batch);
+  if (G32 && mode != 1)
+    hipLaunchKernelGGL((power_kernel<2, float>), grid, dim3(1024), 0, st, G32, sG, W, Y, sWY, n, b, q, mode, cut, T);
+  else
+    hipLaunchKernelGGL((power_kernel<2, double>), grid, dim3(1024), 0, st, G, sG, W, Y, sWY, n, b, q, mode, cut, T);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
