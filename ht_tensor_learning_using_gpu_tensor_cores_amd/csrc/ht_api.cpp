@@ -262,6 +262,7 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
   e.prof = &h->prof;
   e.pin = h->pinned;
   e.pin_bytes = h->pinned_bytes;
+  e.plan = &h->plan;
   if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
   int s = e.run(xd, od);
   if (s && s != HT_ERR_NOCONV) {

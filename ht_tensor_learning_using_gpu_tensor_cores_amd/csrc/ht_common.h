@@ -201,6 +201,7 @@ int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G);
 int sum_strided_launch(hipStream_t st, const double* acc, int n, double* out);
 int transpose_launch(hipStream_t st, const double* A, long m, long n, double* AT);
 int sqrt_clamp_launch(hipStream_t st, const double* w, int k, double* s);
+int ratio_check_launch(hipStream_t st, const double* w, int k, double ratio, int* ok);
 int scale_cols_inv_launch(hipStream_t st, double* A, int m, int n, const double* s);
 // Batched small symmetric eigen solvers (k_small.hip).
 int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* w, long sw, double* V, long sV,

@@ -295,6 +295,19 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   // well-conditioned block; a flat one misses the first round and is widened below.
   if (b <= 0) b = k;
   b = std::min(std::max(b, k), bmax);
+  // a replayed call takes its rounds from the plan; a recorded one appends them
+  const EigCall* pc = nullptr;
+  if (replay) {
+    if (call_idx < plan->calls.size()) {
+      const EigCall& c = plan->calls[call_idx];
+      if (c.n == n && c.k == k && c.batch == batch && c.b0 == b && c.q0 == q0 && c.tol == tol && !c.rounds.empty())
+        pc = &c;
+    }
+    ++call_idx;
+    if (!pc) plan_mismatch = true;  // this call reads its rounds back instead
+  }
+  const size_t rec = recording ? plan->calls.size() : 0;
+  if (recording) plan->calls.push_back(EigCall{n, k, batch, b, q0, tol, {}});
 
   const size_t nbmax = (size_t)n * bmax;
   double* W = alloc<double>(batch * nbmax);
@@ -326,6 +339,11 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   std::vector<double> hwall((size_t)batch * 64), hres(batch);
   int q = std::max(1, q0), mode = 0, iters = 0;
   for (int round = 0;; ++round) {
+    if (pc) {
+      q = pc->rounds[round].q;
+      mode = pc->rounds[round].mode;
+    }
+    const bool last_replayed = pc && round + 1 == (int)pc->rounds.size();
     // CholeskyQR2 first, except on the first round of an oversampled block after very
     // few powers (its extra columns are then numerically rank deficient when the spectrum
     // falls off after k)
@@ -333,9 +351,19 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, nullptr, batch, G32));
     CK(qr_launch(st, Y, Q, nb, n, b, try_chol, batch));
     CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, T, batch));
-    CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, floor, conv, rres, batch));
+    // the last replayed round leaves its convergence flags in the verdict slots
+    int* cflags = last_replayed ? verdict + n_verdict : conv;
+    CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, floor, cflags, rres, batch));
     iters += q + 1;
-    {
+    const int b_rr = b;  // block width of this round's Ritz values
+    int widen = 0;
+    if (pc) {
+      if (last_replayed) {
+        n_verdict += batch;
+        break;
+      }
+      widen = pc->rounds[round].widen;
+    } else {
       // one copy of the result block, through the pinned staging when it fits
       const size_t tot = wb + rb + cb;
       char* hb = (pin && tot <= pin_bytes) ? pin : nullptr;
@@ -349,41 +377,43 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
       std::memcpy(hwall.data(), hb, wb);
       std::memcpy(hres.data(), hb + wb, rb);
       std::memcpy(hconv.data(), hb + wb + rb, cb);
-    }
-    const int b_rr = b;  // block width of this round's Ritz values
-    if (getenv("HT_DEBUG") && getenv("HT_DEBUG")[0] == '2') {
-      static unsigned long long fb_prev = 0;
-      unsigned long long fb = 0;
-      CK(qr_fallback_count(&fb));
-      fprintf(stderr, "ht: sym_topk_batch n=%d k=%d b=%d round %d (q=%d mode %d, chol %d): residual/th1 %.3e, th_k/th_1 "
-              "%.3e, cut/th_1 %.3e, householder %llu\n", n, k, b, round, q, mode, try_chol, hres[0],
-              hwall[k - 1] / hwall[0], hwall[b - 1] / hwall[0], fb - fb_prev);
-      fb_prev = fb;
-    }
-    bool all = true;
-    for (int i = 0; i < batch; ++i) all = all && hconv[i];
-    if (all) break;
-    if (iters >= opt.max_eig_iters) {
-      noconv = 1;
-      if (getenv("HT_DEBUG")) {
-        std::vector<double> hw((size_t)batch * 64), hr(batch);
-        CK(d2h(hw.data(), wall, sizeof(double) * batch * 64));
-        CK(d2h(hr.data(), rres, sizeof(double) * batch));
-        for (int pb = 0; pb < batch; ++pb) {
-          if (hconv[pb]) continue;
-          const double* th = &hw[(size_t)pb * 64];
-          fprintf(stderr, "ht: sym_topk_batch n=%d k=%d b=%d tol=%g floor=%g: problem %d of %d not converged after %d"
-                  " iterations, residual/th1 %.3e; Ritz values (relative):", n, k, b, tol, floor, pb, batch, iters,
-                  hr[pb]);
-          for (int i = 0; i < b && i < 64; ++i) fprintf(stderr, " %.3e", th[i] / std::max(std::fabs(th[0]), 1e-300));
-          fprintf(stderr, "\n");
-        }
+      if (getenv("HT_DEBUG") && getenv("HT_DEBUG")[0] == '2') {
+        static unsigned long long fb_prev = 0;
+        unsigned long long fb = 0;
+        CK(qr_fallback_count(&fb));
+        fprintf(stderr, "ht: sym_topk_batch n=%d k=%d b=%d round %d (q=%d mode %d, chol %d): residual/th1 %.3e, th_k/th_1 "
+                "%.3e, cut/th_1 %.3e, householder %llu\n", n, k, b, round, q, mode, try_chol, hres[0],
+                hwall[k - 1] / hwall[0], hwall[b - 1] / hwall[0], fb - fb_prev);
+        fb_prev = fb;
       }
-      break;
+      bool all = true;
+      for (int i = 0; i < batch; ++i) all = all && hconv[i];
+      if (recording) plan->calls[rec].rounds.push_back(EigRound{q, mode, b, try_chol, 0});
+      if (all) break;
+      if (iters >= opt.max_eig_iters) {
+        noconv = 1;
+        if (getenv("HT_DEBUG")) {
+          std::vector<double> hw((size_t)batch * 64), hr(batch);
+          CK(d2h(hw.data(), wall, sizeof(double) * batch * 64));
+          CK(d2h(hr.data(), rres, sizeof(double) * batch));
+          for (int pb = 0; pb < batch; ++pb) {
+            if (hconv[pb]) continue;
+            const double* th = &hw[(size_t)pb * 64];
+            fprintf(stderr, "ht: sym_topk_batch n=%d k=%d b=%d tol=%g floor=%g: problem %d of %d not converged after %d"
+                    " iterations, residual/th1 %.3e; Ritz values (relative):", n, k, b, tol, floor, pb, batch, iters,
+                    hr[pb]);
+            for (int i = 0; i < b && i < 64; ++i) fprintf(stderr, " %.3e", th[i] / std::max(std::fabs(th[0]), 1e-300));
+            fprintf(stderr, "\n");
+          }
+        }
+        break;
+      }
+      if (b < std::min(bmax, 2 * k)) widen = std::min(bmax, b + 16);
+      if (recording) plan->calls[rec].rounds.back().widen = widen;
     }
-    if (b < std::min(bmax, 2 * k)) {
+    if (widen) {
       // Ritz vectors first, fresh Gaussian columns after them (row-major blocks)
-      const int bn = std::min(bmax, b + 16);
+      const int bn = widen;
       const long nbn = (long)n * bn;
       CK(fill_normals_f64_launch(st, W, batch * nbn, opt.seed, stream + 100 + round));
       HT_HIP_CHECK(hipMemcpy2DAsync(W, sizeof(double) * bn, Uf, sizeof(double) * b, sizeof(double) * b,
@@ -393,6 +423,7 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     } else {
       std::swap(W, Uf);
     }
+    if (pc) continue;
     // Degree of the next filter. Between th_1 and the last wanted Ritz value th_k above
     // the floor the Chebyshev filter on [0, cut] amplifies by T_d(x_1) / T_d(x_k) ~
     // exp(d (acosh x_1 - acosh x_k)), x = 2 th / cut - 1; keeping that below 1e12 keeps
@@ -706,13 +737,20 @@ int Engine::residual(const float* x, double* err_sq, double* x_sq) {
     ProfScope ps(prof, st, PK_RESIDUAL);
     CK(lowrank_launch(st, 1, Lf, Rf, nl, nr, kl, x, nullptr, acc, &nb));
   }
+  passes += 1.0;
+  if (deferred) {
+    // into the head of the verdict block, read with it at the end of the run
+    CK(sum_strided_launch(st, acc, nb, reinterpret_cast<double*>(verdict) - 4));
+    *err_sq = NAN;
+    *x_sq = NAN;
+    return 0;
+  }
   double* red = alloc<double>(2 + 2048);
   CK(sum_strided_launch(st, acc, nb, red));
   double h[2];
   CK(d2h(h, red, sizeof(h)));
   *err_sq = h[0];
   *x_sq = h[1];
-  passes += 1.0;
   return 0;
 }
 
@@ -853,6 +891,25 @@ int Engine::root_transfer(const float* x) {
 
 int Engine::finish_error(const float* x, const double* xsq_dev) {
   rel_err = NAN;
+  if (deferred) {
+    // the error terms go to the head of the verdict block (doubles 0 and 1) and are read
+    // with the verdicts once the run is complete
+    double* head = reinterpret_cast<double*>(verdict) - 4;
+    deferred_mode = opt.err_mode;
+    if (opt.err_mode == HT_ERR_MODE_RESIDUAL) {
+      double esq, xs2;
+      return residual(x, &esq, &xs2);
+    }
+    HT_HIP_CHECK(hipMemcpyAsync(head, xsq_dev, sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (opt.err_mode == HT_ERR_MODE_IDENTITY) {
+      const Problem& p = *prob;
+      const int n = p.rank[p.left[0]] * p.rank[p.right[0]];
+      double* out = alloc<double>(2 + 1024);
+      CK(sumsq_f64_launch(st, Bd[0], n, out));
+      HT_HIP_CHECK(hipMemcpyAsync(head + 1, out, sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    return 0;
+  }
   if (opt.err_mode == HT_ERR_MODE_RESIDUAL) {
     // the residual pass returns ||X||^2 with ||X - X_HT||^2: one host read
     double esq = 0.0, xs2 = 0.0;
@@ -1014,11 +1071,27 @@ int Engine::small_side_svd(const double* C, PMS v, int k, double* U, double* w, 
   int* info = alloc<int>(1);
   double* AV = alloc<double>((size_t)M * k);
   if (!H || !Vh || !wh || !sv || !info || !AV) return HT_ERR_ALLOC;
+  // a replayed run takes the outcome of the rank test from the plan and leaves the test
+  // itself in a verdict slot
+  int planned = -1;
+  if (replay) {
+    if (small_idx < plan->small.size()) planned = plan->small[small_idx];
+    else plan_mismatch = true;
+    ++small_idx;
+  }
+  if (planned == 0) return 0;
   CK(gram_pms_f64_launch(st, C, vs, H));
   CK(jacobi_full_launch(st, H, (long)c * c, c, wh, c, Vh, (long)c * c, 1, info, 30, 1e-14));
-  std::vector<double> hw(c);
-  CK(d2h(hw.data(), wh, sizeof(double) * c));
-  if (!(hw[0] > 0.0) || !(hw[k - 1] > 1e-24 * hw[0])) return 0;
+  if (planned == 1) {
+    CK(ratio_check_launch(st, wh, k, 1e-24, verdict + n_verdict));
+    ++n_verdict;
+  } else {
+    std::vector<double> hw(c);
+    CK(d2h(hw.data(), wh, sizeof(double) * c));
+    const int ok = (hw[0] > 0.0 && hw[k - 1] > 1e-24 * hw[0]) ? 1 : 0;
+    if (recording) plan->small.push_back(ok);
+    if (!ok) return 0;
+  }
   // A V: contraction over the small side of the transposed view.
   if (v.P == 1) {
     CK(proj_pms_f64_launch(st, C, vs, Vh, k, U));  // (M, k) column-major
@@ -1290,7 +1363,7 @@ int Engine::write_outputs(float* out) {
   return 0;
 }
 
-int Engine::run(const float* x, float* out) {
+int Engine::run_once(const float* x, float* out) {
   for (int t = 0; t < HT_MAX_NODES; ++t) {
     Ufull[t] = nullptr;
     Bd[t] = nullptr;
@@ -1310,6 +1383,102 @@ int Engine::run(const float* x, float* out) {
   else return HT_ERR_INVALID_ARG;
   if (s) return s;
   if (out) CK(write_outputs(out));
+  return 0;
+}
+
+// Problem, options and input alignment a plan is recorded for.
+static std::vector<int64_t> plan_key(const Problem& p, const ht_options& o, const float* x) {
+  std::vector<int64_t> k = {p.order, p.n_nodes, p.want_interior_u, o.method, o.precision, o.oversample,
+                            o.power_iters, o.err_mode, o.max_eig_iters, (int64_t)o.seed,
+                            (int64_t)(((uintptr_t)x & 15) == 0)};
+  int64_t tol;
+  std::memcpy(&tol, &o.eig_tol, sizeof(tol));
+  k.push_back(tol);
+  for (int m = 0; m < p.order; ++m) k.push_back(p.dims[m]);
+  for (int t = 0; t < p.n_nodes; ++t)
+    for (int64_t v : {p.lo[t], p.hi[t], p.left[t], p.right[t], p.rank[t]}) k.push_back(v);
+  return k;
+}
+
+// The error terms of a replayed run from the head of its verdict block (as finish_error).
+int Engine::finish_deferred(bool* verified) {
+  char* hb = pin + pin_bytes / 2;
+  const int* vf = reinterpret_cast<const int*>(hb + 32);
+  *verified = true;
+  for (int i = 0; i < n_verdict; ++i) *verified = *verified && vf[i] == 1;
+  if (!*verified) return 0;
+  double head[2];
+  std::memcpy(head, hb, sizeof(head));
+  rel_err = NAN;
+  x_sq = head[0];
+  if (deferred_mode == HT_ERR_MODE_RESIDUAL) {
+    x_sq = head[1];
+    if (!std::isfinite(x_sq)) return HT_ERR_NONFINITE;
+    rel_err = x_sq > 0 ? std::sqrt(head[0] / x_sq) : 0.0;
+    return 0;
+  }
+  if (!std::isfinite(x_sq)) return HT_ERR_NONFINITE;
+  if (deferred_mode == HT_ERR_MODE_IDENTITY)
+    rel_err = x_sq > 0 ? std::sqrt(std::max(x_sq - head[1], 0.0) / x_sq) : 0.0;
+  return 0;
+}
+
+// Leaves-to-root runs of a problem whose host decisions were recorded replay them and read
+// the device once, at the end; a run without a plan (or whose replay fails its verdicts)
+// reads every eigensolver round and records a new plan.
+int Engine::run(const float* x, float* out) {
+  const bool can = plan && pin && pin_bytes >= 16384 && opt.method == HT_METHOD_LTR && !getenv("HT_DEBUG") &&
+                   !getenv("HT_SYNC_ROUNDS");
+  std::vector<int64_t> key;
+  if (can) key = plan_key(*prob, opt, x);
+  if (can && plan->valid && plan->key == key) {
+    size_t cap = plan->small.size();
+    for (const EigCall& c : plan->calls) cap += (size_t)c.batch;
+    if (32 + 4 * cap <= pin_bytes / 2) {
+      const auto m = arena->mark();
+      double* vb = alloc<double>(4 + (cap + 1) / 2);
+      if (!vb) return HT_ERR_ALLOC;
+      verdict = reinterpret_cast<int*>(vb + 4);
+      n_verdict = 0;
+      call_idx = 0;
+      small_idx = 0;
+      plan_mismatch = false;
+      replay = true;
+      deferred = true;
+      const int s = run_once(x, out);
+      replay = false;
+      deferred = false;
+      if (s) {
+        plan->valid = false;
+        return s;
+      }
+      HT_HIP_CHECK(hipMemcpyAsync(pin + pin_bytes / 2, vb, 32 + 4 * (size_t)n_verdict, hipMemcpyDeviceToHost, st));
+      CK(sync());
+      bool verified = false;
+      const int f = finish_deferred(&verified);
+      if (verified) {
+        if (plan_mismatch || call_idx != plan->calls.size() || small_idx != plan->small.size()) plan->valid = false;
+        if (prof && prof->enabled) prof->collect();
+        if (f) return f;
+        if (noconv) return HT_ERR_NOCONV;
+        return 0;
+      }
+      // a replayed eigensolver missed its tolerance: repeat the run with reads
+      arena->rewind(m);
+      if (prof && prof->enabled) prof->n = 0;
+    }
+  }
+  recording = can;
+  if (recording) {
+    plan->valid = false;
+    plan->key = key;
+    plan->calls.clear();
+    plan->small.clear();
+  }
+  const int s = run_once(x, out);
+  if (recording && s == 0 && !noconv) plan->valid = true;
+  recording = false;
+  if (s) return s;
   CK(sync());
   if (prof && prof->enabled) prof->collect();
   if (noconv) return HT_ERR_NOCONV;

@@ -62,6 +62,28 @@ struct Profiler {
   void destroy();
 };
 
+// Host decisions of the leaves-to-root method, recorded by a run that reads every
+// eigensolver round's results and replayed by later runs of the same problem and options,
+// which then read nothing back until the end of the run. Each replayed eigensolver call
+// writes the convergence flags of its last round (and each small-side SVD its rank test)
+// into a device verdict array that is read with the error at the end: any zero there
+// means the replay is discarded and the run is repeated with reads (and re-recorded).
+struct EigRound {
+  int q, mode, b, try_chol;
+  int widen;  // block width of the next round when it grows (0: the Ritz block is reused)
+};
+struct EigCall {
+  int n, k, batch, b0, q0;
+  double tol;
+  std::vector<EigRound> rounds;
+};
+struct RunPlan {
+  std::vector<int64_t> key;  // problem and options the plan was recorded for
+  std::vector<EigCall> calls;
+  std::vector<int> small;    // outcome of each small-side SVD (1: used, 0: fell back)
+  bool valid = false;
+};
+
 struct Engine {
   Profiler* prof = nullptr;
   hipStream_t st;
@@ -82,6 +104,16 @@ struct Engine {
   // copies into pageable memory complete synchronously one by one
   char* pin = nullptr;
   size_t pin_bytes = 0;
+  // record / replay of the host decisions (leaves-to-root only)
+  RunPlan* plan = nullptr;
+  bool recording = false, replay = false, plan_mismatch = false;
+  size_t call_idx = 0, small_idx = 0;
+  int* verdict = nullptr;  // device verdict slots of a replayed run
+  int n_verdict = 0;
+  // reads deferred to the end of a replayed run: the error terms (into the upper half of
+  // the pinned staging) and the verdicts after them
+  bool deferred = false;
+  int deferred_mode = 0;
 
   template <typename T>
   T* alloc(size_t n);
@@ -123,6 +155,8 @@ struct Engine {
   int sketch_node(const float* x, int t, const float* S, const int64_t* off, int w, float* grad, float scale,
                   double* loss_acc, const double* xsq, int* info);
   int write_outputs(float* out);
+  int run_once(const float* x, float* out);
+  int finish_deferred(bool* verified);
   int run(const float* x, float* out);
 };
 
@@ -144,6 +178,8 @@ struct ht_handle_s {
   hipGraph_t bgraph = nullptr;
   hipGraphExec_t bexec = nullptr;
   std::vector<const void*> bkey;
+  // host decisions of the last leaves-to-root run, replayed by the next of the same shape
+  ht::RunPlan plan;
   // pinned staging for small device-to-host reads
   char* pinned = nullptr;
   size_t pinned_bytes = 0;

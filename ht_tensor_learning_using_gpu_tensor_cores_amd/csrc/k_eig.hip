@@ -786,6 +786,16 @@ int sqrt_clamp_launch(hipStream_t st, const double* w, int k, double* s) {
   hipLaunchKernelGGL(sqrt_clamp_kernel, dim3((k + 255) / 256), dim3(256), 0, st, w, k, s);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
+// ok[0] = 1 when w[0] > 0 and w[k-1] > ratio * w[0] (the rank test of a small-side SVD,
+// evaluated on the device when the host does not read the eigenvalues).
+__global__ void ratio_check_kernel(const double* __restrict__ w, int k, double ratio, int* __restrict__ ok) {
+  if (threadIdx.x == 0) ok[0] = (w[0] > 0.0 && w[k - 1] > ratio * w[0]) ? 1 : 0;
+}
+int ratio_check_launch(hipStream_t st, const double* w, int k, double ratio, int* ok) {
+  if (k < 1) return 8;
+  hipLaunchKernelGGL(ratio_check_kernel, dim3(1), dim3(64), 0, st, w, k, ratio, ok);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
 __global__ void scale_cols_inv_kernel(double* __restrict__ A, int m, int n, const double* __restrict__ s) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)m * n) return;

@@ -182,3 +182,33 @@ def test_reconstruct_and_residual_entry_points():
     e = float(torch.linalg.norm(xd - xr) / torch.linalg.norm(xd))
     assert abs(e - res.rel_err) < 1e-6
     assert abs(ht.residual(xd, res) - res.rel_err) < 1e-7
+
+
+def test_replayed_rounds_match_a_run_that_reads_them(monkeypatch):
+    """A repeated leaves-to-root run of one problem on one handle replays the eigensolver
+    rounds the first run chose and reads the device once, at the end. Its outputs must be
+    those of a run that reads every round; on another tensor of the same shape the replay
+    is verified (or repeated with reads) and must match a fresh handle's result."""
+    import torch
+    dims = [128, 128, 24, 24]
+    tr = ht.tree.balanced_4d()
+    ranks = [1] + [16] * 6
+    x = G.tucker(dims, [16] * 4, seed=11, noise_rel=1e-3)
+    y = G.cp_decay(dims, 20, seed=5, noise_rel=1e-4)
+    with ht.Handle() as h:
+        a, xd = run(x, dims, tr, ranks, method="ltr", handle=h)  # records
+        b = ht.decompose(xd, dims, tr, ranks, method="ltr", handle=h)  # replays
+        monkeypatch.setenv("HT_SYNC_ROUNDS", "1")
+        c = ht.decompose(xd, dims, tr, ranks, method="ltr", handle=h)  # reads every round
+        monkeypatch.delenv("HT_SYNC_ROUNDS")
+        yd = torch.from_numpy(y).cuda()
+        d = ht.decompose(yd, dims, tr, ranks, method="ltr", handle=h)  # replay of x's plan
+    pa, pb, pc = to_np(a.packed), to_np(b.packed), to_np(c.packed)
+    assert np.array_equal(pa, pb) and np.array_equal(pa, pc)
+    assert a.rel_err == b.rel_err == c.rel_err
+    with ht.Handle() as h2:
+        e = ht.decompose(yd, dims, tr, ranks, method="ltr", handle=h2)
+    assert abs(d.rel_err - e.rel_err) <= 1e-5
+    for t in (3, 4, 5, 6):
+        ang = np.max(subspace_angles(to_np(d.U[t]).astype(np.float64), to_np(e.U[t]).astype(np.float64)))
+        assert ang < 1e-4
