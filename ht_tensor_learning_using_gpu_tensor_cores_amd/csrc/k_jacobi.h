@@ -190,6 +190,101 @@ __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np
   return 0;
 }
 
+// Cyclic Jacobi on a 16 x 16 symmetric matrix in the registers of one wave, for problems
+// too small to amortise block-wide barriers. Lane l = 8 I + J holds the 2 x 2 blocks (rows
+// 2I, 2I+1; columns 2J, 2J+1) of A and of the rotation product V (V starts as I). A step
+// rotates the eight pairs (2i, 2i+1): the diagonal lanes form the rotations, every lane
+// fetches those of its row and column pair with lane permutes and applies R_I^T A R_J and
+// V R_J. Then the indices move one place along the ring t1 .. t7, b7 .. b0 of the circle
+// round-robin (t0 fixed; ti = 2i, bi = 2i+1) by a row and a column permutation of A and a
+// column permutation of V, so fifteen steps pair every two indices once. Diagonal entries
+// equal to `pad` (decoupled padding) are left out of the convergence test, which is the
+// one of jacobi_sweeps. On return a and v (LDS, ld SJLD) hold the result in the slot
+// order (columns of v follow the diagonal of a); returns 1 when converged.
+__device__ __forceinline__ int jacobi16_wave(double* a, double* v, int max_sweeps, double tol, double pad) {
+  const int l = threadIdx.x & 63, I = l >> 3, J = l & 7;
+  const bool dgl = I == J;
+  double a00 = a[(2 * I) * SJLD + 2 * J], a01 = a[(2 * I) * SJLD + 2 * J + 1];
+  double a10 = a[(2 * I + 1) * SJLD + 2 * J], a11 = a[(2 * I + 1) * SJLD + 2 * J + 1];
+  double v00 = dgl ? 1.0 : 0.0, v01 = 0.0, v10 = 0.0, v11 = dgl ? 1.0 : 0.0;
+  const int up = l >= 8 ? l - 8 : l, dn = l < 56 ? l + 8 : l;
+  const int lf = l >= 1 ? l - 1 : l, rt = l < 63 ? l + 1 : l;
+  int conv = 0;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    double off = dgl ? a01 * a01 + a10 * a10 : a00 * a00 + a01 * a01 + a10 * a10 + a11 * a11;
+    double dg = dgl ? (a00 != pad ? a00 * a00 : 0.0) + (a11 != pad ? a11 * a11 : 0.0) : 0.0;
+    // one lane's sums decide for the whole wave (butterfly sums can differ in the last bit)
+    off = lane_bcast(wave_sum(off), 0);
+    dg = lane_bcast(wave_sum(dg), 0);
+    if (off <= tol * tol * dg || off == 0.0) {
+      conv = 1;
+      break;
+    }
+    for (int step = 0; step < 15; ++step) {
+      double c, s;
+      jacobi_rotation(a00, a11, a01, c, s);  // used from the diagonal lanes only
+      const double cI = __shfl(c, 9 * I, 64), sI = __shfl(s, 9 * I, 64);
+      const double cJ = __shfl(c, 9 * J, 64), sJ = __shfl(s, 9 * J, 64);
+      const double x00 = cJ * a00 - sJ * a01, x01 = sJ * a00 + cJ * a01;
+      const double x10 = cJ * a10 - sJ * a11, x11 = sJ * a10 + cJ * a11;
+      a00 = cI * x00 - sI * x10;
+      a10 = sI * x00 + cI * x10;
+      a01 = cI * x01 - sI * x11;
+      a11 = sI * x01 + cI * x11;
+      if (dgl && sI != 0.0) {
+        a01 = 0.0;
+        a10 = 0.0;
+      }
+      const double w00 = cJ * v00 - sJ * v01, w01 = sJ * v00 + cJ * v01;
+      const double w10 = cJ * v10 - sJ * v11, w11 = sJ * v10 + cJ * v11;
+      // rows: new t_I = t_{I-1} (t_1 = b_0, t_0 stays); new b_I = b_{I+1} (b_7 = t_7)
+      {
+        const double s0 = I == 0 ? a10 : a00, s1 = I == 0 ? a11 : a01;
+        const double u00 = __shfl(s0, up, 64), u01 = __shfl(s1, up, 64);
+        const double d10 = __shfl(a10, dn, 64), d11 = __shfl(a11, dn, 64);
+        const double n10 = I == 7 ? a00 : d10, n11 = I == 7 ? a01 : d11;
+        if (I != 0) {
+          a00 = u00;
+          a01 = u01;
+        }
+        a10 = n10;
+        a11 = n11;
+      }
+      // columns, the same ring (A and V)
+      {
+        const double s0 = J == 0 ? a01 : a00, s1 = J == 0 ? a11 : a10;
+        const double l00 = __shfl(s0, lf, 64), l10 = __shfl(s1, lf, 64);
+        const double r01 = __shfl(a01, rt, 64), r11 = __shfl(a11, rt, 64);
+        const double n01 = J == 7 ? a00 : r01, n11 = J == 7 ? a10 : r11;
+        if (J != 0) {
+          a00 = l00;
+          a10 = l10;
+        }
+        a01 = n01;
+        a11 = n11;
+      }
+      {
+        const double s0 = J == 0 ? w01 : w00, s1 = J == 0 ? w11 : w10;
+        const double l00 = __shfl(s0, lf, 64), l10 = __shfl(s1, lf, 64);
+        const double r01 = __shfl(w01, rt, 64), r11 = __shfl(w11, rt, 64);
+        v00 = J == 0 ? w00 : l00;
+        v10 = J == 0 ? w10 : l10;
+        v01 = J == 7 ? w00 : r01;
+        v11 = J == 7 ? w10 : r11;
+      }
+    }
+  }
+  a[(2 * I) * SJLD + 2 * J] = a00;
+  a[(2 * I) * SJLD + 2 * J + 1] = a01;
+  a[(2 * I + 1) * SJLD + 2 * J] = a10;
+  a[(2 * I + 1) * SJLD + 2 * J + 1] = a11;
+  v[(2 * I) * SJLD + 2 * J] = v00;
+  v[(2 * I) * SJLD + 2 * J + 1] = v01;
+  v[(2 * I + 1) * SJLD + 2 * J] = v10;
+  v[(2 * I + 1) * SJLD + 2 * J + 1] = v11;
+  return conv;
+}
+
 // Load an n x n matrix into a (padded to even np with a decoupled, large negative
 // diagonal entry that sorts last), set v = I.
 __device__ __forceinline__ void jacobi_load(double* a, double* v, const double* A, int lda, int n, int np, bool symmetrise) {
