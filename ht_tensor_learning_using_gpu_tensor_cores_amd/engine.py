@@ -6,6 +6,7 @@ buffers are torch tensors on a ROCm device (torch is used for memory and streams
 every computation runs in libht_amd.so). Host inputs are numpy arrays.
 """
 import ctypes
+from collections.abc import Mapping
 from dataclasses import dataclass, field
 from typing import Dict, Optional
 
@@ -28,6 +29,19 @@ class Handle:
         self._h = ctypes.c_void_p()
         _lib.check(self.lib.ht_create(ctypes.byref(self._h), int(device)), "ht_create")
         self.device = int(device)
+        self._ext = None
+
+    def order_after_torch(self, device):
+        """Queue a wait of the handle's stream for what torch's current stream of `device`
+        holds so far (an event wait on the device; the host does not block)."""
+        torch = _torch()
+        dev = torch.device(device)
+        if dev.index is not None and dev.index != self.device:
+            torch.cuda.synchronize(dev)
+            return
+        if self._ext is None:
+            self._ext = torch.cuda.ExternalStream(self.lib.ht_get_stream(self.ptr), device=dev)
+        self._ext.wait_stream(torch.cuda.current_stream(dev))
 
     @property
     def ptr(self):
@@ -89,6 +103,25 @@ def make_options(method="ltr", precision="bf16", err="residual", oversample=8, p
     return o
 
 
+_LAYOUTS = {}
+
+
+def _layout_entry(dims, tree: Tree, ranks, want_interior_u):
+    """Validated layout and the ctypes arguments of one (dims, tree, ranks), made once."""
+    key = (tuple(dims), tuple(tree.lo), tuple(tree.hi), tuple(tree.left), tuple(tree.right), tuple(ranks),
+           bool(want_interior_u))
+    e = _LAYOUTS.get(key)
+    if e is None:
+        tree.validate()
+        layout = output_layout(dims, tree, ranks, want_interior_u)
+        args = (_lib.i64_array(dims), tree.n_nodes, _lib.i32_array(tree.lo), _lib.i32_array(tree.hi),
+                _lib.i32_array(tree.left), _lib.i32_array(tree.right), _lib.i32_array(ranks))
+        if len(_LAYOUTS) >= 256:
+            _LAYOUTS.clear()
+        e = _LAYOUTS[key] = (layout, args)
+    return e
+
+
 def output_layout(dims, tree: Tree, ranks, want_interior_u=False):
     lib = _lib.load()
     n = tree.n_nodes
@@ -119,26 +152,57 @@ class HTensor:
     want_interior_u: bool = False
 
 
+class _Views(Mapping):
+    """Read-only node -> view mapping over the packed block; a view is made on first use."""
+
+    def __init__(self, keys, make):
+        self._keys = keys
+        self._make = make
+        self._cache = {}
+
+    def __getitem__(self, t):
+        v = self._cache.get(t)
+        if v is None:
+            if t not in self._keys:
+                raise KeyError(t)
+            v = self._cache[t] = self._make(t)
+        return v
+
+    def __iter__(self):
+        return iter(self._keys)
+
+    def __len__(self):
+        return len(self._keys)
+
+
 def _unpack(res: HTensor, layout):
     u_off, b_off, s_off, _ = layout
     tr, rk, pk = res.tree, res.ranks, res.packed
-    rows = {}
-    for t in range(tr.n_nodes):
+
+    def rows(t):
         r = 1
         for m in range(tr.lo[t], tr.hi[t]):
             r *= res.dims[m]
-        rows[t] = r
-    for t in range(tr.n_nodes):
-        if u_off[t] >= 0:
-            n = rows[t] * rk[t]
-            res.U[t] = pk[u_off[t]:u_off[t] + n].reshape(rk[t], rows[t]).T
-        if b_off[t] >= 0:
-            kl, kr = rk[tr.left[t]], rk[tr.right[t]]
-            n = kl * kr * rk[t]
-            res.B[t] = pk[b_off[t]:b_off[t] + n].reshape(rk[t], kr, kl).permute(2, 1, 0) \
-                if hasattr(pk, "permute") else pk[b_off[t]:b_off[t] + n].reshape((kl, kr, rk[t]), order="F")
-        if s_off[t] >= 0:
-            res.sv[t] = pk[s_off[t]:s_off[t] + rk[t]]
+        return r
+
+    def make_u(t):
+        n = rows(t) * rk[t]
+        return pk[u_off[t]:u_off[t] + n].reshape(rk[t], rows(t)).T
+
+    def make_b(t):
+        kl, kr = rk[tr.left[t]], rk[tr.right[t]]
+        n = kl * kr * rk[t]
+        if hasattr(pk, "permute"):
+            return pk[b_off[t]:b_off[t] + n].reshape(rk[t], kr, kl).permute(2, 1, 0)
+        return pk[b_off[t]:b_off[t] + n].reshape((kl, kr, rk[t]), order="F")
+
+    def make_s(t):
+        return pk[s_off[t]:s_off[t] + rk[t]]
+
+    nodes = range(tr.n_nodes)
+    res.U = _Views([t for t in nodes if u_off[t] >= 0], make_u)
+    res.B = _Views([t for t in nodes if b_off[t] >= 0], make_b)
+    res.sv = _Views([t for t in nodes if s_off[t] >= 0], make_s)
 
 
 def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="residual", handle=None,
@@ -151,11 +215,10 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
     """
     lib = _lib.load()
     h = handle or default_handle()
-    tree.validate()
     dims = [int(d) for d in dims]
     ranks = [int(r) for r in ranks]
     opt = make_options(method, precision, err, want_interior_u=want_interior_u, **kw)
-    layout = output_layout(dims, tree, ranks, want_interior_u)
+    layout, targs = _layout_entry(dims, tree, ranks, want_interior_u)
     total = layout[3]
     on_dev = hasattr(x, "is_cuda") and x.is_cuda
     if on_dev:
@@ -164,7 +227,7 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
             raise ValueError("x must be a contiguous float32 tensor (column-major data)")
         out = torch.empty(total, dtype=torch.float32, device=x.device)
         xptr, optr = x.data_ptr(), out.data_ptr()
-        torch.cuda.synchronize(x.device)
+        h.order_after_torch(x.device)
     else:
         xa = np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1))
         out = np.empty(total, dtype=np.float32)
@@ -173,9 +236,7 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
         raise ValueError("x has the wrong number of elements")
     rel = ctypes.c_double()
     st = _lib.Stats()
-    status = lib.ht_decompose(h.ptr, ctypes.c_void_p(xptr), int(on_dev), len(dims), _lib.i64_array(dims),
-                              tree.n_nodes, _lib.i32_array(tree.lo), _lib.i32_array(tree.hi),
-                              _lib.i32_array(tree.left), _lib.i32_array(tree.right), _lib.i32_array(ranks),
+    status = lib.ht_decompose(h.ptr, ctypes.c_void_p(xptr), int(on_dev), len(dims), *targs,
                               ctypes.byref(opt), ctypes.c_void_p(optr), int(on_dev), ctypes.byref(rel),
                               ctypes.byref(st))
     _lib.check(status, "ht_decompose")
