@@ -143,6 +143,9 @@ int gram_pms_launch(hipStream_t st, const float* x, PMS v, int prec, int n_split
 int gram_pms_nsplit(PMS v, int target_blocks);
 // Sum partials into a full symmetric FP64 Gram (ld = M).
 int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G);
+// The two slab Grams' reductions (and the sum of the per-block norms) in one launch.
+int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, double* G1, const float* p2, int M2, double* G2,
+                        int n_split, const double* norms, int n_norms, double* norm_out);
 int gram_reduce_norms_launch(hipStream_t st, const double* norms, int n, double* out);
 
 // Fused slab kernels on the two leading modes (x as n1 x n2 x S, n1,n2 <= 128).
@@ -201,6 +204,15 @@ int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G);
 int sum_strided_launch(hipStream_t st, const double* acc, int n, double* out);
 int transpose_launch(hipStream_t st, const double* A, long m, long n, double* AT);
 int sqrt_clamp_launch(hipStream_t st, const double* w, int k, double* s);
+// One FP64 -> FP32 copy of cast_multi_launch (sqrt_clamp: sqrt(max(x, 0)) before rounding).
+struct CastItem {
+  const double* src;
+  float* dst;
+  long n;
+  int sqrt_clamp;
+};
+constexpr int CAST_MAX = 48;
+int cast_multi_launch(hipStream_t st, const CastItem* items, int n);
 int ratio_check_launch(hipStream_t st, const double* w, int k, double ratio, int* ok);
 int scale_cols_inv_launch(hipStream_t st, double* A, int m, int n, const double* s);
 // Batched small symmetric eigen solvers (k_small.hip).

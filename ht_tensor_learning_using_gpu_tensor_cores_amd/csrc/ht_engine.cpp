@@ -727,8 +727,10 @@ int Engine::residual(const float* x, double* err_sq, double* x_sq) {
   float* Rf = alloc<float>((size_t)nr * kl);
   if (!Rr || !Lf || !Rf) return HT_ERR_ALLOC;
   CK(dgemm_small_launch(st, 0, 1, (int)nr, kl, kr, 1.0, Ur, (int)nr, Bd[0], kl, 0.0, Rr, (int)nr));
-  CK(cast_f64_f32_launch(st, Ul, Lf, nl * kl));
-  CK(cast_f64_f32_launch(st, Rr, Rf, nr * kl));
+  {
+    const CastItem it[2] = {{Ul, Lf, nl * kl, 0}, {Rr, Rf, nr * kl, 0}};
+    CK(cast_multi_launch(st, it, 2));
+  }
   const long nbx = (nl + 127) / 128, nby = (nr + 127) / 128;
   double* acc = alloc<double>((size_t)2 * nbx * nby);
   if (!acc) return HT_ERR_ALLOC;
@@ -788,16 +790,18 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
     CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
     for (int i = 0; i < nb; ++i) {
       Ufull[grp[i]] = U + (size_t)i * n * k;
-      CK(eig_to_sv(w + (size_t)i * k, k, SVd[grp[i]]));
+      CK(eig_to_sv(w + (size_t)i * k, grp[i]));
       done[grp[i]] = 1;
     }
   }
   return 0;
 }
 
-int Engine::eig_to_sv(const double* w, int k, double*& sv) {
-  sv = alloc<double>(k);
-  return sqrt_clamp_launch(st, w, k, sv);
+int Engine::eig_to_sv(const double* w, int t) {
+  // kept as eigenvalues: the output packing takes sqrt(max(w, 0))
+  SVd[t] = const_cast<double*>(w);
+  SVsq[t] = 1;
+  return 0;
 }
 
 int Engine::run_plain(const float* x) {
@@ -817,7 +821,7 @@ int Engine::run_plain(const float* x) {
       double* w = alloc<double>(k);
       CK(gram(x, v, PREC_F32, G, nullptr));
       CK(sym_topk(G, (int)rows, k, Ufull[t], w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16, GRAM_FLOOR));
-      CK(eig_to_sv(w, k, SVd[t]));
+      CK(eig_to_sv(w, t));
     } else if (cols <= 1024 && (p.lo[t] == 0 || p.hi[t] == p.order)) {
       // Gram of the complementary (contiguous) mode range, then U = X_(t) V / sigma via TSQR.
       const PMS vc = p.lo[t] == 0 ? PMS{rows, cols, 1} : PMS{1, cols, rows};
@@ -826,7 +830,7 @@ int Engine::run_plain(const float* x) {
       double* w = alloc<double>(k);
       CK(gram(x, vc, PREC_F32, G, nullptr));
       CK(sym_topk(G, (int)cols, k, Vc, w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16, GRAM_FLOOR));
-      CK(eig_to_sv(w, k, SVd[t]));
+      CK(eig_to_sv(w, t));
       float* Vf = alloc<float>((size_t)cols * k);
       float* Y = alloc<float>((size_t)rows * k);
       float* Qf = alloc<float>((size_t)rows * k);
@@ -847,9 +851,11 @@ int Engine::run_plain(const float* x) {
       double* sv = alloc<double>(kk);
       CK(exact_svd_subspace(x, v, k, kW, Ufull[t], Wr, sv, 3000 + t * 64));
       SVd[t] = sv;
+      SVsq[t] = 0;
       if (kW) {
         Ufull[sib] = Wr;
         SVd[sib] = sv;
+        SVsq[sib] = 0;
         done[sib] = 1;
       }
     } else {
@@ -961,9 +967,7 @@ int Engine::run_ltr(const float* x) {
       ProfScope ps(prof, st, PK_SLAB_GRAM);
       CK(slab_gram_launch(st, x, (int)n0, (int)n1, S, prec, nb, p1, p2, norms));
     }
-    CK(gram_reduce_launch(st, p1, (int)n0, nb, G0));
-    CK(gram_reduce_launch(st, p2, (int)n1, nb, G1));
-    CK(gram_reduce_norms_launch(st, norms, nb, xsq));
+    CK(gram_reduce2_launch(st, p1, (int)n0, G0, p2, (int)n1, G1, nb, norms, nb, xsq));
     passes += 1.0;
   } else {
     if (n0 > 1024 || n1 > 1024) return HT_ERR_UNSUPPORTED;
@@ -980,15 +984,17 @@ int Engine::run_ltr(const float* x) {
     CK(sym_topk_batch(G0, (int)n0, r0, 1, Ufull[t0], w0, 1e-9, 2, 0, 1000 + t0 * 16, GRAM_FLOOR));
     CK(sym_topk_batch(G1, (int)n1, r1, 1, Ufull[t1], w1, 1e-9, 2, 0, 1000 + t1 * 16, GRAM_FLOOR));
   }
-  CK(eig_to_sv(w0, r0, SVd[t0]));
-  CK(eig_to_sv(w1, r1, SVd[t1]));
+  CK(eig_to_sv(w0, t0));
+  CK(eig_to_sv(w1, t1));
   // Pass 2: Z = X x_1 U_0^T x_2 U_1^T  (r0, r1, n_2, ...), FP32.
   float* U0f = alloc<float>((size_t)n0 * r0);
   float* U1f = alloc<float>((size_t)n1 * r1);
   float* Z = alloc<float>((size_t)r0 * r1 * S);
   if (!U0f || !U1f || !Z) return HT_ERR_ALLOC;
-  CK(cast_f64_f32_launch(st, Ufull[t0], U0f, n0 * r0));
-  CK(cast_f64_f32_launch(st, Ufull[t1], U1f, n1 * r1));
+  {
+    const CastItem it[2] = {{Ufull[t0], U0f, n0 * r0, 0}, {Ufull[t1], U1f, n1 * r1, 0}};
+    CK(cast_multi_launch(st, it, 2));
+  }
   if (slab) {
     ProfScope ps(prof, st, PK_SLAB_PROJ);
     CK(slab_proj_launch(st, x, (int)n0, (int)n1, S, U0f, r0, U1f, r1, Z));
@@ -1027,7 +1033,7 @@ int Engine::run_ltr(const float* x) {
       for (int i = 0; i < nb; ++i) {
         const int t = p.leaf_of_mode(grp[i]);
         Ufull[t] = U + (size_t)i * n * k;
-        CK(eig_to_sv(w + (size_t)i * k, k, SVd[t]));
+        CK(eig_to_sv(w + (size_t)i * k, t));
         done[grp[i]] = 1;
       }
     }
@@ -1053,9 +1059,9 @@ int Engine::run_ltr(const float* x) {
 // Top-k left singular vectors of the FP64 matricisation A (M x cols) of the view v when
 // the column side is the small one (cols < M, cols <= 64, P == 1 or S == 1): the exact
 // eigen decomposition of A^T A (Jacobi), then U = A V diag(1/sigma) with canonical signs
-// and w = sigma^2. *done = 0 (nothing written) when the view does not qualify or
+// *w pointing at sigma^2 (descending). *done = 0 (nothing written) when the view does not qualify or
 // sigma_k / sigma_1 <= 1e-12, where the division would lose the basis.
-int Engine::small_side_svd(const double* C, PMS v, int k, double* U, double* w, int* done) {
+int Engine::small_side_svd(const double* C, PMS v, int k, double* U, const double** w, int* done) {
   *done = 0;
   const long M = v.M, cols = v.P * v.S;
   if (!(cols < M && cols <= 64 && k <= cols && (v.P == 1 || v.S == 1))) return 0;
@@ -1102,7 +1108,7 @@ int Engine::small_side_svd(const double* C, PMS v, int k, double* U, double* w, 
   CK(sqrt_clamp_launch(st, wh, k, sv));
   CK(scale_cols_inv_launch(st, U, (int)M, k, sv));
   CK(canon_signs_launch(st, U, (int)M, k, (int)M));
-  HT_HIP_CHECK(hipMemcpyAsync(w, wh, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
+  *w = wh;
   ++eig_calls;
   *done = 1;
   return 0;
@@ -1138,14 +1144,15 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
     double* w = alloc<double>(k);
     if (!V || !w) return HT_ERR_ALLOC;
     int small = 0;
-    CK(small_side_svd(C, v, k, V, w, &small));
+    const double* wsv = w;
+    CK(small_side_svd(C, v, k, V, &wsv, &small));
     if (!small) {
       double* G = alloc<double>((size_t)M * M);
       if (!G) return HT_ERR_ALLOC;
       CK(gram_pms_f64_launch(st, C, v, G));
       CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-4, 8, 32, 4000 + t * 16, 1e-12));
     }
-    CK(eig_to_sv(w, k, SVd[t]));
+    CK(eig_to_sv(wsv, t));
     Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself
     double* C2 = alloc<double>((size_t)v.P * k * v.S);
     CK(proj_pms_f64_launch(st, C, v, V, k, C2));
@@ -1321,6 +1328,7 @@ int Engine::run_rsvd(const float* x) {
     double* sv = alloc<double>(l);
     CK(sqrt_clamp_launch(st, th, l, sv));
     SVd[t] = sv;
+    SVsq[t] = 0;
     if (kW) {
       double* RS = alloc<double>((size_t)l * kW);
       double* Qzd = alloc<double>((size_t)cols * l);
@@ -1331,6 +1339,7 @@ int Engine::run_rsvd(const float* x) {
       CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, l, 1.0, Qzd, (int)cols, RS, l, 0.0, Ufull[sib], (int)cols));
       CK(canon_signs_launch(st, Ufull[sib], (int)cols, kW, (int)cols));
       SVd[sib] = sv;
+      SVsq[sib] = 0;
       done[sib] = 1;
     }
     done[t] = 1;
@@ -1346,21 +1355,24 @@ int Engine::run_rsvd(const float* x) {
 }
 
 // Write the packed outputs (FP32) into `out` (device).
+// The packed FP32 output block in one launch: U_t, B_t and the singular values of every
+// node (square roots of the kept eigenvalues where those are stored).
 int Engine::write_outputs(float* out) {
   const Problem& p = *prob;
+  std::vector<CastItem> items;
   for (int t = 0; t < p.n_nodes; ++t) {
     if (p.u_off[t] >= 0) {
       double* U = Ufull[t];
       if (p.left[t] >= 0) CK(expand(t, &U));
-      CK(cast_f64_f32_launch(st, U, out + p.u_off[t], p.rows[t] * p.rank[t]));
+      items.push_back(CastItem{U, out + p.u_off[t], p.rows[t] * p.rank[t], 0});
     }
     if (p.b_off[t] >= 0) {
       const long n = (long)p.rank[p.left[t]] * p.rank[p.right[t]] * p.rank[t];
-      CK(cast_f64_f32_launch(st, Bd[t], out + p.b_off[t], n));
+      items.push_back(CastItem{Bd[t], out + p.b_off[t], n, 0});
     }
-    if (p.s_off[t] >= 0 && SVd[t]) CK(cast_f64_f32_launch(st, SVd[t], out + p.s_off[t], p.rank[t]));
+    if (p.s_off[t] >= 0 && SVd[t]) items.push_back(CastItem{SVd[t], out + p.s_off[t], p.rank[t], SVsq[t]});
   }
-  return 0;
+  return cast_multi_launch(st, items.data(), (int)items.size());
 }
 
 int Engine::run_once(const float* x, float* out) {
@@ -1368,6 +1380,7 @@ int Engine::run_once(const float* x, float* out) {
     Ufull[t] = nullptr;
     Bd[t] = nullptr;
     SVd[t] = nullptr;
+    SVsq[t] = 0;
   }
   passes = 0.0;
   eig_iters_max = 0;

@@ -95,6 +95,7 @@ struct Engine {
   double* Ufull[HT_MAX_NODES];
   double* Bd[HT_MAX_NODES];
   double* SVd[HT_MAX_NODES];
+  int SVsq[HT_MAX_NODES] = {};  // 1: SVd[t] holds eigenvalues (squared singular values)
   double passes = 0.0;
   int eig_iters_max = 0, eig_calls = 0, noconv = 0;
   int sketch_fallbacks = 0, sketch_nodes = 0;
@@ -139,13 +140,13 @@ struct Engine {
   int residual(const float* x, double* err_sq, double* x_sq);
   int leaves_from_x_plain(const float* x, double* xsq_dev);
   int leaves_from_x(const float* x, int prec, double* xsq_dev);
-  int eig_to_sv(const double* w, int k, double*& sv);
+  int eig_to_sv(const double* w, int t);
   int root_transfer(const float* x);
   int finish_error(const float* x, const double* xsq_dev);
   int run_plain(const float* x);
   int run_ltr(const float* x);
   int ltr_interior(double* C, int64_t* cd, const double* xsq, const float* x);
-  int small_side_svd(const double* C, PMS v, int k, double* U, double* w, int* done);
+  int small_side_svd(const double* C, PMS v, int k, double* U, const double** w, int* done);
   int run_rsvd(const float* x);
   int range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z, float* Qz,
                    double* R);

@@ -11,6 +11,8 @@
 // dependent columns by fresh Philox vectors -- the orthonormalisation step of the
 // subspace iteration on small Gram matrices (the reference's QR: geqrf + orgqr,
 // .../Optimized/kernel.cu:289-346).
+#include <algorithm>
+
 #include "ht_common.h"
 
 namespace ht {
@@ -627,6 +629,32 @@ static unsigned grid_for(long n) {
   return (unsigned)b;
 }
 
+// Several FP64 -> FP32 copies (optionally sqrt(max(x, 0)) first) in one launch: blockIdx.y
+// picks the copy.
+struct CastTable {
+  CastItem it[CAST_MAX];
+};
+__global__ void cast_multi_kernel(CastTable tab) {
+  const CastItem c = tab.it[blockIdx.y];
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += (long)gridDim.x * blockDim.x) {
+    const double x = c.src[i];
+    c.dst[i] = (float)(c.sqrt_clamp ? sqrt(fmax(x, 0.0)) : x);
+  }
+}
+int cast_multi_launch(hipStream_t st, const CastItem* items, int n) {
+  for (int s0 = 0; s0 < n; s0 += CAST_MAX) {
+    CastTable tab;
+    const int cnt = std::min(CAST_MAX, n - s0);
+    long mx = 1;
+    for (int i = 0; i < cnt; ++i) {
+      tab.it[i] = items[s0 + i];
+      mx = std::max(mx, tab.it[i].n);
+    }
+    hipLaunchKernelGGL(cast_multi_kernel, dim3(std::min<unsigned>(grid_for(mx), 256), cnt), dim3(256), 0, st, tab);
+    if (hipGetLastError() != hipSuccess) return 5;
+  }
+  return 0;
+}
 int cast_f64_f32_launch(hipStream_t st, const double* a, float* b, long n) {
   hipLaunchKernelGGL(cast_f64_f32_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, b, n);
   return hipGetLastError() == hipSuccess ? 0 : 5;

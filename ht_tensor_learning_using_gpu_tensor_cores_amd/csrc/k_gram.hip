@@ -10,6 +10,8 @@
 // slab_gram: the Grams of the two leading modes from one read of each n1 x n2 slab.
 // slab_proj: z_s = U1^T S_s U2 per slab (the leaf-core projection of the two leading
 // modes), one wavefront per slab, FP32 MFMA, U1 and U2 staged in LDS.
+#include <algorithm>
+
 #include "ht_common.h"
 
 namespace ht {
@@ -346,10 +348,29 @@ int gram_pms_launch(hipStream_t st, const float* x, PMS v, int prec, int n_split
 // split the partials (wave w: w, w + 16, ..., eight loads in flight per batch); the
 // waves' sums are added in a fixed order through LDS. Diagonal tiles produce every entry
 // themselves, off-diagonal ones also write the mirrored entry.
-__global__ __launch_bounds__(1024) void gram_reduce_kernel(const float* __restrict__ part, int M, int nt, int n_split,
-                                                           double* __restrict__ G) {
+// Up to two reductions in one launch (blockIdx.z picks the job); block (0, 0, 0) also sums
+// the n_norms per-block norms into *norm_out when norms != nullptr.
+struct GramReduceJob {
+  const float* part;
+  int M;
+  double* G;
+};
+__global__ __launch_bounds__(1024) void gram_reduce_kernel(GramReduceJob j0, GramReduceJob j1, int n_split,
+                                                           const double* __restrict__ norms, int n_norms,
+                                                           double* __restrict__ norm_out) {
   __shared__ double red[16][64];
+  const GramReduceJob job = blockIdx.z ? j1 : j0;
+  const float* __restrict__ part = job.part;
+  const int M = job.M, nt = (M + GT_ROWS - 1) / GT_ROWS;
+  double* __restrict__ G = job.G;
   const int pair = blockIdx.y;
+  if (pair >= nt * (nt + 1) / 2) return;  // the smaller job of a launch with two
+  if (norms && blockIdx.x == 0 && pair == 0 && blockIdx.z == 0 && threadIdx.x < 64) {
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n_norms; i += 64) t += norms[i];
+    t = wave_sum(t);
+    if (threadIdx.x == 0) *norm_out = t;
+  }
   int bi = 0, bj = 0;
   {
     int p = pair;
@@ -395,7 +416,16 @@ int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, do
   const int nt = (M + GT_ROWS - 1) / GT_ROWS;
   const int pairs = nt * (nt + 1) / 2;
   dim3 grid(GT_ROWS * GT_ROWS / 64, pairs);
-  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, part, M, nt, n_split, G);
+  const GramReduceJob j{part, M, G};
+  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, j, j, n_split, nullptr, 0, nullptr);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, double* G1, const float* p2, int M2, double* G2,
+                        int n_split, const double* norms, int n_norms, double* norm_out) {
+  const int nt = (std::max(M1, M2) + GT_ROWS - 1) / GT_ROWS;
+  dim3 grid(GT_ROWS * GT_ROWS / 64, nt * (nt + 1) / 2, 2);
+  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, GramReduceJob{p1, M1, G1}, GramReduceJob{p2, M2, G2},
+                     n_split, norms, n_norms, norm_out);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
