@@ -17,6 +17,16 @@ __device__ __forceinline__ double lane_bcast(double x, int l) {
   return __hiloint2double(hi, lo);
 }
 
+// x from lane l - 1 (ROW_SR1) or l + 1 (ROW_SL1) of the same 16-lane row, by DPP moves;
+// the row's first (last) lane keeps 0.
+constexpr int DPP_ROW_SL1 = 0x101, DPP_ROW_SR1 = 0x111;
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_shift(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
 // Round-robin pairing for np (even) indices: step s pairs (np-1, s) and
 // ((s+t) mod (np-1), (s-t) mod (np-1)), t = 1 .. np/2-1; every pair meets once per sweep.
 // Entry (s, t) = p | q << 8 with p < q.
@@ -197,7 +207,8 @@ __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np
 // fetches those of its row and column pair with lane permutes and applies R_I^T A R_J and
 // V R_J. Then the indices move one place along the ring t1 .. t7, b7 .. b0 of the circle
 // round-robin (t0 fixed; ti = 2i, bi = 2i+1) by a row and a column permutation of A and a
-// column permutation of V, so fifteen steps pair every two indices once. Diagonal entries
+// column permutation of V (rows from lanes l -+ 8, columns from lanes l -+ 1 by DPP row
+// shifts), so fifteen steps pair every two indices once. Diagonal entries
 // equal to `pad` (decoupled padding) are left out of the convergence test, which is the
 // one of jacobi_sweeps. On return a and v (LDS, ld SJLD) hold the result in the slot
 // order (columns of v follow the diagonal of a); returns 1 when converged.
@@ -208,7 +219,6 @@ __device__ __forceinline__ int jacobi16_wave(double* a, double* v, int max_sweep
   double a10 = a[(2 * I + 1) * SJLD + 2 * J], a11 = a[(2 * I + 1) * SJLD + 2 * J + 1];
   double v00 = dgl ? 1.0 : 0.0, v01 = 0.0, v10 = 0.0, v11 = dgl ? 1.0 : 0.0;
   const int up = l >= 8 ? l - 8 : l, dn = l < 56 ? l + 8 : l;
-  const int lf = l >= 1 ? l - 1 : l, rt = l < 63 ? l + 1 : l;
   int conv = 0;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     double off = dgl ? a01 * a01 + a10 * a10 : a00 * a00 + a01 * a01 + a10 * a10 + a11 * a11;
@@ -253,8 +263,8 @@ __device__ __forceinline__ int jacobi16_wave(double* a, double* v, int max_sweep
       // columns, the same ring (A and V)
       {
         const double s0 = J == 0 ? a01 : a00, s1 = J == 0 ? a11 : a10;
-        const double l00 = __shfl(s0, lf, 64), l10 = __shfl(s1, lf, 64);
-        const double r01 = __shfl(a01, rt, 64), r11 = __shfl(a11, rt, 64);
+        const double l00 = dpp_row_shift<DPP_ROW_SR1>(s0), l10 = dpp_row_shift<DPP_ROW_SR1>(s1);
+        const double r01 = dpp_row_shift<DPP_ROW_SL1>(a01), r11 = dpp_row_shift<DPP_ROW_SL1>(a11);
         const double n01 = J == 7 ? a00 : r01, n11 = J == 7 ? a10 : r11;
         if (J != 0) {
           a00 = l00;
@@ -265,8 +275,8 @@ __device__ __forceinline__ int jacobi16_wave(double* a, double* v, int max_sweep
       }
       {
         const double s0 = J == 0 ? w01 : w00, s1 = J == 0 ? w11 : w10;
-        const double l00 = __shfl(s0, lf, 64), l10 = __shfl(s1, lf, 64);
-        const double r01 = __shfl(w01, rt, 64), r11 = __shfl(w11, rt, 64);
+        const double l00 = dpp_row_shift<DPP_ROW_SR1>(s0), l10 = dpp_row_shift<DPP_ROW_SR1>(s1);
+        const double r01 = dpp_row_shift<DPP_ROW_SL1>(w01), r11 = dpp_row_shift<DPP_ROW_SL1>(w11);
         v00 = J == 0 ? w00 : l00;
         v10 = J == 0 ? w10 : l10;
         v01 = J == 7 ? w00 : r01;

@@ -706,16 +706,18 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
   U += bidx * sU;
   w += bidx * sw;
   wall += (long)bidx * SJ;
-  // blocks of up to 16 columns are padded to 16 and solved by one wave in registers
-  constexpr bool WAVE16 = BP == 16;
-  const int np = WAVE16 ? 16 : (b + 1) & ~1;
+  // blocks of up to 16 columns are padded to 16 and solved by one wave in registers (a
+  // one-wave 32 x 32 version, four blocks per lane, measured slower than the block-wide
+  // Jacobi below: its lane permutes outweigh the barriers saved)
+  constexpr int WN = BP == 16 ? 16 : 0;
+  const int np = WN ? WN : (b + 1) & ~1;
   for (int e = tid; e < np * np; e += SNT) {
     const int i = e % np, j = e / np;
     a[i * SJLD + j] = (i < b && j < b) ? 0.5 * (T[i * b + j] + T[j * b + i]) : 0.0;
     v[i * SJLD + j] = (i == j) ? 1.0 : 0.0;
   }
   if (tid < SJ) res[tid] = 0.0;
-  if (!WAVE16) jacobi_schedule(sched, np);
+  if (!WN) jacobi_schedule(sched, np);
   for (int e = tid; e < n * b; e += SNT) Ms[e] = Q[e];
   __syncthreads();
   if (np != b && tid == 0) {
@@ -727,7 +729,7 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
   // the Ritz residuals are tested against tol * theta_1: an off-diagonal remainder 1e3
   // times smaller than that is invisible in them (and never below 1e-14)
   const double jtol = fmax(1e-14, 1e-3 * tol);
-  if constexpr (WAVE16) {
+  if constexpr (WN == 16) {
     const double pad = b < 16 ? a[15 * SJLD + 15] : -INFINITY;
     if (tid < 64) jacobi16_wave(a, v, max_sweeps, jtol, pad);
     __syncthreads();
