@@ -295,6 +295,9 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   // well-conditioned block; a flat one misses the first round and is widened below.
   if (b <= 0) b = k;
   b = std::min(std::max(b, k), bmax);
+  // a block that starts at exactly k columns is widened after a missed round; a block
+  // oversampled by the caller keeps its width
+  const int b0 = b;
   // a replayed call takes its rounds from the plan; a recorded one appends them
   const EigCall* pc = nullptr;
   if (replay) {
@@ -408,7 +411,7 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
         }
         break;
       }
-      if (b < std::min(bmax, 2 * k)) widen = std::min(bmax, b + 16);
+      if (b0 == k && b < std::min(bmax, 2 * k)) widen = std::min(bmax, b + 16);
       if (recording) plan->calls[rec].rounds.back().widen = widen;
     }
     if (widen) {
@@ -1150,6 +1153,9 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
       double* G = alloc<double>((size_t)M * M);
       if (!G) return HT_ERR_ALLOC;
       CK(gram_pms_f64_launch(st, C, v, G));
+      // An interior Gram's wanted spectrum is typically flat (the leaf core's unfolding):
+      // a 32-column block (a multiple of 16 keeps the MFMA CholeskyQR; 24 columns with 16
+      // first powers measured slower) and 8 plain powers first; two rounds usually meet 1e-4.
       CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-4, 8, 32, 4000 + t * 16, 1e-12));
     }
     CK(eig_to_sv(wsv, t));
