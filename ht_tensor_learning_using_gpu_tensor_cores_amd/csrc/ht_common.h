@@ -160,6 +160,8 @@ int slab_proj_launch(hipStream_t st, const float* x, int n1, int n2, long S, con
 // sum_m U[m + M*j] x[p + P*m + P*M*s]. T = float or double.
 int proj_pms_f32_launch(hipStream_t st, const float* x, PMS v, const float* U, int r, float* y);
 int proj_pms_f64_launch(hipStream_t st, const double* x, PMS v, const double* U, int r, double* y);
+// The same with U given transposed: Ut is r x M column-major (U[m, j] = Ut[j + r m]).
+int proj_pms_f64t_launch(hipStream_t st, const double* x, PMS v, const double* Ut, int r, double* y);
 int proj_pms_f32in_f64_launch(hipStream_t st, const float* x, PMS v, const double* U, int r, double* y);
 
 // Skinny FP32 GEMMs over a big column-major A (M x K, lda = M), b <= 64:
@@ -214,6 +216,10 @@ struct CastItem {
 constexpr int CAST_MAX = 48;
 int cast_multi_launch(hipStream_t st, const CastItem* items, int n);
 int ratio_check_launch(hipStream_t st, const double* w, int k, double ratio, int* ok);
+// U[:, j] = A[:, j] / sqrt(max(w_j, 0)) with canonical signs, A[r, j] at in[r rs + j cs]
+// (in may alias U), m <= 4096.
+int cols_unit_sign_launch(hipStream_t st, const double* in, long rs, long cs, double* U, int m, int k, int ldu,
+                          const double* w);
 int scale_cols_inv_launch(hipStream_t st, double* A, int m, int n, const double* s);
 // Batched small symmetric eigen solvers (k_small.hip).
 int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* w, long sw, double* V, long sV,

@@ -696,16 +696,13 @@ int Engine::expand(int t, double** out) {
   const int l = p.left[t], r = p.right[t];
   const long nl = p.rows[l], nr = p.rows[r];
   const int kl = p.rank[l], kr = p.rank[r], kt = p.rank[t];
-  // out[il + nl ir, c] = sum_ab Ul[il,a] Ur[ir,b] B[a,b,c]: expand with U^T copies.
-  double* UlT = alloc<double>((size_t)nl * kl);
-  double* UrT = alloc<double>((size_t)nr * kr);
+  // out[il + nl ir, c] = sum_ab Ul[il,a] Ur[ir,b] B[a,b,c]: contractions with U^T, read
+  // through transposed strides.
   double* T1 = alloc<double>((size_t)nl * kr * kt);
   double* E = alloc<double>((size_t)nl * nr * kt);
-  if (!UlT || !UrT || !T1 || !E) return HT_ERR_ALLOC;
-  CK(transpose_f64(Ul, nl, kl, UlT));
-  CK(transpose_f64(Ur, nr, kr, UrT));
-  CK(proj_pms_f64_launch(st, Bd[t], PMS{1, kl, (long)kr * kt}, UlT, (int)nl, T1));  // (nl, kr, kt)
-  CK(proj_pms_f64_launch(st, T1, PMS{nl, kr, kt}, UrT, (int)nr, E));                // (nl, nr, kt)
+  if (!T1 || !E) return HT_ERR_ALLOC;
+  CK(proj_pms_f64t_launch(st, Bd[t], PMS{1, kl, (long)kr * kt}, Ul, (int)nl, T1));  // (nl, kr, kt)
+  CK(proj_pms_f64t_launch(st, T1, PMS{nl, kr, kt}, Ur, (int)nr, E));                // (nl, nr, kt)
   *out = E;
   return 0;
 }
@@ -1076,10 +1073,9 @@ int Engine::small_side_svd(const double* C, PMS v, int k, double* U, const doubl
   double* H = alloc<double>((size_t)c * c);
   double* Vh = alloc<double>((size_t)c * c);
   double* wh = alloc<double>(c);
-  double* sv = alloc<double>(c);
   int* info = alloc<int>(1);
   double* AV = alloc<double>((size_t)M * k);
-  if (!H || !Vh || !wh || !sv || !info || !AV) return HT_ERR_ALLOC;
+  if (!H || !Vh || !wh || !info || !AV) return HT_ERR_ALLOC;
   // a replayed run takes the outcome of the rank test from the plan and leaves the test
   // itself in a verdict slot
   int planned = -1;
@@ -1101,16 +1097,15 @@ int Engine::small_side_svd(const double* C, PMS v, int k, double* U, const doubl
     if (recording) plan->small.push_back(ok);
     if (!ok) return 0;
   }
-  // A V: contraction over the small side of the transposed view.
+  // A V: contraction over the small side of the transposed view, then unit columns with
+  // canonical signs (one launch, transposing the (k, M) product of the S == 1 case).
   if (v.P == 1) {
     CK(proj_pms_f64_launch(st, C, vs, Vh, k, U));  // (M, k) column-major
+    CK(cols_unit_sign_launch(st, U, 1, M, U, (int)M, k, (int)M, wh));
   } else {
     CK(proj_pms_f64_launch(st, C, vs, Vh, k, AV));  // (k, M)
-    CK(transpose_launch(st, AV, k, M, U));
+    CK(cols_unit_sign_launch(st, AV, k, 1, U, (int)M, k, (int)M, wh));
   }
-  CK(sqrt_clamp_launch(st, wh, k, sv));
-  CK(scale_cols_inv_launch(st, U, (int)M, k, sv));
-  CK(canon_signs_launch(st, U, (int)M, k, (int)M));
   *w = wh;
   ++eig_calls;
   *done = 1;

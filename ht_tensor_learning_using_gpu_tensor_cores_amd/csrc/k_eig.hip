@@ -824,6 +824,69 @@ int ratio_check_launch(hipStream_t st, const double* w, int k, double ratio, int
   hipLaunchKernelGGL(ratio_check_kernel, dim3(1), dim3(64), 0, st, w, k, ratio, ok);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
+// U[:, j] = A[:, j] / sqrt(max(w_j, 0)) (zero where that is 0) with the canonical sign (its
+// largest-magnitude entry, the first on ties, made positive); A[r, j] is read at
+// in[r rs + j cs], which may alias U. One block per column, m <= 4096: the column is
+// held in registers before anything is written.
+__global__ __launch_bounds__(256) void cols_unit_sign_kernel(const double* in, long rs, long cs, double* U, int m,
+                                                             int ldu, const double* __restrict__ w) {
+  __shared__ double bv[4], bs[4];
+  __shared__ int bix[4];
+  const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double sw = sqrt(fmax(w[j], 0.0));
+  double x[16];
+  double best = -1.0, sgn = 1.0;
+  int bi = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = tid + 256 * i;
+    x[i] = 0.0;
+    if (r < m) {
+      const double a = in[r * rs + j * cs];
+      x[i] = sw > 0.0 ? a / sw : 0.0;
+      if (fabs(x[i]) > best) {
+        best = fabs(x[i]);
+        sgn = x[i];
+        bi = r;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64), os = __shfl_xor(sgn, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      sgn = os;
+      bi = oi;
+    }
+  }
+  if (lane == 0) {
+    bv[wv] = best;
+    bs[wv] = sgn;
+    bix[wv] = bi;
+  }
+  __syncthreads();
+  double b0 = bv[0], s0 = bs[0];
+  int i0 = bix[0];
+  for (int k = 1; k < 4; ++k)
+    if (bv[k] > b0 || (bv[k] == b0 && bix[k] < i0)) {
+      b0 = bv[k];
+      s0 = bs[k];
+      i0 = bix[k];
+    }
+  const double sg = s0 < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = tid + 256 * i;
+    if (r < m) U[r + (long)ldu * j] = sg * x[i];
+  }
+}
+int cols_unit_sign_launch(hipStream_t st, const double* in, long rs, long cs, double* U, int m, int k, int ldu,
+                          const double* w) {
+  if (m < 1 || m > 4096 || k < 1) return 8;
+  hipLaunchKernelGGL(cols_unit_sign_kernel, dim3(k), dim3(256), 0, st, in, rs, cs, U, m, ldu, w);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
 __global__ void scale_cols_inv_kernel(double* __restrict__ A, int m, int n, const double* __restrict__ s) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)m * n) return;

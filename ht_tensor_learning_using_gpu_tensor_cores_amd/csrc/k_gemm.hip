@@ -299,8 +299,9 @@ int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* 
 
 // ------------------------------------------------------------------ proj_pms
 // One thread per output element (p, j, s); consecutive threads take consecutive p.
+// U[m, j] is read at U[m * um + j * uj] (um = 1, uj = M: column-major M x r).
 template <typename TI, typename TU, typename TO>
-__global__ void proj_pms_kernel(const TI* __restrict__ x, PMS v, const TU* __restrict__ U, int r,
+__global__ void proj_pms_kernel(const TI* __restrict__ x, PMS v, const TU* __restrict__ U, long um, long uj, int r,
                                 TO* __restrict__ y) {
   const long total = v.P * (long)r * v.S;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
@@ -309,9 +310,9 @@ __global__ void proj_pms_kernel(const TI* __restrict__ x, PMS v, const TU* __res
     const int j = (int)(rest % r);
     const long s = rest / r;
     const TI* xp = x + p + v.P * v.M * s;
-    const TU* up = U + v.M * (long)j;
+    const TU* up = U + uj * (long)j;
     double acc = 0.0;
-    for (long m = 0; m < v.M; ++m) acc += (double)up[m] * (double)xp[v.P * m];
+    for (long m = 0; m < v.M; ++m) acc += (double)up[um * m] * (double)xp[v.P * m];
     y[e] = (TO)acc;
   }
 }
@@ -323,7 +324,7 @@ __global__ void proj_pms_kernel(const TI* __restrict__ x, PMS v, const TU* __res
 // its X column) is fixed for the whole tile.
 template <typename TI, typename TU, typename TO>
 __global__ __launch_bounds__(256) void proj_mfma_kernel(const TI* __restrict__ x, PMS v, const TU* __restrict__ U,
-                                                        int r, TO* __restrict__ y, long ntn) {
+                                                        long um, long uj, int r, TO* __restrict__ y, long ntn) {
   const int lane = threadIdx.x & 63;
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long njt = (r + 15) / 16;
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(256) void proj_mfma_kernel(const TI* __restrict__ x
   const TI* xb = x + pp + v.P * M * ss;
   const long j = JT * 16 + i16;
   const bool jok = j < r;
-  const TU* ub = U + M * (jok ? j : r - 1);
+  const TU* ub = U + uj * (jok ? j : r - 1);
   constexpr int UB = 8;
   f64x4 acc = {};
   for (long m0 = 0; m0 < M; m0 += 4 * UB) {
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(256) void proj_mfma_kernel(const TI* __restrict__ x
     for (int u = 0; u < UB; ++u) {
       const long m = m0 + 4 * u + kk;
       const long mc = m < M ? m : M - 1;
-      a[u] = (double)ub[mc];
+      a[u] = (double)ub[um * mc];
       b[u] = (double)xb[v.P * mc];
     }
 #pragma unroll
@@ -367,30 +368,34 @@ __global__ __launch_bounds__(256) void proj_mfma_kernel(const TI* __restrict__ x
 }
 
 template <typename TI, typename TU, typename TO>
-static int proj_launch(hipStream_t st, const TI* x, PMS v, const TU* U, int r, TO* y) {
+static int proj_launch(hipStream_t st, const TI* x, PMS v, const TU* U, long um, long uj, int r, TO* y) {
   const long total = v.P * (long)r * v.S;
   if (total <= 0) return 0;
   {
     const long ntn = (v.P * v.S + 15) / 16, waves = ((r + 15) / 16) * ntn;
     const long blocks = (waves + 3) / 4;
     if (blocks <= (1L << 30)) {
-      hipLaunchKernelGGL((proj_mfma_kernel<TI, TU, TO>), dim3((unsigned)blocks), dim3(256), 0, st, x, v, U, r, y, ntn);
+      hipLaunchKernelGGL((proj_mfma_kernel<TI, TU, TO>), dim3((unsigned)blocks), dim3(256), 0, st, x, v, U, um, uj, r,
+                         y, ntn);
       return hipGetLastError() == hipSuccess ? 0 : 5;
     }
   }
   long blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL((proj_pms_kernel<TI, TU, TO>), dim3((unsigned)blocks), dim3(256), 0, st, x, v, U, r, y);
+  hipLaunchKernelGGL((proj_pms_kernel<TI, TU, TO>), dim3((unsigned)blocks), dim3(256), 0, st, x, v, U, um, uj, r, y);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 int proj_pms_f32_launch(hipStream_t st, const float* x, PMS v, const float* U, int r, float* y) {
-  return proj_launch(st, x, v, U, r, y);
+  return proj_launch(st, x, v, U, 1, v.M, r, y);
 }
 int proj_pms_f64_launch(hipStream_t st, const double* x, PMS v, const double* U, int r, double* y) {
-  return proj_launch(st, x, v, U, r, y);
+  return proj_launch(st, x, v, U, 1, v.M, r, y);
+}
+int proj_pms_f64t_launch(hipStream_t st, const double* x, PMS v, const double* Ut, int r, double* y) {
+  return proj_launch(st, x, v, Ut, r, 1, r, y);
 }
 int proj_pms_f32in_f64_launch(hipStream_t st, const float* x, PMS v, const double* U, int r, double* y) {
-  return proj_launch(st, x, v, U, r, y);
+  return proj_launch(st, x, v, U, 1, v.M, r, y);
 }
 
 // ------------------------------------------------------------------ lowrank
