@@ -143,9 +143,10 @@ int gram_pms_launch(hipStream_t st, const float* x, PMS v, int prec, int n_split
 int gram_pms_nsplit(PMS v, int target_blocks);
 // Sum partials into a full symmetric FP64 Gram (ld = M).
 int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G);
-// The two slab Grams' reductions (and the sum of the per-block norms) in one launch.
-int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, double* G1, const float* p2, int M2, double* G2,
-                        int n_split, const double* norms, int n_norms, double* norm_out);
+// Two Gram reductions (n1, n2 partials each; and the sum of the per-block norms when
+// norms != nullptr) in one launch.
+int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, int n1, double* G1, const float* p2, int M2, int n2,
+                        double* G2, const double* norms, int n_norms, double* norm_out);
 int gram_reduce_norms_launch(hipStream_t st, const double* norms, int n, double* out);
 
 // Fused slab kernels on the two leading modes (x as n1 x n2 x S, n1,n2 <= 128).

@@ -257,6 +257,26 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
   return 0;
 }
 
+// Grams of two views of the same data (no norms), their reductions in one launch.
+int Engine::gram2(const float* x, PMS v1, PMS v2, int prec, double* G1, double* G2) {
+  const PMS vs[2] = {v1, v2};
+  float* part[2];
+  int ns[2];
+  for (int i = 0; i < 2; ++i) {
+    ns[i] = gram_pms_nsplit(vs[i], 512);
+    const int nt = (int)((vs[i].M + 127) / 128);
+    const int pairs = nt * (nt + 1) / 2;
+    part[i] = alloc<float>((size_t)pairs * ns[i] * 128 * 128);
+    double* norms = alloc<double>((size_t)pairs * ns[i]);
+    if (!part[i] || !norms) return HT_ERR_ALLOC;
+    ProfScope ps(prof, st, PK_GRAM);
+    CK(gram_pms_launch(st, x, vs[i], prec, ns[i], part[i], norms));
+  }
+  CK(gram_reduce2_launch(st, part[0], (int)v1.M, ns[0], G1, part[1], (int)v2.M, ns[1], G2, nullptr, 0, nullptr));
+  passes += 2.0;
+  return 0;
+}
+
 // Top-k eigenpairs (descending, canonical signs) of `batch` symmetric PSD n x n FP64
 // matrices G (stride n*n), written to U (stride n*k) and w (stride k).
 // n <= 64: one Jacobi solve each. n <= 256: rounds of block subspace iteration (block b,
@@ -967,7 +987,7 @@ int Engine::run_ltr(const float* x) {
       ProfScope ps(prof, st, PK_SLAB_GRAM);
       CK(slab_gram_launch(st, x, (int)n0, (int)n1, S, prec, nb, p1, p2, norms));
     }
-    CK(gram_reduce2_launch(st, p1, (int)n0, G0, p2, (int)n1, G1, nb, norms, nb, xsq));
+    CK(gram_reduce2_launch(st, p1, (int)n0, nb, G0, p2, (int)n1, nb, G1, norms, nb, xsq));
     passes += 1.0;
   } else {
     if (n0 > 1024 || n1 > 1024) return HT_ERR_UNSUPPORTED;
@@ -1025,9 +1045,15 @@ int Engine::run_ltr(const float* x) {
       double* U = alloc<double>((size_t)nb * n * k);
       double* w = alloc<double>((size_t)nb * k);
       if (!G || !U || !w) return HT_ERR_ALLOC;
-      for (int i = 0; i < nb; ++i) {
-        CK(gram(Z, view_of(zd, d, grp[i], grp[i] + 1), PREC_F32, G + (size_t)i * n * n, nullptr));
-        passes -= 1.0;  // Z is small; not a pass over X
+      if (nb == 2) {
+        CK(gram2(Z, view_of(zd, d, grp[0], grp[0] + 1), view_of(zd, d, grp[1], grp[1] + 1), PREC_F32, G,
+                 G + (size_t)n * n));
+        passes -= 2.0;  // Z is small; not a pass over X
+      } else {
+        for (int i = 0; i < nb; ++i) {
+          CK(gram(Z, view_of(zd, d, grp[i], grp[i] + 1), PREC_F32, G + (size_t)i * n * n, nullptr));
+          passes -= 1.0;
+        }
       }
       CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
       for (int i = 0; i < nb; ++i) {
@@ -1038,9 +1064,14 @@ int Engine::run_ltr(const float* x) {
       }
     }
   }
+  // the core C (FP64): the first projection reads Z (FP32) directly
   long csize = r0 * r1 * S;
-  double* C = alloc<double>((size_t)csize);
-  CK(cast_f32_f64_launch(st, Z, C, csize));
+  double* C = nullptr;
+  if (d == 2) {
+    C = alloc<double>((size_t)csize);
+    if (!C) return HT_ERR_ALLOC;
+    CK(cast_f32_f64_launch(st, Z, C, csize));
+  }
   int64_t cd[HT_MAX_ORDER];
   for (int m = 0; m < d; ++m) cd[m] = zd[m];
   for (int m = 2; m < d; ++m) {
@@ -1048,7 +1079,9 @@ int Engine::run_ltr(const float* x) {
     const int k = p.rank[t];
     const PMS v = view_of(cd, d, m, m + 1);
     double* C2 = alloc<double>((size_t)(csize / cd[m]) * k);
-    CK(proj_pms_f64_launch(st, C, v, Ufull[t], k, C2));
+    if (!C2) return HT_ERR_ALLOC;
+    if (m == 2) CK(proj_pms_f32in_f64_launch(st, Z, v, Ufull[t], k, C2));
+    else CK(proj_pms_f64_launch(st, C, v, Ufull[t], k, C2));
     csize = csize / cd[m] * k;
     cd[m] = k;
     C = C2;

@@ -121,6 +121,7 @@ struct Engine {
   int sync();
   int d2h(void* dst, const void* src, size_t bytes);
   int gram(const float* x, PMS v, int prec, double* G, double* sumsq);
+  int gram2(const float* x, PMS v1, PMS v2, int prec, double* G1, double* G2);
   // Top-k eigenpairs of symmetric G (FP64). A Ritz pair counts as converged when its
   // residual is <= tol * th_1 or its Ritz value is <= floor * th_1 (below what the Gram's
   // own accumulation error resolves, where any basis of the cluster serves).

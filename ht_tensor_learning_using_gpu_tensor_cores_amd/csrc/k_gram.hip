@@ -352,16 +352,16 @@ int gram_pms_launch(hipStream_t st, const float* x, PMS v, int prec, int n_split
 // the n_norms per-block norms into *norm_out when norms != nullptr.
 struct GramReduceJob {
   const float* part;
-  int M;
+  int M, n_split;
   double* G;
 };
-__global__ __launch_bounds__(1024) void gram_reduce_kernel(GramReduceJob j0, GramReduceJob j1, int n_split,
+__global__ __launch_bounds__(1024) void gram_reduce_kernel(GramReduceJob j0, GramReduceJob j1,
                                                            const double* __restrict__ norms, int n_norms,
                                                            double* __restrict__ norm_out) {
   __shared__ double red[16][64];
   const GramReduceJob job = blockIdx.z ? j1 : j0;
   const float* __restrict__ part = job.part;
-  const int M = job.M, nt = (M + GT_ROWS - 1) / GT_ROWS;
+  const int M = job.M, nt = (M + GT_ROWS - 1) / GT_ROWS, n_split = job.n_split;
   double* __restrict__ G = job.G;
   const int pair = blockIdx.y;
   if (pair >= nt * (nt + 1) / 2) return;  // the smaller job of a launch with two
@@ -416,16 +416,16 @@ int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, do
   const int nt = (M + GT_ROWS - 1) / GT_ROWS;
   const int pairs = nt * (nt + 1) / 2;
   dim3 grid(GT_ROWS * GT_ROWS / 64, pairs);
-  const GramReduceJob j{part, M, G};
-  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, j, j, n_split, nullptr, 0, nullptr);
+  const GramReduceJob j{part, M, n_split, G};
+  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, j, j, nullptr, 0, nullptr);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
-int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, double* G1, const float* p2, int M2, double* G2,
-                        int n_split, const double* norms, int n_norms, double* norm_out) {
+int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, int n1, double* G1, const float* p2, int M2, int n2,
+                        double* G2, const double* norms, int n_norms, double* norm_out) {
   const int nt = (std::max(M1, M2) + GT_ROWS - 1) / GT_ROWS;
   dim3 grid(GT_ROWS * GT_ROWS / 64, nt * (nt + 1) / 2, 2);
-  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, GramReduceJob{p1, M1, G1}, GramReduceJob{p2, M2, G2},
-                     n_split, norms, n_norms, norm_out);
+  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, GramReduceJob{p1, M1, n1, G1},
+                     GramReduceJob{p2, M2, n2, G2}, norms, n_norms, norm_out);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
