@@ -203,7 +203,8 @@ int chol_rinv_launch(hipStream_t st, const double* W, int b, long m, int first, 
 int refresh_zero_cols_launch(hipStream_t st, double* Z, long m, int b, double* W, uint64_t seed, uint32_t stream);
 int ritz_resid_launch(hipStream_t st, const double* GU, const double* U, const double* w, int n, int k,
                       double* out);
-int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G);
+// G32 (optional): an FP32 copy of G written alongside.
+int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G, float* G32 = nullptr);
 int sum_strided_launch(hipStream_t st, const double* acc, int n, double* out);
 int transpose_launch(hipStream_t st, const double* A, long m, long n, double* AT);
 int sqrt_clamp_launch(hipStream_t st, const double* w, int k, double* s);
@@ -216,17 +217,20 @@ struct CastItem {
 };
 constexpr int CAST_MAX = 48;
 int cast_multi_launch(hipStream_t st, const CastItem* items, int n);
-int ratio_check_launch(hipStream_t st, const double* w, int k, double ratio, int* ok);
 // U[:, j] = A[:, j] / sqrt(max(w_j, 0)) with canonical signs, A[r, j] at in[r rs + j cs]
-// (in may alias U), m <= 4096.
+// (in may alias U), m <= 4096; ok (optional) receives the rank test w[0] > 0 and
+// w[k-1] > 1e-24 w[0].
 int cols_unit_sign_launch(hipStream_t st, const double* in, long rs, long cs, double* U, int m, int k, int ldu,
-                          const double* w);
+                          const double* w, int* ok = nullptr);
 int scale_cols_inv_launch(hipStream_t st, double* A, int m, int n, const double* s);
 // Batched small symmetric eigen solvers (k_small.hip).
 int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* w, long sw, double* V, long sV,
                        int batch, int* info, int max_sweeps, double tol);
+// W == nullptr: the start block is the Philox normals (gseed, gstream) of element
+// i = problem * sWY + row * b + column, the values fill_normals_f64 writes.
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
-                 int mode, const double* cut, double* T, int batch, const float* G32 = nullptr);
+                 int mode, const double* cut, double* T, int batch, const float* G32 = nullptr, uint64_t gseed = 0,
+                 uint32_t gstream = 0);
 int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b, int try_chol, int batch);
 // Running count of qr_kernel calls that took the Householder fallback (process-wide).
 int qr_fallback_count(unsigned long long* out);

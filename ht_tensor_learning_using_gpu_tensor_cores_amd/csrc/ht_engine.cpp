@@ -286,7 +286,7 @@ int Engine::gram2(const float* x, PMS v1, PMS v2, int prec, double* G1, double* 
 // Chebyshev filter (degree 8, 16, ...) that damps [0, smallest Ritz value]. A missed
 // round also widens a block narrower than 2k by 16 columns (up to min(64, n)).
 int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
-                           uint32_t stream, double floor) {
+                           uint32_t stream, double floor, const float* G32in) {
   ProfScope ps(prof, st, PK_EIG);
   if (n <= 64) {
     double* V = alloc<double>((size_t)batch * n * n);
@@ -347,16 +347,20 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
   double* wall = reinterpret_cast<double*>(rd);
   double* rres = reinterpret_cast<double*>(rd + wb);
   int* conv = reinterpret_cast<int*>(rd + wb + rb);
-  long nb = (long)n * b;
-  CK(fill_normals_f64_launch(st, W, batch * nb, opt.seed, stream));
+  long nb = (long)n * b;  // round 0 generates its Gaussian start block inside the filter kernel
   // At tolerances of 1e-5 and above the filter steps read an FP32 copy of G: the subspace
   // they converge to is that of G rounded to FP32, within ~6e-8 sqrt(n) theta_1 of G's (far
   // below tol theta_1), and the Ritz values and residuals are still formed with G itself.
-  float* G32 = nullptr;
+  const float* G32 = nullptr;
   if (tol >= 1e-5) {
-    G32 = alloc<float>((size_t)batch * n * n);
-    if (!G32) return HT_ERR_ALLOC;
-    CK(cast_f64_f32_launch(st, G, G32, (long)batch * n * n));
+    if (G32in) {
+      G32 = G32in;
+    } else {
+      float* g = alloc<float>((size_t)batch * n * n);
+      if (!g) return HT_ERR_ALLOC;
+      CK(cast_f64_f32_launch(st, G, g, (long)batch * n * n));
+      G32 = g;
+    }
   }
   std::vector<int> hconv(batch);
   std::vector<double> hwall((size_t)batch * 64), hres(batch);
@@ -371,7 +375,8 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     // few powers (its extra columns are then numerically rank deficient when the spectrum
     // falls off after k)
     const int try_chol = (round > 0 || q >= 4 || b == k) ? 1 : 0;
-    CK(power_launch(st, G, (long)n * n, W, Y, nb, n, b, q, mode, cut, nullptr, batch, G32));
+    CK(power_launch(st, G, (long)n * n, round == 0 ? nullptr : W, Y, nb, n, b, q, mode, cut, nullptr, batch, G32,
+                    opt.seed, stream));
     CK(qr_launch(st, Y, Q, nb, n, b, try_chol, batch));
     CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, T, batch));
     // the last replayed round leaves its convergence flags in the verdict slots
@@ -1120,8 +1125,9 @@ int Engine::small_side_svd(const double* C, PMS v, int k, double* U, const doubl
   if (planned == 0) return 0;
   CK(gram_pms_f64_launch(st, C, vs, H));
   CK(jacobi_full_launch(st, H, (long)c * c, c, wh, c, Vh, (long)c * c, 1, info, 30, 1e-14));
+  int* okslot = nullptr;  // the replayed rank test, written by the column-finishing kernel
   if (planned == 1) {
-    CK(ratio_check_launch(st, wh, k, 1e-24, verdict + n_verdict));
+    okslot = verdict + n_verdict;
     ++n_verdict;
   } else {
     std::vector<double> hw(c);
@@ -1134,10 +1140,10 @@ int Engine::small_side_svd(const double* C, PMS v, int k, double* U, const doubl
   // canonical signs (one launch, transposing the (k, M) product of the S == 1 case).
   if (v.P == 1) {
     CK(proj_pms_f64_launch(st, C, vs, Vh, k, U));  // (M, k) column-major
-    CK(cols_unit_sign_launch(st, U, 1, M, U, (int)M, k, (int)M, wh));
+    CK(cols_unit_sign_launch(st, U, 1, M, U, (int)M, k, (int)M, wh, okslot));
   } else {
     CK(proj_pms_f64_launch(st, C, vs, Vh, k, AV));  // (k, M)
-    CK(cols_unit_sign_launch(st, AV, k, 1, U, (int)M, k, (int)M, wh));
+    CK(cols_unit_sign_launch(st, AV, k, 1, U, (int)M, k, (int)M, wh, okslot));
   }
   *w = wh;
   ++eig_calls;
@@ -1179,12 +1185,13 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
     CK(small_side_svd(C, v, k, V, &wsv, &small));
     if (!small) {
       double* G = alloc<double>((size_t)M * M);
-      if (!G) return HT_ERR_ALLOC;
-      CK(gram_pms_f64_launch(st, C, v, G));
+      float* G32 = alloc<float>((size_t)M * M);  // the filter steps' FP32 copy, written with G
+      if (!G || !G32) return HT_ERR_ALLOC;
+      CK(gram_pms_f64_launch(st, C, v, G, G32));
       // An interior Gram's wanted spectrum is typically flat (the leaf core's unfolding):
       // a 32-column block (a multiple of 16 keeps the MFMA CholeskyQR; 24 columns with 16
       // first powers measured slower) and 8 plain powers first; two rounds usually meet 1e-4.
-      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-4, 8, 32, 4000 + t * 16, 1e-12));
+      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-4, 8, 32, 4000 + t * 16, 1e-12, G32));
     }
     CK(eig_to_sv(wsv, t));
     Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself

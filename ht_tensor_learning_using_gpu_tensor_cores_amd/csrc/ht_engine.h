@@ -127,8 +127,9 @@ struct Engine {
   // own accumulation error resolves, where any basis of the cluster serves).
   int sym_topk(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
                double floor);
+  // G32in (optional): an FP32 copy of G for the filter steps (used when tol >= 1e-5).
   int sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
-                     uint32_t stream, double floor);
+                     uint32_t stream, double floor, const float* G32in = nullptr);
   int sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
                      double floor);
   int tsqr(float* Y, long m, int b, float* Q, double* R);

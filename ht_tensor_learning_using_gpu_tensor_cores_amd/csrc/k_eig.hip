@@ -727,7 +727,8 @@ int ritz_resid_launch(hipStream_t st, const double* GU, const double* U, const d
 // reduction steps k0 = 4 (w + 4 u), eight per batch with all sixteen loads issued before
 // the products (indices past K are clamped and their values zeroed); the four partial
 // tiles are added through LDS and the tile is written with its mirror.
-__global__ __launch_bounds__(256) void gram_pms_f64_kernel(const double* __restrict__ x, PMS v, double* __restrict__ G) {
+__global__ __launch_bounds__(256) void gram_pms_f64_kernel(const double* __restrict__ x, PMS v, double* __restrict__ G,
+                                                           float* __restrict__ G32) {
   const int I = blockIdx.x, J = blockIdx.y;
   if (I > J) return;
   __shared__ double part[4][256];
@@ -767,11 +768,15 @@ __global__ __launch_bounds__(256) void gram_pms_f64_kernel(const double* __restr
     const double sum = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
     G[gi + M * gj] = sum;
     G[gj + M * gi] = sum;
+    if (G32) {
+      G32[gi + M * gj] = (float)sum;
+      G32[gj + M * gi] = (float)sum;
+    }
   }
 }
-int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G) {
+int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G, float* G32) {
   const unsigned nt = (unsigned)((v.M + 15) / 16);
-  hipLaunchKernelGGL(gram_pms_f64_kernel, dim3(nt, nt), dim3(256), 0, st, x, v, G);
+  hipLaunchKernelGGL(gram_pms_f64_kernel, dim3(nt, nt), dim3(256), 0, st, x, v, G, G32);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
@@ -814,25 +819,17 @@ int sqrt_clamp_launch(hipStream_t st, const double* w, int k, double* s) {
   hipLaunchKernelGGL(sqrt_clamp_kernel, dim3((k + 255) / 256), dim3(256), 0, st, w, k, s);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
-// ok[0] = 1 when w[0] > 0 and w[k-1] > ratio * w[0] (the rank test of a small-side SVD,
-// evaluated on the device when the host does not read the eigenvalues).
-__global__ void ratio_check_kernel(const double* __restrict__ w, int k, double ratio, int* __restrict__ ok) {
-  if (threadIdx.x == 0) ok[0] = (w[0] > 0.0 && w[k - 1] > ratio * w[0]) ? 1 : 0;
-}
-int ratio_check_launch(hipStream_t st, const double* w, int k, double ratio, int* ok) {
-  if (k < 1) return 8;
-  hipLaunchKernelGGL(ratio_check_kernel, dim3(1), dim3(64), 0, st, w, k, ratio, ok);
-  return hipGetLastError() == hipSuccess ? 0 : 5;
-}
 // U[:, j] = A[:, j] / sqrt(max(w_j, 0)) (zero where that is 0) with the canonical sign (its
 // largest-magnitude entry, the first on ties, made positive); A[r, j] is read at
 // in[r rs + j cs], which may alias U. One block per column, m <= 4096: the column is
 // held in registers before anything is written.
 __global__ __launch_bounds__(256) void cols_unit_sign_kernel(const double* in, long rs, long cs, double* U, int m,
-                                                             int ldu, const double* __restrict__ w) {
+                                                             int ldu, const double* __restrict__ w, int k,
+                                                             int* __restrict__ ok) {
   __shared__ double bv[4], bs[4];
   __shared__ int bix[4];
   const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (ok && j == 0 && tid == 0) *ok = (w[0] > 0.0 && w[k - 1] > 1e-24 * w[0]) ? 1 : 0;
   const double sw = sqrt(fmax(w[j], 0.0));
   double x[16];
   double best = -1.0, sgn = 1.0;
@@ -882,9 +879,9 @@ __global__ __launch_bounds__(256) void cols_unit_sign_kernel(const double* in, l
   }
 }
 int cols_unit_sign_launch(hipStream_t st, const double* in, long rs, long cs, double* U, int m, int k, int ldu,
-                          const double* w) {
+                          const double* w, int* ok) {
   if (m < 1 || m > 4096 || k < 1) return 8;
-  hipLaunchKernelGGL(cols_unit_sign_kernel, dim3(k), dim3(256), 0, st, in, rs, cs, U, m, ldu, w);
+  hipLaunchKernelGGL(cols_unit_sign_kernel, dim3(k), dim3(256), 0, st, in, rs, cs, U, m, ldu, w, k, ok);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 __global__ void scale_cols_inv_kernel(double* __restrict__ A, int m, int n, const double* __restrict__ s) {

@@ -36,11 +36,39 @@ __global__ __launch_bounds__(1024) void jacobi_full_kernel(const double* __restr
   A += bidx * sA;
   w += bidx * sw;
   V += bidx * sV;
-  const int np = (n + 1) & ~1;
-  jacobi_schedule(sched, np);
   double* a = a0;
-  jacobi_load(a, v, A, n, n, np, false);
-  const int conv = jacobi_sweeps(a, v, n, np, max_sweeps, tol, sched);
+  int conv;
+  int np;
+  if (n <= 16) {
+    // padded to 16 (decoupled diagonal entries below every eigenvalue) for the one-wave
+    // register Jacobi
+    np = 16;
+    for (int e = tid; e < 256; e += SNT) {
+      const int i = e % 16, j = e / 16;
+      a[i * SJLD + j] = (i < n && j < n) ? A[i + (long)n * j] : 0.0;
+      v[i * SJLD + j] = i == j ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (n < 16 && tid == 0) {
+      double m = 0.0;
+      for (int i = 0; i < n; ++i) m = fmax(m, fabs(a[i * SJLD + i]));
+      for (int i = n; i < 16; ++i) a[i * SJLD + i] = -(m * (double)n + 1.0) * 4.0;
+    }
+    __syncthreads();
+    const double pad = n < 16 ? a[15 * SJLD + 15] : -INFINITY;
+    __shared__ int conv_s;
+    if (tid < 64) {
+      const int c = jacobi16_wave(a, v, max_sweeps, tol, pad);
+      if (tid == 0) conv_s = c;
+    }
+    __syncthreads();
+    conv = conv_s;
+  } else {
+    np = (n + 1) & ~1;
+    jacobi_schedule(sched, np);
+    jacobi_load(a, v, A, n, n, np, false);
+    conv = jacobi_sweeps(a, v, n, np, max_sweeps, tol, sched);
+  }
   jacobi_order(a, np, order);
   for (int j = wv; j < n; j += SW) {
     const int src = order[j];
@@ -80,7 +108,8 @@ __global__ __launch_bounds__(1024) void jacobi_full_kernel(const double* __restr
 template <int PW_COLS, typename TG>
 __global__ __launch_bounds__(1024) void power_kernel(const TG* __restrict__ G, long sG, const double* __restrict__ W,
                                                      double* __restrict__ Y, long sWY, int n, int b, int q, int mode,
-                                                     const double* __restrict__ cut, double* __restrict__ T) {
+                                                     const double* __restrict__ cut, double* __restrict__ T,
+                                                     uint64_t gseed, uint32_t gstream) {
   __shared__ __attribute__((aligned(16))) double cur[256 * PW_COLS];
   __shared__ double prv[256 * PW_COLS];
   __shared__ double part[8][256 * PW_COLS];
@@ -90,7 +119,8 @@ __global__ __launch_bounds__(1024) void power_kernel(const TG* __restrict__ G, l
   const int c0 = blockIdx.x * PW_COLS, bidx = blockIdx.y;
   const int nc = min(PW_COLS, b - c0);
   G += bidx * sG;
-  W += bidx * sWY;
+  const bool gen = W == nullptr;  // start block generated here (Philox normals)
+  if (!gen) W += bidx * sWY;
   Y += bidx * sWY;
   double e = 0.0;
   if (mode == 2) {
@@ -99,7 +129,10 @@ __global__ __launch_bounds__(1024) void power_kernel(const TG* __restrict__ G, l
   }
   for (int x = tid; x < 256 * PW_COLS; x += 1024) {
     const int rr = x / PW_COLS, c = x % PW_COLS;
-    cur[x] = (rr < n && c < nc) ? W[(long)rr * b + c0 + c] : 0.0;
+    cur[x] = (rr < n && c < nc)
+                 ? (gen ? philox_normal(gseed, gstream, (uint64_t)(bidx * sWY + (long)rr * b + c0 + c))
+                        : W[(long)rr * b + c0 + c])
+                 : 0.0;
     prv[x] = 0.0;
   }
   __syncthreads();
@@ -857,15 +890,18 @@ int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* 
 // by the work-group's stream of G and its FMAs, not by the grid). G32, when given, is an
 // FP32 copy of G that modes 0 and 2 read instead (half the bytes); mode 1 always reads G.
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
-                 int mode, const double* cut, double* T, int batch, const float* G32) {
+                 int mode, const double* cut, double* T, int batch, const float* G32, uint64_t gseed,
+                 uint32_t gstream) {
   if (n < 1 || n > 256 || b < 1 || b > 64 || q < 1 || mode < 0 || mode > 2 || batch < 1) return 8;
   if (mode == 2 && !cut) return 8;
-  if (T && (mode != 1 || q != 1)) return 8;
+  if (T && (mode != 1 || q != 1 || !W)) return 8;
   const dim3 grid((b + 1) / 2, batch);
   if (G32 && mode != 1)
-    hipLaunchKernelGGL((power_kernel<2, float>), grid, dim3(1024), 0, st, G32, sG, W, Y, sWY, n, b, q, mode, cut, T);
+    hipLaunchKernelGGL((power_kernel<2, float>), grid, dim3(1024), 0, st, G32, sG, W, Y, sWY, n, b, q, mode, cut, T,
+                       gseed, gstream);
   else
-    hipLaunchKernelGGL((power_kernel<2, double>), grid, dim3(1024), 0, st, G, sG, W, Y, sWY, n, b, q, mode, cut, T);
+    hipLaunchKernelGGL((power_kernel<2, double>), grid, dim3(1024), 0, st, G, sG, W, Y, sWY, n, b, q, mode, cut, T,
+                       gseed, gstream);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
