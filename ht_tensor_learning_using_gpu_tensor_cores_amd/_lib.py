@@ -95,6 +95,7 @@ SIGNATURES = {
     "ht_philox_normals": (I32, [H, U64, U32, I64, P]),
     "ht_philox_normals_host": (None, [U64, U32, I64, PD]),
     "ht_profile_enable": (I32, [H, I32]),
+    "ht_profile_select": (I32, [H, I32]),
     "ht_profile_read": (I32, [H, PD, PI64, I32, I32]),
     "ht_debug_copy": (I32, [H, P, P, I64]),
     "ht_debug_qr_fallbacks": (I32, [H, PI64]),

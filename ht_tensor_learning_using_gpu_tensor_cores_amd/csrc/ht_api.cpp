@@ -507,6 +507,12 @@ ht_status ht_profile_enable(ht_handle_t h, int32_t on) {
   return HT_OK;
 }
 
+ht_status ht_profile_select(ht_handle_t h, int32_t mask) {
+  if (!h) return HT_ERR_INVALID_ARG;
+  h->prof.mask = mask;
+  return HT_OK;
+}
+
 ht_status ht_profile_read(ht_handle_t h, double* ms, int64_t* launches, int32_t n, int32_t reset) {
   if (!h || n < 0) return HT_ERR_INVALID_ARG;
   for (int i = 0; i < n && i < PK_COUNT; ++i) {

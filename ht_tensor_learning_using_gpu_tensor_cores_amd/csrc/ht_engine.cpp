@@ -54,7 +54,7 @@ void Arena::release() {
 
 // ------------------------------------------------------------------ profiler
 int Profiler::begin(hipStream_t st, int k) {
-  if (!enabled) return -1;
+  if (!enabled || !((mask >> k) & 1)) return -1;
   if (!created) {
     for (auto& x : ev)
       if (hipEventCreate(&x) != hipSuccess) return -1;

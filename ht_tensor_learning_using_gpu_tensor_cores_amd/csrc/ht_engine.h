@@ -54,6 +54,7 @@ struct Profiler {
   int kid[NREC];
   int n = 0;
   bool created = false, enabled = false;
+  int mask = -1;  // kernel ids timed (bit 1 << id)
   double ms[PK_COUNT] = {};
   long cnt[PK_COUNT] = {};
   int begin(hipStream_t st, int k);

@@ -30,6 +30,7 @@ class Handle:
         _lib.check(self.lib.ht_create(ctypes.byref(self._h), int(device)), "ht_create")
         self.device = int(device)
         self._ext = None
+        self._ev = None
 
     def order_after_torch(self, device):
         """Queue a wait of the handle's stream for what torch's current stream of `device`
@@ -41,7 +42,9 @@ class Handle:
             return
         if self._ext is None:
             self._ext = torch.cuda.ExternalStream(self.lib.ht_get_stream(self.ptr), device=dev)
-        self._ext.wait_stream(torch.cuda.current_stream(dev))
+            self._ev = torch.cuda.Event()
+        self._ev.record(torch.cuda.current_stream(dev))
+        self._ext.wait_event(self._ev)
 
     @property
     def ptr(self):

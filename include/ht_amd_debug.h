@@ -80,6 +80,10 @@ ht_status ht_debug_qr_fallbacks(ht_handle_t h, int64_t* count);
  * reset != 0). */
 ht_status ht_profile_enable(ht_handle_t h, int32_t on);
 ht_status ht_profile_read(ht_handle_t h, double* ms, int64_t* launches, int32_t n, int32_t reset);
+/* Restrict the timing to the kernel ids whose bit (1 << id) is set in mask (default: all).
+ * Each timed launch adds two event records to the stream; timing only the passes over X
+ * keeps that cost off a throughput measurement. */
+ht_status ht_profile_select(ht_handle_t h, int32_t mask);
 
 #ifdef __cplusplus
 }
