@@ -263,13 +263,14 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
   e.pin = h->pinned;
   e.pin_bytes = h->pinned_bytes;
   e.plan = &h->plan;
+  e.done_ev = h->ev[1];  // recorded by the run before its final wait
   if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
   int s = e.run(xd, od);
   if (s && s != HT_ERR_NOCONV) {
     (void)hipStreamSynchronize(h->stream);
     return (ht_status)s;
   }
-  if (hipEventRecord(h->ev[1], h->stream) != hipSuccess) return HT_ERR_HIP;
+  if (!e.done_recorded && hipEventRecord(h->ev[1], h->stream) != hipSuccess) return HT_ERR_HIP;
   if (hipEventSynchronize(h->ev[1]) != hipSuccess) return HT_ERR_HIP;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);

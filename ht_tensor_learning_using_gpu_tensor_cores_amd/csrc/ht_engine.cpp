@@ -1507,6 +1507,10 @@ int Engine::run(const float* x, float* out) {
         return s;
       }
       HT_HIP_CHECK(hipMemcpyAsync(pin + pin_bytes / 2, vb, 32 + 4 * (size_t)n_verdict, hipMemcpyDeviceToHost, st));
+      if (done_ev) {
+        HT_HIP_CHECK(hipEventRecord(done_ev, st));
+        done_recorded = true;
+      }
       CK(sync());
       bool verified = false;
       const int f = finish_deferred(&verified);
@@ -1533,6 +1537,10 @@ int Engine::run(const float* x, float* out) {
   if (recording && s == 0 && !noconv) plan->valid = true;
   recording = false;
   if (s) return s;
+  if (done_ev) {
+    HT_HIP_CHECK(hipEventRecord(done_ev, st));
+    done_recorded = true;
+  }
   CK(sync());
   if (prof && prof->enabled) prof->collect();
   if (noconv) return HT_ERR_NOCONV;

@@ -116,6 +116,9 @@ struct Engine {
   // the pinned staging) and the verdicts after them
   bool deferred = false;
   int deferred_mode = 0;
+  // event recorded after the last work of the run, before its final wait (optional)
+  hipEvent_t done_ev = nullptr;
+  bool done_recorded = false;
 
   template <typename T>
   T* alloc(size_t n);

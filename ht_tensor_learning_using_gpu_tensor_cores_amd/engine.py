@@ -6,6 +6,7 @@ buffers are torch tensors on a ROCm device (torch is used for memory and streams
 every computation runs in libht_amd.so). Host inputs are numpy arrays.
 """
 import ctypes
+import math
 from collections.abc import Mapping
 from dataclasses import dataclass, field
 from typing import Dict, Optional
@@ -107,6 +108,7 @@ def make_options(method="ltr", precision="bf16", err="residual", oversample=8, p
 
 
 _LAYOUTS = {}
+_OPTIONS = {}  # ht_options per call signature (the C side copies them; read-only here)
 
 
 def _layout_entry(dims, tree: Tree, ranks, want_interior_u):
@@ -220,7 +222,14 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
     h = handle or default_handle()
     dims = [int(d) for d in dims]
     ranks = [int(r) for r in ranks]
-    opt = make_options(method, precision, err, want_interior_u=want_interior_u, **kw)
+    okey = (method, precision, err, bool(want_interior_u), tuple(sorted(kw.items())))
+    opt = _OPTIONS.get(okey) if "sketch" not in kw else None
+    if opt is None:
+        opt = make_options(method, precision, err, want_interior_u=want_interior_u, **kw)
+        if "sketch" not in kw:
+            if len(_OPTIONS) >= 64:
+                _OPTIONS.clear()
+            _OPTIONS[okey] = opt
     layout, targs = _layout_entry(dims, tree, ranks, want_interior_u)
     total = layout[3]
     on_dev = hasattr(x, "is_cuda") and x.is_cuda
@@ -235,7 +244,7 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
         xa = np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1))
         out = np.empty(total, dtype=np.float32)
         xptr, optr = xa.ctypes.data, out.ctypes.data
-    if int(np.prod(dims)) != (x.numel() if on_dev else xa.size):
+    if math.prod(dims) != (x.numel() if on_dev else xa.size):
         raise ValueError("x has the wrong number of elements")
     rel = ctypes.c_double()
     st = _lib.Stats()
