@@ -732,6 +732,40 @@ int Engine::expand(int t, double** out) {
   return 0;
 }
 
+// The full bases of two nodes; when both are interior their two contractions each go out
+// as one paired launch.
+int Engine::expand2(int ta, int tb, double** oa, double** ob) {
+  const Problem& p = *prob;
+  if (p.left[ta] < 0 || p.left[tb] < 0) {
+    CK(expand(ta, oa));
+    return expand(tb, ob);
+  }
+  const int ts[2] = {ta, tb};
+  double *Ul[2], *Ur[2], *T1[2], *E[2];
+  long nl[2], nr[2];
+  int kl[2], kr[2], kt[2];
+  for (int i = 0; i < 2; ++i) {
+    const int t = ts[i], l = p.left[t], r = p.right[t];
+    CK(expand(l, &Ul[i]));
+    CK(expand(r, &Ur[i]));
+    nl[i] = p.rows[l];
+    nr[i] = p.rows[r];
+    kl[i] = p.rank[l];
+    kr[i] = p.rank[r];
+    kt[i] = p.rank[t];
+    T1[i] = alloc<double>((size_t)nl[i] * kr[i] * kt[i]);
+    E[i] = alloc<double>((size_t)nl[i] * nr[i] * kt[i]);
+    if (!T1[i] || !E[i]) return HT_ERR_ALLOC;
+  }
+  CK(proj_pms_f64t_pair_launch(st, Bd[ta], PMS{1, kl[0], (long)kr[0] * kt[0]}, Ul[0], (int)nl[0], T1[0], Bd[tb],
+                               PMS{1, kl[1], (long)kr[1] * kt[1]}, Ul[1], (int)nl[1], T1[1]));
+  CK(proj_pms_f64t_pair_launch(st, T1[0], PMS{nl[0], kr[0], kt[0]}, Ur[0], (int)nr[0], E[0], T1[1],
+                               PMS{nl[1], kr[1], kt[1]}, Ur[1], (int)nr[1], E[1]));
+  *oa = E[0];
+  *ob = E[1];
+  return 0;
+}
+
 int Engine::transpose_f64(const double* A, long m, long n, double* AT) {
   // AT (n x m) via a copy with swapped strides: use dgemm with identity-free form.
   return transpose_launch(st, A, m, n, AT);
@@ -744,8 +778,7 @@ int Engine::residual(const float* x, double* err_sq, double* x_sq) {
   const long nl = p.rows[l], nr = p.rows[r];
   const int kl = p.rank[l], kr = p.rank[r];
   double *Ul, *Ur;
-  CK(expand(l, &Ul));
-  CK(expand(r, &Ur));
+  CK(expand2(l, r, &Ul, &Ur));
   // Rr = Ur B_root^T (nr x kl)
   double* Rr = alloc<double>((size_t)nr * kl);
   float* Lf = alloc<float>((size_t)nl * kl);

@@ -142,6 +142,7 @@ struct Engine {
   int contract_transfer(const double* Ut, const double* Ul, const double* Ur, long nl, long nr, int kl, int kr,
                         int kt, double* B);
   int expand(int t, double** out);
+  int expand2(int ta, int tb, double** oa, double** ob);
   int transpose_f64(const double* A, long m, long n, double* AT);
   int residual(const float* x, double* err_sq, double* x_sq);
   int leaves_from_x_plain(const float* x, double* xsq_dev);

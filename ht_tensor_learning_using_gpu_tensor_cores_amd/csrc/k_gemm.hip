@@ -323,10 +323,9 @@ __global__ void proj_pms_kernel(const TI* __restrict__ x, PMS v, const TU* __res
 // r or N are clamped and their values zeroed). A lane's n (hence p, s and the base of
 // its X column) is fixed for the whole tile.
 template <typename TI, typename TU, typename TO>
-__global__ __launch_bounds__(256) void proj_mfma_kernel(const TI* __restrict__ x, PMS v, const TU* __restrict__ U,
-                                                        long um, long uj, int r, TO* __restrict__ y, long ntn) {
+__device__ __forceinline__ void proj_mfma_tile(const TI* __restrict__ x, PMS v, const TU* __restrict__ U, long um,
+                                               long uj, int r, TO* __restrict__ y, long ntn, long wave) {
   const int lane = threadIdx.x & 63;
-  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long njt = (r + 15) / 16;
   if (wave >= njt * ntn) return;
   const long NT = wave / njt, JT = wave % njt;
@@ -366,6 +365,27 @@ __global__ __launch_bounds__(256) void proj_mfma_kernel(const TI* __restrict__ x
     }
   }
 }
+template <typename TI, typename TU, typename TO>
+__global__ __launch_bounds__(256) void proj_mfma_kernel(const TI* __restrict__ x, PMS v, const TU* __restrict__ U,
+                                                        long um, long uj, int r, TO* __restrict__ y, long ntn) {
+  proj_mfma_tile(x, v, U, um, uj, r, y, ntn, ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+}
+// Two independent FP64 projections in one launch: blocks below j0.blocks take job 0.
+struct ProjJob {
+  const double* x;
+  PMS v;
+  const double* U;
+  long um, uj;
+  int r;
+  double* y;
+  long ntn, blocks;
+};
+__global__ __launch_bounds__(256) void proj_mfma2_kernel(ProjJob j0, ProjJob j1) {
+  const bool second = blockIdx.x >= j0.blocks;
+  const ProjJob& j = second ? j1 : j0;
+  const long blk = second ? blockIdx.x - j0.blocks : blockIdx.x;
+  proj_mfma_tile(j.x, j.v, j.U, j.um, j.uj, j.r, j.y, j.ntn, (blk * blockDim.x + threadIdx.x) >> 6);
+}
 
 template <typename TI, typename TU, typename TO>
 static int proj_launch(hipStream_t st, const TI* x, PMS v, const TU* U, long um, long uj, int r, TO* y) {
@@ -393,6 +413,17 @@ int proj_pms_f64_launch(hipStream_t st, const double* x, PMS v, const double* U,
 }
 int proj_pms_f64t_launch(hipStream_t st, const double* x, PMS v, const double* Ut, int r, double* y) {
   return proj_launch(st, x, v, Ut, r, 1, r, y);
+}
+int proj_pms_f64t_pair_launch(hipStream_t st, const double* x0, PMS v0, const double* Ut0, int r0, double* y0,
+                              const double* x1, PMS v1, const double* Ut1, int r1, double* y1) {
+  ProjJob j[2] = {{x0, v0, Ut0, r0, 1, r0, y0, 0, 0}, {x1, v1, Ut1, r1, 1, r1, y1, 0, 0}};
+  for (auto& q : j) {
+    q.ntn = (q.v.P * q.v.S + 15) / 16;
+    q.blocks = (((q.r + 15) / 16) * q.ntn + 3) / 4;
+    if (q.v.P * (long)q.r * q.v.S <= 0 || q.blocks > (1L << 29)) return 8;
+  }
+  hipLaunchKernelGGL(proj_mfma2_kernel, dim3((unsigned)(j[0].blocks + j[1].blocks)), dim3(256), 0, st, j[0], j[1]);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 int proj_pms_f32in_f64_launch(hipStream_t st, const float* x, PMS v, const double* U, int r, double* y) {
   return proj_launch(st, x, v, U, 1, v.M, r, y);
