@@ -86,27 +86,6 @@ __device__ __forceinline__ void jacobi_rotation(double app, double aqq, double a
   s = t * c;
 }
 
-// A rotation from an FP32 tangent: t to ~1e-7 relative from the single-precision
-// reciprocal and square root, then c = 1/sqrt(1 + t^2) and s = t c in FP64, so the rotation
-// is orthogonal to FP64 rounding and leaves ~1e-7 of a_pq (reduced again by later sweeps).
-__device__ __forceinline__ void jacobi_rotation_f32t(double app, double aqq, double apq, double& c, double& s) {
-  c = 1.0;
-  s = 0.0;
-  if (apq == 0.0) return;
-  const double tau = (aqq - app) * __builtin_amdgcn_rcp(2.0 * apq);  // ~1e-7 is all t needs
-  const float at = fabsf((float)tau);
-  float tf;
-  if (at > 1e18f) {
-    tf = 0.5f / at;
-  } else {
-    tf = 1.0f / (at + sqrtf(fmaf(at, at, 1.0f)));
-  }
-  double t = (double)tf;
-  if (tau < 0.0) t = -t;
-  c = rsq_nr(fma(t, t, 1.0));
-  s = t * c;
-}
-
 // Cyclic parallel Jacobi on the np x np symmetric matrix a (LDS, ld SJLD), accumulating
 // the rotations into v, for a block of NT threads (256 or 1024). Each step rotates np/2
 // disjoint pairs (phase A, one thread per pair) and then updates every 2x2 block (rows of
@@ -115,12 +94,9 @@ __device__ __forceinline__ void jacobi_rotation_f32t(double app, double aqq, dou
 // tol^2 * diagonal mass. The schedule must be in sched (jacobi_schedule + barrier).
 // With TAB the per-thread items are kept in `tab` (an LDS array of 3072 entries) instead
 // of in registers, which small blocks need to stay within their register budget.
-// With fast (for tolerances of 1e-5 and above) the rotations come from FP32 tangents
-// (jacobi_rotation_f32t) and the rotated entry is kept rather than set to zero.
 template <int NT = SNT, bool TAB = false>
 __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np, int max_sweeps, double tol,
-                                             const unsigned short* sched, unsigned short* tab = nullptr,
-                                             bool fast = false) {
+                                             const unsigned short* sched, unsigned short* tab = nullptr) {
   constexpr int NW = NT / 64;
   constexpr int NI = (3072 + NT - 1) / NT;  // items per thread: (np/2)^2 + np * np/2 <= 3072
   static_assert(NI <= 12, "block too small");
@@ -159,7 +135,7 @@ __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np
       const double xqP = C * aqP - S * aqQ, xqQ = S * aqP + C * aqQ;
       double ypP = c * xpP - s * xqP, yqP = s * xpP + c * xqP;
       double ypQ = c * xpQ - s * xqQ, yqQ = s * xpQ + c * xqQ;
-      if (bi == bj && s != 0.0 && !fast) {
+      if (bi == bj && s != 0.0) {
         ypQ = 0.0;
         yqP = 0.0;
       }
@@ -206,8 +182,7 @@ __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np
         const int pq = sched[step * half + tid];
         const int p = pq & 255, q = pq >> 8;
         double c, s;
-        if (fast) jacobi_rotation_f32t(a[p * SJLD + p], a[q * SJLD + q], a[p * SJLD + q], c, s);
-        else jacobi_rotation(a[p * SJLD + p], a[q * SJLD + q], a[p * SJLD + q], c, s);
+        jacobi_rotation(a[p * SJLD + p], a[q * SJLD + q], a[p * SJLD + q], c, s);
         rpq[tid] = pq;
         rcs[tid] = make_double2(c, s);
       }

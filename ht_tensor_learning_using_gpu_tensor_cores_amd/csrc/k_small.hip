@@ -775,7 +775,7 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
     if (tid < 64) jacobi16_wave(a, v, max_sweeps, jtol, pad);
     __syncthreads();
   } else {
-    jacobi_sweeps(a, v, b, np, max_sweeps, jtol, sched, nullptr, jtol >= 1e-6);
+    jacobi_sweeps(a, v, b, np, max_sweeps, jtol, sched);
   }
   jacobi_order(a, np, order);
   // Ritz vectors u = Q S[:, order[c]] and G u = Z S[:, order[c]] on this thread's rows.
