@@ -21,7 +21,7 @@ Workloads (generated on the device):
       against the untrained Gaussian sketch, then decomposes --reps held-out tensors at
       ranks min(12, width) with the randomised method (BF16, no power iteration) using the
       learned sketch (fallback above --tol), the Gaussian test matrix, and the plain method
-      (the exact judge); --save writes the trained sketch (BF16) to the given .npz
+      (the exact comparator); --save writes the trained sketch (BF16) to the given .npz
 """
 import argparse
 import time

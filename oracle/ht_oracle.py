@@ -158,7 +158,7 @@ def tree_reference_3d():
 
 
 def tree_first_vs_rest_3d():
-    """Tree {1},{2,3} used by the 3rd-order BASELINE workloads."""
+    """Tree {1},{2,3} used by the 3rd-order benchmark workloads (8^3 ranks on 128^3, 32^3 on 1024^3)."""
     return tree_from_spec((0, (1, 2)))
 
 
