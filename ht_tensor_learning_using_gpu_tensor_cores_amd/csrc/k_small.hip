@@ -326,17 +326,21 @@ int qr_fallback_count(unsigned long long* out) {
 // zeros below) is in R and X (upper) in B; returns 0 (uniformly) when a pivot is not
 // above piv_tol.
 __device__ int chol_rinv_block(double* B, double* R, int b, double piv_tol) {
+  __shared__ double rdiag_inv[SJ];  // 1 / R[k][k] of the steps done
   const int tid = threadIdx.x;
   const int jx = tid >> 4, part = tid & 15;
   for (int k = 0; k < b; ++k) {
     const double d = B[k * SJLD + k];
     if (!(d > piv_tol)) return 0;
-    const double rkk = sqrt(d), rinv = 1.0 / rkk, dinv = 1.0 / d;
+    // 1/sqrt(d) from v_rsq_f64 and two Newton steps (~1 ulp): no divide or square root
+    // on the step's dependent path
+    const double rinv = rsq_nr(d), rkk = d * rinv, dinv = rinv * rinv;
+    if (tid == 0) rdiag_inv[k] = rinv;
     {
       double t = 0.0;
       if (jx < k)
         for (int i = jx + part; i < k; i += 16)
-          t += (i == jx ? 1.0 / R[jx * SJLD + jx] : B[jx * SJLD + i]) * R[i * SJLD + k];
+          t += (i == jx ? rdiag_inv[jx] : B[jx * SJLD + i]) * R[i * SJLD + k];
 #pragma unroll
       for (int o = 8; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
       if (part == 0 && jx < k) B[jx * SJLD + k] = -t * rinv;
@@ -356,7 +360,7 @@ __device__ int chol_rinv_block(double* B, double* R, int b, double piv_tol) {
   for (int e = tid; e < b * b; e += SNT) {
     const int r = e / b, c = e % b;
     if (r > c) B[r * SJLD + c] = 0.0;
-    else if (r == c) B[r * SJLD + c] = 1.0 / R[r * SJLD + r];
+    else if (r == c) B[r * SJLD + c] = rdiag_inv[r];
   }
   __syncthreads();
   return 1;
