@@ -41,10 +41,13 @@ class Handle:
         if dev.index is not None and dev.index != self.device:
             torch.cuda.synchronize(dev)
             return
+        cs = torch.cuda.current_stream(dev)
+        if cs.query():  # nothing queued there: no ordering needed
+            return
         if self._ext is None:
             self._ext = torch.cuda.ExternalStream(self.lib.ht_get_stream(self.ptr), device=dev)
             self._ev = torch.cuda.Event()
-        self._ev.record(torch.cuda.current_stream(dev))
+        self._ev.record(cs)
         self._ext.wait_event(self._ev)
 
     @property
