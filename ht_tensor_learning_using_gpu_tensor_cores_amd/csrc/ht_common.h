@@ -209,7 +209,6 @@ int ritz_resid_launch(hipStream_t st, const double* GU, const double* U, const d
 // G32 (optional): an FP32 copy of G written alongside.
 int gram_pms_f64_launch(hipStream_t st, const double* x, PMS v, double* G, float* G32 = nullptr);
 int sum_strided_launch(hipStream_t st, const double* acc, int n, double* out);
-int transpose_launch(hipStream_t st, const double* A, long m, long n, double* AT);
 int sqrt_clamp_launch(hipStream_t st, const double* w, int k, double* s);
 // One FP64 -> FP32 copy of cast_multi_launch (sqrt_clamp: sqrt(max(x, 0)) before rounding).
 struct CastItem {

@@ -766,11 +766,6 @@ int Engine::expand2(int ta, int tb, double** oa, double** ob) {
   return 0;
 }
 
-int Engine::transpose_f64(const double* A, long m, long n, double* AT) {
-  // AT (n x m) via a copy with swapped strides: use dgemm with identity-free form.
-  return transpose_launch(st, A, m, n, AT);
-}
-
 // ||X - X_HT||^2 via one low-rank residual pass over X_(root-left) (N_l x N_r).
 int Engine::residual(const float* x, double* err_sq, double* x_sq) {
   const Problem& p = *prob;

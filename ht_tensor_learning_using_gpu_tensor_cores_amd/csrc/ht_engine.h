@@ -143,7 +143,6 @@ struct Engine {
                         int kt, double* B);
   int expand(int t, double** out);
   int expand2(int ta, int tb, double** oa, double** ob);
-  int transpose_f64(const double* A, long m, long n, double* AT);
   int residual(const float* x, double* err_sq, double* x_sq);
   int leaves_from_x_plain(const float* x, double* xsq_dev);
   int leaves_from_x(const float* x, int prec, double* xsq_dev);

@@ -798,18 +798,6 @@ int sum_strided_launch(hipStream_t st, const double* acc, int n, double* out) {
   hipLaunchKernelGGL(sum_pairs_kernel, dim3(1), dim3(256), 0, st, acc, n, out);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
-__global__ void transpose_f64_kernel(const double* __restrict__ A, long m, long n, double* __restrict__ AT) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m * n) return;
-  const long i = e % m, j = e / m;
-  AT[j + n * i] = A[e];
-}
-int transpose_launch(hipStream_t st, const double* A, long m, long n, double* AT) {
-  const long tot = m * n;
-  if (tot <= 0) return 0;
-  hipLaunchKernelGGL(transpose_f64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A, m, n, AT);
-  return hipGetLastError() == hipSuccess ? 0 : 5;
-}
 __global__ void sqrt_clamp_kernel(const double* __restrict__ w, int k, double* __restrict__ s) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < k) s[i] = sqrt(fmax(w[i], 0.0));
