@@ -414,6 +414,9 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
                 hwall[k - 1] / hwall[0], hwall[b - 1] / hwall[0], fb - fb_prev);
         fb_prev = fb;
       }
+      // a non-finite Gram (NaN or Inf in the input) never converges: report it at once
+      for (int i = 0; i < batch; ++i)
+        if (!std::isfinite(hwall[(size_t)i * 64]) || !std::isfinite(hres[i])) return HT_ERR_NONFINITE;
       bool all = true;
       for (int i = 0; i < batch; ++i) all = all && hconv[i];
       if (recording) plan->calls[rec].rounds.push_back(EigRound{q, mode, b, try_chol, 0});
@@ -551,6 +554,7 @@ int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, 
     CK(ritz_resid_launch(st, YS, QS, th, n, k, res));
     CK(d2h(hres.data(), res, sizeof(double) * k));
     CK(d2h(hth.data(), th, sizeof(double) * b));
+    if (!std::isfinite(hth[0])) return HT_ERR_NONFINITE;  // a NaN/Inf Gram never converges
     const double scale = std::max(std::fabs(hth[0]), 1e-300);
     double rmax = 0.0;
     for (int i = 0; i < k; ++i)
@@ -838,6 +842,7 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
     for (int i = 0; i < nb; ++i) {
       const int m = p.lo[grp[i]];
       CK(gram(x, view_of(p.dims, p.order, m, m + 1), prec, G + (size_t)i * n * n, have_norm ? nullptr : xsq_dev));
+      if (!have_norm) CK(early_finite_check(xsq_dev));
       have_norm = true;
     }
     CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
@@ -847,6 +852,18 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
       done[grp[i]] = 1;
     }
   }
+  return 0;
+}
+
+// With check_finite, a run that reads the device between its steps checks ||X||^2 from
+// the first pass over X before any eigen problem is solved: a NaN or Inf in X makes it
+// non-finite. A replayed run reads nothing until its end and checks it there, before its
+// verdicts (a non-finite input then returns at once, with no repeat).
+int Engine::early_finite_check(const double* xsq_dev) {
+  if (!opt.check_finite || replay || deferred) return 0;
+  double v = 0.0;
+  CK(d2h(&v, xsq_dev, sizeof(double)));
+  if (!std::isfinite(v)) return HT_ERR_NONFINITE;
   return 0;
 }
 
@@ -1027,6 +1044,7 @@ int Engine::run_ltr(const float* x) {
     CK(gram(x, view_of(p.dims, d, 0, 1), prec, G0, xsq));
     CK(gram(x, view_of(p.dims, d, 1, 2), prec, G1, nullptr));
   }
+  CK(early_finite_check(xsq));
   Ufull[t0] = alloc<double>((size_t)n0 * r0 + (size_t)n1 * r1);
   Ufull[t1] = Ufull[t0] + (size_t)n0 * r0;
   double* w0 = alloc<double>(r0 + r1);
@@ -1486,11 +1504,19 @@ static std::vector<int64_t> plan_key(const Problem& p, const ht_options& o, cons
 int Engine::finish_deferred(bool* verified) {
   char* hb = pin + pin_bytes / 2;
   const int* vf = reinterpret_cast<const int*>(hb + 32);
+  double head[2];
+  std::memcpy(head, hb, sizeof(head));
+  // a non-finite input fails the verdicts too; it is reported as such, without a repeat
+  const double xs = deferred_mode == HT_ERR_MODE_RESIDUAL ? head[1] : head[0];
+  if (!std::isfinite(xs)) {
+    *verified = true;
+    x_sq = xs;
+    rel_err = NAN;
+    return HT_ERR_NONFINITE;
+  }
   *verified = true;
   for (int i = 0; i < n_verdict; ++i) *verified = *verified && vf[i] == 1;
   if (!*verified) return 0;
-  double head[2];
-  std::memcpy(head, hb, sizeof(head));
   rel_err = NAN;
   x_sq = head[0];
   if (deferred_mode == HT_ERR_MODE_RESIDUAL) {

@@ -147,6 +147,7 @@ struct Engine {
   int leaves_from_x_plain(const float* x, double* xsq_dev);
   int leaves_from_x(const float* x, int prec, double* xsq_dev);
   int eig_to_sv(const double* w, int t);
+  int early_finite_check(const double* xsq_dev);
   int root_transfer(const float* x);
   int finish_error(const float* x, const double* xsq_dev);
   int run_plain(const float* x);

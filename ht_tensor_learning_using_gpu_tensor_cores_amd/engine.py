@@ -86,7 +86,8 @@ def default_handle(device=0):
 
 
 def make_options(method="ltr", precision="bf16", err="residual", oversample=8, power_iters=1, seed=1234,
-                 eig_tol=1e-10, max_eig_iters=200, want_interior_u=False, sketch=None, sketch_tol=1e-3):
+                 eig_tol=1e-10, max_eig_iters=200, want_interior_u=False, sketch=None, sketch_tol=1e-3,
+                 check_finite=1):
     """ht_options. `sketch` (a sketch.Sketch on the device) selects the learned Khatri-Rao
     test matrix of the randomised method, with the Gaussian one as its fallback above
     `sketch_tol`."""
@@ -107,6 +108,7 @@ def make_options(method="ltr", precision="bf16", err="residual", oversample=8, p
     o.eig_tol = float(eig_tol)
     o.max_eig_iters = int(max_eig_iters)
     o.want_interior_u = int(bool(want_interior_u))
+    o.check_finite = int(bool(check_finite))
     return o
 
 
@@ -270,6 +272,9 @@ def residual(x, res: HTensor, handle=None):
     h = handle or default_handle()
     tr = res.tree
     rel = ctypes.c_double()
+    # the library's stream reads x and the packed block: it waits for what torch's
+    # current stream has queued on them
+    h.order_after_torch(x.device)
     _lib.check(lib.ht_residual(h.ptr, ctypes.c_void_p(x.data_ptr()), len(res.dims), _lib.i64_array(res.dims),
                                tr.n_nodes, _lib.i32_array(tr.lo), _lib.i32_array(tr.hi), _lib.i32_array(tr.left),
                                _lib.i32_array(tr.right), _lib.i32_array(res.ranks), int(res.want_interior_u),
@@ -284,6 +289,9 @@ def reconstruct(res: HTensor, handle=None):
     h = handle or default_handle()
     tr = res.tree
     out = torch.empty(int(np.prod(res.dims)), dtype=torch.float32, device=res.packed.device)
+    # the library's stream writes `out` (a block torch's allocator may have just recycled
+    # from work still queued on its current stream) and reads the packed block
+    h.order_after_torch(res.packed.device)
     _lib.check(lib.ht_reconstruct(h.ptr, len(res.dims), _lib.i64_array(res.dims), tr.n_nodes,
                                   _lib.i32_array(tr.lo), _lib.i32_array(tr.hi), _lib.i32_array(tr.left),
                                   _lib.i32_array(tr.right), _lib.i32_array(res.ranks), int(res.want_interior_u),

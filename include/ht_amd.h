@@ -87,7 +87,12 @@ typedef struct {
   double eig_tol;          /* relative Ritz-value tolerance of the exact solvers (1e-10) */
   uint64_t seed;           /* seed of the Philox test matrices (1234)                    */
   int32_t want_interior_u; /* also return U_t of interior non-root nodes                 */
-  int32_t check_finite;    /* scan the input for NaN/Inf first (default 1)               */
+  /* NaN/Inf in X make ||X||^2 (formed by the first pass over X) non-finite, and the call
+   * returns HT_ERR_NONFINITE. check_finite = 1 (default): a run that reads the device
+   * between its steps checks ||X||^2 right after that first pass, before any eigen problem;
+   * a replayed leaves-to-root run (it reads the device once, at its end) checks it there,
+   * before anything else. 0: only the end of the run checks. */
+  int32_t check_finite;
   /* Test matrix of HT_METHOD_RSVD (ht_sketch_kind). With HT_SKETCH_LEARNED the range
    * finder of node t uses Omega_t = Khatri-Rao product of the per-mode sketches of t's
    * column modes (width sketch_width, which replaces k + oversample), and an FP32
@@ -288,6 +293,26 @@ void htd4_tc(float* x, long a, long b, long c, long d, int* k, void* blas_handle
 const char* ht_version(void);
 
 #ifdef __cplusplus
+}
+
+/* C++ overloads with the exact signatures the reference's optimized drivers declare and
+ * call, so that their call lines compile unchanged against this header:
+ *   error = htd(X,a,b,c,k);              third_order_tensor_decompsition/CUDA/Optimized/ht_init.cu:47
+ *   htd(X,a,b,c,k);                      third_order_tensor_decompsition/CUDA/TSQR/ht_init.cu:35
+ *   htd4(X,a,b,c,d,k,handle,cusolverH);  fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/ht_init.cu:53
+ * (the C names above are unchanged: these are inline and add no exported symbol). */
+
+/* float htd(dt*,long,long,long,int*): third_order_tensor_decompsition/CUDA/Optimized/function.h:6
+ * (and the void variant of third_order_tensor_decompsition/CUDA/TSQR/function.h:6, whose
+ * callers discard the result): the mixed-precision decomposition of htd_tc. */
+inline float htd(float* x, long a, long b, long c, int* k) { return htd_tc(x, a, b, c, k); }
+
+/* void htd4(dt*,long,long,long,long,int*,cublasHandle_t,cusolverDnHandle_t):
+ * fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/function.h:6. The two
+ * library handles are opaque pointers here (no cuBLAS/cuSOLVER on this platform) and are
+ * ignored, as htd4_tc does. */
+inline void htd4(float* x, long a, long b, long c, long d, int* k, void* blas_handle, void* solver_handle) {
+  htd4_tc(x, a, b, c, d, k, blas_handle, solver_handle);
 }
 #endif
 

@@ -321,3 +321,110 @@ def test_planted_dropped_noise_term_is_caught():
     identity = np.sqrt(max(np.sum(x ** 2) - np.sum(xw ** 2), 0)) / np.linalg.norm(x)
     assert abs(identity - O.rel_error(x, xw)) > 1e-3
     assert abs(O.rel_error(x, xr) - np.sqrt(np.sum(x ** 2) - np.sum(xr ** 2)) / np.linalg.norm(x)) < 1e-10
+
+
+# ------------------------------------------------------- randomised root-to-leaves HT
+def _rsvd_variant(old, new):
+    """A copy of oracle.rsvd_ht with one source fragment replaced (a planted mistake)."""
+    import inspect
+    src = inspect.getsource(O.rsvd_ht)
+    assert old in src
+    ns = dict(vars(O))
+    exec(compile(src.replace(old, new), "<planted rsvd_ht>", "exec"), ns)
+    return ns["rsvd_ht"]
+
+
+def _rsvd_matches_plain_on_captured_range(rsvd):
+    """When the sketched matricisation X_(12) has rank <= l = k + p the range finder spans
+    it exactly, so the randomised HT equals the exact HT (and the complementary sibling
+    {3,4} gets the exact basis from the right singular vectors)."""
+    from scipy.linalg import subspace_angles
+    tr = O.tree_balanced_4d()
+    dims = [6, 5, 5, 6]  # X_(12) square, so a sibling given the wrong side still has the right shape
+    x = G.tucker(dims, [2, 2, 2, 2], seed=3, noise_rel=0.0, dtype=np.float64)
+    ranks = [1, 3, 3, 2, 2, 2, 2]  # rank(X_(12)) = 4 > 3: the interior nodes truncate
+    e = O.plain_ht(x, dims, tr, ranks)
+    ee = O.rel_error(x, e.reconstruct())
+    ok = True
+    for q in (0, 1):
+        r = rsvd(x, dims, tr, ranks, p=8, q=q)
+        ok &= abs(O.rel_error(x, r.reconstruct()) - ee) <= 1e-12
+        for t in range(1, tr.n_nodes):
+            ok &= np.max(subspace_angles(r.full_basis(t), e.full_basis(t))) <= 1e-10
+    return ok, ee
+
+
+def test_rsvd_ht_equals_plain_when_the_sketch_captures_the_range():
+    ok, ee = _rsvd_matches_plain_on_captured_range(O.rsvd_ht)
+    assert ok
+    assert ee > 1e-2  # the interior truncation is not trivial
+
+
+def test_rsvd_ht_converges_to_plain_with_power_iterations():
+    """q power iterations drive the range finder to the dominant subspace: on a noisy
+    input the error gap to the exact HT shrinks with q and vanishes at q = 30."""
+    from scipy.linalg import subspace_angles
+    tr = O.tree_balanced_4d()
+    dims = [6, 6, 6, 6]
+    x = G.tucker(dims, [3, 3, 3, 3], seed=4, noise_rel=0.3, dtype=np.float64)
+    ranks = [1, 4, 4, 3, 3, 3, 3]
+    e = O.plain_ht(x, dims, tr, ranks)
+    ee = O.rel_error(x, e.reconstruct())
+    gaps = []
+    for q in (0, 2, 30):
+        r = O.rsvd_ht(x, dims, tr, ranks, p=2, q=q)
+        gaps.append(abs(O.rel_error(x, r.reconstruct()) - ee))
+        ang = max(np.max(subspace_angles(r.full_basis(t), e.full_basis(t))) for t in (1, 2))
+    assert gaps[0] > 1e-2 and gaps[1] < gaps[0] and gaps[2] < 1e-12
+    assert ang < 1e-10
+
+
+def test_rsvd_ht_uses_the_documented_test_matrix():
+    """At q = 0 node t's basis is the top-k left singular subspace of A projected onto
+    range(A Omega_t), Omega_t = the FP32-rounded Philox normals of stream 256 + t (the
+    engine's test matrix), formed here as P_Y A instead of Q^T A."""
+    from scipy.linalg import subspace_angles
+    tr = O.tree_balanced_4d()
+    dims = [5, 6, 4, 5]
+    x = G.tucker(dims, [3, 3, 3, 3], seed=6, noise_rel=0.5, dtype=np.float64)
+    ranks = [1, 4, 4, 3, 3, 3, 3]
+    r = O.rsvd_ht(x, dims, tr, ranks, p=3, q=0, seed=77)
+    A = O.unfold(x, dims, 0, 2)
+    l = 4 + 3
+    om = philox.normals(77, 256 + 1, A.shape[1] * l).astype(np.float32).astype(np.float64)
+    Y = A @ om.reshape((A.shape[1], l), order="F")
+    Qy, _ = np.linalg.qr(Y)
+    U, _ = O.top_left_singular(Qy @ (Qy.T @ A), 4)
+    # (U[t] is the node's own basis; full_basis(t) would be its projection onto the
+    # children's bases)
+    assert np.max(subspace_angles(r.U[1], U)) < 1e-10
+    # the sibling {3,4}: right singular vectors of the same projected problem
+    _, _, Vt = np.linalg.svd(Qy @ (Qy.T @ A), full_matrices=False)
+    assert np.max(subspace_angles(r.U[2], Vt[:4].T)) < 1e-10
+
+
+def test_planted_rsvd_sibling_left_vectors_is_caught():
+    """A sibling that takes the node's left instead of the right singular vectors fails
+    the captured-range closed form."""
+    bad = _rsvd_variant("canon_signs(Vbt.T[:, :kW])", "canon_signs((Q @ Ub)[:, :kW])")
+    ok, _ = _rsvd_matches_plain_on_captured_range(bad)
+    assert not ok
+
+
+def test_planted_rsvd_wrong_stream_is_caught(monkeypatch):
+    """A test matrix from another Philox stream fails the documented-test-matrix check."""
+    bad = _rsvd_variant("philox.normals(seed, 256 + t,", "philox.normals(seed, 257 + t,")
+    monkeypatch.setattr(O, "rsvd_ht", bad)
+    with pytest.raises(AssertionError):
+        test_rsvd_ht_uses_the_documented_test_matrix()
+
+
+def test_planted_rsvd_dropped_power_iteration_is_caught():
+    """An rSVD that ignores q fails the convergence check."""
+    bad = _rsvd_variant("for _ in range(q):", "for _ in range(0):")
+    tr = O.tree_balanced_4d()
+    dims = [6, 6, 6, 6]
+    x = G.tucker(dims, [3, 3, 3, 3], seed=4, noise_rel=0.3, dtype=np.float64)
+    ranks = [1, 4, 4, 3, 3, 3, 3]
+    ee = O.rel_error(x, O.plain_ht(x, dims, tr, ranks).reconstruct())
+    assert abs(O.rel_error(x, bad(x, dims, tr, ranks, p=2, q=30).reconstruct()) - ee) > 1e-3
