@@ -173,7 +173,7 @@ def test_c4_all_4096_members_against_oracle():
     print(f"C4: largest error gap to the oracle {worst:.2e}", flush=True)
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16", 5e-4)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16", 1e-4)])
 def test_c5_learned_sketch_128x4_width20(precision, tol):
     n, r = 128, 12
     tr = ht.tree.balanced_4d()
@@ -239,4 +239,11 @@ def test_fp16_leaves_to_root(order):
     assert abs(r16.rel_err - e_ref) <= 1e-4
     assert abs(r16.rel_err - e_exact) <= 1e-4
     rr = ht.decompose(xd, dims, tr, ranks, method="rsvd", precision="fp16")
-    assert abs(rr.rel_err - e_exact) <= 1e-4  # the sketched ranges hold the Tucker rank
+    rb = ht.decompose(xd, dims, tr, ranks, method="rsvd", precision="bf16")
+    e_rs = O.rel_error(x, O.rsvd_ht(x, dims, otree(tr), ranks, p=8, q=1, seed=1234).reconstruct())
+    print(f"  ltr fp16 {r16.rel_err:.9f} oracle {e_ref:.9f} exact {e_exact:.9f}; rsvd fp16 {rr.rel_err:.9f} "
+          f"bf16 {rb.rel_err:.9f} oracle {e_rs:.9f}", flush=True)
+    assert abs(rr.rel_err - e_rs) <= 1e-3
+    if order == 3:
+        # X_(23) has rank 8 <= k + p: the sketched range holds the signal
+        assert abs(rr.rel_err - e_exact) <= 1e-4
