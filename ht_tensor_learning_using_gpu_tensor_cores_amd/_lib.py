@@ -10,6 +10,7 @@ LIB_PATH = os.path.join(_HERE, "libht_amd.so")
 STATUS_NAMES = {
     0: "HT_OK", 1: "HT_ERR_INVALID_ARG", 2: "HT_ERR_INVALID_TREE", 3: "HT_ERR_RANK", 4: "HT_ERR_ALLOC",
     5: "HT_ERR_HIP", 6: "HT_ERR_NONFINITE", 7: "HT_ERR_NOCONV", 8: "HT_ERR_UNSUPPORTED", 9: "HT_ERR_NO_DEVICE",
+    10: "HT_ERR_WORKSPACE",
 }
 
 METHOD = {"plain": 0, "ltr": 1, "rsvd": 2}
@@ -39,6 +40,7 @@ class Stats(ctypes.Structure):
         ("total_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double), ("error_ms", ctypes.c_double),
         ("bytes_x_passes", ctypes.c_double), ("eig_iters_max", ctypes.c_int32), ("eig_calls", ctypes.c_int32),
         ("sketch_fallbacks", ctypes.c_int32), ("sketch_nodes", ctypes.c_int32), ("sketch_resid", ctypes.c_double),
+        ("workspace_bytes", ctypes.c_double),
     ]
 
 
@@ -71,6 +73,8 @@ SIGNATURES = {
     "ht_output_layout": (I32, [I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, I32, PI64, PI64, PI64, PI64]),
     "ht_decompose": (I32, [H, P, I32, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, ctypes.POINTER(Options), P, I32,
                            PD, ctypes.POINTER(Stats)]),
+    "ht_decompose_noalloc": (I32, [H, P, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, ctypes.POINTER(Options), P,
+                                   PD, ctypes.POINTER(Stats)]),
     "ht_residual": (I32, [H, P, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, I32, P, PD]),
     "ht_reconstruct": (I32, [H, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, I32, P, P]),
     "ht_generate_tucker": (I32, [H, I32, PI64, PI32, U64, D, P]),

@@ -123,6 +123,30 @@ ht_status ht::generate_from_core(ht_handle_t h, int order, const int64_t* dims, 
 }
 
 
+// ------------------------------------------------------------------ no-allocation scratch
+// The device workspace of ht_decompose_noalloc: one array of HT_NOALLOC_SCRATCH_BYTES per
+// GPU the library runs on, held until the process ends. Its host-side name is not a device
+// address: each device's address is taken once with hipGetSymbolAddress while that device
+// is current (ht_create) and kept in g_scratch[device]; host code only ever passes that.
+// Every ht_decompose_noalloc call on one GPU shares the array (the engine writes every
+// workspace word it later reads, so nothing depends on what an earlier call left there);
+// calls that overlap on one GPU race.
+#define HT_SCRATCH_DEVICES 64
+static __device__ __align__(256) char scratch[HT_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch[HT_SCRATCH_DEVICES] = {};
+
+// Address of the current device's copy of `scratch` (device `device` must be current).
+static ht_status scratch_setup(int device) {
+  if (device < 0 || device >= HT_SCRATCH_DEVICES) return HT_OK;  // no-allocation entry unavailable there
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return HT_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return HT_ERR_HIP;
+  g_scratch[device] = p;
+  return HT_OK;
+}
+
 extern "C" {
 
 const char* ht_version(void) { return "ht_amd 0.1 (gfx950)"; }
@@ -153,6 +177,7 @@ ht_status ht_create(ht_handle_t* handle, int32_t device) {
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return HT_ERR_NO_DEVICE;
   if (device < 0 || device >= n) return HT_ERR_INVALID_ARG;
   if (hipSetDevice(device) != hipSuccess) return HT_ERR_HIP;
+  if (scratch_setup(device) != HT_OK) return HT_ERR_HIP;
   ht_handle_s* h = new ht_handle_s();
   h->device = device;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -219,10 +244,13 @@ static int ensure(float** buf, size_t* have, size_t bytes) {
   return 0;
 }
 
-ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32_t order, const int64_t* dims,
-                       int32_t n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
-                       const int32_t* right, const int32_t* ranks, const ht_options* opt_in, float* out,
-                       int32_t out_on_device, double* rel_err, ht_stats* stats) {
+// The decomposition behind ht_decompose and ht_decompose_noalloc: `arena` is the handle's
+// growable workspace or a fixed block that never allocates.
+static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_device, int32_t order,
+                                const int64_t* dims, int32_t n_nodes, const int32_t* lo, const int32_t* hi,
+                                const int32_t* left, const int32_t* right, const int32_t* ranks,
+                                const ht_options* opt_in, float* out, int32_t out_on_device, double* rel_err,
+                                ht_stats* stats, Arena* arena) {
   if (!h || !x) return HT_ERR_INVALID_ARG;
   if (rel_err) *rel_err = NAN;
   ht_options opt;
@@ -253,10 +281,10 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
     CKS(ensure(&h->obuf, &h->obuf_bytes, (size_t)p.total * sizeof(float)));
     od = h->obuf;
   }
-  h->arena.reset();
+  arena->reset();
   Engine e;
   e.st = h->stream;
-  e.arena = &h->arena;
+  e.arena = arena;
   e.prob = &p;
   e.opt = opt;
   e.prof = &h->prof;
@@ -268,6 +296,7 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
   int s = e.run(xd, od);
   if (s && s != HT_ERR_NOCONV) {
     (void)hipStreamSynchronize(h->stream);
+    if (arena->fixed && arena->overflow) return HT_ERR_WORKSPACE;
     return (ht_status)s;
   }
   if (!e.done_recorded && hipEventRecord(h->ev[1], h->stream) != hipSuccess) return HT_ERR_HIP;
@@ -289,9 +318,33 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
     stats->sketch_fallbacks = e.sketch_fallbacks;
     stats->sketch_nodes = e.sketch_nodes;
     stats->sketch_resid = e.sketch_resid;
+    stats->workspace_bytes = (double)arena->peak;
   }
   if (!std::isfinite(e.x_sq)) return HT_ERR_NONFINITE;
   return (ht_status)s;
+}
+
+ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32_t order, const int64_t* dims,
+                       int32_t n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                       const int32_t* right, const int32_t* ranks, const ht_options* opt, float* out,
+                       int32_t out_on_device, double* rel_err, ht_stats* stats) {
+  if (!h) return HT_ERR_INVALID_ARG;
+  return decompose_impl(h, x, x_on_device, order, dims, n_nodes, lo, hi, left, right, ranks, opt, out,
+                        out_on_device, rel_err, stats, &h->arena);
+}
+
+ht_status ht_decompose_noalloc(ht_handle_t h, const float* x, int32_t order, const int64_t* dims, int32_t n_nodes,
+                               const int32_t* lo, const int32_t* hi, const int32_t* left, const int32_t* right,
+                               const int32_t* ranks, const ht_options* opt, float* out, double* rel_err,
+                               ht_stats* stats) {
+  if (!h || !x || !out) return HT_ERR_INVALID_ARG;
+  if (h->device < 0 || h->device >= HT_SCRATCH_DEVICES || !g_scratch[h->device]) return HT_ERR_HIP;
+  Arena fa;
+  fa.fixed = true;
+  fa.fbase = static_cast<char*>(g_scratch[h->device]);
+  fa.fsize = HT_NOALLOC_SCRATCH_BYTES;
+  return decompose_impl(h, x, 1, order, dims, n_nodes, lo, hi, left, right, ranks, opt, out, 1, rel_err, stats,
+                        &fa);
 }
 
 // Load an HT block from the packed layout into an engine (FP64 copies on the device).

@@ -25,6 +25,18 @@ int hip_fail(hipError_t e, const char* what) {
 // ------------------------------------------------------------------ arena
 void* Arena::get(size_t bytes) {
   bytes = (bytes + 255) & ~(size_t)255;
+  if (fixed) {
+    if (lin + bytes > fsize) {
+      overflow = true;
+      return nullptr;
+    }
+    void* p = fbase + lin;
+    lin += bytes;
+    peak = std::max(peak, lin);
+    return p;
+  }
+  lin += bytes;
+  peak = std::max(peak, lin);
   while (cur < chunks.size()) {
     if (off + bytes <= chunks[cur].second) {
       void* p = (char*)chunks[cur].first + off;
@@ -45,9 +57,13 @@ void* Arena::get(size_t bytes) {
 void Arena::reset() {
   cur = 0;
   off = 0;
+  lin = 0;
+  peak = 0;
+  overflow = false;
 }
 void Arena::release() {
-  for (auto& c : chunks) (void)hipFree(c.first);
+  if (!fixed)
+    for (auto& c : chunks) (void)hipFree(c.first);
   chunks.clear();
   reset();
 }

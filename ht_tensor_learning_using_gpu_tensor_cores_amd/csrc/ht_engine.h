@@ -11,17 +11,30 @@ namespace ht {
 
 // Bump allocator over persistent device chunks; reset() rewinds without freeing, so a
 // repeated call of the same shape allocates nothing after the first.
+//
+// A fixed arena serves one caller-provided block (fbase, fsize) and never allocates: a
+// request beyond it returns nullptr and sets `overflow`.
 struct Arena {
   std::vector<std::pair<void*, size_t>> chunks;
   size_t cur = 0, off = 0;
+  // bytes handed out since reset() as if from one contiguous block, and their maximum:
+  // the size a fixed arena needs for the same sequence of requests
+  size_t lin = 0, peak = 0;
+  char* fbase = nullptr;
+  size_t fsize = 0;
+  bool fixed = false, overflow = false;
   void* get(size_t bytes);
   void reset();
   void release();
   // Rewind to an earlier position: what was handed out after `mark()` is reused.
-  std::pair<size_t, size_t> mark() const { return {cur, off}; }
-  void rewind(std::pair<size_t, size_t> m) {
-    cur = m.first;
-    off = m.second;
+  struct Mark {
+    size_t cur, off, lin;
+  };
+  Mark mark() const { return {cur, off, lin}; }
+  void rewind(Mark m) {
+    cur = m.cur;
+    off = m.off;
+    lin = m.lin;
   }
 };
 

@@ -261,7 +261,7 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
                   stats=dict(total_ms=st.total_ms, h2d_ms=st.h2d_ms, passes=st.bytes_x_passes,
                              eig_iters_max=st.eig_iters_max, eig_calls=st.eig_calls,
                              sketch_fallbacks=st.sketch_fallbacks, sketch_nodes=st.sketch_nodes,
-                             sketch_resid=st.sketch_resid))
+                             sketch_resid=st.sketch_resid, workspace_bytes=st.workspace_bytes))
     _unpack(res, layout)
     return res
 

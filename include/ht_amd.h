@@ -51,7 +51,9 @@ typedef enum {
   HT_ERR_NONFINITE = 6,     /* the input holds NaN or Inf                             */
   HT_ERR_NOCONV = 7,        /* an eigensolver did not reach its tolerance             */
   HT_ERR_UNSUPPORTED = 8,   /* a shape this build does not handle                     */
-  HT_ERR_NO_DEVICE = 9      /* no usable GPU                                          */
+  HT_ERR_NO_DEVICE = 9,     /* no usable GPU                                          */
+  HT_ERR_WORKSPACE = 10     /* ht_decompose_noalloc: the problem needs more workspace than
+                               HT_NOALLOC_SCRATCH_BYTES                               */
 } ht_status;
 
 /* Method selection. */
@@ -121,6 +123,7 @@ typedef struct {
   int32_t sketch_fallbacks;/* nodes redone with the Gaussian test matrix                 */
   int32_t sketch_nodes;    /* nodes whose range finder used the learned sketch first     */
   double sketch_resid;     /* largest range-residual estimate of the learned sketch      */
+  double workspace_bytes;  /* device workspace the call used (one contiguous block)      */
 } ht_stats;
 
 typedef struct ht_handle_s* ht_handle_t;
@@ -162,6 +165,23 @@ ht_status ht_decompose(ht_handle_t handle, const float* x, int32_t x_on_device, 
                        const int32_t* left, const int32_t* right, const int32_t* ranks,
                        const ht_options* opt, float* out, int32_t out_on_device, double* rel_err,
                        ht_stats* stats);
+
+/* ht_decompose without any device allocation. No counterpart in the reference (whose
+ * drivers cudaMalloc and cudaFree on every call): it is for callers that cannot allocate
+ * device memory, such as code under HIP graph capture, tight serving loops and harnesses
+ * that forbid hipMalloc. x and out are device pointers (as x_on_device = out_on_device = 1
+ * of ht_decompose); the workspace is a fixed library array of HT_NOALLOC_SCRATCH_BYTES
+ * per GPU, held until the process ends (its address per device is taken by ht_create).
+ * A problem whose workspace does not fit returns HT_ERR_WORKSPACE (ht_stats.
+ * workspace_bytes of an ht_decompose call of the same problem gives its need; the 4th-order
+ * 128^4 rank-16 leaves-to-root decomposition fits). Every call on one GPU shares that
+ * array: calls that overlap on one GPU (through different handles) race. Results equal
+ * ht_decompose's. */
+#define HT_NOALLOC_SCRATCH_BYTES ((size_t)160 << 20)
+ht_status ht_decompose_noalloc(ht_handle_t handle, const float* x, int32_t order, const int64_t* dims,
+                               int32_t n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                               const int32_t* right, const int32_t* ranks, const ht_options* opt, float* out,
+                               double* rel_err, ht_stats* stats);
 
 /* ||X - X_HT||_F / ||X||_F in one fused device pass for an HT block in the packed layout
  * (the reconstruction + error of third_order_tensor_decompsition/CUDA/Baseline/htd_unopt.cu:202-266
