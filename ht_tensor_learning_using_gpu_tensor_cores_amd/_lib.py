@@ -40,7 +40,7 @@ class Stats(ctypes.Structure):
         ("total_ms", ctypes.c_double), ("h2d_ms", ctypes.c_double), ("error_ms", ctypes.c_double),
         ("bytes_x_passes", ctypes.c_double), ("eig_iters_max", ctypes.c_int32), ("eig_calls", ctypes.c_int32),
         ("sketch_fallbacks", ctypes.c_int32), ("sketch_nodes", ctypes.c_int32), ("sketch_resid", ctypes.c_double),
-        ("workspace_bytes", ctypes.c_double),
+        ("workspace_bytes", ctypes.c_double), ("replayed", ctypes.c_int32),
     ]
 
 
@@ -73,6 +73,8 @@ SIGNATURES = {
     "ht_output_layout": (I32, [I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, I32, PI64, PI64, PI64, PI64]),
     "ht_decompose": (I32, [H, P, I32, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, ctypes.POINTER(Options), P, I32,
                            PD, ctypes.POINTER(Stats)]),
+    "ht_decompose_many": (I32, [H, I32, P, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, ctypes.POINTER(Options), P,
+                                PD, I32, ctypes.POINTER(Stats)]),
     "ht_decompose_noalloc": (I32, [H, P, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, ctypes.POINTER(Options), P,
                                    PD, ctypes.POINTER(Stats)]),
     "ht_residual": (I32, [H, P, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, I32, P, PD]),

@@ -4,7 +4,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <chrono>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/ht_amd_debug.h"
@@ -197,6 +200,8 @@ ht_status ht_create(ht_handle_t* handle, int32_t device) {
 
 ht_status ht_destroy(ht_handle_t h) {
   if (!h) return HT_ERR_INVALID_ARG;
+  for (ht_handle_s* w : h->workers) (void)ht_destroy(w);
+  h->workers.clear();
   (void)hipSetDevice(h->device);
   (void)hipStreamSynchronize(h->stream);
   if (h->bexec) (void)hipGraphExecDestroy(h->bexec);
@@ -319,6 +324,7 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
     stats->sketch_nodes = e.sketch_nodes;
     stats->sketch_resid = e.sketch_resid;
     stats->workspace_bytes = (double)arena->peak;
+    stats->replayed = e.replayed;
   }
   if (!std::isfinite(e.x_sq)) return HT_ERR_NONFINITE;
   return (ht_status)s;
@@ -331,6 +337,86 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
   if (!h) return HT_ERR_INVALID_ARG;
   return decompose_impl(h, x, x_on_device, order, dims, n_nodes, lo, hi, left, right, ranks, opt, out,
                         out_on_device, rel_err, stats, &h->arena);
+}
+
+ht_status ht_decompose_many(ht_handle_t h, int32_t count, const float* const* xs, int32_t order, const int64_t* dims,
+                            int32_t n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                            const int32_t* right, const int32_t* ranks, const ht_options* opt, float* const* outs,
+                            double* rel_errs, int32_t inflight, ht_stats* stats) {
+  if (!h || count < 0 || inflight < 1 || inflight > HT_MAX_INFLIGHT) return HT_ERR_INVALID_ARG;
+  if (count > 0 && (!xs || !outs)) return HT_ERR_INVALID_ARG;
+  for (int i = 0; i < count; ++i)
+    if (!xs[i] || !outs[i]) return HT_ERR_INVALID_ARG;
+  {
+    ht_options o;
+    if (opt) o = *opt;
+    else ht_default_options(&o);
+    Problem p;  // shape and tree errors before any work
+    CKS(p.setup(order, dims, n_nodes, lo, hi, left, right, ranks, o.want_interior_u));
+  }
+  if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
+  // decomposition stream 0 is the handle's own; streams 1.. are worker handles
+  while ((int)h->workers.size() < inflight - 1) {
+    ht_handle_t w = nullptr;
+    CKS(ht_create(&w, h->device));
+    w->prof.enabled = h->prof.enabled;
+    w->prof.mask = h->prof.mask;
+    h->workers.push_back(w);
+    if (w->prof.enabled && w->prof.create()) return HT_ERR_HIP;
+    // first use of the worker's stream here, one worker after another (the stream then
+    // holds its own hardware queue while queues are free)
+    if (touch_stream(w->stream)) return HT_ERR_HIP;
+  }
+  // the workers start after what the caller's stream holds so far
+  if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
+  for (int w = 0; w + 1 < inflight; ++w)
+    if (hipStreamWaitEvent(h->workers[w]->stream, h->ev[0], 0) != hipSuccess) return HT_ERR_HIP;
+  std::atomic<int> first_err{0};
+  std::vector<ht_stats> wst(inflight);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto work = [&](int w) {
+    ht_handle_s* wh = w == 0 ? h : h->workers[w - 1];
+    if (hipSetDevice(h->device) != hipSuccess) {
+      int z = 0;
+      first_err.compare_exchange_strong(z, (int)HT_ERR_HIP);
+      return;
+    }
+    std::memset(&wst[w], 0, sizeof(ht_stats));
+    for (int i = w; i < count && first_err.load() == 0; i += inflight) {
+      double e = NAN;
+      ht_stats st;
+      std::memset(&st, 0, sizeof(st));
+      const ht_status s = decompose_impl(wh, xs[i], 1, order, dims, n_nodes, lo, hi, left, right, ranks, opt, outs[i],
+                                         1, &e, &st, &wh->arena);
+      if (rel_errs) rel_errs[i] = e;
+      if (s != HT_OK && s != HT_ERR_NOCONV) {
+        int z = 0;
+        first_err.compare_exchange_strong(z, (int)s);
+      }
+      wst[w].bytes_x_passes = st.bytes_x_passes;
+      wst[w].eig_iters_max = std::max(wst[w].eig_iters_max, st.eig_iters_max);
+      wst[w].eig_calls += st.eig_calls;
+      wst[w].workspace_bytes = std::max(wst[w].workspace_bytes, st.workspace_bytes);
+      wst[w].replayed += st.replayed;
+    }
+  };
+  std::vector<std::thread> threads;
+  for (int w = 1; w < inflight; ++w) threads.emplace_back(work, w);
+  work(0);
+  for (auto& t : threads) t.join();
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->total_ms = ms;
+    for (const ht_stats& w : wst) {
+      stats->bytes_x_passes = std::max(stats->bytes_x_passes, w.bytes_x_passes);
+      stats->eig_iters_max = std::max(stats->eig_iters_max, w.eig_iters_max);
+      stats->eig_calls += w.eig_calls;
+      stats->workspace_bytes += w.workspace_bytes;
+      stats->replayed += w.replayed;
+    }
+  }
+  return (ht_status)first_err.load();
 }
 
 ht_status ht_decompose_noalloc(ht_handle_t h, const float* x, int32_t order, const int64_t* dims, int32_t n_nodes,
@@ -555,29 +641,50 @@ void htd4_tc(float* x, long a, long b, long c, long d, int* k, void* blas_handle
 }
 
 // ------------------------------------------------------------------ profiling
+// The profiling calls act on the handle and on the worker handles of ht_decompose_many;
+// reads sum over them.
 ht_status ht_profile_enable(ht_handle_t h, int32_t on) {
   if (!h) return HT_ERR_INVALID_ARG;
   h->prof.enabled = on != 0;
+  for (ht_handle_s* w : h->workers) w->prof.enabled = on != 0;
+  if (on) {
+    // the events are made here, not inside the work that is timed
+    if (hipSetDevice(h->device) != hipSuccess || h->prof.create()) return HT_ERR_HIP;
+    for (ht_handle_s* w : h->workers)
+      if (w->prof.create()) return HT_ERR_HIP;
+  }
   return HT_OK;
 }
 
 ht_status ht_profile_select(ht_handle_t h, int32_t mask) {
   if (!h) return HT_ERR_INVALID_ARG;
   h->prof.mask = mask;
+  for (ht_handle_s* w : h->workers) w->prof.mask = mask;
   return HT_OK;
 }
 
 ht_status ht_profile_read(ht_handle_t h, double* ms, int64_t* launches, int32_t n, int32_t reset) {
   if (!h || n < 0) return HT_ERR_INVALID_ARG;
   for (int i = 0; i < n && i < PK_COUNT; ++i) {
-    if (ms) ms[i] = h->prof.ms[i];
-    if (launches) launches[i] = h->prof.cnt[i];
+    double m = h->prof.ms[i];
+    int64_t c = h->prof.cnt[i];
+    for (ht_handle_s* w : h->workers) {
+      m += w->prof.ms[i];
+      c += w->prof.cnt[i];
+    }
+    if (ms) ms[i] = m;
+    if (launches) launches[i] = c;
   }
-  if (reset)
+  if (reset) {
     for (int i = 0; i < PK_COUNT; ++i) {
       h->prof.ms[i] = 0.0;
       h->prof.cnt[i] = 0;
+      for (ht_handle_s* w : h->workers) {
+        w->prof.ms[i] = 0.0;
+        w->prof.cnt[i] = 0;
+      }
     }
+  }
   return HT_OK;
 }
 

@@ -258,6 +258,7 @@ int tsqr_launch(hipStream_t st, float* W, long m, int b, float* Q, double* R, vo
 int gen_noise_sumsq_launch(hipStream_t st, long n, uint64_t seed, uint32_t stream, double* acc, int* nb);
 int gen_add_noise_launch(hipStream_t st, float* x, long n, uint64_t seed, uint32_t stream, double scale);
 int copy_f32_launch(hipStream_t st, const float* src, float* dst, long n);
+int touch_stream(hipStream_t st);
 // Batched HT (k_batch.hip): one descriptor per tensor, prepared on the host.
 struct BatchItem {
   long x_off;     // element offset of the tensor in x

@@ -69,13 +69,17 @@ void Arena::release() {
 }
 
 // ------------------------------------------------------------------ profiler
-int Profiler::begin(hipStream_t st, int k) {
-  if (!enabled || !((mask >> k) & 1)) return -1;
+int Profiler::create() {
   if (!created) {
     for (auto& x : ev)
       if (hipEventCreate(&x) != hipSuccess) return -1;
     created = true;
   }
+  return 0;
+}
+int Profiler::begin(hipStream_t st, int k) {
+  if (!enabled || !((mask >> k) & 1)) return -1;
+  if (create()) return -1;
   if (n >= NREC) return -1;
   if (hipEventRecord(ev[2 * n], st) != hipSuccess) return -1;
   kid[n] = k;
@@ -1586,6 +1590,7 @@ int Engine::run(const float* x, float* out) {
       const int f = finish_deferred(&verified);
       if (verified) {
         if (plan_mismatch || call_idx != plan->calls.size() || small_idx != plan->small.size()) plan->valid = false;
+        replayed = 1;
         if (prof && prof->enabled) prof->collect();
         if (f) return f;
         if (noconv) return HT_ERR_NOCONV;

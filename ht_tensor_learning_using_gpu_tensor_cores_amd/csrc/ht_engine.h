@@ -70,6 +70,7 @@ struct Profiler {
   int mask = -1;  // kernel ids timed (bit 1 << id)
   double ms[PK_COUNT] = {};
   long cnt[PK_COUNT] = {};
+  int create();  // the events, made once (ht_profile_enable makes them before timed work)
   int begin(hipStream_t st, int k);
   void end(hipStream_t st, int slot);
   void collect();
@@ -112,6 +113,7 @@ struct Engine {
   int SVsq[HT_MAX_NODES] = {};  // 1: SVd[t] holds eigenvalues (squared singular values)
   double passes = 0.0;
   int eig_iters_max = 0, eig_calls = 0, noconv = 0;
+  int replayed = 0;  // 1: the run replayed the recorded plan and passed its verdicts
   int sketch_fallbacks = 0, sketch_nodes = 0;
   double sketch_resid = 0.0;
   double rel_err = 0.0, x_sq = 0.0;
@@ -204,6 +206,8 @@ struct ht_handle_s {
   // pinned staging for small device-to-host reads
   char* pinned = nullptr;
   size_t pinned_bytes = 0;
+  // ht_decompose_many: worker handles (own stream, workspace, plan), made on first use
+  std::vector<ht_handle_s*> workers;
 };
 
 namespace ht {

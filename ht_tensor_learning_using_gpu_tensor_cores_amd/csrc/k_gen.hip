@@ -189,4 +189,14 @@ int copy_f32_launch(hipStream_t st, const float* src, float* dst, long n) {
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
+// An empty launch (one work-group, no element: no memory is touched) and a wait for it:
+// the runtime binds a stream to a hardware queue when the stream is first used, so new
+// streams touched one after another get distinct queues where streams first used at the
+// same moment from several threads can end up sharing one.
+int touch_stream(hipStream_t st) {
+  hipLaunchKernelGGL(copy_f32_kernel, dim3(1), dim3(64), 0, st, nullptr, nullptr, 0L, 0);
+  if (hipGetLastError() != hipSuccess) return 5;
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : 5;
+}
+
 }  // namespace ht

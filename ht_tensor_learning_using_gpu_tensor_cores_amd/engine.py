@@ -261,9 +261,54 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
                   stats=dict(total_ms=st.total_ms, h2d_ms=st.h2d_ms, passes=st.bytes_x_passes,
                              eig_iters_max=st.eig_iters_max, eig_calls=st.eig_calls,
                              sketch_fallbacks=st.sketch_fallbacks, sketch_nodes=st.sketch_nodes,
-                             sketch_resid=st.sketch_resid, workspace_bytes=st.workspace_bytes))
+                             sketch_resid=st.sketch_resid, workspace_bytes=st.workspace_bytes,
+                             replayed=st.replayed))
     _unpack(res, layout)
     return res
+
+
+def decompose_many(xs, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="residual", handle=None,
+                   want_interior_u=False, inflight=2, outs=None, **kw):
+    """ht_decompose_many: decompose several device tensors of one shape with `inflight`
+    decompositions running at once (their passes over X ordered on the device). `outs`
+    (optional) is a (len(xs), layout total) float32 device tensor to write into. Returns
+    (list of HTensor, stats)."""
+    torch = _torch()
+    lib = _lib.load()
+    h = handle or default_handle()
+    dims = [int(d) for d in dims]
+    ranks = [int(r) for r in ranks]
+    opt = make_options(method, precision, err, want_interior_u=want_interior_u, **kw)
+    layout, targs = _layout_entry(dims, tree, ranks, want_interior_u)
+    total = layout[3]
+    n = math.prod(dims)
+    for x in xs:
+        if not (hasattr(x, "is_cuda") and x.is_cuda) or x.dtype != torch.float32 or not x.is_contiguous():
+            raise ValueError("xs must be contiguous float32 device tensors (column-major data)")
+        if x.numel() != n:
+            raise ValueError("every tensor must have prod(dims) elements")
+    count = len(xs)
+    if outs is None:
+        outs = torch.empty((count, total), dtype=torch.float32, device=xs[0].device)
+    if tuple(outs.shape) != (count, total):
+        raise ValueError("outs must be (len(xs), layout total)")
+    xp = (ctypes.c_void_p * count)(*[x.data_ptr() for x in xs])
+    op = (ctypes.c_void_p * count)(*[outs[i].data_ptr() for i in range(count)])
+    errs = np.full(count, np.nan, dtype=np.float64)
+    st = _lib.Stats()
+    h.order_after_torch(xs[0].device)
+    _lib.check(lib.ht_decompose_many(h.ptr, count, ctypes.cast(xp, ctypes.c_void_p), len(dims), *targs,
+                                     ctypes.byref(opt), ctypes.cast(op, ctypes.c_void_p),
+                                     errs.ctypes.data_as(_lib.PD), int(inflight), ctypes.byref(st)),
+               "ht_decompose_many")
+    results = []
+    for i in range(count):
+        res = HTensor(dims=dims, tree=tree, ranks=ranks, packed=outs[i], rel_err=float(errs[i]),
+                      want_interior_u=want_interior_u)
+        _unpack(res, layout)
+        results.append(res)
+    return results, dict(total_ms=st.total_ms, passes=st.bytes_x_passes, workspace_bytes=st.workspace_bytes,
+                         replayed=st.replayed)
 
 
 def residual(x, res: HTensor, handle=None):

@@ -124,6 +124,8 @@ typedef struct {
   int32_t sketch_nodes;    /* nodes whose range finder used the learned sketch first     */
   double sketch_resid;     /* largest range-residual estimate of the learned sketch      */
   double workspace_bytes;  /* device workspace the call used (one contiguous block)      */
+  int32_t replayed;        /* decompositions that replayed recorded solver rounds (read
+                              the device once, at their end)                          */
 } ht_stats;
 
 typedef struct ht_handle_s* ht_handle_t;
@@ -165,6 +167,25 @@ ht_status ht_decompose(ht_handle_t handle, const float* x, int32_t x_on_device, 
                        const int32_t* left, const int32_t* right, const int32_t* ranks,
                        const ht_options* opt, float* out, int32_t out_on_device, double* rel_err,
                        ht_stats* stats);
+
+/* Decompose `count` device tensors of one shape and tree (xs[i] -> outs[i], rel_errs[i]),
+ * with up to `inflight` decompositions running at once (1..HT_MAX_INFLIGHT), each on its
+ * own stream, workspace and recorded solver plan (worker handles that `handle` creates on
+ * first use and keeps): the latency-bound small solver kernels of one decomposition run
+ * beside the passes over X and solver kernels of the others. Each stream needs a hardware
+ * queue of its own for that (HIP's GPU_MAX_HW_QUEUES, 4 by default, counts every stream
+ * of the process, torch's included); streams that share a queue run one after the other,
+ * with the same results. Outputs equal those of
+ * ht_decompose one tensor at a time. The work starts after what the handle's stream holds
+ * and the call returns when every decomposition is done; stats (optional) receives the
+ * host wall time of the whole call in total_ms. No counterpart in the reference, whose
+ * drivers decompose one tensor per call (third_order_tensor_decompsition/CUDA/Optimized/
+ * ht_init.cu:9-67 loops over sizes). Returns the first failing status. */
+#define HT_MAX_INFLIGHT 8
+ht_status ht_decompose_many(ht_handle_t handle, int32_t count, const float* const* xs, int32_t order,
+                            const int64_t* dims, int32_t n_nodes, const int32_t* lo, const int32_t* hi,
+                            const int32_t* left, const int32_t* right, const int32_t* ranks, const ht_options* opt,
+                            float* const* outs, double* rel_errs, int32_t inflight, ht_stats* stats);
 
 /* ht_decompose without any device allocation. No counterpart in the reference (whose
  * drivers cudaMalloc and cudaFree on every call): it is for callers that cannot allocate
