@@ -5,7 +5,8 @@
 """
 from . import sketch, tree
 from ._lib import HTError, LIB_PATH, load as load_library
-from .engine import (Batch, Handle, HTensor, batch_ranks, batch_shapes, decompose, decompose_batched, decompose_many,
+from .engine import (Batch, Handle, HTensor, batch_ranks, batch_shapes, decompose, decompose_adaptive, decompose_batched,
+                     decompose_many,
                      default_handle, generate_batch, generate_bumps, generate_cp_decay, generate_tucker, htd, htd4,
                      htd4_tc, htd_tc, make_options, output_layout, reconstruct, residual)
 
@@ -13,5 +14,5 @@ __all__ = [
     "sketch", "tree", "HTError", "LIB_PATH", "load_library", "Handle", "HTensor", "decompose", "default_handle",
     "generate_bumps", "generate_cp_decay", "generate_tucker", "htd", "htd4", "htd4_tc", "htd_tc",
     "make_options", "output_layout", "reconstruct", "residual", "Batch", "batch_ranks", "batch_shapes",
-    "decompose_batched", "decompose_many", "generate_batch",
+    "decompose_adaptive", "decompose_batched", "decompose_many", "generate_batch",
 ]

@@ -73,6 +73,8 @@ SIGNATURES = {
     "ht_output_layout": (I32, [I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, I32, PI64, PI64, PI64, PI64]),
     "ht_decompose": (I32, [H, P, I32, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, ctypes.POINTER(Options), P, I32,
                            PD, ctypes.POINTER(Stats)]),
+    "ht_decompose_adaptive": (I32, [H, P, I32, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, D, I32,
+                                    ctypes.POINTER(Options), P, I32, PI32, PD, ctypes.POINTER(Stats)]),
     "ht_decompose_many": (I32, [H, I32, P, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, ctypes.POINTER(Options), P,
                                 PD, I32, ctypes.POINTER(Stats)]),
     "ht_decompose_noalloc": (I32, [H, P, I32, PI64, I32, PI32, PI32, PI32, PI32, PI32, ctypes.POINTER(Options), P,

@@ -255,7 +255,8 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
                                 const int64_t* dims, int32_t n_nodes, const int32_t* lo, const int32_t* hi,
                                 const int32_t* left, const int32_t* right, const int32_t* ranks,
                                 const ht_options* opt_in, float* out, int32_t out_on_device, double* rel_err,
-                                ht_stats* stats, Arena* arena) {
+                                ht_stats* stats, Arena* arena, double rel_eps = 0.0, int32_t max_rank = 0,
+                                int32_t* ranks_out = nullptr) {
   if (!h || !x) return HT_ERR_INVALID_ARG;
   if (rel_err) *rel_err = NAN;
   ht_options opt;
@@ -270,6 +271,18 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
     return HT_ERR_INVALID_ARG;
   Problem p;
   CKS(p.setup(order, dims, n_nodes, lo, hi, left, right, ranks, opt.want_interior_u));
+  const bool adaptive = rel_eps > 0.0;
+  if (adaptive) {
+    if (!std::isfinite(rel_eps) || max_rank < 0) return HT_ERR_INVALID_ARG;
+    // the ranks are caps: max_rank on every node, then k_t <= k_l k_r, deepest first
+    std::vector<int> inner;
+    for (int t = 1; t < p.n_nodes; ++t) {
+      if (max_rank > 0) p.rank[t] = std::min(p.rank[t], (int)max_rank);
+      if (p.left[t] >= 0) inner.push_back(t);
+    }
+    std::sort(inner.begin(), inner.end(), [&](int a, int b) { return p.depth(a) > p.depth(b); });
+    for (int t : inner) p.rank[t] = std::min(p.rank[t], p.rank[p.left[t]] * p.rank[p.right[t]]);
+  }
   if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
   const size_t xbytes = (size_t)p.N * sizeof(float);
   float h2d_ms = 0.f;
@@ -297,6 +310,8 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
   e.pin_bytes = h->pinned_bytes;
   e.plan = &h->plan;
   e.done_ev = h->ev[1];  // recorded by the run before its final wait
+  e.adaptive = adaptive;
+  e.adapt_eps = rel_eps;
   if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
   int s = e.run(xd, od);
   if (s && s != HT_ERR_NOCONV) {
@@ -313,6 +328,8 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
     if (hipMemcpy(out, od, (size_t)p.total * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return HT_ERR_HIP;
   }
   if (rel_err) *rel_err = e.rel_err;
+  if (ranks_out)
+    for (int t = 0; t < p.n_nodes; ++t) ranks_out[t] = p.rank[t];
   if (stats) {
     stats->total_ms = ms;
     stats->h2d_ms = h2d_ms;
@@ -337,6 +354,16 @@ ht_status ht_decompose(ht_handle_t h, const float* x, int32_t x_on_device, int32
   if (!h) return HT_ERR_INVALID_ARG;
   return decompose_impl(h, x, x_on_device, order, dims, n_nodes, lo, hi, left, right, ranks, opt, out,
                         out_on_device, rel_err, stats, &h->arena);
+}
+
+ht_status ht_decompose_adaptive(ht_handle_t h, const float* x, int32_t x_on_device, int32_t order, const int64_t* dims,
+                                int32_t n_nodes, const int32_t* lo, const int32_t* hi, const int32_t* left,
+                                const int32_t* right, const int32_t* max_ranks, double rel_eps, int32_t max_rank,
+                                const ht_options* opt, float* out, int32_t out_on_device, int32_t* ranks_out,
+                                double* rel_err, ht_stats* stats) {
+  if (!h || !ranks_out || !(rel_eps > 0.0)) return HT_ERR_INVALID_ARG;
+  return decompose_impl(h, x, x_on_device, order, dims, n_nodes, lo, hi, left, right, max_ranks, opt, out,
+                        out_on_device, rel_err, stats, &h->arena, rel_eps, max_rank, ranks_out);
 }
 
 ht_status ht_decompose_many(ht_handle_t h, int32_t count, const float* const* xs, int32_t order, const int64_t* dims,
@@ -598,12 +625,32 @@ static const int32_t T3_LO[5] = {0, 0, 2, 0, 1}, T3_HI[5] = {3, 2, 3, 1, 2}, T3_
 static const int32_t T4_LO[7] = {0, 0, 2, 0, 1, 2, 3}, T4_HI[7] = {4, 2, 4, 1, 2, 3, 4},
                      T4_L[7] = {1, 3, 5, -1, -1, -1, -1}, T4_R[7] = {2, 4, 6, -1, -1, -1, -1};
 
+// rel_eps > 0: the ranks k[1..4] become caps (with max_rank > 0 as a further cap) and each
+// node's rank is chosen for ||X - X_HT|| <= rel_eps ||X|| (ht_decompose_adaptive); the ranks
+// chosen are written back into k. rel_eps <= 0: the fixed ranks k, as in the reference.
 float htd(float* x, long a, long b, long c, int* k, float rel_eps, float max_rank) {
-  (void)rel_eps;
-  (void)max_rank;
   const int64_t dims[3] = {a, b, c};
   double ms = 0;
-  const double err = compat_run(x, 3, dims, T3_LO, T3_HI, T3_L, T3_R, k, HT_METHOD_PLAIN, &ms);
+  double err = -1.0;
+  if (rel_eps > 0.0f) {
+    ht_handle_t h = default_handle();
+    if (!h || !x || !k) return -1.0f;
+    int32_t caps[5], chosen[5];
+    for (int t = 0; t < 5; ++t) caps[t] = k[t];
+    ht_options opt;
+    ht_default_options(&opt);
+    opt.method = HT_METHOD_PLAIN;
+    opt.precision = HT_PREC_FP32;
+    ht_stats stats;
+    const ht_status s = ht_decompose_adaptive(h, x, 0, 3, dims, 5, T3_LO, T3_HI, T3_L, T3_R, caps, rel_eps,
+                                              max_rank > 0.0f ? (int32_t)max_rank : 0, &opt, nullptr, 0, chosen,
+                                              &err, &stats);
+    if (s != HT_OK) return -1.0f - (float)s;
+    for (int t = 0; t < 5; ++t) k[t] = chosen[t];
+    ms = stats.total_ms + stats.h2d_ms;
+  } else {
+    err = compat_run(x, 3, dims, T3_LO, T3_HI, T3_L, T3_R, k, HT_METHOD_PLAIN, &ms);
+  }
   if (err >= 0) {
     printf("cost time :%gs\n", ms / 1000.0);
     printf("error rate %g\n", err);

@@ -176,12 +176,17 @@ int Problem::setup(int order_, const int64_t* dims_, int n_nodes_, const int32_t
     if (k < 1 || k > HT_MAX_RANK || k > rows[t] || k > cols) return HT_ERR_RANK;
     if (left[t] >= 0 && k > (int64_t)rank[left[t]] * rank[right[t]]) return HT_ERR_RANK;
   }
-  // Packed output layout.
-  int64_t off = 0;
   want_interior_u = want_int_u;
+  layout();
+  return HT_OK;
+}
+
+// Packed output layout of the current ranks.
+void Problem::layout() {
+  int64_t off = 0;
   for (int t = 0; t < n_nodes; ++t) {
     u_off[t] = b_off[t] = s_off[t] = -1;
-    if (t != 0 && (left[t] < 0 || want_int_u)) {
+    if (t != 0 && (left[t] < 0 || want_interior_u)) {
       u_off[t] = off;
       off += rows[t] * rank[t];
     }
@@ -196,7 +201,6 @@ int Problem::setup(int order_, const int64_t* dims_, int n_nodes_, const int32_t
     off += rank[t];
   }
   total = off;
-  return HT_OK;
 }
 
 int Problem::depth(int t) const {
@@ -862,13 +866,18 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
     for (int i = 0; i < nb; ++i) {
       const int m = p.lo[grp[i]];
       CK(gram(x, view_of(p.dims, p.order, m, m + 1), prec, G + (size_t)i * n * n, have_norm ? nullptr : xsq_dev));
-      if (!have_norm) CK(early_finite_check(xsq_dev));
+      if (!have_norm) {
+        CK(early_finite_check(xsq_dev));
+        // root-to-leaves: the two root children share one spectrum, so 2d - 3 truncations
+        if (adaptive) CK(adapt_tau(xsq_dev, 2 * p.order - 3));
+      }
       have_norm = true;
     }
     CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
     for (int i = 0; i < nb; ++i) {
       Ufull[grp[i]] = U + (size_t)i * n * k;
       CK(eig_to_sv(w + (size_t)i * k, grp[i]));
+      if (adaptive) CK(choose_rank(grp[i], w + (size_t)i * k, k, adapt_xsq, 1));  // leaf of X: trace ||X||^2
       done[grp[i]] = 1;
     }
   }
@@ -887,6 +896,46 @@ int Engine::early_finite_check(const double* xsq_dev) {
   return 0;
 }
 
+// ------------------------------------------------------------------ rank-adaptive truncation
+// Per-node energy budget tau^2 = (rel_eps ||X||)^2 / m, m the number of truncations whose
+// removed energies the method's error adds up over: then ||X - X_HT||^2 <= (rel_eps ||X||)^2.
+int Engine::adapt_tau(const double* xsq_dev, int truncations) {
+  double xs = 0.0;
+  CK(d2h(&xs, xsq_dev, sizeof(double)));
+  if (!std::isfinite(xs)) return HT_ERR_NONFINITE;
+  adapt_xsq = xs;
+  tau2 = adapt_eps * adapt_eps * xs / std::max(1, truncations);
+  return 0;
+}
+
+int Engine::choose_rank(int t, const double* w_dev, int k, double trace, int sq) {
+  std::vector<double> w(k);
+  CK(d2h(w.data(), w_dev, sizeof(double) * k));
+  double kept = 0.0;
+  int r = k;
+  for (int i = 0; i < k; ++i) {
+    kept += sq ? std::max(w[i], 0.0) : w[i] * w[i];
+    if (trace - kept <= tau2) {
+      r = i + 1;
+      break;
+    }
+  }
+  prob->rank[t] = std::max(1, std::min(r, prob->rank[t]));
+  return 0;
+}
+
+// Trace of an n x n FP64 Gram on the device (its diagonal, read with one strided copy).
+int Engine::gram_trace(const double* G, long n, double* tr) {
+  std::vector<double> d(n);
+  HT_HIP_CHECK(hipMemcpy2DAsync(d.data(), sizeof(double), G, sizeof(double) * (n + 1), sizeof(double), n,
+                                hipMemcpyDeviceToHost, st));
+  HT_HIP_CHECK(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (long i = 0; i < n; ++i) s += d[i];
+  *tr = s;
+  return 0;
+}
+
 int Engine::eig_to_sv(const double* w, int t) {
   // kept as eigenvalues: the output packing takes sqrt(max(w, 0))
   SVd[t] = const_cast<double*>(w);
@@ -896,6 +945,7 @@ int Engine::eig_to_sv(const double* w, int t) {
 
 int Engine::run_plain(const float* x) {
   const Problem& p = *prob;
+  const std::vector<int> cap(p.rank, p.rank + p.n_nodes);  // caps of a rank-adaptive run
   double* xsq = alloc<double>(1);
   CK(leaves_from_x_plain(x, xsq));
   // Interior non-root nodes: exact basis of X_(t).
@@ -912,6 +962,7 @@ int Engine::run_plain(const float* x) {
       CK(gram(x, v, PREC_F32, G, nullptr));
       CK(sym_topk(G, (int)rows, k, Ufull[t], w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16, GRAM_FLOOR));
       CK(eig_to_sv(w, t));
+      if (adaptive) CK(choose_rank(t, w, k, adapt_xsq, 1));
     } else if (cols <= 1024 && (p.lo[t] == 0 || p.hi[t] == p.order)) {
       // Gram of the complementary (contiguous) mode range, then U = X_(t) V / sigma via TSQR.
       const PMS vc = p.lo[t] == 0 ? PMS{rows, cols, 1} : PMS{1, cols, rows};
@@ -921,6 +972,9 @@ int Engine::run_plain(const float* x) {
       CK(gram(x, vc, PREC_F32, G, nullptr));
       CK(sym_topk(G, (int)cols, k, Vc, w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16, GRAM_FLOOR));
       CK(eig_to_sv(w, t));
+      if (adaptive) CK(choose_rank(t, w, k, adapt_xsq, 1));
+      // (the basis keeps all k columns: Q of the TSQR of X_(t) V spans X_(t) V[:, :r] in its
+      // first r columns, so a smaller rank is a prefix)
       float* Vf = alloc<float>((size_t)cols * k);
       float* Y = alloc<float>((size_t)rows * k);
       float* Qf = alloc<float>((size_t)rows * k);
@@ -942,16 +996,32 @@ int Engine::run_plain(const float* x) {
       CK(exact_svd_subspace(x, v, k, kW, Ufull[t], Wr, sv, 3000 + t * 64));
       SVd[t] = sv;
       SVsq[t] = 0;
+      if (adaptive) CK(choose_rank(t, sv, k, adapt_xsq, 0));
       if (kW) {
         Ufull[sib] = Wr;
         SVd[sib] = sv;
         SVsq[sib] = 0;
         done[sib] = 1;
+        if (adaptive) CK(choose_rank(sib, sv, kW, adapt_xsq, 0));
       }
     } else {
       return HT_ERR_UNSUPPORTED;
     }
     done[t] = 1;
+  }
+  if (adaptive) {
+    // The two root children truncate one spectrum: one rank for both (within their caps).
+    // Then nested ranks: an interior node keeps at most k_l k_r columns, deepest first.
+    int* rk = prob->rank;
+    const int cl = p.left[0], cr = p.right[0];
+    const int common = std::max(rk[cl], rk[cr]);
+    rk[cl] = std::min(common, cap[cl]);
+    rk[cr] = std::min(common, cap[cr]);
+    std::vector<int> inner;
+    for (int t = 1; t < p.n_nodes; ++t)
+      if (p.left[t] >= 0) inner.push_back(t);
+    std::sort(inner.begin(), inner.end(), [&](int a, int b) { return p.depth(a) > p.depth(b); });
+    for (int t : inner) rk[t] = std::min(rk[t], rk[p.left[t]] * rk[p.right[t]]);
   }
   // Transfer tensors of interior non-root nodes.
   for (int t = 1; t < p.n_nodes; ++t) {
@@ -1038,7 +1108,7 @@ int Engine::run_ltr(const float* x) {
   const int prec = opt.precision == HT_PREC_FP32 ? PREC_F32 : (opt.precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
   const int t0 = p.leaf_of_mode(0), t1 = p.leaf_of_mode(1);
   const long n0 = p.dims[0], n1 = p.dims[1];
-  const int r0 = p.rank[t0], r1 = p.rank[t1];
+  int r0 = p.rank[t0], r1 = p.rank[t1];  // caps until chosen, in a rank-adaptive run
   double* xsq = alloc<double>(1);
   const bool slab = n0 <= 128 && n1 <= 128 && n0 % 16 == 0 && n1 % 16 == 0 && r0 <= 32 && r1 <= 32 &&
                     prec != PREC_F32 && (((uintptr_t)x & 15) == 0);
@@ -1077,6 +1147,14 @@ int Engine::run_ltr(const float* x) {
   }
   CK(eig_to_sv(w0, t0));
   CK(eig_to_sv(w1, t1));
+  if (adaptive) {
+    // leaves-to-root: the energies removed by the 2d - 2 truncations add up exactly
+    CK(adapt_tau(xsq, 2 * d - 2));
+    CK(choose_rank(t0, w0, r0, adapt_xsq, 1));
+    CK(choose_rank(t1, w1, r1, adapt_xsq, 1));
+    r0 = p.rank[t0];
+    r1 = p.rank[t1];
+  }
   // Pass 2: Z = X x_1 U_0^T x_2 U_1^T  (r0, r1, n_2, ...), FP32.
   float* U0f = alloc<float>((size_t)n0 * r0);
   float* U1f = alloc<float>((size_t)n1 * r1);
@@ -1131,6 +1209,11 @@ int Engine::run_ltr(const float* x) {
         const int t = p.leaf_of_mode(grp[i]);
         Ufull[t] = U + (size_t)i * n * k;
         CK(eig_to_sv(w + (size_t)i * k, t));
+        if (adaptive) {
+          double tr = 0.0;  // ||Z||^2: the leaf is truncated in the projected tensor
+          CK(gram_trace(G + (size_t)i * n * n, n, &tr));
+          CK(choose_rank(t, w + (size_t)i * k, k, tr, 1));
+        }
         done[grp[i]] = 1;
       }
     }
@@ -1240,9 +1323,12 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
     PMS v{1, cd[i] * cd[i + 1], 1};
     for (int s = 0; s < i; ++s) v.P *= cd[s];
     for (int s = i + 2; s < (int)cd.size(); ++s) v.S *= cd[s];
-    const int k = p.rank[t];
     const long M = v.M;
     if (M > 4096) return HT_ERR_UNSUPPORTED;
+    // (a rank-adaptive run may have left the children fewer columns than the cap assumed;
+    // a fixed-rank run keeps the ranks it was given, as the reference does)
+    if (adaptive) prob->rank[t] = (int)std::min<long>(std::min<long>(p.rank[t], M), v.P * v.S);
+    int k = p.rank[t];
     double* V = alloc<double>((size_t)M * k);
     double* w = alloc<double>(k);
     if (!V || !w) return HT_ERR_ALLOC;
@@ -1260,7 +1346,16 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
       CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-4, 8, 32, 4000 + t * 16, 1e-12, G32));
     }
     CK(eig_to_sv(wsv, t));
-    Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself
+    if (adaptive) {
+      double* cs = alloc<double>(2 + 1024);
+      if (!cs) return HT_ERR_ALLOC;
+      CK(sumsq_f64_launch(st, C, v.P * M * v.S, cs));  // ||C||^2: the node is truncated in the core
+      double tr = 0.0;
+      CK(d2h(&tr, cs, sizeof(double)));
+      CK(choose_rank(t, wsv, k, tr, 1));
+      k = p.rank[t];
+    }
+    Bd[t] = V;  // (k_l, k_r, k_t) layout is V itself (a smaller rank is a prefix)
     double* C2 = alloc<double>((size_t)v.P * k * v.S);
     CK(proj_pms_f64_launch(st, C, v, V, k, C2));
     C = C2;
@@ -1365,6 +1460,7 @@ int Engine::sketch_node(const float* x, int t, const float* S, const int64_t* of
 
 int Engine::run_rsvd(const float* x) {
   const Problem& p = *prob;
+  if (adaptive) return HT_ERR_UNSUPPORTED;  // the range finder's width is fixed by the rank
   const int prec = opt.precision == HT_PREC_FP32 ? PREC_F32 : (opt.precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
   const bool learned = opt.sketch == HT_SKETCH_LEARNED;
   int64_t soff[HT_MAX_ORDER], stotal = 0;
@@ -1502,6 +1598,7 @@ int Engine::run_once(const float* x, float* out) {
   else if (opt.method == HT_METHOD_RSVD) s = run_rsvd(x);
   else return HT_ERR_INVALID_ARG;
   if (s) return s;
+  if (adaptive) prob->layout();  // the packed layout of the ranks chosen
   if (out) CK(write_outputs(out));
   return 0;
 }
@@ -1555,8 +1652,8 @@ int Engine::finish_deferred(bool* verified) {
 // the device once, at the end; a run without a plan (or whose replay fails its verdicts)
 // reads every eigensolver round and records a new plan.
 int Engine::run(const float* x, float* out) {
-  const bool can = plan && pin && pin_bytes >= 16384 && opt.method == HT_METHOD_LTR && !getenv("HT_DEBUG") &&
-                   !getenv("HT_SYNC_ROUNDS");
+  const bool can = plan && pin && pin_bytes >= 16384 && opt.method == HT_METHOD_LTR && !adaptive &&
+                   !getenv("HT_DEBUG") && !getenv("HT_SYNC_ROUNDS");
   std::vector<int64_t> key;
   if (can) key = plan_key(*prob, opt, x);
   if (can && plan->valid && plan->key == key) {

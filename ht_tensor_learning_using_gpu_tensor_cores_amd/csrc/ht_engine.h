@@ -53,6 +53,7 @@ struct Problem {
   int64_t total = 0;
   int setup(int order, const int64_t* dims, int n_nodes, const int32_t* lo, const int32_t* hi,
             const int32_t* left, const int32_t* right, const int32_t* ranks, int want_interior_u);
+  void layout();  // u_off, b_off, s_off and total of the current ranks
   int depth(int t) const;
   int leaf_of_mode(int m) const;
 };
@@ -103,8 +104,20 @@ struct Engine {
   Profiler* prof = nullptr;
   hipStream_t st;
   Arena* arena;
-  const Problem* prob;
+  Problem* prob;  // ranks change during a rank-adaptive run
   ht_options opt;
+  // Rank-adaptive truncation: a non-root node's rank on entry is its cap; once the node's
+  // spectrum is known its rank becomes the smallest r whose truncation removes at most
+  // tau2 of energy (trace of the current Gram minus its r largest eigenvalues).
+  bool adaptive = false;
+  double adapt_eps = 0.0;  // rel_eps
+  double tau2 = 0.0;       // (rel_eps ||X||_F)^2 / (number of truncations the error adds up over)
+  double adapt_xsq = 0.0;  // ||X||_F^2 (host)
+  int adapt_tau(const double* xsq_dev, int truncations);
+  // w_dev: k eigenvalues (sq = 1) or singular values (sq = 0), descending; trace: energy of
+  // the tensor the node is truncated in
+  int choose_rank(int t, const double* w_dev, int k, double trace, int sq);
+  int gram_trace(const double* G, long n, double* tr);
   // per-node device data (FP64): full bases (leaves; exact interior bases for plain),
   // transfer tensors, singular values
   double* Ufull[HT_MAX_NODES];

@@ -267,6 +267,40 @@ def decompose(x, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="r
     return res
 
 
+def decompose_adaptive(x, dims, tree: Tree, max_ranks, rel_eps, max_rank=0, method="plain", precision=None,
+                       err="residual", handle=None, **kw):
+    """ht_decompose_adaptive: each node's rank is the smallest (within max_ranks[t] and
+    max_rank) that keeps ||X - X_HT|| <= rel_eps ||X|| (per-node energy budget
+    (rel_eps ||X||)^2 / m, m = 2d-3 for plain, 2d-2 for ltr). The rel_eps/max_rank of
+    htd (third_order_tensor_decompsition/CUDA/Baseline/function.h:6). Returns an HTensor
+    whose `ranks` are the ranks chosen."""
+    torch = _torch()
+    lib = _lib.load()
+    h = handle or default_handle()
+    dims = [int(d) for d in dims]
+    max_ranks = [int(r) for r in max_ranks]
+    if precision is None:
+        precision = "fp32" if method == "plain" else "bf16"
+    opt = make_options(method, precision, err, **kw)
+    layout, targs = _layout_entry(dims, tree, max_ranks, False)
+    if not (hasattr(x, "is_cuda") and x.is_cuda) or x.dtype != torch.float32 or not x.is_contiguous():
+        raise ValueError("x must be a contiguous float32 device tensor (column-major data)")
+    out = torch.empty(layout[3], dtype=torch.float32, device=x.device)
+    chosen = (ctypes.c_int32 * tree.n_nodes)()
+    rel = ctypes.c_double()
+    st = _lib.Stats()
+    h.order_after_torch(x.device)
+    _lib.check(lib.ht_decompose_adaptive(h.ptr, ctypes.c_void_p(x.data_ptr()), 1, len(dims), *targs, float(rel_eps),
+                                         int(max_rank), ctypes.byref(opt), ctypes.c_void_p(out.data_ptr()), 1,
+                                         chosen, ctypes.byref(rel), ctypes.byref(st)), "ht_decompose_adaptive")
+    ranks = [int(v) for v in chosen]
+    lay = output_layout(dims, tree, ranks)
+    res = HTensor(dims=dims, tree=tree, ranks=ranks, packed=out[:lay[3]], rel_err=rel.value,
+                  stats=dict(total_ms=st.total_ms, passes=st.bytes_x_passes, eig_calls=st.eig_calls))
+    _unpack(res, lay)
+    return res
+
+
 def decompose_many(xs, dims, tree: Tree, ranks, method="ltr", precision="bf16", err="residual", handle=None,
                    want_interior_u=False, inflight=2, outs=None, **kw):
     """ht_decompose_many: decompose several device tensors of one shape with `inflight`
@@ -484,11 +518,16 @@ def htd(x, a, b, c, k, rel_eps=0.0, max_rank=0.0):
     """float htd(dt *x, long a, long b, long c, int *k, dt rel_eps, dt max_rank) --
     third_order_tensor_decompsition/CUDA/Baseline/function.h:6 (plain FP32, tree {1,2},{3},
     k heap-indexed with k[0]=1). Prints "cost time"/"error rate" like the reference and
-    returns the relative error (negative on failure)."""
+    returns the relative error (negative on failure). With rel_eps > 0 the ranks chosen
+    are written back into k (a list or an integer array), as the C entry writes them."""
     lib = _lib.load()
     xa = _host_f32(x)
-    return float(lib.htd(ctypes.c_void_p(xa.ctypes.data), a, b, c, _lib.i32_array(k), float(rel_eps),
-                         float(max_rank)))
+    karr = _lib.i32_array(k)
+    e = float(lib.htd(ctypes.c_void_p(xa.ctypes.data), a, b, c, karr, float(rel_eps), float(max_rank)))
+    if rel_eps > 0:
+        for i in range(len(k)):
+            k[i] = int(karr[i])
+    return e
 
 
 def htd_tc(x, a, b, c, k):

@@ -168,6 +168,24 @@ ht_status ht_decompose(ht_handle_t handle, const float* x, int32_t x_on_device, 
                        const ht_options* opt, float* out, int32_t out_on_device, double* rel_err,
                        ht_stats* stats);
 
+/* Rank-adaptive decomposition: what the reference's rel_eps and max_rank arguments
+ * (float htd(dt*,long,long,long,int*,dt rel_eps,dt max_rank),
+ * third_order_tensor_decompsition/CUDA/Baseline/function.h:6) stand for; the reference
+ * accepts and ignores them. max_ranks[t] caps node t (max_ranks[0] = 1), max_rank > 0 caps
+ * every node, and an interior node keeps at most k_l k_r columns. Once a node's spectrum is
+ * known its rank becomes the smallest r within its cap whose truncation removes at most
+ * (rel_eps ||X||_F)^2 / m of energy, m the number of truncations the method's error adds up
+ * over: 2d - 3 for HT_METHOD_PLAIN (the two root children truncate one spectrum and get one
+ * rank), 2d - 2 for HT_METHOD_LTR. Then ||X - X_HT||_F <= rel_eps ||X||_F whenever the caps
+ * leave room (the reported rel_err is measured, not the bound). ranks_out receives the
+ * ranks chosen; `out` must hold ht_output_layout(max_ranks) floats and receives the packed
+ * layout of ranks_out (a shorter block). HT_METHOD_RSVD returns HT_ERR_UNSUPPORTED. */
+ht_status ht_decompose_adaptive(ht_handle_t handle, const float* x, int32_t x_on_device, int32_t order,
+                                const int64_t* dims, int32_t n_nodes, const int32_t* lo, const int32_t* hi,
+                                const int32_t* left, const int32_t* right, const int32_t* max_ranks,
+                                double rel_eps, int32_t max_rank, const ht_options* opt, float* out,
+                                int32_t out_on_device, int32_t* ranks_out, double* rel_err, ht_stats* stats);
+
 /* Decompose `count` device tensors of one shape and tree (xs[i] -> outs[i], rel_errs[i]),
  * with up to `inflight` decompositions running at once (1..HT_MAX_INFLIGHT), each on its
  * own stream, workspace and recorded solver plan (worker handles that `handle` creates on
@@ -315,7 +333,9 @@ ht_status ht_sketch_eval_bumps(ht_handle_t handle, int32_t order, int64_t n, int
 
 /* third_order_tensor_decompsition/CUDA/Baseline/function.h:6: plain FP32 HT of the
  * reference's tree {1,2},{3}; x is host memory (a*b*c floats), k[5] heap-indexed ranks
- * with k[0]=1. rel_eps and max_rank are accepted and ignored, as in the reference.
+ * with k[0]=1. rel_eps <= 0 (what the reference's driver effectively gets: it ignores
+ * both): the ranks k. rel_eps > 0: k[1..4] are caps, max_rank > 0 a further cap, and the
+ * ranks are chosen as ht_decompose_adaptive chooses them, then written back into k.
  * Returns the relative error, or a negative value on failure. */
 float htd(float* x, long a, long b, long c, int* k, float rel_eps, float max_rank);
 /* third_order_tensor_decompsition/CUDA/Optimized/function.h:6: the mixed-precision

@@ -565,3 +565,99 @@ def rsvd_ht(x, dims, tree, ranks, p=8, q=1, seed=1234, max_block=64):
         else:
             ht.B[t] = contract_transfer(full[t], full[l], full[r], nl, nr)
     return ht
+
+
+# ------------------------------------------------------------- rank-adaptive truncation
+
+def choose_rank(energies, trace, tau2, cap):
+    """Smallest r <= cap with trace - sum(energies[:r]) <= tau2 (energies descending), else cap."""
+    kept = 0.0
+    for i in range(min(cap, len(energies))):
+        kept += max(float(energies[i]), 0.0)
+        if trace - kept <= tau2:
+            return i + 1
+    return cap
+
+
+def _nest(tree, ranks):
+    """k_t <= k_l k_r, deepest interior nodes first."""
+    inner = sorted([t for t in tree.interior() if t != 0], key=lambda t: -tree.depth(t))
+    for t in inner:
+        ranks[t] = min(ranks[t], ranks[tree.left[t]] * ranks[tree.right[t]])
+    return ranks
+
+
+def adaptive_caps(tree, max_ranks, max_rank=0):
+    caps = list(max_ranks)
+    for t in range(1, tree.n_nodes):
+        if max_rank > 0:
+            caps[t] = min(caps[t], max_rank)
+    return _nest(tree, caps)
+
+
+def plain_ht_adaptive(x, dims, tree, max_ranks, rel_eps, max_rank=0):
+    """The rel_eps/max_rank truncation the reference's htd declares and ignores
+    (third_order_tensor_decompsition/CUDA/Baseline/function.h:6), on the root-to-leaves
+    (plain) HT: each non-root node's rank is the smallest within its cap whose SVD tail of
+    X_(t) is at most tau^2 = (rel_eps ||X||)^2 / (2d - 3); the two root children share one
+    spectrum and get one rank; then k_t <= k_l k_r. Returns (HT, ranks)."""
+    x = np.asarray(x, dtype=np.float64)
+    caps = adaptive_caps(tree, max_ranks, max_rank)
+    xsq = float(np.sum(x * x))
+    tau2 = rel_eps ** 2 * xsq / max(1, 2 * len(dims) - 3)
+    ranks = list(caps)
+    for t in range(1, tree.n_nodes):
+        s = np.linalg.svd(unfold(x, dims, tree.lo[t], tree.hi[t]), compute_uv=False)
+        ranks[t] = max(1, choose_rank(s ** 2, xsq, tau2, caps[t]))
+    cl, cr = tree.left[0], tree.right[0]
+    common = max(ranks[cl], ranks[cr])
+    ranks[cl], ranks[cr] = min(common, caps[cl]), min(common, caps[cr])
+    ranks = _nest(tree, ranks)
+    return plain_ht(x, dims, tree, ranks), ranks
+
+
+def ltr_ht_adaptive(x, dims, tree, max_ranks, rel_eps, max_rank=0, leaf_groups=None):
+    """Leaves-to-root HT (ltr_ht) with the same per-node rule, the energy of each truncation
+    measured in the tensor it is applied to (X for the first leaf group, the projected tensor
+    for later groups, the core for interior nodes), tau^2 = (rel_eps ||X||)^2 / (2d - 2): the
+    removed energies of leaves-to-root truncation add up exactly. Returns (HT, ranks)."""
+    x = np.asarray(x, dtype=np.float64)
+    d = len(dims)
+    caps = adaptive_caps(tree, max_ranks, max_rank)
+    ranks = list(caps)
+    tau2 = rel_eps ** 2 * float(np.sum(x * x)) / max(1, 2 * d - 2)
+    ht = HT(dims, tree, ranks)
+    if leaf_groups is None:
+        leaf_groups = [list(range(d))]
+    y, ydims = x, list(dims)
+    for group in leaf_groups:
+        bases = {}
+        energy = float(np.sum(y * y))
+        for mode in group:
+            t = tree.leaf_of_mode(mode)
+            A = unfold(y, ydims, mode, mode + 1)
+            U, w = gram_topk(A @ A.T, caps[t])
+            ranks[t] = max(1, choose_rank(w, energy, tau2, caps[t]))
+            bases[mode] = U[:, :ranks[t]]
+            ht.U[t] = bases[mode]
+            ht.sv[t] = np.sqrt(np.maximum(w[:ranks[t]], 0.0))
+        for mode in group:
+            y, ydims = project_modes(y, ydims, mode, bases[mode])
+    slots = [tree.leaf_of_mode(m) for m in range(d)]
+    todo = sorted([t for t in tree.interior() if t != 0], key=lambda t: (-tree.depth(t), tree.lo[t]))
+    for t in todo:
+        l, r = tree.left[t], tree.right[t]
+        i = slots.index(l)
+        A = unfold(y, ydims, i, i + 2)
+        cap = min(caps[t], A.shape[0], A.shape[1])
+        V, s = top_left_singular(A, cap)
+        ranks[t] = max(1, choose_rank(s ** 2, float(np.sum(y * y)), tau2, cap))
+        V, s = V[:, :ranks[t]], s[:ranks[t]]
+        ht.B[t] = V.reshape((ydims[i], ydims[i + 1], ranks[t]), order="F")
+        ht.sv[t] = s
+        y = fold(V.T @ A, ydims[:i] + [ranks[t]] + ydims[i + 2:], i, i + 1)
+        ydims = ydims[:i] + [ranks[t]] + ydims[i + 2:]
+        slots = slots[:i] + [t] + slots[i + 2:]
+    ht.B[0] = y.reshape((ydims[0], ydims[1], 1), order="F")
+    ht.ranks = ranks
+    return ht, ranks

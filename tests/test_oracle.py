@@ -428,3 +428,93 @@ def test_planted_rsvd_dropped_power_iteration_is_caught():
     ranks = [1, 4, 4, 3, 3, 3, 3]
     ee = O.rel_error(x, O.plain_ht(x, dims, tr, ranks).reconstruct())
     assert abs(O.rel_error(x, bad(x, dims, tr, ranks, p=2, q=30).reconstruct()) - ee) > 1e-3
+
+
+# ---------------------------------------------------------------- rank-adaptive truncation
+ADAPT_DIMS = [8, 7, 6, 8]
+ADAPT_FULL = [1, 48, 48, 8, 7, 6, 8]   # the full rank of every matricisation: no cap binds
+
+
+def _adaptive_input(seed):
+    return G.tucker(ADAPT_DIMS, [4, 4, 4, 4], seed=seed, noise_rel=0.3, dtype=np.float64)
+
+
+@pytest.mark.parametrize("eps", [0.05, 0.1, 0.2, 0.3])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_adaptive_error_within_rel_eps(seed, eps):
+    """With caps that do not bind, ||X - X_HT|| <= rel_eps ||X|| for both truncation orders,
+    and the tail-sum bound of the chosen ranks (ht_error_bound) also sits below rel_eps."""
+    x = _adaptive_input(seed)
+    tr = O.tree_balanced_4d()
+    for fn in (O.plain_ht_adaptive, O.ltr_ht_adaptive):
+        h, ranks = fn(x, ADAPT_DIMS, tr, ADAPT_FULL, eps)
+        assert O.rel_error(x, h.reconstruct()) <= eps
+    _, ranks = O.plain_ht_adaptive(x, ADAPT_DIMS, tr, ADAPT_FULL, eps)
+    assert O.ht_error_bound(x, ADAPT_DIMS, tr, ranks) <= eps
+
+
+def test_adaptive_ranks_are_minimal_for_plain():
+    """Lowering any one node's chosen rank (keeping the shared root-children rank and the
+    nesting) pushes that node's own SVD tail past the per-node budget."""
+    x = _adaptive_input(0)
+    tr = O.tree_balanced_4d()
+    eps = 0.2
+    _, ranks = O.plain_ht_adaptive(x, ADAPT_DIMS, tr, ADAPT_FULL, eps)
+    xsq = float(np.sum(x * x))
+    tau2 = eps ** 2 * xsq / (2 * len(ADAPT_DIMS) - 3)
+    for t in range(1, tr.n_nodes):
+        s = np.linalg.svd(O.unfold(x, ADAPT_DIMS, tr.lo[t], tr.hi[t]), compute_uv=False)
+        tail = lambda r: float(np.sum(s[r:] ** 2))  # noqa: E731
+        assert tail(ranks[t]) <= tau2 * (1 + 1e-12)
+        if t not in (tr.left[0], tr.right[0]) and ranks[t] > 1:
+            assert tail(ranks[t] - 1) > tau2
+
+
+def test_adaptive_closed_form_tucker_ranks():
+    """An exact Tucker tensor (plus 1e-6 noise) with core (3,4,2,5): a small rel_eps recovers
+    the multilinear ranks at the leaves and min(3*4, 2*5) = 10 at the two interior nodes."""
+    dims = [9, 8, 7, 10]
+    x = G.tucker(dims, [3, 4, 2, 5], seed=3, noise_rel=1e-6, dtype=np.float64)
+    tr = O.tree_balanced_4d()
+    caps = [1, 20, 20, 9, 8, 7, 10]
+    for fn in (O.plain_ht_adaptive, O.ltr_ht_adaptive):
+        h, ranks = fn(x, dims, tr, caps, 1e-3)
+        assert ranks == [1, 10, 10, 3, 4, 2, 5], (fn.__name__, ranks)
+        assert O.rel_error(x, h.reconstruct()) <= 1e-5
+    # 3-D, tree {1,2},{3}: leaves (4,3), node {1,2} and node {3} both rank 5
+    x3 = G.tucker([12, 10, 9], [4, 3, 5], seed=4, noise_rel=1e-6, dtype=np.float64)
+    tr3 = O.tree_reference_3d()
+    h, ranks = O.plain_ht_adaptive(x3, [12, 10, 9], tr3, [1, 9, 9, 12, 10], 1e-3)
+    assert ranks == [1, 5, 5, 4, 3]
+
+
+def test_adaptive_max_rank_caps_and_nests():
+    x = _adaptive_input(1)
+    tr = O.tree_balanced_4d()
+    for fn in (O.plain_ht_adaptive, O.ltr_ht_adaptive):
+        _, ranks = fn(x, ADAPT_DIMS, tr, ADAPT_FULL, 1e-6, max_rank=3)
+        assert ranks[1:] == [3] * 6, ranks
+    caps = O.adaptive_caps(tr, [1, 48, 48, 2, 2, 6, 8], 0)
+    assert caps[1] == 4 and caps[2] == 48
+
+
+def _planted_adaptive(fn, old, new):
+    import inspect
+    src = inspect.getsource(fn)
+    assert old in src
+    ns = dict(vars(O))
+    exec(src.replace(old, new), ns)
+    return ns[fn.__name__]
+
+
+def test_planted_adaptive_budget_count_is_caught():
+    """Spending the whole budget at every node (dividing by 1 instead of the number of
+    truncations) breaks the rel_eps guarantee that the bound test checks."""
+    bad_plain = _planted_adaptive(O.plain_ht_adaptive, "max(1, 2 * len(dims) - 3)", "1")
+    bad_ltr = _planted_adaptive(O.ltr_ht_adaptive, "max(1, 2 * d - 2)", "1")
+    x = _adaptive_input(0)
+    tr = O.tree_balanced_4d()
+    for bad in (bad_plain, bad_ltr):
+        h, _ = bad(x, ADAPT_DIMS, tr, ADAPT_FULL, 0.1)
+        assert O.rel_error(x, h.reconstruct()) > 0.1
+
