@@ -1470,9 +1470,24 @@ int Engine::run_rsvd(const float* x) {
   std::vector<int> done(p.n_nodes, 0);
   for (int t = 1; t < p.n_nodes; ++t) {
     if (p.left[t] < 0 || done[t]) continue;
-    if (!(p.lo[t] == 0 || p.hi[t] == p.order)) return HT_ERR_UNSUPPORTED;
     const PMS v = view_of(p.dims, p.order, p.lo[t], p.hi[t]);
     const long rows = v.M, cols = v.P * v.S;
+    if (!(p.lo[t] == 0 || p.hi[t] == p.order)) {
+      // A node strictly inside the mode order (trees of order 5 and up): X_(t) is no
+      // contiguous slab of X, so it takes the exact basis from its Gram (in the run's
+      // precision), as the leaves do. It is never a root child, so no sibling shares it.
+      if (rows > 1024) return HT_ERR_UNSUPPORTED;
+      const int k = p.rank[t];
+      double* G = alloc<double>((size_t)rows * rows);
+      double* w = alloc<double>(k);
+      Ufull[t] = alloc<double>((size_t)rows * k);
+      if (!G || !w || !Ufull[t]) return HT_ERR_ALLOC;
+      CK(gram(x, v, prec, G, nullptr));
+      CK(sym_topk(G, (int)rows, k, Ufull[t], w, opt.eig_tol * 1e3, opt.max_eig_iters, 2000 + t * 16, GRAM_FLOOR));
+      CK(eig_to_sv(w, t));
+      done[t] = 1;
+      continue;
+    }
     int sib = -1;
     if (p.parent[t] == 0) sib = (p.left[0] == t) ? p.right[0] : p.left[0];
     const int k = p.rank[t];

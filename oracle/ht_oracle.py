@@ -520,7 +520,9 @@ def rsvd_ht(x, dims, tree, ranks, p=8, q=1, seed=1234, max_block=64):
     at mode 0 or ends at the last mode by a range finder on X_(t) with the FP32-rounded
     Philox test matrix of stream 256+t (l = min(k + p, 64, rows, cols) columns, k the
     larger of the node's and its complementary sibling's rank); the complementary sibling
-    of a root child takes the right singular vectors of the same projected SVD."""
+    of a root child takes the right singular vectors of the same projected SVD. A node
+    strictly inside the mode order (order >= 5 trees) takes the Gram eigenvectors of its
+    X_(t), as the leaves do."""
     validate_ranks(dims, tree, ranks)
     x = np.asarray(x, dtype=np.float64)
     ht = HT(dims, tree, ranks)
@@ -537,6 +539,12 @@ def rsvd_ht(x, dims, tree, ranks, p=8, q=1, seed=1234, max_block=64):
             continue
         A = unfold(x, dims, tree.lo[t], tree.hi[t])
         rows, cols = A.shape
+        if not (tree.lo[t] == 0 or tree.hi[t] == len(dims)):
+            U, w = gram_topk(A @ A.T, ranks[t])
+            full[t] = ht.U[t] = U
+            ht.sv[t] = np.sqrt(np.maximum(w, 0))
+            done.add(t)
+            continue
         sib = -1
         if parent[t] == 0:
             sib = tree.right[0] if tree.left[0] == t else tree.left[0]

@@ -134,6 +134,26 @@ def test_4d_against_oracle(n, core, r, method):
     check_against_oracle(method, x, dims, ht.tree.balanced_4d(), [1] + [r] * 6, [core] * 4)
 
 
+# Orders 5 and 6 (the reference stops at 4; HT_MAX_ORDER is 8). Tucker core 3 per mode:
+# every interior rank is the signal rank min(prod core inside, prod core outside) of its
+# matricisation. The second tree puts node {2,3} strictly inside the mode order.
+CASES_HIGH = [
+    ([12] * 5, "bal5", [1, 9, 9, 9, 4, 4, 4, 4, 4]),
+    ([12, 10, 8, 10, 12], "mid5", [1, 9, 9, 4, 9, 4, 4, 4, 4]),
+    ([8] * 6, "bal6", [1, 27, 27, 9, 4, 9, 4, 4, 4, 4, 4]),
+]
+
+
+@pytest.mark.parametrize("dims,kind,ranks", CASES_HIGH, ids=[c[1] for c in CASES_HIGH])
+@pytest.mark.parametrize("method", ["plain", "ltr", "rsvd"])
+def test_order5_and_6_against_oracle(dims, kind, ranks, method):
+    tr = {"bal5": ht.tree.balanced(5), "mid5": ht.tree.from_spec(((0, (1, 2)), (3, 4))),
+          "bal6": ht.tree.balanced(6)}[kind]
+    x = G.tucker(dims, [3] * len(dims), seed=4, noise_rel=1e-3)
+    print(f"case dims={dims} tree={kind} method={method}", flush=True)
+    check_against_oracle(method, x, dims, tr, ranks, [3] * len(dims))
+
+
 def test_plain_subspace_iteration_path():
     """Interior nodes with more than 1024 rows and columns take the exact subspace solver."""
     dims = [36, 36, 36, 36]

@@ -9,6 +9,7 @@ such mistakes in copies of the oracle and assert that the checks catch them.
 """
 import numpy as np
 import pytest
+from scipy.linalg import subspace_angles
 
 from oracle import gen as G
 from oracle import ht_oracle as O
@@ -224,7 +225,6 @@ def test_rsvd_opt_reference_quirk():
     om = rng.standard_normal((70, 12))
     U = O.rsvd_opt_reference(A, 4, om)
     Ue, _ = O.top_left_singular(A, 4)
-    from scipy.linalg import subspace_angles
     assert np.max(subspace_angles(U, Ue)) < 1e-8          # the subspace is exact ...
     assert np.max(np.abs(np.abs(U.T @ Ue) - np.eye(4))) > 1e-3  # ... its basis is not A's singular vectors
     B = rng.standard_normal((50, 70)) * np.linspace(1, 0.1, 70)
@@ -338,7 +338,6 @@ def _rsvd_matches_plain_on_captured_range(rsvd):
     """When the sketched matricisation X_(12) has rank <= l = k + p the range finder spans
     it exactly, so the randomised HT equals the exact HT (and the complementary sibling
     {3,4} gets the exact basis from the right singular vectors)."""
-    from scipy.linalg import subspace_angles
     tr = O.tree_balanced_4d()
     dims = [6, 5, 5, 6]  # X_(12) square, so a sibling given the wrong side still has the right shape
     x = G.tucker(dims, [2, 2, 2, 2], seed=3, noise_rel=0.0, dtype=np.float64)
@@ -363,7 +362,6 @@ def test_rsvd_ht_equals_plain_when_the_sketch_captures_the_range():
 def test_rsvd_ht_converges_to_plain_with_power_iterations():
     """q power iterations drive the range finder to the dominant subspace: on a noisy
     input the error gap to the exact HT shrinks with q and vanishes at q = 30."""
-    from scipy.linalg import subspace_angles
     tr = O.tree_balanced_4d()
     dims = [6, 6, 6, 6]
     x = G.tucker(dims, [3, 3, 3, 3], seed=4, noise_rel=0.3, dtype=np.float64)
@@ -383,7 +381,6 @@ def test_rsvd_ht_uses_the_documented_test_matrix():
     """At q = 0 node t's basis is the top-k left singular subspace of A projected onto
     range(A Omega_t), Omega_t = the FP32-rounded Philox normals of stream 256 + t (the
     engine's test matrix), formed here as P_Y A instead of Q^T A."""
-    from scipy.linalg import subspace_angles
     tr = O.tree_balanced_4d()
     dims = [5, 6, 4, 5]
     x = G.tucker(dims, [3, 3, 3, 3], seed=6, noise_rel=0.5, dtype=np.float64)
@@ -429,6 +426,36 @@ def test_planted_rsvd_dropped_power_iteration_is_caught():
     ee = O.rel_error(x, O.plain_ht(x, dims, tr, ranks).reconstruct())
     assert abs(O.rel_error(x, bad(x, dims, tr, ranks, p=2, q=30).reconstruct()) - ee) > 1e-3
 
+
+
+def _middle_node_case():
+    """Order 5, tree ((1,(2,3)),(4,5)): node {2,3} lies strictly inside the mode order."""
+    dims = [6, 5, 4, 5, 6]
+    tr = O.tree_from_spec(((0, (1, 2)), (3, 4)))
+    mid = [t for t in tr.interior() if t != 0 and tr.lo[t] != 0 and tr.hi[t] != len(dims)]
+    assert len(mid) == 1
+    ranks = [1] + [3] * (tr.n_nodes - 1)
+    x = G.tucker(dims, [3] * 5, seed=2, noise_rel=0.5, dtype=np.float64)
+    return x, dims, tr, ranks, mid[0]
+
+
+def _middle_angle(rsvd):
+    x, dims, tr, ranks, t = _middle_node_case()
+    r = rsvd(x, dims, tr, ranks, p=0, q=0)
+    e = O.plain_ht(x, dims, tr, ranks)
+    return float(np.max(subspace_angles(r.U[t], e.U[t])))
+
+
+def test_rsvd_ht_middle_node_takes_the_exact_basis():
+    """A node strictly inside the mode order takes its Gram eigenvectors: the exact top-k
+    left singular subspace of X_(t), even with no oversampling or power iteration on a
+    half-noise input."""
+    assert _middle_angle(O.rsvd_ht) < 1e-8
+
+
+def test_planted_rsvd_middle_node_sketched_is_caught():
+    bad = _rsvd_variant("if not (tree.lo[t] == 0 or tree.hi[t] == len(dims)):", "if False:")
+    assert _middle_angle(bad) > 1e-3
 
 # ---------------------------------------------------------------- rank-adaptive truncation
 ADAPT_DIMS = [8, 7, 6, 8]
