@@ -23,7 +23,7 @@ constexpr int BT_SLD = BT_NMAX + 1;        // LDS leading dimension of staged sl
 constexpr int BT_SCR = 3 * 64 * 64 + 3 * 64 * BT_RMAX;  // doubles of scratch per block
 
 // LDS plan (bytes): phase A (Grams) 2 x 64 x 65 floats; phase B (Jacobi) a, v 2 x 64 x 65
-// doubles + schedule; phase C (projections) slab + U1..U3 + T, V + W + B23 accumulators.
+// doubles + item table; phase C (projections) slab + U1..U3 + T, V + W + B23 accumulators.
 constexpr int BT_LDS_DOUBLES = 2 * SJ * SJLD + 1024;
 // Jacobi item table (3072 uint16) right after a and v, inside the same LDS array.
 static_assert(2 * SJ * SJLD * 8 + 3072 * 2 <= BT_LDS_DOUBLES * 8, "LDS plan");
@@ -34,7 +34,6 @@ __global__ __launch_bounds__(BT_NT, 2) void batch_ht_kernel(const float* __restr
                                                          float* __restrict__ out, double* __restrict__ rel_err,
                                                          double* __restrict__ scratch) {
   __shared__ double lds[BT_LDS_DOUBLES];
-  __shared__ unsigned short sched[(SJ - 1) * (SJ / 2)];
   __shared__ int ordj[SJ];
   __shared__ double red[2][4];
   __shared__ double sgn_s[BT_RMAX];
@@ -139,9 +138,8 @@ __global__ __launch_bounds__(BT_NT, 2) void batch_ht_kernel(const float* __restr
       const int n = m == 0 ? n1 : (m == 1 ? n2 : n3), np = (n + 1) & ~1;
       const int km = m == 0 ? k1x : (m == 1 ? k2 : k3);
       __syncthreads();
-      jacobi_schedule(sched, np);
       jacobi_load(a, v, G + m * 64 * 64, 64, n, np, true);
-      jacobi_sweeps<BT_NT, true>(a, v, n, np, 30, 1e-14, sched,
+      jacobi_sweeps<BT_NT, true>(a, v, n, np, 30, 1e-14,
                            reinterpret_cast<unsigned short*>(lds + 2 * SJ * SJLD));
       jacobi_order(a, np, ordj);
       // canonical signs of the first km columns; one wave per column

@@ -29,26 +29,22 @@ __device__ __forceinline__ double dpp_row_shift(double x) {
 
 // Round-robin pairing for np (even) indices: step s pairs (np-1, s) and
 // ((s+t) mod (np-1), (s-t) mod (np-1)), t = 1 .. np/2-1; every pair meets once per sweep.
-// Entry (s, t) = p | q << 8 with p < q.
-__device__ __forceinline__ void jacobi_schedule(unsigned short* sched, int np) {
-  const int half = np / 2;
-  for (int e = threadIdx.x; e < (np - 1) * half; e += blockDim.x) {
-    const int s = e / half, t = e % half;
-    int p, q;
-    if (t == 0) {
-      p = np - 1;
-      q = s;
-    } else {
-      p = (s + t) % (np - 1);
-      q = (s - t + (np - 1)) % (np - 1);
-    }
-    if (p > q) {
-      const int x = p;
-      p = q;
-      q = x;
-    }
-    sched[e] = (unsigned short)(p | (q << 8));
+// Pair t of step s as (p, q), p < q, computed where it is needed (no table, no LDS read on
+// the step's dependent path).
+__device__ __forceinline__ void jacobi_pair(int s, int t, int np, int& p, int& q) {
+  const int m = np - 1;
+  int x, y;
+  if (t == 0) {
+    x = m;
+    y = s;
+  } else {
+    x = s + t;
+    if (x >= m) x -= m;
+    y = s - t;
+    if (y < 0) y += m;
   }
+  p = min(x, y);
+  q = max(x, y);
 }
 
 // 1/x and 1/sqrt(x) in FP64 from the hardware approximations (v_rcp_f64, v_rsq_f64) and
@@ -91,17 +87,16 @@ __device__ __forceinline__ void jacobi_rotation(double app, double aqq, double a
 // disjoint pairs (phase A, one thread per pair) and then updates every 2x2 block (rows of
 // pair i, columns of pair j) as R_i^T A_ij R_j and the two columns of v per pair (phase
 // B, over one combined item space). Returns 1 when the off-diagonal mass fell below
-// tol^2 * diagonal mass. The schedule must be in sched (jacobi_schedule + barrier).
+// tol^2 * diagonal mass. Pairs come from jacobi_pair.
 // With TAB the per-thread items are kept in `tab` (an LDS array of 3072 entries) instead
 // of in registers, which small blocks need to stay within their register budget.
 template <int NT = SNT, bool TAB = false>
 __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np, int max_sweeps, double tol,
-                                             const unsigned short* sched, unsigned short* tab = nullptr) {
+                                             unsigned short* tab = nullptr) {
   constexpr int NW = NT / 64;
   constexpr int NI = (3072 + NT - 1) / NT;  // items per thread: (np/2)^2 + np * np/2 <= 3072
   static_assert(NI <= 12, "block too small");
   __shared__ double2 rcs[SJ / 2];
-  __shared__ int rpq[SJ / 2];
   __shared__ double red[2][NW];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int half = np / 2, nblk = half * half, nitem = nblk + np * half;
@@ -120,12 +115,13 @@ __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np
 #pragma unroll
     for (int u = 0; u < NI; ++u) it[u] = item(tid + NT * u);
   }
-  auto apply = [&](int packed) {
+  auto apply = [&](int packed, int step) {
     const int kind = packed & 3, x0 = (packed >> 2) & 63, x1 = packed >> 8;
     if (kind == 1) {
       const int bi = x0, bj = x1;
-      const int pq = rpq[bi], PQ = rpq[bj];
-      const int p = pq & 255, q = pq >> 8, P = PQ & 255, Q = PQ >> 8;
+      int p, q, P, Q;
+      jacobi_pair(step, bi, np, p, q);
+      jacobi_pair(step, bj, np, P, Q);
       const double2 cs = rcs[bi], CS = rcs[bj];
       const double c = cs.x, s = cs.y, C = CS.x, S = CS.y;
       const double apP = a[p * SJLD + P], apQ = a[p * SJLD + Q];
@@ -145,8 +141,8 @@ __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np
       a[q * SJLD + Q] = yqQ;
     } else if (kind == 2) {
       const int r = x0, j = x1;
-      const int PQ = rpq[j];
-      const int P = PQ & 255, Q = PQ >> 8;
+      int P, Q;
+      jacobi_pair(step, j, np, P, Q);
       const double2 CS = rcs[j];
       const double vP = v[r * SJLD + P], vQ = v[r * SJLD + Q];
       v[r * SJLD + P] = CS.x * vP - CS.y * vQ;
@@ -179,20 +175,19 @@ __device__ __forceinline__ int jacobi_sweeps(double* a, double* v, int n, int np
     if (off <= tol * tol * dg || off == 0.0) return 1;
     for (int step = 0; step < np - 1; ++step) {
       if (tid < half) {
-        const int pq = sched[step * half + tid];
-        const int p = pq & 255, q = pq >> 8;
+        int p, q;
+        jacobi_pair(step, tid, np, p, q);
         double c, s;
         jacobi_rotation(a[p * SJLD + p], a[q * SJLD + q], a[p * SJLD + q], c, s);
-        rpq[tid] = pq;
         rcs[tid] = make_double2(c, s);
       }
       __syncthreads();
       if constexpr (TAB) {
 #pragma unroll 1
-        for (int e = tid; e < nitem; e += NT) apply(tab[e]);
+        for (int e = tid; e < nitem; e += NT) apply(tab[e], step);
       } else {
 #pragma unroll
-        for (int u = 0; u < NI; ++u) apply(it[u]);
+        for (int u = 0; u < NI; ++u) apply(it[u], step);
       }
       __syncthreads();
     }

@@ -30,7 +30,6 @@ __global__ __launch_bounds__(1024) void jacobi_full_kernel(const double* __restr
                                                           int max_sweeps, double tol) {
   __shared__ double a0[SJ * SJLD];
   __shared__ double v[SJ * SJLD];
-  __shared__ unsigned short sched[(SJ - 1) * (SJ / 2)];
   __shared__ int order[SJ];
   const int bidx = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   A += bidx * sA;
@@ -65,9 +64,8 @@ __global__ __launch_bounds__(1024) void jacobi_full_kernel(const double* __restr
     conv = conv_s;
   } else {
     np = (n + 1) & ~1;
-    jacobi_schedule(sched, np);
     jacobi_load(a, v, A, n, n, np, false);
-    conv = jacobi_sweeps(a, v, n, np, max_sweeps, tol, sched);
+    conv = jacobi_sweeps(a, v, n, np, max_sweeps, tol);
   }
   jacobi_order(a, np, order);
   for (int j = wv; j < n; j += SW) {
@@ -252,6 +250,139 @@ __global__ __launch_bounds__(1024) void power_kernel(const TG* __restrict__ G, l
       if (l2 < b && c2 < nc)
         T[(long)bidx * 64 * 64 + l2 * b + c0 + c2] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
     }
+  }
+}
+
+// Modes 0 and 2 of power_kernel with G in registers for the whole launch, loaded once, so
+// a step costs the work-group its FMAs and one LDS reduction instead of another stream of
+// G through L2 (the repeated stream is what bounds power_kernel). Wave p (of 16) holds
+// columns [p L, p L + L) of G (L = NR / 16), lane l its rows l + 64 i, i < RPT = NR / 64:
+// RPT * L values per thread. Every lane of a wave reads the same block row (an LDS
+// broadcast) and each read feeds RPT FMAs per column. The 16 partial sums of a row meet in
+// LDS. A step is bound by the work-group's FP64 FMAs and LDS traffic, so each work-group
+// takes PW = 1 column: b work-groups, each with the least work per step. TG = float (the
+// FP32 copy of G) or double; NR = 256 (n <= 256: 64 values per thread) or 128 (n <= 128:
+// 16).
+template <typename TG, int NR, int PW>
+__global__ __launch_bounds__(1024) void power_reg_kernel(const TG* __restrict__ G, long sG, const double* __restrict__ W,
+                                                         double* __restrict__ Y, long sWY, int n, int b, int q, int mode,
+                                                         const double* __restrict__ cut, uint64_t gseed,
+                                                         uint32_t gstream) {
+  constexpr int NPART = 16, L = NR / NPART, RPT = NR / 64, CH = RPT * PW >= 4 ? 4 : 8;
+  static_assert(L % CH == 0, "chunks of CH block rows");
+  __shared__ double cur[NR * PW];
+  __shared__ double prv[NR * PW];
+  __shared__ double part[NPART][NR * PW];
+  __shared__ double red[SW][PW];
+  __shared__ double scl[PW];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int c0 = blockIdx.x * PW, bidx = blockIdx.y;
+  const int nc = min(PW, b - c0);
+  G += bidx * sG;
+  const bool gen = W == nullptr;  // start block generated here (Philox normals)
+  if (!gen) W += bidx * sWY;
+  Y += bidx * sWY;
+  double e = 0.0;
+  if (mode == 2) {
+    e = cut[bidx];
+    if (!(e > 0.0)) mode = 0;
+  }
+  const int l0 = wv * L;
+  TG g[RPT][L];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i) {
+    const int r = lane + 64 * i;
+#pragma unroll
+    for (int j = 0; j < L; ++j) g[i][j] = (r < n && l0 + j < n) ? G[r + (long)n * (l0 + j)] : TG(0);
+  }
+  for (int x = tid; x < NR * PW; x += 1024) {
+    const int rr = x / PW, c = x % PW;
+    cur[x] = (rr < n && c < nc)
+                 ? (gen ? philox_normal(gseed, gstream, (uint64_t)(bidx * sWY + (long)rr * b + c0 + c))
+                        : W[(long)rr * b + c0 + c])
+                 : 0.0;
+    prv[x] = 0.0;
+  }
+  __syncthreads();
+  const bool own = tid < NR * PW;  // tid = row * PW + column
+  for (int it = 0; it < q; ++it) {
+    double acc[PW][RPT];
+#pragma unroll
+    for (int c = 0; c < PW; ++c)
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) acc[c][i] = 0.0;
+    // chunks of CH block rows, the scheduler kept from hoisting later chunks' reads (the
+    // resident values of G leave room for one chunk in flight)
+#pragma unroll
+    for (int j0 = 0; j0 < L; j0 += CH) {
+      double cc[CH][PW];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+#pragma unroll
+        for (int c = 0; c < PW; ++c) cc[u][c] = cur[(l0 + j0 + u) * PW + c];
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          // (an empty asm on the value keeps its widening inside the step: hoisted out of
+          // the step loop, the widened values would need twice the registers)
+          TG gv = g[i][j0 + u];
+          asm volatile("" : "+v"(gv));
+          const double gj = (double)gv;
+#pragma unroll
+          for (int c = 0; c < PW; ++c) acc[c][i] = fma(gj, cc[u][c], acc[c][i]);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = lane + 64 * i;
+#pragma unroll
+      for (int c = 0; c < PW; ++c) part[wv][r * PW + c] = acc[c][i];
+    }
+    __syncthreads();
+    double y = 0.0, x = 0.0;
+    if (own) {
+#pragma unroll
+      for (int pp = 0; pp < NPART; ++pp) y += part[pp][tid];
+      x = cur[tid];
+      if (mode == 2) {
+        const double ly = (2.0 * y - e * x) / e;
+        y = (it == 0) ? ly : 2.0 * ly - prv[tid];
+      }
+    }
+    // column normalisation every fourth step and on the last (the span is unchanged)
+    if ((it & 3) == 3 || it == q - 1) {
+      double sq[PW];
+#pragma unroll
+      for (int c = 0; c < PW; ++c) sq[c] = wave_sum((own && (tid % PW) == c) ? y * y : 0.0);
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < PW; ++c) red[wv][c] = sq[c];
+      }
+      __syncthreads();
+      if (tid < PW) {
+        double t = 0.0;
+        for (int w = 0; w < SW; ++w) t += red[w][tid];
+        scl[tid] = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
+      }
+      __syncthreads();
+      if (own) {
+        const double f = scl[tid % PW];
+        y *= f;
+        x *= f;
+      }
+    }
+    if (own) {
+      prv[tid] = x;
+      cur[tid] = y;
+    }
+    __syncthreads();
+  }
+  for (int x = tid; x < n * nc; x += 1024) {
+    const int rr = x / nc, c = x % nc;
+    Y[(long)rr * b + c0 + c] = cur[rr * PW + c];
   }
 }
 
@@ -735,7 +866,6 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
   __shared__ double a[SJ * SJLD];
   __shared__ double v[SJ * SJLD];
   __shared__ double Ms[QR_LDS];
-  __shared__ unsigned short sched[(SJ - 1) * (SJ / 2)];
   __shared__ int order[SJ];
   __shared__ double res[SJ];
   __shared__ double bests[SW][BP];
@@ -761,7 +891,6 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
     v[i * SJLD + j] = (i == j) ? 1.0 : 0.0;
   }
   if (tid < SJ) res[tid] = 0.0;
-  if (!WN) jacobi_schedule(sched, np);
   for (int e = tid; e < n * b; e += SNT) Ms[e] = Q[e];
   __syncthreads();
   if (np != b && tid == 0) {
@@ -779,7 +908,7 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
     if (tid < 64) jacobi16_wave(a, v, max_sweeps, jtol, pad);
     __syncthreads();
   } else {
-    jacobi_sweeps(a, v, b, np, max_sweeps, jtol, sched);
+    jacobi_sweeps(a, v, b, np, max_sweeps, jtol);
   }
   jacobi_order(a, np, order);
   // Ritz vectors u = Q S[:, order[c]] and G u = Z S[:, order[c]] on this thread's rows.
@@ -890,22 +1019,34 @@ int jacobi_full_launch(hipStream_t st, const double* A, long sA, int n, double* 
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
-// Two columns per work-group (twice the work-groups, half the FMAs each: a step is bound
-// by the work-group's stream of G and its FMAs, not by the grid). G32, when given, is an
-// FP32 copy of G that modes 0 and 2 read instead (half the bytes); mode 1 always reads G.
+// G32, when given, is an FP32 copy of G that modes 0 and 2 read instead; mode 1 always
+// reads G. Filter steps keep G in registers (power_reg_kernel, one column per work-group)
+// for FP32 G and for FP64 G with n <= 128; the rest take power_kernel, two columns per
+// work-group.
 int power_launch(hipStream_t st, const double* G, long sG, const double* W, double* Y, long sWY, int n, int b, int q,
                  int mode, const double* cut, double* T, int batch, const float* G32, uint64_t gseed,
                  uint32_t gstream) {
   if (n < 1 || n > 256 || b < 1 || b > 64 || q < 1 || mode < 0 || mode > 2 || batch < 1) return 8;
   if (mode == 2 && !cut) return 8;
   if (T && (mode != 1 || q != 1 || !W)) return 8;
-  const dim3 grid((b + 1) / 2, batch);
-  if (G32 && mode != 1)
-    hipLaunchKernelGGL((power_kernel<2, float>), grid, dim3(1024), 0, st, G32, sG, W, Y, sWY, n, b, q, mode, cut, T,
-                       gseed, gstream);
-  else
+  const dim3 grid((b + 1) / 2, batch), grid1(b, batch);
+  // filter steps (modes 0 and 2) with G held in registers, one column per work-group
+  if (mode != 1 && G32) {
+    if (n <= 128)
+      hipLaunchKernelGGL((power_reg_kernel<float, 128, 1>), grid1, dim3(1024), 0, st, G32, sG, W, Y, sWY, n, b, q,
+                         mode, cut, gseed, gstream);
+    else
+      hipLaunchKernelGGL((power_reg_kernel<float, 256, 1>), grid1, dim3(1024), 0, st, G32, sG, W, Y, sWY, n, b, q,
+                         mode, cut, gseed, gstream);
+  } else if (mode != 1 && n <= 128) {
+    hipLaunchKernelGGL((power_reg_kernel<double, 128, 1>), grid1, dim3(1024), 0, st, G, sG, W, Y, sWY, n, b, q, mode,
+                       cut, gseed, gstream);
+  } else {
+    // one product (mode 1, with T), or FP64 filter steps on n > 128 (128 values per thread
+    // would not fit the register file)
     hipLaunchKernelGGL((power_kernel<2, double>), grid, dim3(1024), 0, st, G, sG, W, Y, sWY, n, b, q, mode, cut, T,
                        gseed, gstream);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
