@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-sample-n", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only time the CPU baseline on the full --n^4 workload (minutes; prints each node)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                                            "pmc_traffic_c2.txt"))
@@ -67,18 +69,20 @@ def read_traffic(path, kernel):
     return None
 
 
-def cpu_baseline(n, rank):
+def cpu_baseline(n, rank, warm=True, progress=None):
     """The reference's CPU-path semantics (oracle.tensorly_ht_fp32) on an n^4 member of
-    the same family, timed on the host cores: one warm-up, then one timed run."""
+    the same family, timed on the host cores: one warm-up (unless warm is False), then
+    one timed run."""
     from oracle import gen as G
     from oracle import ht_oracle as O
     dims = [n] * 4
     x = G.tucker(dims, [16] * 4, seed=1, noise_rel=1e-4)
     tr = O.tree_balanced_4d()
     ranks = [1] + [rank] * 6
-    O.tensorly_ht_fp32(x, dims, tr, ranks)
+    if warm:
+        O.tensorly_ht_fp32(x, dims, tr, ranks)
     t0 = time.perf_counter()
-    _, err = O.tensorly_ht_fp32(x, dims, tr, ranks)
+    _, err = O.tensorly_ht_fp32(x, dims, tr, ranks, progress=progress)
     dt = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": 1.0 / dt, "unit": "decomps/s", "cores": threads, "kind": "port",
@@ -89,6 +93,15 @@ def cpu_baseline(n, rank):
 
 def main():
     args = parse()
+    if args.cpu_full:
+        t0 = time.perf_counter()
+
+        def progress(msg):
+            print(f"[{time.perf_counter() - t0:7.1f} s] {msg}", flush=True)
+
+        print(json.dumps({"cpu_baseline_full": cpu_baseline(args.n, args.rank, warm=False, progress=progress)}),
+              flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one MI355X; "
               "run with --gpus 1", file=sys.stderr)

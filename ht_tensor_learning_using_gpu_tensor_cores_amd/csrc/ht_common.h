@@ -238,11 +238,13 @@ int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b
 int qr_fallback_count(unsigned long long* out);
 int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, const double* T, int n, int b, int k,
               double* Uf, double* U, long sU, double* w, long sw, double* wall, double* cut, double tol,
-              double floor, int* converged, double* rres, int batch);
+              double tol_rel, double floor, int* converged, double* rres, int batch);
 // Casts and sign canonicalisation.
 int cast_f64_f32_launch(hipStream_t st, const double* a, float* b, long n);
 int cast_f32_f64_launch(hipStream_t st, const float* a, double* b, long n);
-int canon_signs_launch(hipStream_t st, double* U, int m, int k, int ldu);
+// ws: canon_signs_ws_doubles(m, k) doubles (none below the split size).
+size_t canon_signs_ws_doubles(int m, int k);
+int canon_signs_launch(hipStream_t st, double* U, int m, int k, int ldu, double* ws);
 int fill_normals_f64_launch(hipStream_t st, double* a, long n, uint64_t seed, uint32_t stream);
 int fill_normals_f32_launch(hipStream_t st, float* a, long n, uint64_t seed, uint32_t stream);
 int copy_cols_f64_launch(hipStream_t st, const double* a, int lda, double* b, int ldb, int m, int n);

@@ -310,7 +310,7 @@ int Engine::gram2(const float* x, PMS v1, PMS v2, int prec, double* G1, double* 
 // Chebyshev filter (degree 8, 16, ...) that damps [0, smallest Ritz value]. A missed
 // round also widens a block narrower than 2k by 16 columns (up to min(64, n)).
 int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
-                           uint32_t stream, double floor, const float* G32in) {
+                           uint32_t stream, double floor, const float* G32in, double tol_rel) {
   ProfScope ps(prof, st, PK_EIG);
   if (n <= 64) {
     double* V = alloc<double>((size_t)batch * n * n);
@@ -405,7 +405,8 @@ int Engine::sym_topk_batch(const double* G, int n, int k, int batch, double* U, 
     CK(power_launch(st, G, (long)n * n, Q, Z, nb, n, b, 1, 1, cut, T, batch));
     // the last replayed round leaves its convergence flags in the verdict slots
     int* cflags = last_replayed ? verdict + n_verdict : conv;
-    CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, floor, cflags, rres, batch));
+    CK(rr_launch(st, Q, Z, nb, T, n, b, k, Uf, U, (long)n * k, w, k, wall, cut, tol, tol_rel, floor, cflags, rres,
+                 batch));
     iters += q + 1;
     const int b_rr = b;  // block width of this round's Ritz values
     int widen = 0;
@@ -597,10 +598,19 @@ int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, 
     }
     steps = std::min(8, steps * 2);
   }
-  CK(canon_signs_launch(st, U, n, k, n));
+  CK(canon_signs(U, n, k, n));
   eig_iters_max = std::max(eig_iters_max, it);
   ++eig_calls;
   return 0;
+}
+
+// Canonical signs of the k columns of U (largest-magnitude entry positive), with the
+// workspace the split pass over tall columns needs.
+int Engine::canon_signs(double* U, int m, int k, int ldu) {
+  const size_t wsd = canon_signs_ws_doubles(m, k);
+  double* ws = wsd ? alloc<double>(wsd) : nullptr;
+  if (wsd && !ws) return HT_ERR_ALLOC;
+  return canon_signs_launch(st, U, m, k, ldu, ws);
 }
 
 // Orthonormal basis of the columns of Y (m x b, FP32) by TSQR; Q (FP32) and R (FP64).
@@ -609,6 +619,47 @@ int Engine::tsqr(float* Y, long m, int b, float* Q, double* R) {
   if (!ws) return HT_ERR_ALLOC;
   ProfScope ps(prof, st, PK_TSQR);
   return tsqr_launch(st, Y, m, b, Q, R, ws);
+}
+
+// Orthonormal basis Q (FP32) of the columns of a tall FP32 Y (m x b) by shifted
+// CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 2020): three passes of W = Y^T Y (FP64
+// accumulation), R^{-1} from the Cholesky factor of W (the first pass equilibrated and
+// shifted, which keeps it defined for a numerically rank-deficient Y), Y <- Y R^{-1}. A
+// pass reads Y twice and writes it once, against TSQR's column-serial Householder per row
+// block; for m = 1,048,576 and b = 40 the passes are bound by HBM, not by latency. Y is
+// left unchanged. When a pivot of a later pass is not positive (an exactly dependent
+// column the shift did not lift), the basis comes from TSQR instead (one host read of the
+// pivot flag).
+static constexpr long TALL_ORTH_ROWS = 1L << 18;  // sketches this tall take tall_orth
+
+int Engine::tall_orth(float* Y, long m, int b, float* Q) {
+  ProfScope ps(prof, st, PK_TSQR);
+  double* W = alloc<double>((size_t)b * b);
+  double* Ri = alloc<double>((size_t)b * b);
+  double* part = alloc<double>((size_t)tall_gram_blocks(m) * b * b);
+  float* T = alloc<float>((size_t)m * b);
+  int* info = alloc<int>(1);
+  if (!W || !Ri || !part || !T || !info) return HT_ERR_ALLOC;
+  HT_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), st));
+  const float* src = Y;
+  for (int pass = 0; pass < 3; ++pass) {
+    float* dst = (pass == 1) ? T : Q;  // Y -> Q -> T -> Q
+    CK(tall_gram_launch(st, src, b, src, b, m, part, W));
+    CK(chol_rinv_launch(st, W, b, m, pass == 0, Ri, info));
+    CK(tall_small_launch(st, src, m, b, Ri, nullptr, nullptr, dst));
+    src = dst;
+  }
+  int hinfo = 0;
+  CK(d2h(&hinfo, info, sizeof(int)));
+  if (hinfo) {
+    double* R = alloc<double>((size_t)b * b);
+    if (!R) return HT_ERR_ALLOC;
+    void* ws = arena->get(tsqr_ws_bytes(m, b));
+    if (!ws) return HT_ERR_ALLOC;
+    HT_HIP_CHECK(hipMemcpyAsync(T, Y, sizeof(float) * m * b, hipMemcpyDeviceToDevice, st));
+    return tsqr_launch(st, T, m, b, Q, R, ws);
+  }
+  return 0;
 }
 
 // Skinny products with a (P,M,S) view where P == 1 (A = X_(t), lda = M) or S == 1
@@ -701,7 +752,7 @@ int Engine::exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U,
   if (!Qd) return HT_ERR_ALLOC;
   CK(cast_f32_f64_launch(st, Q, Qd, rows * b));
   CK(dgemm_small_launch(st, 0, 0, (int)rows, kU, b, 1.0, Qd, (int)rows, Sv, b, 0.0, U, (int)rows));
-  CK(canon_signs_launch(st, U, (int)rows, kU, (int)rows));
+  CK(canon_signs(U, (int)rows, kU, (int)rows));
   std::vector<double> hs(b);
   for (int i = 0; i < b; ++i) hs[i] = std::sqrt(std::max(cur[i], 0.0));
   double* svd = alloc<double>(b);
@@ -721,7 +772,7 @@ int Engine::exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U,
     CK(dgemm_small_launch(st, 0, 0, b, kW, b, 1.0, RtSfull, b, D, b, 0.0, RtS, b));
     CK(cast_f32_f64_launch(st, Qz, Qzd, cols * b));
     CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, b, 1.0, Qzd, (int)cols, RtS, b, 0.0, Wr, (int)cols));
-    CK(canon_signs_launch(st, Wr, (int)cols, kW, (int)cols));
+    CK(canon_signs(Wr, (int)cols, kW, (int)cols));
   }
   return sync();
 }
@@ -983,7 +1034,7 @@ int Engine::run_plain(const float* x) {
       CK(view_gemm(x, v, 0, Vf, k, PREC_F32, Y));
       CK(tsqr(Y, rows, k, Qf, R));
       CK(cast_f32_f64_launch(st, Qf, Ufull[t], rows * k));
-      CK(canon_signs_launch(st, Ufull[t], (int)rows, k, (int)rows));
+      CK(canon_signs(Ufull[t], (int)rows, k, (int)rows));
     } else if (p.lo[t] == 0 || p.hi[t] == p.order) {
       // Both dimensions large: subspace iteration; the complementary sibling (a root
       // child) takes the right singular vectors of the same problem.
@@ -1342,12 +1393,14 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
       CK(gram_pms_f64_launch(st, C, v, G, G32));
       // An interior Gram's wanted spectrum is typically flat (the leaf core's unfolding): a
       // 32-column block (a multiple of 16 keeps the MFMA CholeskyQR) and 24 plain powers
-      // first, so one round usually meets the tolerance. The tolerance is on the Ritz
-      // residuals, 2e-3 theta_1: what the truncation error depends on is the energy the k
-      // kept vectors capture, and Rayleigh-Ritz makes that error quadratic in the residual
-      // over the gap to the block's unconverged rest (the k wanted values sit in a block of
-      // 2k), ~1e-6 theta_1 here, far below the error's own scale.
-      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 2e-3, 24, 32, 4000 + t * 16, 1e-12, G32));
+      // first, so one round usually converges. A Ritz pair converges when its residual is
+      // at most 1e-4 theta_1 or 2e-3 of its own value: what the truncation error depends
+      // on is the energy the k kept vectors capture, and Rayleigh-Ritz makes the energy
+      // error of a pair quadratic in its residual over the gap to the block's unconverged
+      // rest (the k wanted values sit in a block of 2k), so 2e-3 of a pair's own value
+      // costs ~4e-6 of its energy. Pairs far below theta_1 (steep spectra) keep the
+      // absolute test.
+      CK(sym_topk_batch(G, (int)M, k, 1, V, w, 1e-4, 24, 32, 4000 + t * 16, 1e-12, G32, 2e-3));
     }
     CK(eig_to_sv(wsv, t));
     if (adaptive) {
@@ -1380,13 +1433,16 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
 int Engine::range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z,
                          float* Qz, double* R) {
   const long rows = v.M, cols = v.P * v.S;
+  // tall sketches (the million-row side of a root child) take shifted CholeskyQR3, the
+  // rest TSQR
+  auto orth = [&](float* A, long m, float* Qo) { return m >= TALL_ORTH_ROWS ? tall_orth(A, m, l, Qo) : tsqr(A, m, l, Qo, R); };
   CK(view_gemm(x, v, 0, Om, l, prec, Y));
-  CK(tsqr(Y, rows, l, Q, R));
+  CK(orth(Y, rows, Q));
   for (int q = 0; q < opt.power_iters; ++q) {
     CK(view_gemm(x, v, 1, Q, l, prec, Z));
-    CK(tsqr(Z, cols, l, Qz, R));
+    CK(orth(Z, cols, Qz));
     CK(view_gemm(x, v, 0, Qz, l, prec, Y));
-    CK(tsqr(Y, rows, l, Q, R));
+    CK(orth(Y, rows, Q));
   }
   return 0;
 }
@@ -1546,7 +1602,7 @@ int Engine::run_rsvd(const float* x) {
     Ufull[t] = alloc<double>((size_t)rows * k);
     CK(cast_f32_f64_launch(st, Q, Qd, rows * l));
     CK(dgemm_small_launch(st, 0, 0, (int)rows, k, l, 1.0, Qd, (int)rows, Sv, l, 0.0, Ufull[t], (int)rows));
-    CK(canon_signs_launch(st, Ufull[t], (int)rows, k, (int)rows));
+    CK(canon_signs(Ufull[t], (int)rows, k, (int)rows));
     double* sv = alloc<double>(l);
     CK(sqrt_clamp_launch(st, th, l, sv));
     SVd[t] = sv;
@@ -1559,7 +1615,7 @@ int Engine::run_rsvd(const float* x) {
       CK(scale_cols_inv_launch(st, RS, l, kW, sv));
       CK(cast_f32_f64_launch(st, Qz, Qzd, cols * l));
       CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, l, 1.0, Qzd, (int)cols, RS, l, 0.0, Ufull[sib], (int)cols));
-      CK(canon_signs_launch(st, Ufull[sib], (int)cols, kW, (int)cols));
+      CK(canon_signs(Ufull[sib], (int)cols, kW, (int)cols));
       SVd[sib] = sv;
       SVsq[sib] = 0;
       done[sib] = 1;

@@ -161,10 +161,12 @@ struct Engine {
                double floor);
   // G32in (optional): an FP32 copy of G for the filter steps (used when tol >= 1e-5).
   int sym_topk_batch(const double* G, int n, int k, int batch, double* U, double* w, double tol, int q0, int b,
-                     uint32_t stream, double floor, const float* G32in = nullptr);
+                     uint32_t stream, double floor, const float* G32in = nullptr, double tol_rel = 0.0);
+  int canon_signs(double* U, int m, int k, int ldu);
   int sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
                      double floor);
   int tsqr(float* Y, long m, int b, float* Q, double* R);
+  int tall_orth(float* Y, long m, int b, float* Q);
   int view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y);
   int exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U, double* Wr, double* sv, uint32_t stream);
   int contract_transfer(const double* Ut, const double* Ul, const double* Ur, long nl, long nr, int kl, int kr,

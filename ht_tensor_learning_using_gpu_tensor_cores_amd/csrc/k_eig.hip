@@ -163,9 +163,16 @@ __global__ __launch_bounds__(256) void jacobi_eig_kernel(const double* __restric
   }
 }
 
+// Contiguous problems (lda = ldv = n) take the 1024-thread solver of the batched eigen
+// problems (jacobi_full_kernel: the same sorted, sign-canonical result, its steps spread
+// over 16 waves); tolerances below 1e-14, which FP64 rounding of a sweep does not reach
+// reliably, are raised to 1e-14.
 int jacobi_eig_launch(hipStream_t st, const double* A, int n, int lda, double* w, double* V, int ldv, int* info,
                       int max_sweeps, double tol) {
   if (n < 1 || n > JMAX) return 8;
+  if (lda == n && ldv == n)
+    return jacobi_full_launch(st, A, (long)n * n, n, w, n, V, (long)n * n, 1, info, max_sweeps,
+                              tol < 1e-14 ? 1e-14 : tol);
   hipLaunchKernelGGL(jacobi_eig_kernel, dim3(1), dim3(256), 0, st, A, n, lda, 0L, w, 0L, V, ldv, 0L, info,
                      max_sweeps, tol);
   return hipGetLastError() == hipSuccess ? 0 : 5;
@@ -294,7 +301,7 @@ int dgemm_small_launch(hipStream_t st, int ta, int tb, int m, int n, int k, doub
 int dgemm_splitk_slices(int m, int n, int k) {
   const long tiles = (long)((m + DG_T - 1) / DG_T) * ((n + DG_T - 1) / DG_T);
   long z = 512 / tiles;  // about two work-groups per CU
-  const long zk = k / 1024;  // at least 1024 of k per slice
+  const long zk = k / 128;  // at least 128 of k per slice (a b x b Gram of 1024 rows: 8 slices)
   if (z > zk) z = zk;
   if (z > 1024) z = 1024;
   return z < 1 ? 1 : (int)z;
@@ -361,20 +368,44 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
     __syncthreads();
     for (int i = k + 1 + tid; i < b; i += 256) L[i * JLD + k] /= L[k * JLD + k];
     __syncthreads();
-    const int r = b - k - 1;
-    for (int e = tid; e < r * r; e += 256) {
-      const int i = k + 1 + e % r, j = k + 1 + e / r;
-      if (j <= i) L[i * JLD + j] -= L[i * JLD + k] * L[j * JLD + k];
+    // trailing update of the lower triangle, four entries a round: their loads are all
+    // issued before the stores (stores between loads would order every load behind them)
+    const int r = b - k - 1, nl = r * (r + 1) / 2;
+    for (int e0 = tid; e0 < nl; e0 += 4 * 256) {
+      int ii[4], jj[4];
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + 256 * u;
+        // e -> (i, j), j <= i, over the packed lower triangle of the trailing square
+        int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        if ((i + 1) * (i + 2) / 2 <= e) ++i;
+        if (i * (i + 1) / 2 > e) --i;
+        ii[u] = k + 1 + i;
+        jj[u] = k + 1 + (e - i * (i + 1) / 2);
+        v[u] = e < nl ? L[ii[u] * JLD + jj[u]] - L[ii[u] * JLD + k] * L[jj[u] * JLD + k] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (e0 + 256 * u < nl) L[ii[u] * JLD + jj[u]] = v[u];
     }
     __syncthreads();
   }
-  // T = L^-1 (lower), column j by thread j
+  // T = L^-1 (lower), column j by thread j; four partial sums keep the row's loads in
+  // flight instead of one dependent chain of length i - j
   if (tid < b) {
     const int j = tid;
     for (int i = 0; i < b; ++i) {
-      double s = (i == j) ? 1.0 : 0.0;
-      for (int q = j; q < i; ++q) s -= L[i * JLD + q] * T[q * JLD + j];
-      T[i * JLD + j] = i < j ? 0.0 : s / L[i * JLD + i];
+      double s0 = (i == j) ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int q = j;
+      for (; q + 4 <= i; q += 4) {
+        s0 -= L[i * JLD + q] * T[q * JLD + j];
+        s1 -= L[i * JLD + q + 1] * T[(q + 1) * JLD + j];
+        s2 -= L[i * JLD + q + 2] * T[(q + 2) * JLD + j];
+        s3 -= L[i * JLD + q + 3] * T[(q + 3) * JLD + j];
+      }
+      for (; q < i; ++q) s0 -= L[i * JLD + q] * T[q * JLD + j];
+      T[i * JLD + j] = i < j ? 0.0 : ((s0 + s1) + (s2 + s3)) / L[i * JLD + i];
     }
   }
   __syncthreads();
@@ -599,6 +630,104 @@ __global__ void canon_signs_kernel(double* __restrict__ U, int m, int ldu) {
   if (sg < 0.0)
     for (int r = tid; r < m; r += 256) U[r + (long)ldu * j] = -U[r + (long)ldu * j];
 }
+// Canonical signs of tall columns (m > CS_SPLIT rows) in two passes over many blocks: a
+// one-block-per-column scan streams the column serially, which for the million-row bases
+// of a root child is most of the time of a decomposition. Pass 1: block (j, slice) finds
+// the largest |U[r][j]| over its CS_ROWS rows (ties to the smaller row) and records
+// (|value|, row, value) in part[j][slice]. Pass 2: block (j, slice) reduces the column's
+// records the same way and negates its rows when the recorded value is negative (the
+// value is taken from the record, not re-read, so no block reads a row another block is
+// negating).
+constexpr int CS_ROWS = 8192, CS_SPLIT = 16384;
+__global__ __launch_bounds__(256) void canon_max_kernel(const double* __restrict__ U, int m, int ldu,
+                                                        double* __restrict__ part, int S) {
+  __shared__ double bv[4], bs[4];
+  __shared__ int bix[4];
+  const int j = blockIdx.x, sl = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int r0 = sl * CS_ROWS, r1 = min(m, r0 + CS_ROWS);
+  double best = -1.0, sv = 0.0;
+  int bi = 0;
+#pragma unroll 8
+  for (int r = r0 + tid; r < r1; r += 256) {
+    const double x = U[r + (long)ldu * j];
+    if (fabs(x) > best) {
+      best = fabs(x);
+      bi = r;
+      sv = x;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64), os = __shfl_xor(sv, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+      sv = os;
+    }
+  }
+  if (lane == 0) {
+    bv[wv] = best;
+    bix[wv] = bi;
+    bs[wv] = sv;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < 4; ++w)
+      if (bv[w] > bv[0] || (bv[w] == bv[0] && bix[w] < bix[0])) {
+        bv[0] = bv[w];
+        bix[0] = bix[w];
+        bs[0] = bs[w];
+      }
+    double* o = part + 3 * ((long)j * S + sl);
+    o[0] = bv[0];
+    o[1] = (double)bix[0];
+    o[2] = bs[0];
+  }
+}
+__global__ __launch_bounds__(256) void canon_flip_kernel(double* __restrict__ U, int m, int ldu,
+                                                         const double* __restrict__ part, int S) {
+  __shared__ double bv[4], bs[4];
+  __shared__ int bix[4];
+  const int j = blockIdx.x, sl = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double best = -1.0, sv = 0.0;
+  int bi = 0;
+  for (int e = tid; e < S; e += 256) {
+    const double* o = part + 3 * ((long)j * S + e);
+    const int oi = (int)o[1];
+    if (o[0] > best || (o[0] == best && oi < bi)) {
+      best = o[0];
+      bi = oi;
+      sv = o[2];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64), os = __shfl_xor(sv, o, 64);
+    const int oi = __shfl_xor(bi, o, 64);
+    if (ob > best || (ob == best && oi < bi)) {
+      best = ob;
+      bi = oi;
+      sv = os;
+    }
+  }
+  if (lane == 0) {
+    bv[wv] = best;
+    bix[wv] = bi;
+    bs[wv] = sv;
+  }
+  __syncthreads();
+  double s0 = bs[0], b0 = bv[0];
+  int i0 = bix[0];
+  for (int w = 1; w < 4; ++w)
+    if (bv[w] > b0 || (bv[w] == b0 && bix[w] < i0)) {
+      b0 = bv[w];
+      i0 = bix[w];
+      s0 = bs[w];
+    }
+  if (s0 < 0.0) {
+    const int r0 = sl * CS_ROWS, r1 = min(m, r0 + CS_ROWS);
+    for (int r = r0 + tid; r < r1; r += 256) U[r + (long)ldu * j] = -U[r + (long)ldu * j];
+  }
+}
 template <typename T>
 __global__ void sumsq_stage1(const T* __restrict__ a, long n, double* __restrict__ part) {
   __shared__ double red[4];
@@ -677,9 +806,19 @@ int copy_cols_f64_launch(hipStream_t st, const double* a, int lda, double* b, in
   hipLaunchKernelGGL(copy_cols_f64_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a, lda, b, ldb, m, n);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
-int canon_signs_launch(hipStream_t st, double* U, int m, int k, int ldu) {
-  if (k < 1) return 0;
-  hipLaunchKernelGGL(canon_signs_kernel, dim3(k), dim3(256), 0, st, U, m, ldu);
+size_t canon_signs_ws_doubles(int m, int k) {
+  return m > CS_SPLIT ? (size_t)3 * k * ((m + CS_ROWS - 1) / CS_ROWS) : 0;
+}
+int canon_signs_launch(hipStream_t st, double* U, int m, int k, int ldu, double* ws) {
+  if (k < 1 || m < 1) return 0;
+  if (m <= CS_SPLIT) {
+    hipLaunchKernelGGL(canon_signs_kernel, dim3(k), dim3(256), 0, st, U, m, ldu);
+  } else {
+    if (!ws) return 8;
+    const int S = (m + CS_ROWS - 1) / CS_ROWS;
+    hipLaunchKernelGGL(canon_max_kernel, dim3(k, S), dim3(256), 0, st, U, m, ldu, ws, S);
+    hipLaunchKernelGGL(canon_flip_kernel, dim3(k, S), dim3(256), 0, st, U, m, ldu, ws, S);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 // out must hold 1 + 1024 doubles (part scratch after the result).

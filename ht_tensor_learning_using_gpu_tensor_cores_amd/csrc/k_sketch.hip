@@ -233,7 +233,10 @@ __global__ __launch_bounds__(256) void sketch_solve_kernel(const double* __restr
   }
 }
 
-// out (rows x w, FP32) = A (rows x w, FP32) C (w x w, FP64)  [+ beta[0] * B].
+// out (rows x w, FP32) = A (rows x w, FP32) C (w x w, FP64)  [+ beta[0] * B]. One
+// thread per row: its w values of A are read once (coalesced across the wave for each
+// column) and kept in registers, and the outputs are formed 16 columns at a time, so A
+// is streamed once instead of once per output column.
 __global__ __launch_bounds__(256) void tall_small_kernel(const float* __restrict__ A, long rows, int w,
                                                          const double* __restrict__ C, const float* __restrict__ B,
                                                          const double* __restrict__ beta, float* __restrict__ out) {
@@ -241,14 +244,32 @@ __global__ __launch_bounds__(256) void tall_small_kernel(const float* __restrict
   for (int e = threadIdx.x; e < w * w; e += blockDim.x) cs[e] = C[e];
   __syncthreads();
   const double bt = B ? beta[0] : 0.0;
-  const long total = rows * w;
-  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    const long r = idx % rows;
-    const int j = (int)(idx / rows);
-    double s = 0.0;
-    for (int k = 0; k < w; ++k) s += (double)A[r + rows * k] * cs[k + w * j];
-    if (B) s += bt * (double)B[idx];
-    out[idx] = (float)s;
+  for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (long)gridDim.x * blockDim.x) {
+    float a[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) a[k] = k < w ? A[r + rows * k] : 0.f;
+    for (int j0 = 0; j0 < w; j0 += 16) {
+      double acc[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) acc[u] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 64; ++k) {
+        if (k < w) {
+          const double ak = (double)a[k];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) acc[u] = fma(ak, cs[k + w * min(j0 + u, w - 1)], acc[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int j = j0 + u;
+        if (j < w) {
+          double v = acc[u];
+          if (B) v += bt * (double)B[r + rows * j];
+          out[r + rows * j] = (float)v;
+        }
+      }
+    }
   }
 }
 
@@ -370,7 +391,7 @@ int sketch_solve_launch(hipStream_t st, const double* W, const double* E, const 
 int tall_small_launch(hipStream_t st, const float* A, long rows, int w, const double* C, const float* B,
                       const double* beta, float* out) {
   if (w < 1 || w > 64) return 8;
-  hipLaunchKernelGGL(tall_small_kernel, dim3(grid_for(rows * w)), dim3(256), 0, st, A, rows, w, C, B, beta, out);
+  hipLaunchKernelGGL(tall_small_kernel, dim3(grid_for(rows)), dim3(256), 0, st, A, rows, w, C, B, beta, out);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 

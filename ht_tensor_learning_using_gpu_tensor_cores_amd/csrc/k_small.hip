@@ -850,7 +850,7 @@ __global__ __launch_bounds__(1024) void qr_kernel(const double* __restrict__ Y, 
 // Rayleigh-Ritz from T = Q^T G Q (row-major b x b, formed with Z = G Q): Jacobi on
 // sym(T) = S Th S^T; Uf = Q S (all b Ritz vectors, descending, row-major), U = first k
 // (column-major, canonical signs), w = Th[:k], wall = Th, cut = Th[b-1], converged = 1
-// when every top-k Ritz residual ||Z s_i - th_i Q s_i|| <= tol * |th_1|, pairs with
+// when every top-k Ritz residual ||Z s_i - th_i Q s_i|| <= max(tol |th_1|, tol_rel |th_i|), pairs with
 // |th_i| <= floor * |th_1| exempt (a cluster below what the Gram resolves); rres (may be
 // null) receives that largest residual over |th_1|. Q and then Z are
 // staged in LDS; thread (g, c, h) forms column c of the Ritz vectors on its rows.
@@ -859,9 +859,9 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
                                                   const double* __restrict__ T, int n, int b, int k,
                                                   double* __restrict__ Uf, double* __restrict__ U, long sU,
                                                   double* __restrict__ w, long sw, double* __restrict__ wall,
-                                                  double* __restrict__ cut, double tol, double floor,
-                                                  int* __restrict__ converged, double* __restrict__ rres,
-                                                  int max_sweeps) {
+                                                  double* __restrict__ cut, double tol, double tol_rel,
+                                                  double floor, int* __restrict__ converged,
+                                                  double* __restrict__ rres, int max_sweeps) {
   constexpr int RS = 64 / BP;
   __shared__ double a[SJ * SJLD];
   __shared__ double v[SJ * SJLD];
@@ -1004,9 +1004,16 @@ __global__ __launch_bounds__(1024) void rr_kernel(const double* __restrict__ Q, 
   if (tid == 0) {
     const double th1 = fabs(a[order[0] * SJLD + order[0]]);
     double rmax = 0.0;
-    for (int j = 0; j < k; ++j)
-      if (fabs(a[order[j] * SJLD + order[j]]) > floor * th1) rmax = fmax(rmax, sqrt(res[j]));
-    converged[bidx] = (rmax <= tol * th1 || th1 == 0.0) ? 1 : 0;
+    int ok = 1;
+    for (int j = 0; j < k; ++j) {
+      const double thj = fabs(a[order[j] * SJLD + order[j]]);
+      if (thj > floor * th1) {
+        const double rj = sqrt(res[j]);
+        rmax = fmax(rmax, rj);
+        if (!(rj <= fmax(tol * th1, tol_rel * thj))) ok = 0;
+      }
+    }
+    converged[bidx] = (ok || th1 == 0.0) ? 1 : 0;
     if (rres) rres[bidx] = th1 > 0.0 ? rmax / th1 : 0.0;
     cut[bidx] = a[order[b - 1] * SJLD + order[b - 1]];
   }
@@ -1064,14 +1071,14 @@ int qr_launch(hipStream_t st, const double* Y, double* Q, long sYQ, int n, int b
 }
 
 int rr_launch(hipStream_t st, const double* Q, const double* Z, long sQ, const double* T, int n, int b, int k,
-              double* Uf, double* U, long sU, double* w, long sw, double* wall, double* cut, double tol, double floor,
-              int* converged, double* rres, int batch) {
+              double* Uf, double* U, long sU, double* w, long sw, double* wall, double* cut, double tol,
+              double tol_rel, double floor, int* converged, double* rres, int batch) {
   if (n < 1 || n > SW * SR || b < 1 || b > SJ || b > n || n * b > QR_LDS || k < 1 || k > b || batch < 1) return 8;
   const dim3 grid(batch), block(SNT);
   const int sweeps = 20;
 #define HT_RR(BPv, SRv)                                                                                          \
   hipLaunchKernelGGL((rr_kernel<BPv, SRv>), grid, block, 0, st, Q, Z, sQ, T, n, b, k, Uf, U, sU, w, sw, wall, cut, \
-                     tol, floor, converged, rres, sweeps)
+                     tol, tol_rel, floor, converged, rres, sweeps)
   if (b <= 16 && n <= 128) HT_RR(16, 2);
   else if (b <= 32 && n <= 128) HT_RR(32, 4);
   else if (b <= 32) HT_RR(32, 8);

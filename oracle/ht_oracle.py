@@ -452,12 +452,13 @@ def ltr_ht(x, dims, tree, ranks, leaf_groups=None):
 
 # ---------------------------------------------------- TensorLy CPU-baseline port
 
-def tensorly_ht_fp32(x, dims, tree, ranks):
+def tensorly_ht_fp32(x, dims, tree, ranks, progress=None):
     """FP32 CPU restatement of the reference's TensorLy path (3P/HT_TensorLy.py:51-87,
     4P/HT_4d_TensorLy.py:50-98): leaves by Gram X_(n) X_(n)^T + symmetric eig (top-k,
     sorted -- torch.eig's unsorted order is a documented deviation), interior nodes by
     a truncated SVD of the matricisation, B_t by mode products with the child bases,
-    then reconstruction and the relative error. Returns (HT, rel_err).
+    then reconstruction and the relative error. Returns (HT, rel_err). progress, if
+    given, is called with a short text after each node and stage.
     """
     x32 = np.asarray(x, dtype=np.float32)
     ht = HT(dims, tree, ranks)
@@ -480,6 +481,8 @@ def tensorly_ht_fp32(x, dims, tree, ranks):
         full[t] = U.astype(np.float32)
         if tree.is_leaf(t):
             ht.U[t] = full[t]
+        if progress:
+            progress(f"node {t} ({A.shape[0]} x {A.shape[1]}) done")
     for t in tree.interior():
         l, r = tree.left[t], tree.right[t]
         nl, nr = node_rows(dims, tree, l), node_rows(dims, tree, r)
@@ -488,8 +491,12 @@ def tensorly_ht_fp32(x, dims, tree, ranks):
             ht.B[0] = (full[l].T @ Xm @ full[r])[:, :, None]
         else:
             ht.B[t] = contract_transfer(full[t], full[l], full[r], nl, nr).astype(np.float32)
+    if progress:
+        progress("transfer tensors done")
     xr = ht.reconstruct().astype(np.float32)
     err = float(np.linalg.norm(x32 - xr) / np.linalg.norm(x32))
+    if progress:
+        progress("reconstruction and error done")
     return ht, err
 
 
