@@ -16,6 +16,12 @@ latency is measured beside it, as are plain FP32 exact truncated-SVD HT (the com
 on the same GPU), the randomised method at p = 8, q = 1, and the reference's CPU path
 (its TensorLy semantics, restated in FP32 numpy) on a bounded sample on the host.
 
+Timed regions: (1) the K steps with `--inflight` decompositions in flight (the `value`);
+(2) decompositions one at a time with HIP events around the three passes over X on every
+fourth step (the per-kernel roofline: events on concurrent streams would serialise them).
+`--cpu-full` instead times only the CPU baseline on the full workload (minutes; prints each
+node of the tree as it finishes).
+
 Prints one JSON line (rank 0, N = 1).
 """
 import argparse
