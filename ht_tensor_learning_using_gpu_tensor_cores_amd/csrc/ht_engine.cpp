@@ -523,10 +523,10 @@ int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double
 // FP64 GEMM), the triangular inverse (one work-group) and Z <- Z R^-1; the first pass
 // equilibrates the columns and shifts W (chol_rinv_kernel). Zt is an m x b scratch block.
 static int cholqr3(hipStream_t st, double* Z, double* Zt, long m, int b, double* W, double* Rinv, int* info,
-                   uint64_t seed, uint32_t stream, double* ws) {
+                   uint64_t seed, uint32_t stream, double* ws, int passes = 3) {
   double* src = Z;
   double* dst = Zt;
-  for (int pass = 0; pass < 3; ++pass) {
+  for (int pass = 0; pass < passes; ++pass) {
     CK(dgemm_splitk_launch(st, 1, 0, b, b, (int)m, 1.0, src, (int)m, src, (int)m, 0.0, W, b, ws));
     if (pass == 0) CK(refresh_zero_cols_launch(st, src, m, b, W, seed, stream));
     CK(chol_rinv_launch(st, W, b, m, pass == 0, Rinv, info));
@@ -563,7 +563,10 @@ int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, 
   while (!done) {
     for (int s = 0; s < steps; ++s) {
       CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, G, n, Q, n, 0.0, Y, n));
-      CK(cholqr3(st, Y, Zt, n, b, W, Ri, info, opt.seed, stream + 2 + it, ws));
+      // between Rayleigh-Ritz steps one shifted CholeskyQR pass keeps the block well
+      // conditioned (kappa ~ sqrt(u) kappa(Y)) for the next product; the block the
+      // Rayleigh-Ritz step reads is orthonormalised by all three passes
+      CK(cholqr3(st, Y, Zt, n, b, W, Ri, info, opt.seed, stream + 2 + it, ws, s + 1 == steps ? 3 : 1));
       std::swap(Q, Y);
       ++it;
     }

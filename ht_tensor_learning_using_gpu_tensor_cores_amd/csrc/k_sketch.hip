@@ -120,12 +120,123 @@ __global__ __launch_bounds__(256) void tall_gram_kernel(const float* __restrict_
   }
 }
 
-__global__ void tall_gram_reduce_kernel(const double* __restrict__ part, int nb, int np, double* __restrict__ out) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
+// Partial Y^T Y of a tall FP32 Y (rows x b, ld rows, 16 <= b <= 64) on FP64 MFMA
+// (16x16x4): block blk takes 128-row chunks blk, blk + gridDim.x, ..., stages each as FP64
+// in LDS (row-major, one conversion per value), and wave w accumulates every upper 16 x 16
+// tile pair (I <= J, at most 10) over rows [32 w, 32 w + 32) of each chunk, so the waves
+// carry equal MFMA work (FP64 MFMA issues at a quarter of the BF16 rate: it is the unit
+// that bounds this kernel). The next chunk is loaded into registers while the current one
+// is multiplied. The waves' tiles are summed in a fixed order through LDS and the block's
+// b x b partial (column-major, both triangles) goes to part[blk]. The products of FP32
+// values are exact in FP64, so the sums are FP64 sums of exact terms, as in
+// tall_gram_kernel, whose VALU form spends two conversions per product.
+constexpr int TS_ROWS = 128, TS_LD = 65, TS_PER = TS_ROWS * 64 / 256;
+__global__ __launch_bounds__(256) void tall_syrk_kernel(const float* __restrict__ Y, long rows, int b,
+                                                        double* __restrict__ part) {
+  __shared__ double Ys[TS_ROWS * TS_LD];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int bt = (b + 15) / 16, np = bt * (bt + 1) / 2;
+  const int bp = 16 * bt;  // staged (padded) width
+  f64x4 acc[10];
+#pragma unroll
+  for (int u = 0; u < 10; ++u) acc[u] = f64x4{};
+  const long n_chunks = (rows + TS_ROWS - 1) / TS_ROWS;
+  float nxt[TS_PER];
+  auto load = [&](long ch) {
+    const long r0 = ch * TS_ROWS;
+#pragma unroll
+    for (int q = 0; q < TS_PER; ++q) {
+      const int e = tid + 256 * q, row = e % TS_ROWS, col = e / TS_ROWS;
+      const long r = r0 + row;
+      nxt[q] = (col < b && r < rows) ? Y[r + rows * col] : 0.f;
+    }
+  };
+  long ch = blockIdx.x;
+  if (ch < n_chunks) load(ch);
+  for (; ch < n_chunks; ch += gridDim.x) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < TS_PER; ++q) {
+      const int e = tid + 256 * q, row = e % TS_ROWS, col = e / TS_ROWS;
+      if (col < bp) Ys[row * TS_LD + col] = (double)nxt[q];
+    }
+    __syncthreads();
+    if (ch + gridDim.x < n_chunks) load(ch + gridDim.x);
+#pragma unroll
+    for (int k0 = 0; k0 < 32; k0 += 4) {
+      const double* yr = Ys + (32 * w + k0 + (lane >> 4)) * TS_LD + (lane & 15);
+      double v[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) v[t] = t < bt ? yr[16 * t] : 0.0;
+      int u = 0;
+#pragma unroll
+      for (int I = 0; I < 4; ++I)
+#pragma unroll
+        for (int J = I; J < 4; ++J) {
+          if (J < bt) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(v[I], v[J], acc[u], 0, 0, 0);
+          ++u;
+        }
+    }
+  }
+  // sum the four waves' tiles in LDS (wave order fixed), then write the b x b partial
+  __syncthreads();
+  double* red = Ys;  // 4 x np x 256 doubles <= 10240 fits the staging array (8320) in two halves
+  double* out = part + (long)blockIdx.x * b * b;
+  for (int half = 0; half < 2; ++half) {
+    const int u0 = 5 * half;
+    __syncthreads();
+    int u = 0;
+#pragma unroll
+    for (int I = 0; I < 4; ++I)
+#pragma unroll
+      for (int J = I; J < 4; ++J) {
+        if (u >= u0 && u < u0 + 5 && J < bt) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) red[((w * 5 + (u - u0)) * 4 + e) * 64 + lane] = acc[u][e];
+        }
+        ++u;
+      }
+    __syncthreads();
+    if (w == 0) {
+      int uu = 0;
+#pragma unroll
+      for (int I = 0; I < 4; ++I)
+#pragma unroll
+        for (int J = I; J < 4; ++J) {
+          if (uu >= u0 && uu < u0 + 5 && J < bt) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int o = ((uu - u0) * 4 + e) * 64 + lane;
+              const double t = (red[o] + red[o + 5 * 256]) + (red[o + 10 * 256] + red[o + 15 * 256]);
+              const int i = 16 * I + acc16_row_f64(e, lane), j = 16 * J + (lane & 15);
+              if (i < b && j < b) {
+                out[i + (long)b * j] = t;
+                if (I != J) out[j + (long)b * i] = t;
+              }
+            }
+          }
+          ++uu;
+        }
+    }
+  }
+}
+
+// out[p] = sum_b part[b][p]: a block takes 64 entries (one per lane, consecutive in
+// memory) and its 4 waves split the partials (wave w: w, w + 4, ...); the waves' sums are
+// added in a fixed order through LDS, so the result does not depend on timing.
+__global__ __launch_bounds__(256) void tall_gram_reduce_kernel(const double* __restrict__ part, int nb, int np,
+                                                               double* __restrict__ out) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + lane;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(long)b * np + p];
-  out[p] = s;
+  if (p < np) {
+#pragma unroll 8
+    for (int b = w; b < nb; b += 4) s += part[(long)b * np + p];
+  }
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && p < np) out[p] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
 // Small FP64 solve of one node's loss (one work-group, w <= 64):
@@ -368,15 +479,18 @@ int kr_grad_launch(hipStream_t st, const float* S, const float* dom, const int64
 
 int tall_gram_blocks(long rows) {
   long nb = (rows + TG_ROWS - 1) / TG_ROWS;
-  return (int)(nb > 256 ? 256 : (nb < 1 ? 1 : nb));
+  return (int)(nb > 1024 ? 1024 : (nb < 1 ? 1 : nb));
 }
 
 int tall_gram_launch(hipStream_t st, const float* A, int wa, const float* B, int wb, long rows, double* part,
                      double* out) {
   if (wa < 1 || wa > 64 || wb < 1 || wb > 64 || rows < 1) return 8;
   const int nb = tall_gram_blocks(rows);
-  hipLaunchKernelGGL(tall_gram_kernel, dim3(nb), dim3(256), 0, st, A, B, rows, wa, wb, part);
-  hipLaunchKernelGGL(tall_gram_reduce_kernel, dim3((wa * wb + 255) / 256), dim3(256), 0, st, part, nb, wa * wb, out);
+  if (A == B && wa == wb && wa >= 16)
+    hipLaunchKernelGGL(tall_syrk_kernel, dim3(nb), dim3(256), 0, st, A, rows, wa, part);
+  else
+    hipLaunchKernelGGL(tall_gram_kernel, dim3(nb), dim3(256), 0, st, A, B, rows, wa, wb, part);
+  hipLaunchKernelGGL(tall_gram_reduce_kernel, dim3((wa * wb + 63) / 64), dim3(256), 0, st, part, nb, wa * wb, out);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
@@ -431,7 +545,7 @@ int probe_resid_launch(hipStream_t st, const float* Q, const float* P, const dou
   if (b < 1 || b > 64 || np < 1 || np > 8) return 8;
   const int nb = probe_resid_blocks(rows);
   hipLaunchKernelGGL(probe_resid_kernel, dim3(nb), dim3(256), 0, st, Q, P, QtP, rows, b, np, part);
-  hipLaunchKernelGGL(tall_gram_reduce_kernel, dim3(1), dim3(64), 0, st, part, nb, 1, out);
+  hipLaunchKernelGGL(tall_gram_reduce_kernel, dim3(1), dim3(256), 0, st, part, nb, 1, out);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
