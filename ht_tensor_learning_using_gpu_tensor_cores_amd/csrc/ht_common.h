@@ -135,18 +135,22 @@ struct PMS {
 // ------------------------------------------------------------------ launchers
 // All launchers enqueue on `stream` and return 0 or an ht_status code.
 
-// Leaf Gram partials of an M-row matricisation (M <= 1024) in 128x128 tiles.
-// part: [n_pairs][n_split][128*128] FP32; norms: [n_pairs*n_split] FP64 sum of squares
-// of the diagonal-pair tiles (their sum is ||X||^2).
+// Leaf Gram partials of an M-row matricisation (M <= 1024) in tile x tile pieces,
+// tile = gram_pms_tile(x, v, prec) (128, or 256 for 16-bit products of more than 256
+// rows in contiguous layouts). part: [gram_pms_pairs][n_split][tile*tile] FP32; norms:
+// [pairs*n_split] FP64 sums of squares whose total is ||X||^2.
 int gram_pms_launch(hipStream_t st, const float* x, PMS v, int prec, int n_split, float* part,
                     double* norms);
-int gram_pms_nsplit(PMS v, int target_blocks);
+int gram_pms_tile(const float* x, PMS v, int prec);
+int gram_pms_pairs(const float* x, PMS v, int prec);
+int gram_pms_nsplit(const float* x, PMS v, int prec);
 // Sum partials into a full symmetric FP64 Gram (ld = M).
-int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G);
+int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G, int tile = 128);
 // Two Gram reductions (n1, n2 partials each; and the sum of the per-block norms when
 // norms != nullptr) in one launch.
 int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, int n1, double* G1, const float* p2, int M2, int n2,
-                        double* G2, const double* norms, int n_norms, double* norm_out);
+                        double* G2, const double* norms, int n_norms, double* norm_out, int tile1 = 128,
+                        int tile2 = 128);
 int gram_reduce_norms_launch(hipStream_t st, const double* norms, int n, double* out);
 
 // Fused slab kernels on the two leading modes (x as n1 x n2 x S, n1,n2 <= 128).

@@ -265,17 +265,17 @@ template int* Engine::alloc<int>(size_t);
 // Gram of an (P,M,S) view of FP32 data, reduced to FP64 (M x M). Returns ||x||^2 of the
 // data in *sumsq (device scalar) when sumsq != nullptr.
 int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
-  const int ns = gram_pms_nsplit(v, 512);
-  const int nt = (int)((v.M + 127) / 128);
-  const int pairs = nt * (nt + 1) / 2;
-  float* part = alloc<float>((size_t)pairs * ns * 128 * 128);
+  const int ns = gram_pms_nsplit(x, v, prec);
+  const int tile = gram_pms_tile(x, v, prec);
+  const int pairs = gram_pms_pairs(x, v, prec);
+  float* part = alloc<float>((size_t)pairs * ns * tile * tile);
   double* norms = alloc<double>((size_t)pairs * ns);
   if (!part || !norms) return HT_ERR_ALLOC;
   {
     ProfScope ps(prof, st, PK_GRAM);
     CK(gram_pms_launch(st, x, v, prec, ns, part, norms));
   }
-  CK(gram_reduce_launch(st, part, (int)v.M, ns, G));
+  CK(gram_reduce_launch(st, part, (int)v.M, ns, G, tile));
   if (sumsq) CK(gram_reduce_norms_launch(st, norms, pairs * ns, sumsq));
   passes += 1.0;
   return 0;
@@ -285,18 +285,19 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
 int Engine::gram2(const float* x, PMS v1, PMS v2, int prec, double* G1, double* G2) {
   const PMS vs[2] = {v1, v2};
   float* part[2];
-  int ns[2];
+  int ns[2], tile[2];
   for (int i = 0; i < 2; ++i) {
-    ns[i] = gram_pms_nsplit(vs[i], 512);
-    const int nt = (int)((vs[i].M + 127) / 128);
-    const int pairs = nt * (nt + 1) / 2;
-    part[i] = alloc<float>((size_t)pairs * ns[i] * 128 * 128);
+    ns[i] = gram_pms_nsplit(x, vs[i], prec);
+    tile[i] = gram_pms_tile(x, vs[i], prec);
+    const int pairs = gram_pms_pairs(x, vs[i], prec);
+    part[i] = alloc<float>((size_t)pairs * ns[i] * tile[i] * tile[i]);
     double* norms = alloc<double>((size_t)pairs * ns[i]);
     if (!part[i] || !norms) return HT_ERR_ALLOC;
     ProfScope ps(prof, st, PK_GRAM);
     CK(gram_pms_launch(st, x, vs[i], prec, ns[i], part[i], norms));
   }
-  CK(gram_reduce2_launch(st, part[0], (int)v1.M, ns[0], G1, part[1], (int)v2.M, ns[1], G2, nullptr, 0, nullptr));
+  CK(gram_reduce2_launch(st, part[0], (int)v1.M, ns[0], G1, part[1], (int)v2.M, ns[1], G2, nullptr, 0, nullptr,
+                         tile[0], tile[1]));
   passes += 2.0;
   return 0;
 }

@@ -304,12 +304,234 @@ __global__ __launch_bounds__(256) void gram_pms_kernel(const float* __restrict__
   if (tid == 0) norms[(long)pair * n_split + split] = red[0] + red[1] + red[2] + red[3];
 }
 
-int gram_pms_nsplit(PMS v, int target_blocks) {
-  const long K = v.P * v.S;
-  const long n_chunks = (K + GT_K - 1) / GT_K;
-  const int nt = (int)((v.M + GT_ROWS - 1) / GT_ROWS);
-  const int pairs = nt * (nt + 1) / 2;
-  long ns = target_blocks / pairs;
+
+// ------------------------------------------------------------------ gram_big
+// Matricisations of more than 256 rows with 16-bit products. A block owns a 256 x 256
+// tile pair (i, j) of G over one range of columns: 512 threads = 8 waves in a 4 x 2 grid,
+// wave (wr, wc) multiplying rows 64 wr.. of tile i against rows 128 wc.. of tile j
+// (2 x 4 MFMA 32x32x16 accumulators). Columns advance in chunks of 64 through two LDS
+// buffers: chunk c + 1 is loaded into registers while chunk c is multiplied, then stored
+// to the other buffer, so one barrier per chunk orders both. Against 128 x 128 tiles each
+// column chunk is read half as many times (4 tile pairs per row tile of a 1024-row
+// matricisation instead of 8); the bytes each CU pulls from L2 per MFMA are what bound
+// the smaller tiles at these sizes. Diagonal pairs (i == j) load one tile and run their
+// own copy of the loop (no branch on i == j inside it).
+// Tile b starts at row gb_origin(b) = min(256 b, M - 256): the last tile overlaps its
+// neighbour instead of running past row M, so every row a block reads exists (entries
+// of the overlap are produced by two tile pairs, with the same value) and the loads need
+// no row bounds. Their addresses are a block- or wave-uniform base plus a fixed 32-bit
+// lane offset.
+constexpr int GB_ROWS = 256, GB_K = 64, GB_LD = GB_K + 8;  // LDS rows of 144 B (9 x 16 B)
+constexpr int GB_TILE = GB_ROWS * GB_LD;                    // one tile, 16-bit elements
+
+__device__ __forceinline__ long gb_origin(int b, long M) { return min((long)GB_ROWS * b, M - GB_ROWS); }
+
+// One 256 x 64 tile (rows m0.., columns col0..) into registers, 32 floats per thread.
+template <int LOADER>
+__device__ __forceinline__ void gb_load(const float* __restrict__ x, PMS v, long K, long m0, long col0, int tid,
+                                        float (&r)[32]) {
+  if constexpr (LOADER == LD_COLS_CONTIG) {
+    // P % 64 == 0: the chunk's 64 columns share one s and are contiguous in every row (K
+    // is a multiple of 64, so chunks are whole); 16 lanes read a row's 256 bytes (a
+    // float4 each), rows tid / 16 + 32 i.
+    const long s = col0 / v.P, p0 = col0 - s * v.P;
+    const float* base = x + p0 + v.P * (v.M * s + m0);
+    const unsigned off = (unsigned)v.P * (unsigned)(tid >> 4) + 4u * (unsigned)(tid & 15);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const float4 q = *reinterpret_cast<const float4*>(base + v.P * (32 * i) + off);
+      r[4 * i + 0] = q.x; r[4 * i + 1] = q.y; r[4 * i + 2] = q.z; r[4 * i + 3] = q.w;
+    }
+  } else {
+    // P == 1. Thread owns row m0 + tid % 256 and columns col0 + 32 (tid / 256) + i, i < 32
+    // (the column clamped to K - 1; gb_store zeroes the copies): a wave's load i reads
+    // 256 contiguous bytes of one column.
+    const long cb = col0 + 32 * __builtin_amdgcn_readfirstlane(tid >> 8);
+    const unsigned off = (unsigned)(tid & 255);
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      const float* col = x + (v.M * min(cb + i, K - 1) + m0);
+      r[i] = col[off];
+    }
+  }
+}
+
+// Registers -> LDS tile [row][k] as 16-bit values (row stride GB_LD). Columns past K are
+// zeroed in r as well (the diagonal pairs' norm sums r after the store).
+template <int PREC, int LOADER>
+__device__ __forceinline__ void gb_store(uint16_t* __restrict__ T, int tid, float (&r)[32], long K, long col0) {
+  if constexpr (LOADER == LD_COLS_CONTIG) {
+    const int kq = tid & 15;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int row = (tid >> 4) + 32 * i;
+      uint2 w;
+      w.x = cvt2<PREC>(r[4 * i], r[4 * i + 1]);
+      w.y = cvt2<PREC>(r[4 * i + 2], r[4 * i + 3]);
+      *reinterpret_cast<uint2*>(&T[row * GB_LD + 4 * kq]) = w;
+    }
+  } else {
+    const int row = tid & 255, k0 = 32 * (tid >> 8);
+    const long nv = K - (col0 + k0);  // columns of this thread inside the matrix
+#pragma unroll
+    for (int i = 0; i < 32; ++i)
+      if (i >= nv) r[i] = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      uint4 w;
+      w.x = cvt2<PREC>(r[8 * q + 0], r[8 * q + 1]);
+      w.y = cvt2<PREC>(r[8 * q + 2], r[8 * q + 3]);
+      w.z = cvt2<PREC>(r[8 * q + 4], r[8 * q + 5]);
+      w.w = cvt2<PREC>(r[8 * q + 6], r[8 * q + 7]);
+      *reinterpret_cast<uint4*>(&T[row * GB_LD + k0 + 8 * q]) = w;
+    }
+  }
+}
+
+// Sum of squares of the rows of r that tile b owns (rows from 256 b on: the overlapped
+// rows of the last tile belong to its neighbour), so the diagonal pairs' norms add up to
+// ||X||^2.
+template <int LOADER>
+__device__ __forceinline__ float gb_sumsq(const float (&r)[32], int tid, int own_lo) {
+  float s = 0.f;
+  if constexpr (LOADER == LD_COLS_CONTIG) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const float t = r[4 * i] * r[4 * i] + r[4 * i + 1] * r[4 * i + 1] + r[4 * i + 2] * r[4 * i + 2] +
+                      r[4 * i + 3] * r[4 * i + 3];
+      s += (tid >> 4) + 32 * i >= own_lo ? t : 0.f;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 32; ++i) s += r[i] * r[i];
+    if ((tid & 255) < own_lo) s = 0.f;
+  }
+  return s;
+}
+
+template <int PREC, int LOADER, bool DIAG>
+__device__ __forceinline__ void gb_body(const float* __restrict__ x, PMS v, long K, long c_begin, long c_end,
+                                        long mi, long mj, int own_lo, uint16_t* __restrict__ lds,
+                                        f32x16 (&acc)[2][4], double& sumsq) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  constexpr int BUF = DIAG ? GB_TILE : 2 * GB_TILE;
+  float ri[32], rj[32];
+  gb_load<LOADER>(x, v, K, mi, c_begin * GB_K, tid, ri);
+  if constexpr (!DIAG) gb_load<LOADER>(x, v, K, mj, c_begin * GB_K, tid, rj);
+  gb_store<PREC, LOADER>(lds, tid, ri, K, c_begin * GB_K);
+  if constexpr (!DIAG) gb_store<PREC, LOADER>(lds + GB_TILE, tid, rj, K, c_begin * GB_K);
+  __syncthreads();
+  int cur = 0;
+  for (long c = c_begin; c < c_end; ++c) {
+    if constexpr (DIAG) sumsq += (double)gb_sumsq<LOADER>(ri, tid, own_lo);  // ri = chunk c
+    // the next chunk (the last one again at the end: loaded and stored, never used)
+    const long cn = c + 1 < c_end ? c + 1 : c;
+    gb_load<LOADER>(x, v, K, mi, cn * GB_K, tid, ri);
+    if constexpr (!DIAG) gb_load<LOADER>(x, v, K, mj, cn * GB_K, tid, rj);
+    // keep the loads ahead of the multiply (the scheduler would otherwise sink them to
+    // their stores and wait on them inside the MFMA sequence)
+    __builtin_amdgcn_sched_barrier(0);
+    const uint16_t* Ai = lds + cur * BUF;
+    const uint16_t* Aj = DIAG ? Ai : Ai + GB_TILE;
+#pragma unroll
+    for (int kk = 0; kk < GB_K / 16; ++kk) {
+      u16x8 af[2], bf[4];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        af[a] = *reinterpret_cast<const u16x8*>(&Ai[(64 * wr + 32 * a + (lane & 31)) * GB_LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        bf[b] = *reinterpret_cast<const u16x8*>(&Aj[(128 * wc + 32 * b + (lane & 31)) * GB_LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = mfma32x32x16<PREC>(af[a], bf[b], acc[a][b]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    cur ^= 1;
+    gb_store<PREC, LOADER>(lds + cur * BUF, tid, ri, K, cn * GB_K);
+    if constexpr (!DIAG) gb_store<PREC, LOADER>(lds + cur * BUF + GB_TILE, tid, rj, K, cn * GB_K);
+    __syncthreads();
+  }
+}
+
+template <int PREC, int LOADER>
+__global__ __launch_bounds__(512) void gram_big_kernel(const float* __restrict__ x, PMS v, long K, long n_chunks,
+                                                       int n_split, int n_tiles, float* __restrict__ part,
+                                                       double* __restrict__ norms) {
+  __shared__ __attribute__((aligned(16))) uint16_t lds[4 * GB_TILE];
+  __shared__ double red[8];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const int split = blockIdx.x, pair = blockIdx.y;
+  int bi = 0, bj = 0;
+  {
+    int p = pair;
+    for (bi = 0; bi < n_tiles; ++bi) {
+      const int cnt = n_tiles - bi;
+      if (p < cnt) { bj = bi + p; break; }
+      p -= cnt;
+    }
+  }
+  const long c_begin = (long)split * n_chunks / n_split;
+  const long c_end = (long)(split + 1) * n_chunks / n_split;
+  const long mi = gb_origin(bi, v.M), mj = gb_origin(bj, v.M);
+  f32x16 acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f32x16{};
+  double sumsq = 0.0;
+  if (c_begin < c_end) {
+    if (bi == bj)
+      gb_body<PREC, LOADER, true>(x, v, K, c_begin, c_end, mi, mi, (int)((long)GB_ROWS * bi - mi), lds, acc, sumsq);
+    else
+      gb_body<PREC, LOADER, false>(x, v, K, c_begin, c_end, mi, mj, 0, lds, acc, sumsq);
+  }
+  float* out = part + ((long)pair * n_split + split) * (GB_ROWS * GB_ROWS);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int row = 64 * wr + 32 * a + acc32_row(e, lane);
+        const int col = 128 * wc + 32 * b + (lane & 31);
+        out[row * GB_ROWS + col] = acc[a][b][e];
+      }
+  sumsq = wave_sum(sumsq);
+  if (lane == 0) red[w] = sumsq;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int q = 0; q < 8; ++q) t += red[q];
+    norms[(long)pair * n_split + split] = t;
+  }
+}
+
+// Tile edge of the Gram partials: 256 for 16-bit products of more than 256 rows whose
+// columns are single floats of a row's stretch (P == 1) or aligned runs of 64
+// (P % 64 == 0, 16-byte aligned data, lane offsets P * 15 + 60 within 32 bits).
+int gram_pms_tile(const float* x, PMS v, int prec) {
+  const bool rows = v.P == 1;
+  const bool cols = v.P % 64 == 0 && ((uintptr_t)x & 15) == 0 && v.P <= (1L << 24);
+  return (prec != PREC_F32 && v.M > GB_ROWS && (rows || cols)) ? GB_ROWS : GT_ROWS;
+}
+
+int gram_pms_pairs(const float* x, PMS v, int prec) {
+  const int tile = gram_pms_tile(x, v, prec);
+  const int nt = (int)((v.M + tile - 1) / tile);
+  return nt * (nt + 1) / 2;
+}
+
+// Column splits: 256-row tiles take one block per CU (256 blocks, LDS-limited to one per
+// CU), 128-row tiles two per CU.
+int gram_pms_nsplit(const float* x, PMS v, int prec) {
+  const int tile = gram_pms_tile(x, v, prec);
+  const int kc = tile == GB_ROWS ? GB_K : GT_K;
+  const long n_chunks = (v.P * v.S + kc - 1) / kc;
+  long ns = (tile == GB_ROWS ? 256 : 512) / gram_pms_pairs(x, v, prec);
   if (ns < 1) ns = 1;
   if (ns > n_chunks) ns = n_chunks;
   return (int)ns;
@@ -318,11 +540,25 @@ int gram_pms_nsplit(PMS v, int target_blocks) {
 template <int PREC>
 static void gram_dispatch(hipStream_t st, const float* x, PMS v, int n_split, float* part, double* norms) {
   const long K = v.P * v.S;
+  const bool aligned = ((uintptr_t)x & 15) == 0;
+  if constexpr (PREC != PREC_F32) {
+    if (gram_pms_tile(x, v, PREC) == GB_ROWS) {
+      const long n_chunks = (K + GB_K - 1) / GB_K;
+      const int nt = (int)((v.M + GB_ROWS - 1) / GB_ROWS);
+      dim3 grid(n_split, nt * (nt + 1) / 2);
+      if (v.P == 1)
+        hipLaunchKernelGGL((gram_big_kernel<PREC, LD_ROWS_CONTIG>), grid, dim3(512), 0, st, x, v, K, n_chunks,
+                           n_split, nt, part, norms);
+      else
+        hipLaunchKernelGGL((gram_big_kernel<PREC, LD_COLS_CONTIG>), grid, dim3(512), 0, st, x, v, K, n_chunks,
+                           n_split, nt, part, norms);
+      return;
+    }
+  }
   const long n_chunks = (K + GT_K - 1) / GT_K;
   const int nt = (int)((v.M + GT_ROWS - 1) / GT_ROWS);
   const int pairs = nt * (nt + 1) / 2;
   dim3 grid(n_split, pairs);
-  const bool aligned = ((uintptr_t)x & 15) == 0;
   if (aligned && v.P % 64 == 0)
     hipLaunchKernelGGL((gram_pms_kernel<PREC, LD_COLS_CONTIG>), grid, dim3(256), 0, st, x, v, K, n_chunks,
                        n_split, nt, part, norms);
@@ -354,6 +590,7 @@ struct GramReduceJob {
   const float* part;
   int M, n_split;
   double* G;
+  int tile;  // 128 or 256
 };
 __global__ __launch_bounds__(1024) void gram_reduce_kernel(GramReduceJob j0, GramReduceJob j1,
                                                            const double* __restrict__ norms, int n_norms,
@@ -361,7 +598,7 @@ __global__ __launch_bounds__(1024) void gram_reduce_kernel(GramReduceJob j0, Gra
   __shared__ double red[16][64];
   const GramReduceJob job = blockIdx.z ? j1 : j0;
   const float* __restrict__ part = job.part;
-  const int M = job.M, nt = (M + GT_ROWS - 1) / GT_ROWS, n_split = job.n_split;
+  const int M = job.M, tile = job.tile, nt = (M + tile - 1) / tile, n_split = job.n_split;
   double* __restrict__ G = job.G;
   const int pair = blockIdx.y;
   if (pair >= nt * (nt + 1) / 2) return;  // the smaller job of a launch with two
@@ -380,7 +617,8 @@ __global__ __launch_bounds__(1024) void gram_reduce_kernel(GramReduceJob j0, Gra
       p -= cnt;
     }
   }
-  constexpr int T2 = GT_ROWS * GT_ROWS;
+  const int T2 = tile * tile;
+  if (blockIdx.x * 64 >= T2) return;  // the smaller tile of a launch with two
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int e = blockIdx.x * 64 + lane;
   const float* p = part + (long)pair * n_split * T2 + e;
@@ -403,8 +641,11 @@ __global__ __launch_bounds__(1024) void gram_reduce_kernel(GramReduceJob j0, Gra
     double t = 0.0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) t += red[q][lane];
-    const int r = e / GT_ROWS, c = e % GT_ROWS;
-    const int i = GT_ROWS * bi + r, j = GT_ROWS * bj + c;
+    // 256-row tiles: the last one starts at M - 256 (see gram_big_kernel)
+    const int r = e / tile, c = e % tile;
+    const int oi = tile == GT_ROWS ? tile * bi : std::min(tile * bi, M - tile);
+    const int oj = tile == GT_ROWS ? tile * bj : std::min(tile * bj, M - tile);
+    const int i = oi + r, j = oj + c;
     if (i < M && j < M) {
       G[i + (long)M * j] = t;
       if (bi != bj) G[j + (long)M * i] = t;
@@ -412,20 +653,21 @@ __global__ __launch_bounds__(1024) void gram_reduce_kernel(GramReduceJob j0, Gra
   }
 }
 
-int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G) {
-  const int nt = (M + GT_ROWS - 1) / GT_ROWS;
+int gram_reduce_launch(hipStream_t st, const float* part, int M, int n_split, double* G, int tile) {
+  const int nt = (M + tile - 1) / tile;
   const int pairs = nt * (nt + 1) / 2;
-  dim3 grid(GT_ROWS * GT_ROWS / 64, pairs);
-  const GramReduceJob j{part, M, n_split, G};
+  dim3 grid(tile * tile / 64, pairs);
+  const GramReduceJob j{part, M, n_split, G, tile};
   hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, j, j, nullptr, 0, nullptr);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, int n1, double* G1, const float* p2, int M2, int n2,
-                        double* G2, const double* norms, int n_norms, double* norm_out) {
-  const int nt = (std::max(M1, M2) + GT_ROWS - 1) / GT_ROWS;
-  dim3 grid(GT_ROWS * GT_ROWS / 64, nt * (nt + 1) / 2, 2);
-  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, GramReduceJob{p1, M1, n1, G1},
-                     GramReduceJob{p2, M2, n2, G2}, norms, n_norms, norm_out);
+                        double* G2, const double* norms, int n_norms, double* norm_out, int tile1, int tile2) {
+  const int nt = std::max((M1 + tile1 - 1) / tile1, (M2 + tile2 - 1) / tile2);
+  const int tile = std::max(tile1, tile2);
+  dim3 grid(tile * tile / 64, nt * (nt + 1) / 2, 2);
+  hipLaunchKernelGGL(gram_reduce_kernel, grid, dim3(1024), 0, st, GramReduceJob{p1, M1, n1, G1, tile1},
+                     GramReduceJob{p2, M2, n2, G2, tile2}, norms, n_norms, norm_out);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 

@@ -65,10 +65,16 @@ def test_mfma_lane_layouts(lib, handle, rng):
     np.testing.assert_allclose(host(o5).reshape(16, 16), a2 @ b2.T, rtol=1e-5, atol=1e-4)
 
 
+# More than 256 rows with 16-bit products take 256 x 256 tile pairs: column-contiguous rows
+# (P == 1: (300,20,18) mode 0, (20,16,600) modes 0-1, (1024,3,5) mode 0), runs of 64
+# (P % 64 == 0: (64,520,3) mode 1, (16,20,288) mode 2), partial column chunks and rows
+# that fill no whole tile; (4,300,16) mode 1 (P == 4) stays on 128 x 128 tiles.
 @pytest.mark.parametrize("dims,lo,hi", [((24, 20, 18), 0, 1), ((24, 20, 18), 1, 2), ((24, 20, 18), 2, 3),
                                         ((64, 32, 40), 1, 2), ((128, 64, 8, 4), 0, 1), ((16, 128, 8, 4), 1, 2),
-                                        ((8, 8, 130, 4), 2, 3), ((12, 10, 6), 0, 2), ((3, 5, 7, 2), 1, 3)])
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+                                        ((8, 8, 130, 4), 2, 3), ((12, 10, 6), 0, 2), ((3, 5, 7, 2), 1, 3),
+                                        ((300, 20, 18), 0, 1), ((20, 16, 600), 0, 2), ((1024, 3, 5), 0, 1),
+                                        ((64, 520, 3), 1, 2), ((16, 20, 288), 2, 3), ((4, 300, 16), 1, 2)])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
 def test_mode_gram(lib, handle, rng, dims, lo, hi, precision):
     x = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
     A = O.unfold(x.astype(np.float64), list(dims), lo, hi)
@@ -83,13 +89,17 @@ def test_mode_gram(lib, handle, rng, dims, lo, hi, precision):
     if precision == "fp32":
         ref = A @ A.T
         np.testing.assert_allclose(Gh, ref, rtol=1e-5, atol=1e-4 * np.abs(ref).max())
-    else:
+    elif precision == "bf16":
         Ab = O.unfold(bf16_round(x).astype(np.float64), list(dims), lo, hi)
         ref = Ab @ Ab.T
         np.testing.assert_allclose(Gh, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
         # forward error bound against the exact FP64 Gram: |G_bf16 - G| <= 2^-7 |A||A|^T
         bound = 2.0 ** -7 * (np.abs(A) @ np.abs(A).T) + 1e-6
         assert np.all(np.abs(Gh - A @ A.T) <= bound)
+    else:
+        Ah = O.unfold(x.astype(np.float16).astype(np.float64), list(dims), lo, hi)
+        ref = Ah @ Ah.T
+        np.testing.assert_allclose(Gh, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
     np.testing.assert_allclose(s.value, float(np.sum(x.astype(np.float64) ** 2)), rtol=1e-6)
     assert np.allclose(Gh, Gh.T)
 
