@@ -18,282 +18,382 @@
 
 namespace ht {
 
-constexpr int SK_B = 64;     // padded skinny width
-constexpr int SK_TILE = 128; // rows of the output tile
-constexpr int SK_K = 64;     // reduction chunk
+constexpr int SK_B = 64;  // padded skinny width
+constexpr int SK_K = 64;  // reduction chunk
+
+// LDS element type and operand fetch of the skinny products: 16-bit tiles for the BF16 /
+// FP16 precisions (converted once, when the chunk is stored), FP32 tiles for FP32.
+template <int PREC>
+struct SkT {
+  typedef uint16_t T;
+  static constexpr int LD = SK_K + 8;  // 144-byte rows: 16-byte operand reads without conflicts
+};
+template <>
+struct SkT<PREC_F32> {
+  typedef float T;
+  static constexpr int LD = SK_K + 4;
+};
+
+// Four consecutive values of a tile row into LDS (converted for 16-bit tiles).
+template <int PREC>
+__device__ __forceinline__ void sk_put4(typename SkT<PREC>::T* dst, float a, float b, float c, float d) {
+  if constexpr (PREC == PREC_F32) {
+    *reinterpret_cast<float4*>(dst) = make_float4(a, b, c, d);
+  } else {
+    uint2 w;
+    w.x = cvt2<PREC>(a, b);
+    w.y = cvt2<PREC>(c, d);
+    *reinterpret_cast<uint2*>(dst) = w;
+  }
+}
+
+// acc[bb] += (rows 32 bb.. of tile R) x (rows 32 w.. of tile C)^T over one 64-deep chunk:
+// D[r][c] = sum_k R[r][k] C[c][k], R and C row-major [row][k] in LDS. D's lane layout puts
+// consecutive c in consecutive lanes, which the output stores rely on.
+template <int PREC, int NB>
+__device__ __forceinline__ void sk_chunk(const typename SkT<PREC>::T* __restrict__ R,
+                                         const typename SkT<PREC>::T* __restrict__ C, int w, int lane,
+                                         f32x16 (&acc)[NB]) {
+  constexpr int LD = SkT<PREC>::LD;
+  if constexpr (PREC == PREC_F32) {
+#pragma unroll 8
+    for (int kk = 0; kk < SK_K / 2; ++kk) {
+      const float cf = C[(32 * w + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
+#pragma unroll
+      for (int bb = 0; bb < NB; ++bb) {
+        const float rf = R[(32 * bb + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
+        acc[bb] = __builtin_amdgcn_mfma_f32_32x32x2f32(rf, cf, acc[bb], 0, 0, 0);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < SK_K / 16; ++kk) {
+      const u16x8 cf = *reinterpret_cast<const u16x8*>(&C[(32 * w + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int bb = 0; bb < NB; ++bb) {
+        const u16x8 rf =
+            *reinterpret_cast<const u16x8*>(&R[(32 * bb + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+        acc[bb] = mfma32x32x16<PREC>(rf, cf, acc[bb]);
+      }
+    }
+  }
+}
+
+// Output of a block: acc[bb] holds D[j][c] for j = 32 bb + acc32_row(e, lane), c = c0 +
+// 32 w + lane % 32. With one split the values go straight to Y (column-major, rows x b:
+// lanes write consecutive rows); otherwise to part[split][j][row] for skinny_sum_kernel.
+template <int NB>
+__device__ __forceinline__ void sk_write(const f32x16 (&acc)[NB], long c0, int w, int lane, long rows, int b,
+                                         int n_split, int split, float* __restrict__ Y, float* __restrict__ part) {
+  const long c = c0 + 32 * w + (lane & 31);
+  if (c >= rows) return;
+  float* dst = n_split == 1 ? Y : part + (long)split * SK_B * rows;
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int j = 32 * bb + acc32_row(e, lane);
+      if (j < b) dst[c + rows * j] = acc[bb][e];
+    }
+}
 
 // ------------------------------------------------------------------ gemm_atb
-// Y(K x b) = A^T B. Block: 128 columns of A (128 output rows) x 64; 4 waves, wave w owns
-// output rows 32w..32w+31 and all 64 columns (two 32x32 accumulators). LDS tiles are
-// [column][m] for A and [j][m] for B: both are column-major in memory along m.
-// NB = number of 32-column accumulators (1 when b <= 32: half the MFMA work).
-template <int PREC, int NB>
+// Y(K x b) = A^T B, A: M x K (lda M), B: M x b (ldb M). A block owns 128 columns c of A
+// (output rows) and all b outputs: 256 threads = 4 waves, wave w the columns 32 w.. (NB
+// 32-row accumulators over j). The reduction runs over m in chunks of 64: chunk ch + 1 is
+// loaded into registers (every load unconditional, its index clamped into the matrix:
+// a load behind a divergent branch is waited for at the join) while chunk ch is
+// multiplied; stored values with m >= M are zero, columns c >= K and j >= b are copies
+// whose outputs are never written.
+// VEC (M % 4 == 0, A and B 16-byte aligned): float4 loads along m.
+template <int PREC, int NB, bool VEC>
 __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__ A, long M, long K,
                                                        const float* __restrict__ B, int b, int n_split,
-                                                       float* __restrict__ part) {
-  constexpr int LD = SK_K + 4;
-  __shared__ __attribute__((aligned(16))) float As[SK_TILE * LD];
-  __shared__ __attribute__((aligned(16))) float Bs[SK_B * LD];
+                                                       float* __restrict__ Y, float* __restrict__ part) {
+  typedef typename SkT<PREC>::T T;
+  constexpr int LD = SkT<PREC>::LD;
+  constexpr int NBJ = 32 * NB;                    // j rows of the B tile
+  constexpr int RA = 32, RB = NBJ * SK_K / 256;   // floats per thread per chunk
+  __shared__ __attribute__((aligned(16))) T As[128 * LD];
+  __shared__ __attribute__((aligned(16))) T Bs[NBJ * LD];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const long c0 = (long)blockIdx.x * SK_TILE;
+  const long c0 = (long)blockIdx.x * 128;
   const int split = blockIdx.y;
   const long n_chunks = (M + SK_K - 1) / SK_K;
   const long k_begin = (long)split * n_chunks / n_split, k_end = (long)(split + 1) * n_chunks / n_split;
-  const bool vecA = (M % 4 == 0) && (((uintptr_t)A & 15) == 0);
-  const bool vecB = (M % 4 == 0) && (((uintptr_t)B & 15) == 0);
-
-  f32x16 acc[NB];
-#pragma unroll
-  for (int bb = 0; bb < NB; ++bb) acc[bb] = f32x16{};
-  for (long ch = k_begin; ch < k_end; ++ch) {
+  float ra[RA], rb[RB];
+  // VEC: thread element q of A is (col f / 16, m 4 (f % 16) + 0..3), f = tid + 256 (q / 4);
+  // scalar: (col e / 64, m e % 64), e = tid + 256 q. B likewise with j for col.
+  auto load = [&](long ch) {
     const long m0 = ch * SK_K;
-    __syncthreads();
-    // A tile: 128 columns x 64 m (float4 along m): 2048 float4 / 256 threads = 8 each.
+    if constexpr (VEC) {
 #pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      const int f = tid + 256 * it;
-      const int col = f >> 4, mq = f & 15;
-      const long c = c0 + col, m = m0 + 4 * mq;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (c < K) {
-        const float* p = A + m + M * c;
-        if (vecA && m + 3 < M) v = *reinterpret_cast<const float4*>(p);
-        else {
-          if (m < M) v.x = p[0];
-          if (m + 1 < M) v.y = p[1];
-          if (m + 2 < M) v.z = p[2];
-          if (m + 3 < M) v.w = p[3];
-        }
+      for (int it = 0; it < RA / 4; ++it) {
+        const int f = tid + 256 * it;
+        const long c = min(c0 + (f >> 4), K - 1), m = min(m0 + 4 * (f & 15), M - 4);
+        const float4 v = *reinterpret_cast<const float4*>(A + m + M * c);
+        ra[4 * it] = v.x; ra[4 * it + 1] = v.y; ra[4 * it + 2] = v.z; ra[4 * it + 3] = v.w;
       }
-      *reinterpret_cast<float4*>(&As[col * LD + 4 * mq]) = v;
-    }
 #pragma unroll
-    for (int it = 0; it < 2 * NB; ++it) {
-      const int f = tid + 256 * it;
-      const int j = f >> 4, mq = f & 15;
-      const long m = m0 + 4 * mq;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (j < b) {
-        const float* p = B + m + M * j;
-        if (vecB && m + 3 < M) v = *reinterpret_cast<const float4*>(p);
-        else {
-          if (m < M) v.x = p[0];
-          if (m + 1 < M) v.y = p[1];
-          if (m + 2 < M) v.z = p[2];
-          if (m + 3 < M) v.w = p[3];
-        }
-      }
-      *reinterpret_cast<float4*>(&Bs[j * LD + 4 * mq]) = v;
-    }
-    __syncthreads();
-    if constexpr (PREC == PREC_F32) {
-#pragma unroll 4
-      for (int kk = 0; kk < SK_K / 2; ++kk) {
-        const float af = As[(32 * w + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
-#pragma unroll
-        for (int bb = 0; bb < NB; ++bb) {
-          const float bf = Bs[(32 * bb + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
-          acc[bb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af, bf, acc[bb], 0, 0, 0);
-        }
+      for (int it = 0; it < RB / 4; ++it) {
+        const int f = tid + 256 * it;
+        const long j = min(f >> 4, b - 1), m = min(m0 + 4 * (f & 15), M - 4);
+        const float4 v = *reinterpret_cast<const float4*>(B + m + M * j);
+        rb[4 * it] = v.x; rb[4 * it + 1] = v.y; rb[4 * it + 2] = v.z; rb[4 * it + 3] = v.w;
       }
     } else {
 #pragma unroll
-      for (int kk = 0; kk < SK_K / 16; ++kk) {
-        u16x8 af;
-        const float* ap = &As[(32 * w + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)];
+      for (int q = 0; q < RA; ++q) {
+        const int e = tid + 256 * q;
+        const long c = min(c0 + (e >> 6), K - 1), m = min(m0 + (e & 63), M - 1);
+        ra[q] = A[m + M * c];
+      }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) af[i] = to16<PREC>(ap[i]);
-#pragma unroll
-        for (int bb = 0; bb < NB; ++bb) {
-          u16x8 bf;
-          const float* bp = &Bs[(32 * bb + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)];
-#pragma unroll
-          for (int i = 0; i < 8; ++i) bf[i] = to16<PREC>(bp[i]);
-          acc[bb] = mfma32x32x16<PREC>(af, bf, acc[bb]);
-        }
+      for (int q = 0; q < RB; ++q) {
+        const int e = tid + 256 * q;
+        const long j = min(e >> 6, b - 1), m = min(m0 + (e & 63), M - 1);
+        rb[q] = B[m + M * j];
       }
     }
-  }
-  // part[split][row c][64]
-  float* out = part + (long)split * K * SK_B;
+  };
+  auto store = [&](long ch) {
+    const long m0 = ch * SK_K;
+    if constexpr (VEC) {
 #pragma unroll
-  for (int bb = 0; bb < NB; ++bb)
+      for (int it = 0; it < RA / 4; ++it) {
+        const int f = tid + 256 * it;
+        const bool ok = m0 + 4 * (f & 15) < M;
+        sk_put4<PREC>(&As[(f >> 4) * LD + 4 * (f & 15)], ok ? ra[4 * it] : 0.f, ok ? ra[4 * it + 1] : 0.f,
+                      ok ? ra[4 * it + 2] : 0.f, ok ? ra[4 * it + 3] : 0.f);
+      }
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const long c = c0 + 32 * w + acc32_row(e, lane);
-      const int j = 32 * bb + (lane & 31);
-      if (c < K) out[c * SK_B + j] = acc[bb][e];
+      for (int it = 0; it < RB / 4; ++it) {
+        const int f = tid + 256 * it;
+        const bool ok = m0 + 4 * (f & 15) < M;
+        sk_put4<PREC>(&Bs[(f >> 4) * LD + 4 * (f & 15)], ok ? rb[4 * it] : 0.f, ok ? rb[4 * it + 1] : 0.f,
+                      ok ? rb[4 * it + 2] : 0.f, ok ? rb[4 * it + 3] : 0.f);
+      }
+    } else {
+      const bool ok = m0 + (tid & 63) < M;  // e % 64 == tid % 64 for every q
+#pragma unroll
+      for (int q = 0; q < RA; ++q) {
+        const int e = tid + 256 * q;
+        const float v = ok ? ra[q] : 0.f;
+        if constexpr (PREC == PREC_F32) As[(e >> 6) * LD + (e & 63)] = v;
+        else As[(e >> 6) * LD + (e & 63)] = to16<PREC>(v);
+      }
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        const int e = tid + 256 * q;
+        const float v = ok ? rb[q] : 0.f;
+        if constexpr (PREC == PREC_F32) Bs[(e >> 6) * LD + (e & 63)] = v;
+        else Bs[(e >> 6) * LD + (e & 63)] = to16<PREC>(v);
+      }
     }
+  };
+  f32x16 acc[NB];
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb) acc[bb] = f32x16{};
+  if (k_begin < k_end) load(k_begin);
+  for (long ch = k_begin; ch < k_end; ++ch) {
+    __syncthreads();
+    store(ch);
+    __syncthreads();
+    load(ch + 1 < k_end ? ch + 1 : ch);  // the last chunk again at the end (never stored)
+    __builtin_amdgcn_sched_barrier(0);
+    sk_chunk<PREC, NB>(Bs, As, w, lane, acc);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  sk_write<NB>(acc, c0, w, lane, K, b, n_split, split, Y, part);
 }
 
 // ------------------------------------------------------------------ gemm_ab
-// Y(M x b) = A B. Block: 128 rows of A x 64; reduction over columns of A in chunks of
-// 64. A tile stored transposed in LDS ([m][c]), B tile as [j][c].
-template <int PREC, int NB>
-__global__ __launch_bounds__(256) void gemm_ab_kernel(const float* __restrict__ A, long M, long K,
+// Y(M x b) = A B, A: M x K (lda M), B: K x b (ldb K). A block owns 256 rows m (output
+// rows) and all b outputs: 512 threads = 8 waves, wave w the rows 32 w... The reduction
+// runs over columns k of A in chunks of 64, pipelined as in gemm_atb. The A tile is
+// transposed on its way to LDS ([m][k]): thread t reads row m0 + t % 256 of the 32
+// columns 32 (t / 256).. (a wave's load reads 64 consecutive rows of a column) and
+// stores them as one row piece. Rows m >= M are copies whose outputs are never
+// written; columns k >= K are stored as zero.
+// VEC (K % 4 == 0, B 16-byte aligned): float4 loads of B along k.
+template <int PREC, int NB, bool VEC>
+__global__ __launch_bounds__(512) void gemm_ab_kernel(const float* __restrict__ A, long M, long K,
                                                       const float* __restrict__ B, int b, int n_split,
-                                                      float* __restrict__ part) {
-  constexpr int LD = SK_K + 4;
-  __shared__ __attribute__((aligned(16))) float As[SK_TILE * LD];
-  __shared__ __attribute__((aligned(16))) float Bs[SK_B * LD];
+                                                      float* __restrict__ Y, float* __restrict__ part) {
+  typedef typename SkT<PREC>::T T;
+  constexpr int LD = SkT<PREC>::LD;
+  constexpr int NBJ = 32 * NB;
+  constexpr int RB = NBJ * SK_K / 512;
+  __shared__ __attribute__((aligned(16))) T As[256 * LD];
+  __shared__ __attribute__((aligned(16))) T Bs[NBJ * LD];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const long m0 = (long)blockIdx.x * SK_TILE;
+  const long r0 = (long)blockIdx.x * 256;
   const int split = blockIdx.y;
   const long n_chunks = (K + SK_K - 1) / SK_K;
   const long k_begin = (long)split * n_chunks / n_split, k_end = (long)(split + 1) * n_chunks / n_split;
-  const bool vecA = (M % 4 == 0) && (((uintptr_t)A & 15) == 0);
-
-  f32x16 acc[NB];
+  const long arow = min(r0 + (tid & 255), M - 1);
+  const int kh = 32 * (tid >> 8);
+  float ra[32], rb[RB];
+  auto load = [&](long ch) {
+    const long k0 = ch * SK_K;
+    const float* a = A + arow;
 #pragma unroll
-  for (int bb = 0; bb < NB; ++bb) acc[bb] = f32x16{};
-  for (long ch = k_begin; ch < k_end; ++ch) {
-    const long cc0 = ch * SK_K;
-    __syncthreads();
-    // A tile: 64 columns x 128 m; float4 along m (32 per column): 2048 float4 / 256 = 8.
+    for (int q = 0; q < 32; ++q) ra[q] = a[M * min(k0 + kh + q, K - 1)];
+    if constexpr (VEC) {
 #pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      const int f = tid + 256 * it;
-      const int k = f >> 5, mq = f & 31;
-      const long c = cc0 + k, m = m0 + 4 * mq;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (c < K) {
-        const float* p = A + m + M * c;
-        if (vecA && m + 3 < M) v = *reinterpret_cast<const float4*>(p);
-        else {
-          if (m < M) v.x = p[0];
-          if (m + 1 < M) v.y = p[1];
-          if (m + 2 < M) v.z = p[2];
-          if (m + 3 < M) v.w = p[3];
-        }
-      }
-      As[(4 * mq + 0) * LD + k] = v.x;
-      As[(4 * mq + 1) * LD + k] = v.y;
-      As[(4 * mq + 2) * LD + k] = v.z;
-      As[(4 * mq + 3) * LD + k] = v.w;
-    }
-    // B tile: 64 (j) x 64 (c); B col-major (K x b): B[c + K*j].
-#pragma unroll
-    for (int it = 0; it < 8 * NB; ++it) {
-      const int f = tid + 256 * it;
-      const int j = f >> 6, k = f & 63;
-      const long c = cc0 + k;
-      Bs[j * LD + k] = (j < b && c < K) ? B[c + K * j] : 0.f;
-    }
-    __syncthreads();
-    if constexpr (PREC == PREC_F32) {
-#pragma unroll 4
-      for (int kk = 0; kk < SK_K / 2; ++kk) {
-        const float af = As[(32 * w + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
-#pragma unroll
-        for (int bb = 0; bb < NB; ++bb) {
-          const float bf = Bs[(32 * bb + (lane & 31)) * LD + 2 * kk + (lane >> 5)];
-          acc[bb] = __builtin_amdgcn_mfma_f32_32x32x2f32(af, bf, acc[bb], 0, 0, 0);
-        }
+      for (int it = 0; it < RB / 4; ++it) {
+        const int f = tid + 512 * it;
+        const long j = min(f >> 4, b - 1), k = min(k0 + 4 * (f & 15), K - 4);
+        const float4 v = *reinterpret_cast<const float4*>(B + k + K * j);
+        rb[4 * it] = v.x; rb[4 * it + 1] = v.y; rb[4 * it + 2] = v.z; rb[4 * it + 3] = v.w;
       }
     } else {
 #pragma unroll
-      for (int kk = 0; kk < SK_K / 16; ++kk) {
-        u16x8 af;
-        const float* ap = &As[(32 * w + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) af[i] = to16<PREC>(ap[i]);
-#pragma unroll
-        for (int bb = 0; bb < NB; ++bb) {
-          u16x8 bf;
-          const float* bp = &Bs[(32 * bb + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)];
-#pragma unroll
-          for (int i = 0; i < 8; ++i) bf[i] = to16<PREC>(bp[i]);
-          acc[bb] = mfma32x32x16<PREC>(af, bf, acc[bb]);
-        }
+      for (int q = 0; q < RB; ++q) {
+        const int e = tid + 512 * q;
+        const long j = min(e >> 6, b - 1), k = min(k0 + (e & 63), K - 1);
+        rb[q] = B[k + K * j];
       }
     }
-  }
-  float* out = part + (long)split * M * SK_B;
+  };
+  auto store = [&](long ch) {
+    const long k0 = ch * SK_K;
+    const long nk = K - (k0 + kh);  // valid columns among this thread's 32
 #pragma unroll
-  for (int bb = 0; bb < NB; ++bb)
+    for (int q = 0; q < 32; q += 4)
+      sk_put4<PREC>(&As[(tid & 255) * LD + kh + q], q < nk ? ra[q] : 0.f, q + 1 < nk ? ra[q + 1] : 0.f,
+                    q + 2 < nk ? ra[q + 2] : 0.f, q + 3 < nk ? ra[q + 3] : 0.f);
+    if constexpr (VEC) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      const long m = m0 + 32 * w + acc32_row(e, lane);
-      const int j = 32 * bb + (lane & 31);
-      if (m < M) out[m * SK_B + j] = acc[bb][e];
+      for (int it = 0; it < RB / 4; ++it) {
+        const int f = tid + 512 * it;
+        const bool ok = k0 + 4 * (f & 15) < K;
+        sk_put4<PREC>(&Bs[(f >> 4) * LD + 4 * (f & 15)], ok ? rb[4 * it] : 0.f, ok ? rb[4 * it + 1] : 0.f,
+                      ok ? rb[4 * it + 2] : 0.f, ok ? rb[4 * it + 3] : 0.f);
+      }
+    } else {
+      const bool ok = k0 + (tid & 63) < K;
+#pragma unroll
+      for (int q = 0; q < RB; ++q) {
+        const int e = tid + 512 * q;
+        const float v = ok ? rb[q] : 0.f;
+        if constexpr (PREC == PREC_F32) Bs[(e >> 6) * LD + (e & 63)] = v;
+        else Bs[(e >> 6) * LD + (e & 63)] = to16<PREC>(v);
+      }
     }
+  };
+  f32x16 acc[NB];
+#pragma unroll
+  for (int bb = 0; bb < NB; ++bb) acc[bb] = f32x16{};
+  if (k_begin < k_end) load(k_begin);
+  for (long ch = k_begin; ch < k_end; ++ch) {
+    __syncthreads();
+    store(ch);
+    __syncthreads();
+    load(ch + 1 < k_end ? ch + 1 : ch);
+    __builtin_amdgcn_sched_barrier(0);
+    sk_chunk<PREC, NB>(Bs, As, w, lane, acc);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  sk_write<NB>(acc, r0, w, lane, M, b, n_split, split, Y, part);
 }
 
-// part[split][row][64] -> Y (rows x b, column-major, ld = rows)
-__global__ void skinny_reduce_kernel(const float* __restrict__ part, long rows, int b, int n_split,
-                                     float* __restrict__ Y) {
+// Y[r + rows j] = sum over splits of part[s][j][r] (FP64 sum, fixed order), j < b.
+__global__ void skinny_sum_kernel(const float* __restrict__ part, long rows, int b, int n_split,
+                                  float* __restrict__ Y) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= rows * SK_B) return;
-  const long row = e / SK_B;
-  const int j = (int)(e % SK_B);
-  if (j >= b) return;
+  if (e >= rows * b) return;
   double s = 0.0;
   for (int k = 0; k < n_split; ++k) s += (double)part[(long)k * rows * SK_B + e];
-  Y[row + rows * j] = (float)s;
+  Y[e] = (float)s;
 }
 
-static int pick_split(long tiles, long chunks) {
-  long s = 1024 / (tiles > 0 ? tiles : 1);
-  if (s < 1) s = 1;
+// Splits of the reduction: enough blocks to fill the chip (target blocks), at most 256
+// and at most one per chunk. The splits' FP32 partials are summed in FP64, so short
+// reductions split finely also keep their FP32 rounding small.
+static int pick_split(long tiles, long chunks, long target) {
+  long s = target / (tiles > 0 ? tiles : 1);
   if (s > chunks) s = chunks;
-  if (s > 64) s = 64;
+  if (s > 256) s = 256;
+  if (s < 1) s = 1;
   return (int)s;
 }
+static constexpr long ATB_TARGET = 1024, AB_TARGET = 512;
 
 size_t gemm_ws_floats(long M, long K, int b) {
   (void)b;
-  const long tA = (K + SK_TILE - 1) / SK_TILE, cA = (M + SK_K - 1) / SK_K;
-  const long tB = (M + SK_TILE - 1) / SK_TILE, cB = (K + SK_K - 1) / SK_K;
-  const long s1 = pick_split(tA, cA), s2 = pick_split(tB, cB);
-  const long a = s1 * K * SK_B, c = s2 * M * SK_B;
-  return (size_t)(a > c ? a : c);
+  const long s1 = pick_split((K + 127) / 128, (M + SK_K - 1) / SK_K, ATB_TARGET);
+  const long s2 = pick_split((M + 255) / 256, (K + SK_K - 1) / SK_K, AB_TARGET);
+  const long a = s1 > 1 ? s1 * K * SK_B : 0, c = s2 > 1 ? s2 * M * SK_B : 0;
+  return (size_t)std::max(std::max(a, c), 1L);
+}
+
+template <int PREC, int NB>
+static void atb_dispatch(hipStream_t st, dim3 grid, bool vec, const float* A, long M, long K, const float* B, int b,
+                         int ns, float* Y, float* ws) {
+  if (vec)
+    hipLaunchKernelGGL((gemm_atb_kernel<PREC, NB, true>), grid, dim3(256), 0, st, A, M, K, B, b, ns, Y, ws);
+  else
+    hipLaunchKernelGGL((gemm_atb_kernel<PREC, NB, false>), grid, dim3(256), 0, st, A, M, K, B, b, ns, Y, ws);
+}
+template <int PREC, int NB>
+static void ab_dispatch(hipStream_t st, dim3 grid, bool vec, const float* A, long M, long K, const float* B, int b,
+                        int ns, float* Y, float* ws) {
+  if (vec)
+    hipLaunchKernelGGL((gemm_ab_kernel<PREC, NB, true>), grid, dim3(512), 0, st, A, M, K, B, b, ns, Y, ws);
+  else
+    hipLaunchKernelGGL((gemm_ab_kernel<PREC, NB, false>), grid, dim3(512), 0, st, A, M, K, B, b, ns, Y, ws);
 }
 
 int gemm_atb_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec, float* Y,
                     float* ws, size_t ws_floats) {
   if (b < 1 || b > SK_B || M < 1 || K < 1) return 8;
-  const long tiles = (K + SK_TILE - 1) / SK_TILE, chunks = (M + SK_K - 1) / SK_K;
-  const int ns = pick_split(tiles, chunks);
-  if ((size_t)ns * K * SK_B > ws_floats) return 4;
+  const long tiles = (K + 127) / 128, chunks = (M + SK_K - 1) / SK_K;
+  const int ns = pick_split(tiles, chunks, ATB_TARGET);
+  if (ns > 1 && (size_t)ns * K * SK_B > ws_floats) return 4;
+  const bool vec = M % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
   dim3 grid((unsigned)tiles, ns);
+  const bool nb1 = b <= 32;
   if (prec == PREC_BF16) {
-    if (b <= 32) hipLaunchKernelGGL((gemm_atb_kernel<PREC_BF16, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-    else hipLaunchKernelGGL((gemm_atb_kernel<PREC_BF16, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+    if (nb1) atb_dispatch<PREC_BF16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    else atb_dispatch<PREC_BF16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+  } else if (prec == PREC_F16) {
+    if (nb1) atb_dispatch<PREC_F16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    else atb_dispatch<PREC_F16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+  } else {
+    if (nb1) atb_dispatch<PREC_F32, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    else atb_dispatch<PREC_F32, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
   }
-  else if (prec == PREC_F16) {
-    if (b <= 32) hipLaunchKernelGGL((gemm_atb_kernel<PREC_F16, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-    else hipLaunchKernelGGL((gemm_atb_kernel<PREC_F16, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  if (ns > 1) {
+    const long tot = K * b;
+    hipLaunchKernelGGL(skinny_sum_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ws, K, b, ns, Y);
   }
-  else {
-    if (b <= 32) hipLaunchKernelGGL((gemm_atb_kernel<PREC_F32, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-    else hipLaunchKernelGGL((gemm_atb_kernel<PREC_F32, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-  }
-  const long tot = K * SK_B;
-  hipLaunchKernelGGL(skinny_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ws, K, b, ns, Y);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
 int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec, float* Y,
                    float* ws, size_t ws_floats) {
   if (b < 1 || b > SK_B || M < 1 || K < 1) return 8;
-  const long tiles = (M + SK_TILE - 1) / SK_TILE, chunks = (K + SK_K - 1) / SK_K;
-  const int ns = pick_split(tiles, chunks);
-  if ((size_t)ns * M * SK_B > ws_floats) return 4;
+  const long tiles = (M + 255) / 256, chunks = (K + SK_K - 1) / SK_K;
+  const int ns = pick_split(tiles, chunks, AB_TARGET);
+  if (ns > 1 && (size_t)ns * M * SK_B > ws_floats) return 4;
+  const bool vec = K % 4 == 0 && ((uintptr_t)B & 15) == 0;
   dim3 grid((unsigned)tiles, ns);
+  const bool nb1 = b <= 32;
   if (prec == PREC_BF16) {
-    if (b <= 32) hipLaunchKernelGGL((gemm_ab_kernel<PREC_BF16, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-    else hipLaunchKernelGGL((gemm_ab_kernel<PREC_BF16, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+    if (nb1) ab_dispatch<PREC_BF16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    else ab_dispatch<PREC_BF16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+  } else if (prec == PREC_F16) {
+    if (nb1) ab_dispatch<PREC_F16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    else ab_dispatch<PREC_F16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+  } else {
+    if (nb1) ab_dispatch<PREC_F32, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    else ab_dispatch<PREC_F32, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
   }
-  else if (prec == PREC_F16) {
-    if (b <= 32) hipLaunchKernelGGL((gemm_ab_kernel<PREC_F16, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-    else hipLaunchKernelGGL((gemm_ab_kernel<PREC_F16, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
+  if (ns > 1) {
+    const long tot = M * b;
+    hipLaunchKernelGGL(skinny_sum_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ws, M, b, ns, Y);
   }
-  else {
-    if (b <= 32) hipLaunchKernelGGL((gemm_ab_kernel<PREC_F32, 1>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-    else hipLaunchKernelGGL((gemm_ab_kernel<PREC_F32, 2>), grid, dim3(256), 0, st, A, M, K, B, b, ns, ws);
-  }
-  const long tot = M * SK_B;
-  hipLaunchKernelGGL(skinny_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ws, M, b, ns, Y);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 

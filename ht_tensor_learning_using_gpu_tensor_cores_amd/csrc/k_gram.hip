@@ -464,7 +464,20 @@ __global__ __launch_bounds__(512) void gram_big_kernel(const float* __restrict__
   __shared__ double red[8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wr = w >> 1, wc = w & 1;
-  const int split = blockIdx.x, pair = blockIdx.y;
+  // Work-groups are dealt to the 8 XCDs in turn (linear id mod 8). With n_split a
+  // multiple of 8, every tile pair of a column split runs on the same XCD, so a row
+  // panel's chunk, read by the n_tiles pairs that contain its tile, comes from HBM once
+  // into that XCD's L2 rather than once per pair.
+  const int n_pairs = n_tiles * (n_tiles + 1) / 2;
+  int split, pair;
+  if (n_split % 8 == 0) {
+    const int q = blockIdx.x >> 3;
+    split = (blockIdx.x & 7) + 8 * (q / n_pairs);
+    pair = q % n_pairs;
+  } else {
+    split = blockIdx.x % n_split;
+    pair = blockIdx.x / n_split;
+  }
   int bi = 0, bj = 0;
   {
     int p = pair;
@@ -525,13 +538,15 @@ int gram_pms_pairs(const float* x, PMS v, int prec) {
   return nt * (nt + 1) / 2;
 }
 
-// Column splits: 256-row tiles take one block per CU (256 blocks, LDS-limited to one per
-// CU), 128-row tiles two per CU.
+// Column splits: 256-row tiles take at most one block per CU (256 blocks, LDS-limited to
+// one per CU), a multiple of 8 splits where there are that many (the XCD placement of
+// gram_big_kernel); 128-row tiles two blocks per CU.
 int gram_pms_nsplit(const float* x, PMS v, int prec) {
   const int tile = gram_pms_tile(x, v, prec);
   const int kc = tile == GB_ROWS ? GB_K : GT_K;
   const long n_chunks = (v.P * v.S + kc - 1) / kc;
   long ns = (tile == GB_ROWS ? 256 : 512) / gram_pms_pairs(x, v, prec);
+  if (tile == GB_ROWS && ns >= 8) ns -= ns % 8;
   if (ns < 1) ns = 1;
   if (ns > n_chunks) ns = n_chunks;
   return (int)ns;
@@ -545,7 +560,7 @@ static void gram_dispatch(hipStream_t st, const float* x, PMS v, int n_split, fl
     if (gram_pms_tile(x, v, PREC) == GB_ROWS) {
       const long n_chunks = (K + GB_K - 1) / GB_K;
       const int nt = (int)((v.M + GB_ROWS - 1) / GB_ROWS);
-      dim3 grid(n_split, nt * (nt + 1) / 2);
+      dim3 grid(n_split * (nt * (nt + 1) / 2));
       if (v.P == 1)
         hipLaunchKernelGGL((gram_big_kernel<PREC, LD_ROWS_CONTIG>), grid, dim3(512), 0, st, x, v, K, n_chunks,
                            n_split, nt, part, norms);

@@ -201,9 +201,14 @@ def test_tsqr(lib, handle, rng, m, b):
     assert np.allclose(np.tril(Rh[:kk, :kk], -1), 0)
 
 
+# Shapes cover one and many reduction splits, b in one or two 32-column tiles, rows and
+# reduction lengths that fill no whole tile or chunk, and the unaligned (scalar) loads:
+# M % 4 != 0 for A^T B, K % 4 != 0 for A B.
 @pytest.mark.parametrize("M,K,b,trans", [(4096, 3000, 64, 1), (4096, 3000, 24, 0), (1000, 777, 17, 1),
-                                         (130, 5000, 8, 0), (16384, 512, 40, 1)])
-@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+                                         (130, 5000, 8, 0), (16384, 512, 40, 1), (1022, 3001, 40, 1),
+                                         (1024, 65537, 40, 0), (777, 4099, 33, 0), (300, 64, 64, 0),
+                                         (62, 70001, 40, 1)])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
 def test_skinny_gemm(lib, handle, rng, M, K, b, trans, precision):
     import torch
     A = rng.standard_normal((M, K)).astype(np.float32)
@@ -216,6 +221,8 @@ def test_skinny_gemm(lib, handle, rng, M, K, b, trans, precision):
     Yh = host(Y).reshape((rows, b), order="F")
     if precision == "bf16":
         A, B = bf16_round(A), bf16_round(B)
+    elif precision == "fp16":
+        A, B = A.astype(np.float16), B.astype(np.float16)
     A64, B64 = A.astype(np.float64), B.astype(np.float64)
     ref = A64.T @ B64 if trans else A64 @ B64
     np.testing.assert_allclose(Yh, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
