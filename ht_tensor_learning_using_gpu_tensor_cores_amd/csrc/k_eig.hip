@@ -212,30 +212,53 @@ __global__ __launch_bounds__(256) void dgemm_tiled_kernel(int m, int n, int k, d
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  for (int k0 = kb; k0 < ke; k0 += DG_K) {
+  // Tiles of the next k step are loaded into registers while this one is multiplied.
+  // Every load is issued unconditionally with its index clamped into the matrix (a load
+  // behind a divergent branch is waited for at the join); entries outside are stored
+  // as zero.
+  double ra[4], rb[4];
+  auto load = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = tid + 256 * q;
       int ii, kk;
       if (TA == 0) { ii = e % DG_T; kk = e / DG_T; } else { kk = e % DG_K; ii = e / DG_K; }
-      const long gi = i0 + ii;
-      const int gk = k0 + kk;
-      double v = 0.0;
-      if (gi < m && gk < ke) v = TA == 0 ? A[gi + (long)lda * gk] : A[gk + (long)lda * gi];
-      As[kk][ii] = v;
+      const long gi = min(i0 + ii, (long)m - 1);
+      const int gk = min(k0 + kk, ke - 1);
+      ra[q] = TA == 0 ? A[gi + (long)lda * gk] : A[gk + (long)lda * gi];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = tid + 256 * q;
       int jj, kk;
       if (TB == 0) { kk = e % DG_K; jj = e / DG_K; } else { jj = e % DG_T; kk = e / DG_T; }
-      const long gj = j0 + jj;
-      const int gk = k0 + kk;
-      double v = 0.0;
-      if (gj < n && gk < ke) v = TB == 0 ? B[gk + (long)ldb * gj] : B[gj + (long)ldb * gk];
-      Bs[kk][jj] = v;
+      const long gj = min(j0 + jj, (long)n - 1);
+      const int gk = min(k0 + kk, ke - 1);
+      rb[q] = TB == 0 ? B[gk + (long)ldb * gj] : B[gj + (long)ldb * gk];
     }
+  };
+  auto store = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int ii, kk;
+      if (TA == 0) { ii = e % DG_T; kk = e / DG_T; } else { kk = e % DG_K; ii = e / DG_K; }
+      As[kk][ii] = (i0 + ii < m && k0 + kk < ke) ? ra[q] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = tid + 256 * q;
+      int jj, kk;
+      if (TB == 0) { kk = e % DG_K; jj = e / DG_K; } else { jj = e % DG_T; kk = e / DG_T; }
+      Bs[kk][jj] = (j0 + jj < n && k0 + kk < ke) ? rb[q] : 0.0;
+    }
+  };
+  if (kb < ke) load(kb);
+  for (int k0 = kb; k0 < ke; k0 += DG_K) {
     __syncthreads();
+    store(k0);
+    __syncthreads();
+    if (k0 + DG_K < ke) load(k0 + DG_K);
 #pragma unroll
     for (int kk = 0; kk < DG_K; ++kk) {
       double av[4], bv[4];
@@ -248,7 +271,6 @@ __global__ __launch_bounds__(256) void dgemm_tiled_kernel(int m, int n, int k, d
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
     }
-    __syncthreads();
   }
 #pragma unroll
   for (int a = 0; a < 4; ++a)

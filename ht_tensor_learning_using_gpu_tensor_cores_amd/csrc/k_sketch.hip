@@ -345,30 +345,30 @@ __global__ __launch_bounds__(256) void sketch_solve_kernel(const double* __restr
 }
 
 // out (rows x w, FP32) = A (rows x w, FP32) C (w x w, FP64)  [+ beta[0] * B]. One
-// thread per row: its w values of A are read once (coalesced across the wave for each
-// column) and kept in registers, and the outputs are formed 16 columns at a time, so A
-// is streamed once instead of once per output column.
+// thread per row: its w values of A are loaded at once (every load issued
+// unconditionally: columns past w read column w - 1 and are never used; a wave's load
+// of a column covers 64 consecutive rows) and kept in registers, and the outputs are
+// formed 16 columns at a time in FP64, so A is streamed once. C is read with
+// wave-uniform indices (scalar loads; no LDS staging). W: w rounded up to 8.
+template <int W>
 __global__ __launch_bounds__(256) void tall_small_kernel(const float* __restrict__ A, long rows, int w,
                                                          const double* __restrict__ C, const float* __restrict__ B,
                                                          const double* __restrict__ beta, float* __restrict__ out) {
-  __shared__ double cs[64 * 64];
-  for (int e = threadIdx.x; e < w * w; e += blockDim.x) cs[e] = C[e];
-  __syncthreads();
   const double bt = B ? beta[0] : 0.0;
   for (long r = (long)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (long)gridDim.x * blockDim.x) {
-    float a[64];
+    float a[W];
 #pragma unroll
-    for (int k = 0; k < 64; ++k) a[k] = k < w ? A[r + rows * k] : 0.f;
+    for (int k = 0; k < W; ++k) a[k] = A[r + rows * min(k, w - 1)];
     for (int j0 = 0; j0 < w; j0 += 16) {
       double acc[16];
 #pragma unroll
       for (int u = 0; u < 16; ++u) acc[u] = 0.0;
 #pragma unroll
-      for (int k = 0; k < 64; ++k) {
+      for (int k = 0; k < W; ++k) {
         if (k < w) {
           const double ak = (double)a[k];
 #pragma unroll
-          for (int u = 0; u < 16; ++u) acc[u] = fma(ak, cs[k + w * min(j0 + u, w - 1)], acc[u]);
+          for (int u = 0; u < 16; ++u) acc[u] = fma(ak, C[k + w * min(j0 + u, w - 1)], acc[u]);
         }
       }
 #pragma unroll
@@ -505,7 +505,18 @@ int sketch_solve_launch(hipStream_t st, const double* W, const double* E, const 
 int tall_small_launch(hipStream_t st, const float* A, long rows, int w, const double* C, const float* B,
                       const double* beta, float* out) {
   if (w < 1 || w > 64) return 8;
-  hipLaunchKernelGGL(tall_small_kernel, dim3(grid_for(rows)), dim3(256), 0, st, A, rows, w, C, B, beta, out);
+  // one row per thread where the grid allows (the occupancy hides the loads' latency)
+  const dim3 g(grid_for(rows, 1L << 20)), blk(256);
+  switch ((w + 7) / 8) {
+    case 1: hipLaunchKernelGGL(tall_small_kernel<8>, g, blk, 0, st, A, rows, w, C, B, beta, out); break;
+    case 2: hipLaunchKernelGGL(tall_small_kernel<16>, g, blk, 0, st, A, rows, w, C, B, beta, out); break;
+    case 3: hipLaunchKernelGGL(tall_small_kernel<24>, g, blk, 0, st, A, rows, w, C, B, beta, out); break;
+    case 4: hipLaunchKernelGGL(tall_small_kernel<32>, g, blk, 0, st, A, rows, w, C, B, beta, out); break;
+    case 5: hipLaunchKernelGGL(tall_small_kernel<40>, g, blk, 0, st, A, rows, w, C, B, beta, out); break;
+    case 6: hipLaunchKernelGGL(tall_small_kernel<48>, g, blk, 0, st, A, rows, w, C, B, beta, out); break;
+    case 7: hipLaunchKernelGGL(tall_small_kernel<56>, g, blk, 0, st, A, rows, w, C, B, beta, out); break;
+    default: hipLaunchKernelGGL(tall_small_kernel<64>, g, blk, 0, st, A, rows, w, C, B, beta, out); break;
+  }
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
