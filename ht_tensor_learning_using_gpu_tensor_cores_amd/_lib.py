@@ -99,6 +99,7 @@ SIGNATURES = {
     "ht_sym_eig": (I32, [H, P, I32, P, P]),
     "ht_sym_topk": (I32, [H, P, I32, I32, D, P, P, PI32]),
     "ht_tsqr": (I32, [H, P, I64, I32, P, P]),
+    "ht_tall_orth": (I32, [H, P, I64, I32, P, P]),
     "ht_skinny_gemm": (I32, [H, P, I64, I64, I32, P, I32, I32, P]),
     "ht_philox_normals": (I32, [H, U64, U32, I64, P]),
     "ht_philox_normals_host": (None, [U64, U32, I64, PD]),

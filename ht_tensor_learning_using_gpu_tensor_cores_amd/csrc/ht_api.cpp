@@ -212,6 +212,11 @@ ht_status ht_destroy(ht_handle_t h) {
   if (h->xbuf) (void)hipFree(h->xbuf);
   if (h->obuf) (void)hipFree(h->obuf);
   for (auto& e : h->ev) (void)hipEventDestroy(e);
+  if (h->side) {
+    (void)hipStreamSynchronize(h->side);
+    for (auto& e : h->side_ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(h->side);
+  }
   (void)hipStreamDestroy(h->stream);
   delete h;
   return HT_OK;
@@ -308,6 +313,8 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
   e.prof = &h->prof;
   e.pin = h->pinned;
   e.pin_bytes = h->pinned_bytes;
+  e.side = &h->side;
+  e.side_ev = h->side_ev;
   e.plan = &h->plan;
   e.done_ev = h->ev[1];  // recorded by the run before its final wait
   e.adaptive = adaptive;
@@ -838,6 +845,19 @@ ht_status ht_tsqr(ht_handle_t h, float* W, int64_t m, int32_t b, float* Q, doubl
   void* ws = h->arena.get(tsqr_ws_bytes(m, b));
   if (!ws) return HT_ERR_ALLOC;
   CKS(tsqr_launch(h->stream, W, m, b, Q, R, ws));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+ht_status ht_tall_orth(ht_handle_t h, float* Y, int64_t m, int32_t b, float* Q, int32_t* fallback) {
+  if (!h || !Y || !Q || m < 1 || b < 1 || b > 64 || m < b) return HT_ERR_INVALID_ARG;
+  if (hipSetDevice(h->device) != hipSuccess) return HT_ERR_HIP;
+  h->arena.reset();
+  Engine e;
+  e.st = h->stream;
+  e.arena = &h->arena;
+  ht_default_options(&e.opt);
+  CKS(e.tall_orth(Y, m, b, Q));
+  if (fallback) *fallback = e.orth_fallbacks;
   return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
 }
 

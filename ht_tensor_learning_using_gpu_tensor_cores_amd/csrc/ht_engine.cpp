@@ -633,9 +633,8 @@ int Engine::tsqr(float* Y, long m, int b, float* Q, double* R) {
 // block; for m = 1,048,576 and b = 40 the passes are bound by HBM, not by latency. Y is
 // left unchanged. When a pivot of a later pass is not positive (an exactly dependent
 // column the shift did not lift), the basis comes from TSQR instead (one host read of the
-// pivot flag).
-static constexpr long TALL_ORTH_ROWS = 1L << 18;  // sketches this tall take tall_orth
-
+// pivot flag). The randomised method orthonormalises every sketch this way, the
+// 1024-row co-range sketches of the 1024^3 workload as well as the million-row ones.
 int Engine::tall_orth(float* Y, long m, int b, float* Q) {
   ProfScope ps(prof, st, PK_TSQR);
   double* W = alloc<double>((size_t)b * b);
@@ -656,6 +655,7 @@ int Engine::tall_orth(float* Y, long m, int b, float* Q) {
   int hinfo = 0;
   CK(d2h(&hinfo, info, sizeof(int)));
   if (hinfo) {
+    ++orth_fallbacks;
     double* R = alloc<double>((size_t)b * b);
     if (!R) return HT_ERR_ALLOC;
     void* ws = arena->get(tsqr_ws_bytes(m, b));
@@ -897,7 +897,30 @@ int Engine::leaves_from_x_plain(const float* x, double* xsq_dev) {
   return leaves_from_x(x, PREC_F32, xsq_dev);
 }
 
+// The side stream and its events exist (made here on first use; never with a fixed
+// arena, whose callers may not allow resources to be created).
+bool Engine::side_ready() {
+  if (!side || !side_ev || arena->fixed) return false;
+  if (*side) return true;
+  if (hipStreamCreateWithFlags(side, hipStreamNonBlocking) != hipSuccess) {
+    *side = nullptr;
+    return false;
+  }
+  for (int i = 0; i < SIDE_EVENTS; ++i)
+    if (hipEventCreateWithFlags(&side_ev[i], hipEventDisableTiming) != hipSuccess) {
+      for (int j = 0; j < i; ++j) (void)hipEventDestroy(side_ev[j]);
+      (void)hipStreamDestroy(*side);
+      *side = nullptr;
+      return false;
+    }
+  return true;
+}
+
 // Leaf bases from Grams of X (one pass per leaf), eigen problems of equal shape batched.
+// Leaves of more than 256 rows are solved one by one on the side stream, each as soon as
+// its Gram is done, beside the Grams that follow it on st (a Gram kernel leaves CUs free;
+// the solver's kernels are single work-groups and host-checked). The results equal those
+// of the same solves on st.
 int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
   const Problem& p = *prob;
   std::vector<int> leaves;
@@ -918,6 +941,7 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
     double* U = alloc<double>((size_t)nb * n * k);
     double* w = alloc<double>((size_t)nb * k);
     if (!G || !U || !w) return HT_ERR_ALLOC;
+    const bool overlap = n > 256 && nb <= HT_MAX_ORDER && side_ready();
     for (int i = 0; i < nb; ++i) {
       const int m = p.lo[grp[i]];
       CK(gram(x, view_of(p.dims, p.order, m, m + 1), prec, G + (size_t)i * n * n, have_norm ? nullptr : xsq_dev));
@@ -927,8 +951,29 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
         if (adaptive) CK(adapt_tau(xsq_dev, 2 * p.order - 3));
       }
       have_norm = true;
+      if (overlap) HT_HIP_CHECK(hipEventRecord(side_ev[i], st));
     }
-    CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
+    if (overlap) {
+      // the same solves (and Philox streams) as sym_topk_batch's one-at-a-time path
+      const hipStream_t main_st = st;
+      st = *side;
+      int rc = 0;
+      for (int i = 0; i < nb && !rc; ++i) {
+        if (hipStreamWaitEvent(st, side_ev[i], 0) != hipSuccess) rc = HT_ERR_HIP;
+        else
+          rc = sym_topk_large(G + (size_t)i * n * n, (int)n, k, U + (size_t)i * n * k, w + (size_t)i * k, 1e-9,
+                              opt.max_eig_iters, 1000 + grp[0] * 16 + 8 * i, GRAM_FLOOR);
+      }
+      const bool joined = hipEventRecord(side_ev[HT_MAX_ORDER], st) == hipSuccess;
+      st = main_st;
+      if (!joined || hipStreamWaitEvent(st, side_ev[HT_MAX_ORDER], 0) != hipSuccess) {
+        (void)hipStreamSynchronize(*side);
+        return rc ? rc : HT_ERR_HIP;
+      }
+      if (rc) return rc;
+    } else {
+      CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
+    }
     for (int i = 0; i < nb; ++i) {
       Ufull[grp[i]] = U + (size_t)i * n * k;
       CK(eig_to_sv(w + (size_t)i * k, grp[i]));
@@ -1435,11 +1480,11 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
 // Range finder of A = X_(t) (view v) from the test matrix Om (cols x l): Y = A Om, Q R = Y
 // (TSQR), then q power iterations Q <- orth(A orth(A^T Q)).
 int Engine::range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z,
-                         float* Qz, double* R) {
+                         float* Qz) {
   const long rows = v.M, cols = v.P * v.S;
-  // tall sketches (the million-row side of a root child) take shifted CholeskyQR3, the
-  // rest TSQR
-  auto orth = [&](float* A, long m, float* Qo) { return m >= TALL_ORTH_ROWS ? tall_orth(A, m, l, Qo) : tsqr(A, m, l, Qo, R); };
+  // the sketches' bases come from shifted CholeskyQR3 (TSQR when it meets a non-positive
+  // pivot): only Q is needed
+  auto orth = [&](float* A, long m, float* Qo) { return tall_orth(A, m, l, Qo); };
   CK(view_gemm(x, v, 0, Om, l, prec, Y));
   CK(orth(Y, rows, Q));
   for (int q = 0; q < opt.power_iters; ++q) {
@@ -1569,14 +1614,13 @@ int Engine::run_rsvd(const float* x) {
     float* Q = alloc<float>((size_t)rows * lmax);
     float* Z = alloc<float>((size_t)cols * lmax);
     float* Qz = alloc<float>((size_t)cols * lmax);
-    double* R = alloc<double>((size_t)lmax * lmax);
-    if (!Om || !Y || !Q || !Z || !Qz || !R) return HT_ERR_ALLOC;
+    if (!Om || !Y || !Q || !Z || !Qz) return HT_ERR_ALLOC;
     bool gaussian = !learned;
     if (learned) {
       int m0 = 0, m1 = 0;
       CK(node_col_modes(p, t, &m0, &m1));
       CK(kr_omega_launch(st, opt.sketch_weights, p.dims, soff, m0, m1, l, Om));
-      CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz, R));
+      CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz));
       double est = 0.0;
       CK(range_residual(x, v, t, Q, l, xsq, &est));
       ++sketch_nodes;
@@ -1589,16 +1633,19 @@ int Engine::run_rsvd(const float* x) {
     }
     if (gaussian) {
       CK(fill_normals_f32_launch(st, Om, cols * l, opt.seed, 256 + t));
-      CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz, R));
+      CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz));
     }
-    // A^T Q = Qz Rz  =>  Q^T A = Rz^T Qz^T; SVD of Rz^T via eig(Rz^T Rz).
+    // Z = A^T Q; the SVD of Q^T A = Z^T from the eigen problem of Z^T Z (FP64 Gram of the
+    // FP32 Z: exact products, FP64 sums): Z^T Z = Sv diag(th) Sv^T, left vectors Q Sv,
+    // right vectors Z Sv / sigma. (A QR of Z gives the same: Z = Qz Rz, Rz^T Rz = Z^T Z.)
     CK(view_gemm(x, v, 1, Q, l, prec, Z));
-    CK(tsqr(Z, cols, l, Qz, R));
     double* M = alloc<double>((size_t)l * l);
     double* Sv = alloc<double>((size_t)l * l);
     double* th = alloc<double>(l);
     int* info = alloc<int>(4);
-    CK(dgemm_small_launch(st, 1, 0, l, l, l, 1.0, R, l, R, l, 0.0, M, l));  // Rz^T Rz
+    double* gpart = alloc<double>((size_t)tall_gram_blocks(cols) * l * l);
+    if (!M || !Sv || !th || !info || !gpart) return HT_ERR_ALLOC;
+    CK(tall_gram_launch(st, Z, l, Z, l, cols, gpart, M));
     CK(jacobi_eig_launch(st, M, l, l, th, Sv, l, info, 60, 1e-15));
     ++eig_calls;
     // Left vectors of A: Q Sv; right vectors: Qz Rz Sv / sigma.
@@ -1613,12 +1660,14 @@ int Engine::run_rsvd(const float* x) {
     SVsq[t] = 0;
     if (kW) {
       double* RS = alloc<double>((size_t)l * kW);
-      double* Qzd = alloc<double>((size_t)cols * l);
+      double* Zd = alloc<double>((size_t)cols * l);
       Ufull[sib] = alloc<double>((size_t)cols * kW);
-      CK(dgemm_small_launch(st, 0, 0, l, kW, l, 1.0, R, l, Sv, l, 0.0, RS, l));
+      if (!RS || !Zd || !Ufull[sib]) return HT_ERR_ALLOC;
+      HT_HIP_CHECK(hipMemcpy2DAsync(RS, sizeof(double) * l, Sv, sizeof(double) * l, sizeof(double) * l, kW,
+                                    hipMemcpyDeviceToDevice, st));
       CK(scale_cols_inv_launch(st, RS, l, kW, sv));
-      CK(cast_f32_f64_launch(st, Qz, Qzd, cols * l));
-      CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, l, 1.0, Qzd, (int)cols, RS, l, 0.0, Ufull[sib], (int)cols));
+      CK(cast_f32_f64_launch(st, Z, Zd, cols * l));
+      CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, l, 1.0, Zd, (int)cols, RS, l, 0.0, Ufull[sib], (int)cols));
       CK(canon_signs(Ufull[sib], (int)cols, kW, (int)cols));
       SVd[sib] = sv;
       SVsq[sib] = 0;

@@ -100,6 +100,9 @@ struct RunPlan {
   bool valid = false;
 };
 
+// Events of the side stream: one per leaf whose Gram it waits for, and the join.
+constexpr int SIDE_EVENTS = HT_MAX_ORDER + 1;
+
 struct Engine {
   Profiler* prof = nullptr;
   hipStream_t st;
@@ -126,6 +129,7 @@ struct Engine {
   int SVsq[HT_MAX_NODES] = {};  // 1: SVd[t] holds eigenvalues (squared singular values)
   double passes = 0.0;
   int eig_iters_max = 0, eig_calls = 0, noconv = 0;
+  int orth_fallbacks = 0;  // tall_orth calls that took the TSQR basis
   int replayed = 0;  // 1: the run replayed the recorded plan and passed its verdicts
   int sketch_fallbacks = 0, sketch_nodes = 0;
   double sketch_resid = 0.0;
@@ -134,6 +138,11 @@ struct Engine {
   // copies into pageable memory complete synchronously one by one
   char* pin = nullptr;
   size_t pin_bytes = 0;
+  // The handle's side stream and its events (SIDE_EVENTS), made on first use: the eigen
+  // problems of large leaves run there beside the remaining Grams on st. Null: no side
+  // stream (they run on st).
+  hipStream_t* side = nullptr;
+  hipEvent_t* side_ev = nullptr;
   // record / replay of the host decisions (leaves-to-root only)
   RunPlan* plan = nullptr;
   bool recording = false, replay = false, plan_mismatch = false;
@@ -167,6 +176,7 @@ struct Engine {
                      double floor);
   int tsqr(float* Y, long m, int b, float* Q, double* R);
   int tall_orth(float* Y, long m, int b, float* Q);
+  bool side_ready();
   int view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y);
   int exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U, double* Wr, double* sv, uint32_t stream);
   int contract_transfer(const double* Ut, const double* Ul, const double* Ur, long nl, long nr, int kl, int kr,
@@ -185,8 +195,7 @@ struct Engine {
   int ltr_interior(double* C, int64_t* cd, const double* xsq, const float* x);
   int small_side_svd(const double* C, PMS v, int k, double* U, const double** w, int* done);
   int run_rsvd(const float* x);
-  int range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z, float* Qz,
-                   double* R);
+  int range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z, float* Qz);
   int range_residual(const float* x, PMS v, int t, const float* Q, int l, const double* xsq, double* est);
   // One sketched node's loss (accumulated into loss_acc[0]) and, when grad != nullptr,
   // scale times its gradient (accumulated into grad).
@@ -204,6 +213,9 @@ struct ht_handle_s {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {};
+  // side stream for large leaves' eigen problems (Engine::side), made on first use
+  hipStream_t side = nullptr;
+  hipEvent_t side_ev[ht::SIDE_EVENTS] = {};
   ht::Arena arena;
   float* xbuf = nullptr;   // device copy of host inputs
   size_t xbuf_bytes = 0;

@@ -353,13 +353,22 @@ int dgemm_splitk_launch(hipStream_t st, int ta, int tb, int m, int n, int k, dou
 // rank-deficient block defined; later passes use S = I, sigma = 0. Writes
 // Rinv = S (chol(W_s)^T)^-1 (upper triangular, column-major) so that Z Rinv has
 // orthonormal columns up to the pass's accuracy. info[0] |= 1 on a non-positive pivot.
+//
+// 256 threads on LDS-resident matrices with one barrier per step (compact loops: the
+// kernel runs once, so straight-line code would be fetched once and wait on the
+// instruction cache throughout). Factorisation step k reads the pivot, subtracts the
+// outer product of the unscaled column k divided by the pivot from the trailing lower
+// triangle, and after the barrier scales column k; the next step touches neither column
+// k nor what the scaling writes. The inverse T = L^-1 is eliminated the same way, a
+// column of L per step with T's rows scaled at the end.
 __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict__ W, int b, long m, int first,
                                                         double* __restrict__ Rinv, int* __restrict__ info) {
   __shared__ double L[JMAX * JLD];
   __shared__ double T[JMAX * JLD];
   __shared__ double sc[JMAX];
+  __shared__ double rdg[JMAX];
   __shared__ double shift;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
   if (tid < b) {
     const double d = W[tid + (long)b * tid];
     sc[tid] = (first && d > 0.0) ? 1.0 / sqrt(d) : 1.0;
@@ -368,73 +377,52 @@ __global__ __launch_bounds__(256) void chol_rinv_kernel(const double* __restrict
   for (int e = tid; e < b * b; e += 256) {
     const int i = e % b, j = e / b;
     L[i * JLD + j] = sc[i] * W[e] * sc[j];
+    T[i * JLD + j] = i == j ? 1.0 : 0.0;
   }
   __syncthreads();
-  if (tid == 0) {
-    double tr = 0.0;
-    for (int i = 0; i < b; ++i) tr += L[i * JLD + i];
-    shift = first ? 11.0 * ((double)m * b + (double)b * (b + 1)) * 1.1102230246251565e-16 * tr : 0.0;
+  if (tid < 64) {
+    double tr = tid < b ? L[tid * JLD + tid] : 0.0;
+    for (int o = 32; o > 0; o >>= 1) tr += __shfl_xor(tr, o, 64);
+    if (tid == 0) shift = first ? 11.0 * ((double)m * b + (double)b * (b + 1)) * 1.1102230246251565e-16 * tr : 0.0;
   }
   __syncthreads();
   if (tid < b) L[tid * JLD + tid] += shift;
   __syncthreads();
   for (int k = 0; k < b; ++k) {
-    if (tid == 0) {
-      double d = L[k * JLD + k];
-      if (!(d > 0.0)) {
-        if (info) info[0] |= 1;
-        d = 1e-300;
-      }
-      L[k * JLD + k] = sqrt(d);
+    double d = L[k * JLD + k];
+    if (!(d > 0.0)) {
+      if (tid == 0 && info) info[0] |= 1;
+      d = 1e-300;
+    }
+    const double rdd = 1.0 / d;
+    const int R = b - k - 1;
+    for (int i = k + 1 + tx; i < b; i += 16) {  // the trailing lower triangle
+      const double lik = L[i * JLD + k] * rdd;
+      for (int j = k + 1 + ty; j <= i; j += 16) L[i * JLD + j] -= lik * L[j * JLD + k];
     }
     __syncthreads();
-    for (int i = k + 1 + tid; i < b; i += 256) L[i * JLD + k] /= L[k * JLD + k];
-    __syncthreads();
-    // trailing update of the lower triangle, four entries a round: their loads are all
-    // issued before the stores (stores between loads would order every load behind them)
-    const int r = b - k - 1, nl = r * (r + 1) / 2;
-    for (int e0 = tid; e0 < nl; e0 += 4 * 256) {
-      int ii[4], jj[4];
-      double v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int e = e0 + 256 * u;
-        // e -> (i, j), j <= i, over the packed lower triangle of the trailing square
-        int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-        if ((i + 1) * (i + 2) / 2 <= e) ++i;
-        if (i * (i + 1) / 2 > e) --i;
-        ii[u] = k + 1 + i;
-        jj[u] = k + 1 + (e - i * (i + 1) / 2);
-        v[u] = e < nl ? L[ii[u] * JLD + jj[u]] - L[ii[u] * JLD + k] * L[jj[u] * JLD + k] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (e0 + 256 * u < nl) L[ii[u] * JLD + jj[u]] = v[u];
-    }
-    __syncthreads();
-  }
-  // T = L^-1 (lower), column j by thread j; four partial sums keep the row's loads in
-  // flight instead of one dependent chain of length i - j
-  if (tid < b) {
-    const int j = tid;
-    for (int i = 0; i < b; ++i) {
-      double s0 = (i == j) ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int q = j;
-      for (; q + 4 <= i; q += 4) {
-        s0 -= L[i * JLD + q] * T[q * JLD + j];
-        s1 -= L[i * JLD + q + 1] * T[(q + 1) * JLD + j];
-        s2 -= L[i * JLD + q + 2] * T[(q + 2) * JLD + j];
-        s3 -= L[i * JLD + q + 3] * T[(q + 3) * JLD + j];
-      }
-      for (; q < i; ++q) s0 -= L[i * JLD + q] * T[q * JLD + j];
-      T[i * JLD + j] = i < j ? 0.0 : ((s0 + s1) + (s2 + s3)) / L[i * JLD + i];
-    }
+    const double piv = sqrt(d);
+    if (tid < R) L[(k + 1 + tid) * JLD + k] /= piv;
+    if (tid == 0) L[k * JLD + k] = piv;
   }
   __syncthreads();
-  // Rinv[i, j] = sc[i] (L^-T)[i, j] = sc[i] T[j, i]
+  if (tid < b) rdg[tid] = 1.0 / L[tid * JLD + tid];
+  __syncthreads();
+  // T starts as I; step q subtracts L[r][q] (T[q][:] / L[q][q]) from the rows r > q, on
+  // the columns j <= q (T is lower triangular); row q is final (before its scaling) once
+  // step q - 1 is done
+  for (int q = 0; q < b; ++q) {
+    const double rq = rdg[q];
+    for (int r = q + 1 + tx; r < b; r += 16) {
+      const double lrq = L[r * JLD + q] * rq;
+      for (int j = ty; j <= q; j += 16) T[r * JLD + j] -= lrq * T[q * JLD + j];
+    }
+    __syncthreads();
+  }
+  // Rinv[i, j] = sc[i] (L^-T)[i, j] = sc[i] T[j, i] rdg[j]
   for (int e = tid; e < b * b; e += 256) {
     const int i = e % b, j = e / b;
-    Rinv[e] = sc[i] * T[j * JLD + i];
+    Rinv[e] = j < i ? 0.0 : sc[i] * T[j * JLD + i] * rdg[j];
   }
 }
 

@@ -53,6 +53,13 @@ ht_status ht_sym_topk(ht_handle_t h, const double* G, int32_t n, int32_t k, doub
  * TSQR/kernel.cu:500-677. */
 ht_status ht_tsqr(ht_handle_t h, float* W, int64_t m, int32_t b, float* Q, double* R);
 
+/* Orthonormal basis Q (m x b FP32) of the columns of Y (m x b FP32, b <= 64, m >= b) by
+ * shifted CholeskyQR3 with FP64 Grams -- the randomised method's orthonormalisation of
+ * its sketches -- or, when a later pass meets a non-positive pivot, by TSQR
+ * (*fallback, host, optional: 1 then). Y is left unchanged. Replaces the QR step of
+ * tsqr_svd for tall sketches: third_order_tensor_decompsition/CUDA/TSQR/kernel.cu:500-677. */
+ht_status ht_tall_orth(ht_handle_t h, float* Y, int64_t m, int32_t b, float* Q, int32_t* fallback);
+
 /* Skinny products with A (M x K column-major): Y = A^T B (transA=1, B: M x b) or
  * Y = A B (transA=0, B: K x b), b <= 64. The sketch/co-range GEMMs of rsvd:
  * fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/kernel.cu:434-444. */

@@ -201,6 +201,44 @@ def test_tsqr(lib, handle, rng, m, b):
     assert np.allclose(np.tril(Rh[:kk, :kk], -1), 0)
 
 
+# Shifted CholeskyQR3 (the randomised method's orthonormalisation of its sketches): a
+# Gaussian block, one whose column norms span six decades (the first, equilibrated and
+# shifted pass has to keep the small directions), and one with a repeated column (rank
+# b - 1: the basis still spans Y, from CholeskyQR3 or its TSQR fallback). b = 40 is the
+# 1024^3 workload's sketch width (1,048,576 and 1024 rows), b = 64 the largest.
+@pytest.mark.parametrize("m,b,kind", [(300000, 40, "gauss"), (262144, 64, "graded"), (270001, 17, "graded"),
+                                      (262144, 40, "repeat"), (1024, 40, "gauss"), (1000, 40, "graded"),
+                                      (64, 64, "gauss")])
+def test_tall_orth(lib, handle, rng, m, b, kind):
+    import torch
+    Y = rng.standard_normal((m, b)).astype(np.float32)
+    if kind == "graded":
+        Y *= np.logspace(0, -6, b).astype(np.float32)
+    elif kind == "repeat":
+        Y[:, 7] = Y[:, 3]
+    Yd = dev(Y.reshape(-1, order="F"))
+    Q = torch.zeros(m * b, device="cuda")
+    fb = ctypes.c_int32(-1)
+    _lib.check(lib.ht_tall_orth(handle.ptr, ptr(Yd), m, b, ptr(Q), ctypes.byref(fb)), "tall_orth")
+    assert np.array_equal(host(Yd), Y.reshape(-1, order="F"))  # Y is left unchanged
+    Qh = host(Q).reshape((m, b), order="F").astype(np.float64)
+    Y64 = Y.astype(np.float64)
+    print(f"tall_orth {kind} m={m} b={b}: fallback {fb.value}", flush=True)
+    if kind != "repeat":
+        assert fb.value == 0
+        np.testing.assert_allclose(Qh.T @ Qh, np.eye(b), atol=2e-6)
+        # Y = Q (Q^T Y): the basis spans Y column by column, the weakest to FP32 accuracy
+        resid = Y64 - Qh @ (Qh.T @ Y64)
+        assert np.all(np.linalg.norm(resid, axis=0) <= 1e-5 * np.linalg.norm(Y64, axis=0))
+    else:
+        # b - 1 independent columns: Q's columns are orthonormal up to one direction the
+        # data leaves free, and they span Y
+        sv = np.linalg.svd(Qh, compute_uv=False)
+        assert np.all(sv[:b - 1] > 1 - 1e-5) and sv[0] < 1 + 1e-5
+        resid = Y64 - Qh @ np.linalg.lstsq(Qh, Y64, rcond=None)[0]
+        assert np.linalg.norm(resid) <= 1e-5 * np.linalg.norm(Y64)
+
+
 # Shapes cover one and many reduction splits, b in one or two 32-column tiles, rows and
 # reduction lengths that fill no whole tile or chunk, and the unaligned (scalar) loads:
 # M % 4 != 0 for A^T B, K % 4 != 0 for A B.
