@@ -522,8 +522,10 @@ int Engine::sym_topk(const double* G, int n, int k, double* U, double* w, double
 
 // Shifted CholeskyQR3 of Z (m x b, FP64, ld m) in place: three passes of W = Z^T Z (tiled
 // FP64 GEMM), the triangular inverse (one work-group) and Z <- Z R^-1; the first pass
-// equilibrates the columns and shifts W (chol_rinv_kernel). Zt is an m x b scratch block.
-static int cholqr3(hipStream_t st, double* Z, double* Zt, long m, int b, double* W, double* Rinv, int* info,
+// equilibrates the columns and shifts W (chol_rinv_kernel). Zt is an m x b scratch block;
+// the passes alternate between the two, and the pointers are swapped when the result
+// ends in the scratch block (no copy): on return Z holds the basis.
+static int cholqr3(hipStream_t st, double*& Z, double*& Zt, long m, int b, double* W, double* Rinv, int* info,
                    uint64_t seed, uint32_t stream, double* ws, int passes = 3) {
   double* src = Z;
   double* dst = Zt;
@@ -534,18 +536,22 @@ static int cholqr3(hipStream_t st, double* Z, double* Zt, long m, int b, double*
     CK(dgemm_small_launch(st, 0, 0, (int)m, b, b, 1.0, src, (int)m, Rinv, b, 0.0, dst, (int)m));
     std::swap(src, dst);
   }
-  if (src != Z) HT_HIP_CHECK(hipMemcpyAsync(Z, src, sizeof(double) * m * b, hipMemcpyDeviceToDevice, st));
+  if (src != Z) std::swap(Z, Zt);
   return 0;
 }
 
-// Subspace iteration for n > 256 (multi-kernel, host-checked convergence).
+// Block width of sym_topk_large.
+static int large_block(int n, int k) { return std::min(std::min(64, std::max(2 * k, k + 16)), n); }
+
+// Subspace iteration for n > 256 (multi-kernel, host-checked convergence). The starting
+// block is Philox stream `stream` (n x large_block(n, k) normals); Q0, when given, holds
+// it already (and is overwritten).
 int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
-                           double floor) {
+                           double floor, double* Q0) {
   ProfScope ps(prof, st, PK_EIG);
   int* info = alloc<int>(4);
-  int b = std::min(64, std::max(2 * k, k + 16));
-  b = std::min(b, n);
-  double* Q = alloc<double>((size_t)n * b);
+  const int b = large_block(n, k);
+  double* Q = Q0 ? Q0 : alloc<double>((size_t)n * b);
   double* Y = alloc<double>((size_t)n * b);
   double* T = alloc<double>((size_t)b * b);
   double* S = alloc<double>((size_t)b * b);
@@ -556,7 +562,7 @@ int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, 
   double* Ri = alloc<double>((size_t)b * b);
   double* ws = alloc<double>((size_t)dgemm_splitk_slices(b, b, n) * b * b);
   if (!Q || !Y || !T || !S || !th || !W || !res || !Zt || !Ri || !ws) return HT_ERR_ALLOC;
-  CK(fill_normals_f64_launch(st, Q, (long)n * b, opt.seed, stream));
+  if (!Q0) CK(fill_normals_f64_launch(st, Q, (long)n * b, opt.seed, stream));
   CK(cholqr3(st, Q, Zt, n, b, W, Ri, info, opt.seed, stream + 1, ws));
   int it = 0, steps = 2;
   std::vector<double> hres(k), hth(b);
@@ -942,6 +948,15 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
     double* w = alloc<double>((size_t)nb * k);
     if (!G || !U || !w) return HT_ERR_ALLOC;
     const bool overlap = n > 256 && nb <= HT_MAX_ORDER && side_ready();
+    // the solvers' starting blocks are drawn on st first, while the GPU is free
+    std::vector<double*> Q0(nb, nullptr);
+    if (overlap)
+      for (int i = 0; i < nb; ++i) {
+        const int bq = large_block((int)n, k);
+        Q0[i] = alloc<double>((size_t)n * bq);
+        if (!Q0[i]) return HT_ERR_ALLOC;
+        CK(fill_normals_f64_launch(st, Q0[i], n * bq, opt.seed, 1000 + grp[0] * 16 + 8 * i));
+      }
     for (int i = 0; i < nb; ++i) {
       const int m = p.lo[grp[i]];
       CK(gram(x, view_of(p.dims, p.order, m, m + 1), prec, G + (size_t)i * n * n, have_norm ? nullptr : xsq_dev));
@@ -962,7 +977,7 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
         if (hipStreamWaitEvent(st, side_ev[i], 0) != hipSuccess) rc = HT_ERR_HIP;
         else
           rc = sym_topk_large(G + (size_t)i * n * n, (int)n, k, U + (size_t)i * n * k, w + (size_t)i * k, 1e-9,
-                              opt.max_eig_iters, 1000 + grp[0] * 16 + 8 * i, GRAM_FLOOR);
+                              opt.max_eig_iters, 1000 + grp[0] * 16 + 8 * i, GRAM_FLOOR, Q0[i]);
       }
       const bool joined = hipEventRecord(side_ev[HT_MAX_ORDER], st) == hipSuccess;
       st = main_st;

@@ -173,7 +173,7 @@ struct Engine {
                      uint32_t stream, double floor, const float* G32in = nullptr, double tol_rel = 0.0);
   int canon_signs(double* U, int m, int k, int ldu);
   int sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
-                     double floor);
+                     double floor, double* Q0 = nullptr);
   int tsqr(float* Y, long m, int b, float* Q, double* R);
   int tall_orth(float* Y, long m, int b, float* Q);
   bool side_ready();
