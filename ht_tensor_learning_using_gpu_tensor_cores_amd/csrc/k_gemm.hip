@@ -607,8 +607,10 @@ __global__ __launch_bounds__(256) void lowrank_kernel(const float* __restrict__ 
 template <int KP, bool FULL>
 __global__ void residual_kernel(const float* __restrict__ L, const float* __restrict__ R, long M, long N, int k,
                                 const float* __restrict__ X, int nts, double* __restrict__ acc_out);
-__global__ void residual_stream_kernel(const float* __restrict__ L, const float* __restrict__ R, long M, long N,
-                                       int k, const float* __restrict__ X, int nts, double* __restrict__ acc_out);
+template <int KP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP <= 16 ? 4 : 2, KP <= 16 ? 8 : 2))) void
+residual_stream_kernel(const float* __restrict__ L, const float* __restrict__ R, long M, long N, int k,
+                       const float* __restrict__ X, int nts, double* __restrict__ acc_out);
 
 int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, long M, long N, int k,
                    const float* X, float* out, double* acc, int* n_blocks_out) {
@@ -626,7 +628,7 @@ int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, lon
   // subtiles keep their prefetch busy for longer; nts shrinks while the grid would fall
   // below 2048 blocks (8 per CU).
   int nts = 4;
-  if (full && k <= 16) {
+  if (full) {
     nts = 32;
     while (nts > 4 && (N % (32L * nts) != 0 || grid.x * (N / (32L * nts)) < 2048)) nts /= 2;
     grid.y = (unsigned)(N / (32L * nts));
@@ -634,11 +636,11 @@ int lowrank_launch(hipStream_t st, int mode, const float* L, const float* R, lon
   if (n_blocks_out) *n_blocks_out = (int)(grid.x * grid.y);
   if (full) {
     if (k <= 16)
-      hipLaunchKernelGGL(residual_stream_kernel, grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
+      hipLaunchKernelGGL(residual_stream_kernel<16>, grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
     else if (k <= 32)
-      hipLaunchKernelGGL((residual_kernel<32, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
+      hipLaunchKernelGGL(residual_stream_kernel<32>, grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
     else
-      hipLaunchKernelGGL((residual_kernel<64, true>), grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
+      hipLaunchKernelGGL(residual_stream_kernel<64>, grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
   } else if (k <= 16)
     hipLaunchKernelGGL((residual_kernel<16, false>), grid, dim3(256), 0, st, L, R, M, N, k, X, nts, acc);
   else if (k <= 32)
@@ -736,17 +738,19 @@ __global__ __launch_bounds__(256) void residual_kernel(const float* __restrict__
   }
 }
 
-// The residual pass for k <= 16 with M a multiple of 128 and N of 32 nts: the same
-// block, wave and lane layout as residual_kernel, no bounds checks, and the next
-// subtile's X is loaded while the current one is multiplied. X is read once, with
-// streaming (nontemporal) loads; each half-wave load covers whole 128-byte lines. The subtile after the last
-// is the last again (loaded, never used), so every path issues the same loads and each
-// wait covers only the subtile being multiplied; a scheduling barrier keeps the
-// compiler from sinking the loads below the products.
-__global__ __launch_bounds__(256) void residual_stream_kernel(const float* __restrict__ L, const float* __restrict__ R,
-                                                              long M, long N, int k, const float* __restrict__ X,
-                                                              int nts, double* __restrict__ acc_out) {
-  constexpr int KP = 16;
+// The residual pass for k <= KP (16, 32 or 64) with M a multiple of 128 and N of 32 nts:
+// the same block, wave and lane layout as residual_kernel, no bounds checks, and the
+// next subtile's X and R rows are loaded while the current one is multiplied. X is read
+// once, with streaming (nontemporal) loads; each half-wave load covers whole 128-byte
+// lines. The subtile after the last is the last again (loaded, never used), so every
+// path issues the same loads and each wait covers only the subtile being multiplied; a
+// scheduling barrier keeps the compiler from sinking the loads below the products.
+// Wider KP holds more operands in flight: two waves per SIMD leave it 256 registers.
+template <int KP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KP <= 16 ? 4 : 2, KP <= 16 ? 8 : 2))) void
+residual_stream_kernel(
+    const float* __restrict__ L, const float* __restrict__ R, long M, long N, int k, const float* __restrict__ X,
+    int nts, double* __restrict__ acc_out) {
   __shared__ double red[8];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 31, kh = lane >> 5;
@@ -767,19 +771,19 @@ __global__ __launch_bounds__(256) void residual_stream_kernel(const float* __res
     for (int e = 0; e < 16; ++e)
       xv[e] = __builtin_nontemporal_load(base + ((unsigned)li + mu * (unsigned)acc32_row(e, lane)));
   };
-  double se = 0.0, sx = 0.0;
-  auto tile = [&](const float* xv, int t) {
+  // R rows of subtile t as the MFMA's A operand (columns past k are copies of column
+  // k - 1, zeroed at their use)
+  auto load_r = [&](float* rv, int t) {
     const long n = nbase + 32 * t + li;
-    float ar[KP / 2];
 #pragma unroll
-    for (int j = 0; j < KP / 2; ++j) {
-      const int kk = 2 * j + kh;
-      const float v = R[n + N * (long)min(kk, k - 1)];
-      ar[j] = kk < k ? v : 0.f;
-    }
+    for (int j = 0; j < KP / 2; ++j) rv[j] = R[n + N * (long)min(2 * j + kh, k - 1)];
+  };
+  double se = 0.0, sx = 0.0;
+  auto tile = [&](const float* xv, const float* rv) {
     f32x16 acc = f32x16{};
 #pragma unroll
-    for (int j = 0; j < KP / 2; ++j) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[j], bl[j], acc, 0, 0, 0);
+    for (int j = 0; j < KP / 2; ++j)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(2 * j + kh < k ? rv[j] : 0.f, bl[j], acc, 0, 0, 0);
     float e2 = 0.f, x2 = 0.f;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
@@ -790,15 +794,18 @@ __global__ __launch_bounds__(256) void residual_stream_kernel(const float* __res
     se += (double)e2;
     sx += (double)x2;
   };
-  float xa[16], xb[16];
+  float xa[16], xb[16], ra[KP / 2], rb[KP / 2];
   load_x(xa, 0);
+  load_r(ra, 0);
   for (int t = 0; t < nts; t += 2) {
     load_x(xb, min(t + 1, nts - 1));
+    load_r(rb, min(t + 1, nts - 1));
     __builtin_amdgcn_sched_barrier(0);
-    tile(xa, t);
+    tile(xa, ra);
     load_x(xa, min(t + 2, nts - 1));
+    load_r(ra, min(t + 2, nts - 1));
     __builtin_amdgcn_sched_barrier(0);
-    if (t + 1 < nts) tile(xb, t + 1);
+    if (t + 1 < nts) tile(xb, rb);
   }
   se = wave_sum(se);
   sx = wave_sum(sx);
