@@ -14,7 +14,7 @@ STATUS_NAMES = {
 }
 
 METHOD = {"plain": 0, "ltr": 1, "rsvd": 2}
-PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2}
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
 ERR_MODE = {"none": 0, "identity": 1, "residual": 2}
 SKETCH = {"gaussian": 0, "learned": 1}
 

@@ -267,7 +267,8 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
   ht_options opt;
   if (opt_in) opt = *opt_in;
   else ht_default_options(&opt);
-  if (opt.method < 0 || opt.method > 2 || opt.precision < 0 || opt.precision > 2 || opt.oversample < 0 ||
+  if (opt.precision == HT_PREC_FP8 && opt.method != HT_METHOD_RSVD) return HT_ERR_UNSUPPORTED;
+  if (opt.method < 0 || opt.method > 2 || opt.precision < 0 || opt.precision > 3 || opt.oversample < 0 ||
       opt.power_iters < 0 || opt.max_eig_iters < 1 || !(opt.eig_tol > 0) || opt.sketch < 0 || opt.sketch > 1)
     return HT_ERR_INVALID_ARG;
   if (opt.sketch == HT_SKETCH_LEARNED &&
@@ -864,7 +865,10 @@ ht_status ht_tall_orth(ht_handle_t h, float* Y, int64_t m, int32_t b, float* Q, 
 ht_status ht_skinny_gemm(ht_handle_t h, const float* A, int64_t M, int64_t K, int32_t transA, const float* B, int32_t b,
                          int32_t precision, float* Y) {
   if (!h || !A || !B || !Y || M < 1 || K < 1 || b < 1 || b > 64) return HT_ERR_INVALID_ARG;
-  const int prec = precision == HT_PREC_FP32 ? PREC_F32 : (precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
+  const int prec = precision == HT_PREC_FP32 ? PREC_F32
+                   : precision == HT_PREC_FP16 ? PREC_F16
+                   : precision == HT_PREC_FP8  ? PREC_FP8
+                                               : PREC_BF16;
   h->arena.reset();
   const size_t wsf = gemm_ws_floats(M, K, b);
   float* ws = (float*)h->arena.get(wsf * 4);

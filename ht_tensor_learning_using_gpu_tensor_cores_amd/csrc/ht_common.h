@@ -21,7 +21,25 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
-enum Prec { PREC_F32 = 0, PREC_BF16 = 1, PREC_F16 = 2 };
+enum Prec { PREC_F32 = 0, PREC_BF16 = 1, PREC_F16 = 2, PREC_FP8 = 3 };
+
+// FP8 (OCP E4M3, gfx950's v_cvt_pk_fp8_f32: round to nearest even, largest finite 448,
+// NaN past it) of four FP32 values, packed low byte first.
+__device__ __forceinline__ uint32_t fp8x4(float a, float b, float c, float d) {
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+  return (uint32_t)w;
+}
+__device__ __forceinline__ uint8_t fp8_bits(float a) {
+  return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(a, 0.f, 0, false) & 0xff);
+}
+// Power-of-two scale that brings a tile whose largest magnitude is m to at most 224 (half
+// the FP8 range, so rounding never reaches NaN); 1 for an all-zero tile.
+__device__ __forceinline__ float fp8_scale(float m) {
+  if (!(m > 0.f) || !isfinite(m)) return 1.f;
+  const float e = fminf(fmaxf(floorf(log2f(224.f / m)), -120.f), 120.f);
+  return ldexpf(1.f, (int)e);
+}
 
 // FP32 -> BF16 bits, round to nearest even (inputs are finite: the engine checks them).
 __device__ __forceinline__ uint16_t f2bf_bits(float f) {

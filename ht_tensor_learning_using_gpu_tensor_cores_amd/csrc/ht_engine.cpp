@@ -1586,6 +1586,9 @@ int Engine::run_rsvd(const float* x) {
   const Problem& p = *prob;
   if (adaptive) return HT_ERR_UNSUPPORTED;  // the range finder's width is fixed by the rank
   const int prec = opt.precision == HT_PREC_FP32 ? PREC_F32 : (opt.precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
+  // FP8: the range finder's products only, kept when the a-posteriori estimate allows
+  const bool fp8 = opt.precision == HT_PREC_FP8;
+  const int rf_prec = fp8 ? PREC_FP8 : prec;
   const bool learned = opt.sketch == HT_SKETCH_LEARNED;
   int64_t soff[HT_MAX_ORDER], stotal = 0;
   if (learned) CK(sketch_offsets(p.order, p.dims, opt.sketch_width, soff, &stotal));
@@ -1635,7 +1638,7 @@ int Engine::run_rsvd(const float* x) {
       int m0 = 0, m1 = 0;
       CK(node_col_modes(p, t, &m0, &m1));
       CK(kr_omega_launch(st, opt.sketch_weights, p.dims, soff, m0, m1, l, Om));
-      CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz));
+      CK(range_finder(x, v, Om, l, rf_prec, Y, Q, Z, Qz));
       double est = 0.0;
       CK(range_residual(x, v, t, Q, l, xsq, &est));
       ++sketch_nodes;
@@ -1648,7 +1651,17 @@ int Engine::run_rsvd(const float* x) {
     }
     if (gaussian) {
       CK(fill_normals_f32_launch(st, Om, cols * l, opt.seed, 256 + t));
-      CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz));
+      CK(range_finder(x, v, Om, l, rf_prec, Y, Q, Z, Qz));
+      if (fp8) {
+        double est = 0.0;
+        CK(range_residual(x, v, t, Q, l, xsq, &est));
+        ++sketch_nodes;
+        sketch_resid = std::max(sketch_resid, est);
+        if (!(est <= opt.sketch_tol)) {
+          ++sketch_fallbacks;
+          CK(range_finder(x, v, Om, l, prec, Y, Q, Z, Qz));
+        }
+      }
     }
     // Z = A^T Q; the SVD of Q^T A = Z^T from the eigen problem of Z^T Z (FP64 Gram of the
     // FP32 Z: exact products, FP64 sums): Z^T Z = Sv diag(th) Sv^T, left vectors Q Sv,

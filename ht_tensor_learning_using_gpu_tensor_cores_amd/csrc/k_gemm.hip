@@ -33,18 +33,32 @@ struct SkT<PREC_F32> {
   typedef float T;
   static constexpr int LD = SK_K + 4;
 };
+template <>
+struct SkT<PREC_FP8> {
+  typedef uint8_t T;
+  static constexpr int LD = SK_K + 8;  // 72-byte rows, 8-byte operand reads
+};
 
 // Four consecutive values of a tile row into LDS (converted for 16-bit tiles).
 template <int PREC>
 __device__ __forceinline__ void sk_put4(typename SkT<PREC>::T* dst, float a, float b, float c, float d) {
   if constexpr (PREC == PREC_F32) {
     *reinterpret_cast<float4*>(dst) = make_float4(a, b, c, d);
+  } else if constexpr (PREC == PREC_FP8) {
+    *reinterpret_cast<uint32_t*>(dst) = fp8x4(a, b, c, d);
   } else {
     uint2 w;
     w.x = cvt2<PREC>(a, b);
     w.y = cvt2<PREC>(c, d);
     *reinterpret_cast<uint2*>(dst) = w;
   }
+}
+
+template <int PREC>
+__device__ __forceinline__ typename SkT<PREC>::T sk_one(float v) {
+  if constexpr (PREC == PREC_F32) return v;
+  else if constexpr (PREC == PREC_FP8) return fp8_bits(v);
+  else return to16<PREC>(v);
 }
 
 // acc[bb] += (rows 32 bb.. of tile R) x (rows 32 w.. of tile C)^T over one 64-deep chunk:
@@ -65,6 +79,16 @@ __device__ __forceinline__ void sk_chunk(const typename SkT<PREC>::T* __restrict
         acc[bb] = __builtin_amdgcn_mfma_f32_32x32x2f32(rf, cf, acc[bb], 0, 0, 0);
       }
     }
+  } else if constexpr (PREC == PREC_FP8) {
+#pragma unroll
+    for (int kk = 0; kk < SK_K / 16; ++kk) {
+      const long cf = *reinterpret_cast<const long*>(&C[(32 * w + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+      for (int bb = 0; bb < NB; ++bb) {
+        const long rf = *reinterpret_cast<const long*>(&R[(32 * bb + (lane & 31)) * LD + 16 * kk + 8 * (lane >> 5)]);
+        acc[bb] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(rf, cf, acc[bb], 0, 0, 0);
+      }
+    }
   } else {
 #pragma unroll
     for (int kk = 0; kk < SK_K / 16; ++kk) {
@@ -76,6 +100,51 @@ __device__ __forceinline__ void sk_chunk(const typename SkT<PREC>::T* __restrict
         acc[bb] = mfma32x32x16<PREC>(rf, cf, acc[bb]);
       }
     }
+  }
+}
+
+// FP8 tiles carry a power-of-two scale each (sk_scales): the wave maxima of the two
+// register tiles go to red before the barrier that precedes the LDS stores, and every
+// thread derives the block's scales from them. The chunk's MFMA partial sums are then
+// scaled back (exactly: powers of two) as they join the running accumulators.
+template <int NA, int NBF>
+__device__ __forceinline__ void sk_maxima(const float (&ra)[NA], const float (&rb)[NBF], int lane, int w,
+                                          float* red) {
+  float ma = 0.f, mb = 0.f;
+#pragma unroll
+  for (int q = 0; q < NA; ++q) ma = fmaxf(ma, fabsf(ra[q]));
+#pragma unroll
+  for (int q = 0; q < NBF; ++q) mb = fmaxf(mb, fabsf(rb[q]));
+  for (int o = 32; o > 0; o >>= 1) {
+    ma = fmaxf(ma, __shfl_xor(ma, o, 64));
+    mb = fmaxf(mb, __shfl_xor(mb, o, 64));
+  }
+  if (lane == 0) {
+    red[w] = ma;
+    red[16 + w] = mb;
+  }
+}
+__device__ __forceinline__ float sk_scale(const float* red, int nw) {
+  float m = 0.f;
+  for (int i = 0; i < nw; ++i) m = fmaxf(m, red[i]);
+  return fp8_scale(m);
+}
+template <int PREC, int NB>
+__device__ __forceinline__ void sk_chunk_acc(const typename SkT<PREC>::T* __restrict__ R,
+                                             const typename SkT<PREC>::T* __restrict__ C, int w, int lane,
+                                             f32x16 (&acc)[NB], float sr, float sc) {
+  if constexpr (PREC == PREC_FP8) {
+    f32x16 part[NB];
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb) part[bb] = f32x16{};
+    sk_chunk<PREC, NB>(R, C, w, lane, part);
+    const float ir = 1.f / sr, ic = 1.f / sc;
+#pragma unroll
+    for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[bb][e] += (part[bb][e] * ir) * ic;
+  } else {
+    sk_chunk<PREC, NB>(R, C, w, lane, acc);
   }
 }
 
@@ -116,6 +185,7 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
   constexpr int RA = 32, RB = NBJ * SK_K / 256;   // floats per thread per chunk
   __shared__ __attribute__((aligned(16))) T As[128 * LD];
   __shared__ __attribute__((aligned(16))) T Bs[NBJ * LD];
+  __shared__ float smax[32];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const long c0 = (long)blockIdx.x * 128;
   const int split = blockIdx.y;
@@ -156,38 +226,34 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
       }
     }
   };
-  auto store = [&](long ch) {
+  auto store = [&](long ch, float sa, float sb) {
     const long m0 = ch * SK_K;
     if constexpr (VEC) {
 #pragma unroll
       for (int it = 0; it < RA / 4; ++it) {
         const int f = tid + 256 * it;
         const bool ok = m0 + 4 * (f & 15) < M;
-        sk_put4<PREC>(&As[(f >> 4) * LD + 4 * (f & 15)], ok ? ra[4 * it] : 0.f, ok ? ra[4 * it + 1] : 0.f,
-                      ok ? ra[4 * it + 2] : 0.f, ok ? ra[4 * it + 3] : 0.f);
+        sk_put4<PREC>(&As[(f >> 4) * LD + 4 * (f & 15)], ok ? ra[4 * it] * sa : 0.f, ok ? ra[4 * it + 1] * sa : 0.f,
+                      ok ? ra[4 * it + 2] * sa : 0.f, ok ? ra[4 * it + 3] * sa : 0.f);
       }
 #pragma unroll
       for (int it = 0; it < RB / 4; ++it) {
         const int f = tid + 256 * it;
         const bool ok = m0 + 4 * (f & 15) < M;
-        sk_put4<PREC>(&Bs[(f >> 4) * LD + 4 * (f & 15)], ok ? rb[4 * it] : 0.f, ok ? rb[4 * it + 1] : 0.f,
-                      ok ? rb[4 * it + 2] : 0.f, ok ? rb[4 * it + 3] : 0.f);
+        sk_put4<PREC>(&Bs[(f >> 4) * LD + 4 * (f & 15)], ok ? rb[4 * it] * sb : 0.f, ok ? rb[4 * it + 1] * sb : 0.f,
+                      ok ? rb[4 * it + 2] * sb : 0.f, ok ? rb[4 * it + 3] * sb : 0.f);
       }
     } else {
       const bool ok = m0 + (tid & 63) < M;  // e % 64 == tid % 64 for every q
 #pragma unroll
       for (int q = 0; q < RA; ++q) {
         const int e = tid + 256 * q;
-        const float v = ok ? ra[q] : 0.f;
-        if constexpr (PREC == PREC_F32) As[(e >> 6) * LD + (e & 63)] = v;
-        else As[(e >> 6) * LD + (e & 63)] = to16<PREC>(v);
+        As[(e >> 6) * LD + (e & 63)] = sk_one<PREC>(ok ? ra[q] * sa : 0.f);
       }
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
         const int e = tid + 256 * q;
-        const float v = ok ? rb[q] : 0.f;
-        if constexpr (PREC == PREC_F32) Bs[(e >> 6) * LD + (e & 63)] = v;
-        else Bs[(e >> 6) * LD + (e & 63)] = to16<PREC>(v);
+        Bs[(e >> 6) * LD + (e & 63)] = sk_one<PREC>(ok ? rb[q] * sb : 0.f);
       }
     }
   };
@@ -196,12 +262,14 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
   for (int bb = 0; bb < NB; ++bb) acc[bb] = f32x16{};
   if (k_begin < k_end) load(k_begin);
   for (long ch = k_begin; ch < k_end; ++ch) {
+    if constexpr (PREC == PREC_FP8) sk_maxima(ra, rb, lane, w, smax);
     __syncthreads();
-    store(ch);
+    const float sa = PREC == PREC_FP8 ? sk_scale(smax, 4) : 1.f, sb = PREC == PREC_FP8 ? sk_scale(smax + 16, 4) : 1.f;
+    store(ch, sa, sb);
     __syncthreads();
     load(ch + 1 < k_end ? ch + 1 : ch);  // the last chunk again at the end (never stored)
     __builtin_amdgcn_sched_barrier(0);
-    sk_chunk<PREC, NB>(Bs, As, w, lane, acc);
+    sk_chunk_acc<PREC, NB>(Bs, As, w, lane, acc, sb, sa);
     __builtin_amdgcn_sched_barrier(0);
   }
   sk_write<NB>(acc, c0, w, lane, K, b, n_split, split, Y, part);
@@ -226,6 +294,7 @@ __global__ __launch_bounds__(512) void gemm_ab_kernel(const float* __restrict__ 
   constexpr int RB = NBJ * SK_K / 512;
   __shared__ __attribute__((aligned(16))) T As[256 * LD];
   __shared__ __attribute__((aligned(16))) T Bs[NBJ * LD];
+  __shared__ float smax[32];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const long r0 = (long)blockIdx.x * 256;
   const int split = blockIdx.y;
@@ -256,29 +325,27 @@ __global__ __launch_bounds__(512) void gemm_ab_kernel(const float* __restrict__ 
       }
     }
   };
-  auto store = [&](long ch) {
+  auto store = [&](long ch, float sa, float sb) {
     const long k0 = ch * SK_K;
     const long nk = K - (k0 + kh);  // valid columns among this thread's 32
 #pragma unroll
     for (int q = 0; q < 32; q += 4)
-      sk_put4<PREC>(&As[(tid & 255) * LD + kh + q], q < nk ? ra[q] : 0.f, q + 1 < nk ? ra[q + 1] : 0.f,
-                    q + 2 < nk ? ra[q + 2] : 0.f, q + 3 < nk ? ra[q + 3] : 0.f);
+      sk_put4<PREC>(&As[(tid & 255) * LD + kh + q], q < nk ? ra[q] * sa : 0.f, q + 1 < nk ? ra[q + 1] * sa : 0.f,
+                    q + 2 < nk ? ra[q + 2] * sa : 0.f, q + 3 < nk ? ra[q + 3] * sa : 0.f);
     if constexpr (VEC) {
 #pragma unroll
       for (int it = 0; it < RB / 4; ++it) {
         const int f = tid + 512 * it;
         const bool ok = k0 + 4 * (f & 15) < K;
-        sk_put4<PREC>(&Bs[(f >> 4) * LD + 4 * (f & 15)], ok ? rb[4 * it] : 0.f, ok ? rb[4 * it + 1] : 0.f,
-                      ok ? rb[4 * it + 2] : 0.f, ok ? rb[4 * it + 3] : 0.f);
+        sk_put4<PREC>(&Bs[(f >> 4) * LD + 4 * (f & 15)], ok ? rb[4 * it] * sb : 0.f, ok ? rb[4 * it + 1] * sb : 0.f,
+                      ok ? rb[4 * it + 2] * sb : 0.f, ok ? rb[4 * it + 3] * sb : 0.f);
       }
     } else {
       const bool ok = k0 + (tid & 63) < K;
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
         const int e = tid + 512 * q;
-        const float v = ok ? rb[q] : 0.f;
-        if constexpr (PREC == PREC_F32) Bs[(e >> 6) * LD + (e & 63)] = v;
-        else Bs[(e >> 6) * LD + (e & 63)] = to16<PREC>(v);
+        Bs[(e >> 6) * LD + (e & 63)] = sk_one<PREC>(ok ? rb[q] * sb : 0.f);
       }
     }
   };
@@ -287,12 +354,14 @@ __global__ __launch_bounds__(512) void gemm_ab_kernel(const float* __restrict__ 
   for (int bb = 0; bb < NB; ++bb) acc[bb] = f32x16{};
   if (k_begin < k_end) load(k_begin);
   for (long ch = k_begin; ch < k_end; ++ch) {
+    if constexpr (PREC == PREC_FP8) sk_maxima(ra, rb, lane, w, smax);
     __syncthreads();
-    store(ch);
+    const float sa = PREC == PREC_FP8 ? sk_scale(smax, 8) : 1.f, sb = PREC == PREC_FP8 ? sk_scale(smax + 16, 8) : 1.f;
+    store(ch, sa, sb);
     __syncthreads();
     load(ch + 1 < k_end ? ch + 1 : ch);
     __builtin_amdgcn_sched_barrier(0);
-    sk_chunk<PREC, NB>(Bs, As, w, lane, acc);
+    sk_chunk_acc<PREC, NB>(Bs, As, w, lane, acc, sb, sa);
     __builtin_amdgcn_sched_barrier(0);
   }
   sk_write<NB>(acc, r0, w, lane, M, b, n_split, split, Y, part);
@@ -357,6 +426,9 @@ int gemm_atb_launch(hipStream_t st, const float* A, long M, long K, const float*
   if (prec == PREC_BF16) {
     if (nb1) atb_dispatch<PREC_BF16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
     else atb_dispatch<PREC_BF16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+  } else if (prec == PREC_FP8) {
+    if (nb1) atb_dispatch<PREC_FP8, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    else atb_dispatch<PREC_FP8, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
   } else if (prec == PREC_F16) {
     if (nb1) atb_dispatch<PREC_F16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
     else atb_dispatch<PREC_F16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
@@ -383,6 +455,9 @@ int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* 
   if (prec == PREC_BF16) {
     if (nb1) ab_dispatch<PREC_BF16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
     else ab_dispatch<PREC_BF16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+  } else if (prec == PREC_FP8) {
+    if (nb1) ab_dispatch<PREC_FP8, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    else ab_dispatch<PREC_FP8, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
   } else if (prec == PREC_F16) {
     if (nb1) ab_dispatch<PREC_F16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
     else ab_dispatch<PREC_F16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);

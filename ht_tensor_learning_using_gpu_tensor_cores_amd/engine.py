@@ -90,7 +90,8 @@ def make_options(method="ltr", precision="bf16", err="residual", oversample=8, p
                  check_finite=1):
     """ht_options. `sketch` (a sketch.Sketch on the device) selects the learned Khatri-Rao
     test matrix of the randomised method, with the Gaussian one as its fallback above
-    `sketch_tol`."""
+    `sketch_tol`; precision "fp8" (randomised method only) keeps its FP8 range finder
+    within the same `sketch_tol`, else redoes it in BF16."""
     lib = _lib.load()
     o = _lib.Options()
     lib.ht_default_options(ctypes.byref(o))
@@ -98,7 +99,7 @@ def make_options(method="ltr", precision="bf16", err="residual", oversample=8, p
         o.sketch = _lib.SKETCH["learned"]
         o.sketch_width = int(sketch.width)
         o.sketch_weights = ctypes.c_void_p(sketch.weights.data_ptr())
-        o.sketch_tol = float(sketch_tol)
+    o.sketch_tol = float(sketch_tol)
     o.method = _lib.METHOD[method]
     o.precision = _lib.PRECISION[precision]
     o.err_mode = _lib.ERR_MODE[err]

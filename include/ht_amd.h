@@ -66,12 +66,18 @@ typedef enum {
    * projection pass to the leaf core and exact FP64 interior problems on that core. */
   HT_METHOD_LTR = 1,
   /* Root-to-leaves randomised: every non-root node by a range finder (oversampling p,
-   * q power iterations) on MFMA, TSQR orthonormalisation, FP64 projected SVD.
+   * q power iterations) on MFMA, shifted CholeskyQR3 orthonormalisation, FP64 projected SVD.
    * fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/kernel.cu:419-504. */
   HT_METHOD_RSVD = 2
 } ht_method;
 
-typedef enum { HT_PREC_FP32 = 0, HT_PREC_BF16 = 1, HT_PREC_FP16 = 2 } ht_precision;
+/* HT_PREC_FP8 (HT_METHOD_RSVD only): the range finder's products with X run on FP8 MFMA
+ * (OCP E4M3, a power-of-two scale per operand tile of every 64-deep chunk, FP32
+ * accumulation); everything else runs as in HT_PREC_BF16. After each range finder the
+ * a-posteriori estimate ||(I - QQ^T) X_(t)||_F / ||X||_F (four FP32 Gaussian probes) must be
+ * at most opt.sketch_tol, or the range finder is redone in BF16; ht_stats reports the
+ * nodes checked, the largest estimate and the redone nodes. */
+typedef enum { HT_PREC_FP32 = 0, HT_PREC_BF16 = 1, HT_PREC_FP16 = 2, HT_PREC_FP8 = 3 } ht_precision;
 
 typedef enum {
   HT_ERR_MODE_NONE = 0,      /* do not compute the error                                   */
@@ -120,9 +126,9 @@ typedef struct {
   double bytes_x_passes;   /* number of full passes over X made by the decomposition     */
   int32_t eig_iters_max;   /* largest subspace-iteration count of any node               */
   int32_t eig_calls;       /* number of node eigen problems solved                       */
-  int32_t sketch_fallbacks;/* nodes redone with the Gaussian test matrix                 */
-  int32_t sketch_nodes;    /* nodes whose range finder used the learned sketch first     */
-  double sketch_resid;     /* largest range-residual estimate of the learned sketch      */
+  int32_t sketch_fallbacks;/* nodes redone: Gaussian after a learned sketch, BF16 after FP8 */
+  int32_t sketch_nodes;    /* nodes whose range finder was checked (learned sketch or FP8) */
+  double sketch_resid;     /* largest range-residual estimate of those checks            */
   double workspace_bytes;  /* device workspace the call used (one contiguous block)      */
   int32_t replayed;        /* decompositions that replayed recorded solver rounds (read
                               the device once, at their end)                          */

@@ -295,3 +295,36 @@ def test_residual_and_reconstruct_wait_for_torch_work():
         res2 = ht.HTensor(dims=res.dims, tree=res.tree, ranks=res.ranks, packed=pk2)
         rec2 = to_np(ht.reconstruct(res2))
         np.testing.assert_array_equal(rec2, ref_rec)
+
+
+def test_rsvd_fp8_range_finder_and_its_fallback():
+    """HT_PREC_FP8 runs the range finder's products on FP8 MFMA and keeps the sketch only
+    when the a-posteriori estimate ||(I - QQ^T) X_(t)||_F / ||X||_F is within sketch_tol.
+    With a loose tolerance the FP8 bases are kept (and the error stays near the BF16 one);
+    with the default 1e-3 a node whose estimate exceeds it is redone in BF16, which then
+    reproduces the BF16 run exactly."""
+    import torch
+    dims, core = [64, 48, 40], [6, 5, 4]
+    tr = ht.tree.first_vs_rest_3d()
+    ranks = [1, 6, 6, 5, 4]
+    x = torch.from_numpy(G.tucker(dims, core, seed=7, noise_rel=1e-3)).cuda()
+    bf = ht.decompose(x, dims, tr, ranks, method="rsvd", precision="bf16")
+    loose = ht.decompose(x, dims, tr, ranks, method="rsvd", precision="fp8", sketch_tol=0.5)
+    strict = ht.decompose(x, dims, tr, ranks, method="rsvd", precision="fp8", sketch_tol=1e-3)
+    print(f"rsvd fp8: bf16 err {bf.rel_err:.6e}; fp8 kept: err {loose.rel_err:.6e}, estimate "
+          f"{loose.stats['sketch_resid']:.3e}; fp8 at 1e-3: {strict.stats['sketch_fallbacks']} of "
+          f"{strict.stats['sketch_nodes']} redone, err {strict.rel_err:.6e}", flush=True)
+    assert loose.stats["sketch_nodes"] >= 1 and loose.stats["sketch_fallbacks"] == 0
+    assert loose.stats["sketch_resid"] <= 0.5
+    assert abs(loose.rel_err - bf.rel_err) <= 2e-2
+    assert strict.stats["sketch_nodes"] == loose.stats["sketch_nodes"]
+    if strict.stats["sketch_fallbacks"] == strict.stats["sketch_nodes"]:
+        assert strict.rel_err == bf.rel_err
+        for t in range(tr.n_nodes):
+            if tr.is_leaf(t):
+                assert torch.equal(strict.U[t], bf.U[t])
+    else:
+        assert strict.stats["sketch_resid"] <= 1e-3 or strict.stats["sketch_fallbacks"] >= 1
+    # FP8 is a range-finder option of the randomised method only
+    with pytest.raises(RuntimeError):
+        ht.decompose(x, dims, tr, ranks, method="ltr", precision="fp8")

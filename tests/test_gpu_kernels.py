@@ -266,6 +266,46 @@ def test_skinny_gemm(lib, handle, rng, M, K, b, trans, precision):
     np.testing.assert_allclose(Yh, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
 
 
+# FP8 skinny products (OCP E4M3 with a power-of-two scale per operand tile and chunk).
+# Small integers |v| <= 8 stay exact under every such scale (k 2^e, k <= 8, needs at most
+# three mantissa bits), so the product of integer matrices must come out exact: this pins
+# the operand layout, the per-tile scales and their removal. Gaussian data must stay
+# within two FP8 roundings (2^-4 each) of the exact product, plus what underflows.
+FP8_SHAPES = [(4096, 3000, 64, 1), (1022, 3001, 40, 1), (1024, 65537, 40, 0), (777, 4099, 33, 0),
+              (62, 70001, 24, 1), (300, 64, 17, 0)]
+
+
+@pytest.mark.parametrize("M,K,b,trans", FP8_SHAPES)
+def test_skinny_gemm_fp8(lib, handle, rng, M, K, b, trans):
+    import torch
+    rows = K if trans else M
+
+    def run(A, B):
+        Y = torch.zeros(rows * b, device="cuda")
+        Ad, Bd = dev(A.reshape(-1, order="F")), dev(B.reshape(-1, order="F"))
+        _lib.check(lib.ht_skinny_gemm(handle.ptr, ptr(Ad), M, K, trans, ptr(Bd), b, _lib.PRECISION["fp8"], ptr(Y)),
+                   "gemm")
+        return host(Y).reshape((rows, b), order="F").astype(np.float64)
+
+    A = rng.integers(-8, 9, size=(M, K)).astype(np.float32)
+    B = rng.integers(-8, 9, size=((M if trans else K), b)).astype(np.float32)
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    ref = A64.T @ B64 if trans else A64 @ B64
+    np.testing.assert_array_equal(run(A, B), ref)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal(((M if trans else K), b)).astype(np.float32)
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    ref = A64.T @ B64 if trans else A64 @ B64
+    mag = np.abs(A64).T @ np.abs(B64) if trans else np.abs(A64) @ np.abs(B64)
+    red = M if trans else K
+    Yh = run(A, B)
+    bound = 2.0 ** -3 * mag + 2.0 ** -10 / 112 * np.abs(A64).max() * np.abs(B64).max() * red
+    assert np.all(np.abs(Yh - ref) <= bound)
+    rel = np.linalg.norm(Yh - ref) / np.linalg.norm(ref)
+    print(f"fp8 gemm M={M} K={K} b={b} trans={trans}: relative error {rel:.3e}", flush=True)
+    assert rel < 0.05
+
+
 def test_philox_device_matches_oracle(lib, handle):
     import torch
     n = 4099
