@@ -94,6 +94,7 @@ SIGNATURES = {
     "htd4_tc": (None, [P, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long, PI32, P, P]),
     "ht_debug_mfma_probe": (I32, [H, P, P, P, P, P, P, P]),
     "ht_mode_gram": (I32, [H, P, I32, PI64, I32, I32, I32, P, PD]),
+    "ht_mode_gram16": (I32, [H, P, I32, PI64, I32, I32, I32, P, PD]),
     "ht_slab_gram": (I32, [H, P, I32, I32, I64, I32, P, P, PD]),
     "ht_slab_project": (I32, [H, P, I32, I32, I64, P, I32, P, I32, P]),
     "ht_sym_eig": (I32, [H, P, I32, P, P]),
@@ -101,6 +102,7 @@ SIGNATURES = {
     "ht_tsqr": (I32, [H, P, I64, I32, P, P]),
     "ht_tall_orth": (I32, [H, P, I64, I32, P, P]),
     "ht_skinny_gemm": (I32, [H, P, I64, I64, I32, P, I32, I32, P]),
+    "ht_skinny_gemm16": (I32, [H, P, I64, I64, I32, P, I32, I32, P]),
     "ht_philox_normals": (I32, [H, U64, U32, I64, P]),
     "ht_philox_normals_host": (None, [U64, U32, I64, PD]),
     "ht_profile_enable": (I32, [H, I32]),

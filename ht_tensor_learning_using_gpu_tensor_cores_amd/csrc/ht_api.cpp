@@ -786,6 +786,34 @@ ht_status ht_mode_gram(ht_handle_t h, const float* x, int32_t order, const int64
   return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
 }
 
+ht_status ht_mode_gram16(ht_handle_t h, const float* x, int32_t order, const int64_t* dims, int32_t lo,
+                         int32_t hi, int32_t precision, double* G, double* sumsq) {
+  if (!h || !x || !dims || !G || order < 1 || order > HT_MAX_ORDER || lo < 0 || hi > order || lo >= hi)
+    return HT_ERR_INVALID_ARG;
+  PMS v{1, 1, 1};
+  for (int m = 0; m < lo; ++m) v.P *= dims[m];
+  for (int m = lo; m < hi; ++m) v.M *= dims[m];
+  for (int m = hi; m < order; ++m) v.S *= dims[m];
+  if (precision != HT_PREC_BF16 && precision != HT_PREC_FP16) return HT_ERR_UNSUPPORTED;
+  if (!gram16_ok(v) || ((uintptr_t)x & 15)) return HT_ERR_UNSUPPORTED;
+  const int prec = precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16;
+  const long n = v.P * v.M * v.S;
+  h->arena.reset();
+  Engine e;
+  e.st = h->stream;
+  e.arena = &h->arena;
+  uint16_t* y = (uint16_t*)h->arena.get((size_t)n * sizeof(uint16_t));
+  double* part = (double*)h->arena.get((size_t)cast16_part_size() * sizeof(double));
+  double* s = (double*)h->arena.get(8);
+  if (!y || !part || !s) return HT_ERR_ALLOC;
+  CKS(cast16_sumsq_launch(h->stream, x, n, prec, y, part, s));
+  e.x16 = y;
+  e.x16_prec = prec;
+  CKS(e.gram(x, v, prec, G, nullptr));
+  if (sumsq) CKS(e.d2h(sumsq, s, 8));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
 ht_status ht_slab_gram(ht_handle_t h, const float* x, int32_t n1, int32_t n2, int64_t S, int32_t precision,
                        double* G1, double* G2, double* sumsq) {
   if (!h || !x || !G1 || !G2 || n1 < 1 || n2 < 1 || n1 > 128 || n2 > 128 || S < 1) return HT_ERR_INVALID_ARG;
@@ -875,6 +903,24 @@ ht_status ht_skinny_gemm(ht_handle_t h, const float* A, int64_t M, int64_t K, in
   if (!ws) return HT_ERR_ALLOC;
   if (transA) CKS(gemm_atb_launch(h->stream, A, M, K, B, b, prec, Y, ws, wsf));
   else CKS(gemm_ab_launch(h->stream, A, M, K, B, b, prec, Y, ws, wsf));
+  return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
+}
+
+ht_status ht_skinny_gemm16(ht_handle_t h, const float* A, int64_t M, int64_t K, int32_t transA, const float* B,
+                           int32_t b, int32_t precision, float* Y) {
+  if (!h || !A || !B || !Y || M < 1 || K < 1 || b < 1 || b > 64) return HT_ERR_INVALID_ARG;
+  if (precision != HT_PREC_BF16 && precision != HT_PREC_FP16) return HT_ERR_UNSUPPORTED;
+  const int prec = precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16;
+  if (((uintptr_t)A & 15) || ((uintptr_t)B & 15) || M % (transA ? 8 : 4)) return HT_ERR_UNSUPPORTED;
+  h->arena.reset();
+  const size_t wsf = gemm_ws_floats(M, K, b);
+  float* ws = (float*)h->arena.get(wsf * 4);
+  uint16_t* A16 = (uint16_t*)h->arena.get((size_t)M * K * sizeof(uint16_t));
+  double* part = (double*)h->arena.get((size_t)cast16_part_size() * sizeof(double) + 8);
+  if (!ws || !A16 || !part) return HT_ERR_ALLOC;
+  CKS(cast16_sumsq_launch(h->stream, A, M * K, prec, A16, part, part + cast16_part_size()));
+  if (transA) CKS(gemm_atb16_launch(h->stream, A16, M, K, B, b, prec, Y, ws, wsf));
+  else CKS(gemm_ab16_launch(h->stream, A16, M, K, B, b, prec, Y, ws, wsf));
   return hipStreamSynchronize(h->stream) == hipSuccess ? HT_OK : HT_ERR_HIP;
 }
 

@@ -170,6 +170,18 @@ int gram_reduce2_launch(hipStream_t st, const float* p1, int M1, int n1, double*
                         double* G2, const double* norms, int n_norms, double* norm_out, int tile1 = 128,
                         int tile2 = 128);
 int gram_reduce_norms_launch(hipStream_t st, const double* norms, int n, double* out);
+// 16-bit copy y of the n floats of x (16-byte aligned; prec BF16 or FP16, round to nearest
+// even) and *sumsq = ||x||^2 of the FP32 values; part holds cast16_part_size() doubles.
+int cast16_part_size();
+int cast16_sumsq_launch(hipStream_t st, const float* x, long n, int prec, uint16_t* y, double* part,
+                        double* sumsq);
+// Gram partials from such a copy (16-byte aligned), in the layout of gram_pms_launch's
+// 256-row tiles (part: [gram16_pairs][n_split][256*256] FP32), for views where gram16_ok
+// holds (257 to 1024 rows; P == 1 with M % 4 == 0, or P % 64 == 0).
+bool gram16_ok(PMS v);
+int gram16_pairs(PMS v);
+int gram16_nsplit(PMS v);
+int gram16_launch(hipStream_t st, const uint16_t* x, PMS v, int prec, int n_split, float* part);
 
 // Fused slab kernels on the two leading modes (x as n1 x n2 x S, n1,n2 <= 128).
 // GRAM: part1/part2 [n_blocks][128*128] partials of X_(1)X_(1)^T and X_(2)X_(2)^T.
@@ -199,6 +211,14 @@ int gemm_atb_launch(hipStream_t st, const float* A, long M, long K, const float*
 int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec,
                    float* Y, float* ws, size_t ws_floats);
 size_t gemm_ws_floats(long M, long K, int b);
+// The same products with A a 16-bit copy in the precision's format (prec BF16/F16; the
+// values gemm_*_launch would round A to). gemm16_ok: gemm_atb16 (op_atb = 1) takes
+// M % 8 == 0 with A and B 16-byte aligned, gemm_ab16 M % 4 == 0 with A 8-byte aligned.
+bool gemm16_ok(const uint16_t* A, long M, const float* B, int op_atb);
+int gemm_atb16_launch(hipStream_t st, const uint16_t* A, long M, long K, const float* B, int b, int prec,
+                      float* Y, float* ws, size_t ws_floats);
+int gemm_ab16_launch(hipStream_t st, const uint16_t* A, long M, long K, const float* B, int b, int prec,
+                     float* Y, float* ws, size_t ws_floats);
 
 // Low-rank tiles C = L R^T (L: M x k, R: N x k, both column-major, k <= 256).
 //   mode 0: store C into out (ld M);  mode 1: accumulate sum (X - C)^2 and sum X^2 (FP64)

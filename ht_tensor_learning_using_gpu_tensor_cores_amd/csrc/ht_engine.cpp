@@ -263,8 +263,21 @@ template double* Engine::alloc<double>(size_t);
 template int* Engine::alloc<int>(size_t);
 
 // Gram of an (P,M,S) view of FP32 data, reduced to FP64 (M x M). Returns ||x||^2 of the
-// data in *sumsq (device scalar) when sumsq != nullptr.
+// data in *sumsq (device scalar) when sumsq != nullptr. A 16-bit product over a view the
+// run's 16-bit copy of X serves reads the copy (the same rounded values; half the bytes).
 int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
+  if (x16 && prec == x16_prec && !sumsq && gram16_ok(v)) {
+    const int ns = gram16_nsplit(v);
+    float* part = alloc<float>((size_t)gram16_pairs(v) * ns * 256 * 256);
+    if (!part) return HT_ERR_ALLOC;
+    {
+      ProfScope ps(prof, st, PK_GRAM);
+      CK(gram16_launch(st, x16, v, prec, ns, part));
+    }
+    CK(gram_reduce_launch(st, part, (int)v.M, ns, G, 256));
+    passes += 0.5;
+    return 0;
+  }
   const int ns = gram_pms_nsplit(x, v, prec);
   const int tile = gram_pms_tile(x, v, prec);
   const int pairs = gram_pms_pairs(x, v, prec);
@@ -277,6 +290,45 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
   }
   CK(gram_reduce_launch(st, part, (int)v.M, ns, G, tile));
   if (sumsq) CK(gram_reduce_norms_launch(st, norms, pairs * ns, sumsq));
+  passes += 1.0;
+  return 0;
+}
+
+// 16-bit copy of X for the run's 16-bit passes over X: the Grams of its matricisations of
+// 257 to 1024 rows (the leaves and the interior nodes of the randomised method) and the
+// range finders' skinny products, made for tensors of 2^22 elements and more when two or
+// more passes would read it and the arena may grow (not in a fixed workspace): one pass
+// over X, which also gives ||X||^2 (*xsq), after which each of those passes reads half
+// the bytes. x16 stays nullptr when no copy is made (the passes then read X and the first
+// Gram gives ||X||^2). The passes' results are the same either way.
+int Engine::make_x16(const float* x, int prec, double* xsq) {
+  x16 = nullptr;
+  const Problem& p = *prob;
+  // small tensors: the passes are launch-bound and the copy gains nothing
+  if (prec == PREC_F32 || prec == PREC_FP8 || arena->fixed || ((uintptr_t)x & 15) || p.N < (1L << 22)) return 0;
+  int uses = 0;
+  for (int t = 1; t < p.n_nodes; ++t) {
+    const bool leaf = p.left[t] < 0, inner = !leaf && p.lo[t] > 0 && p.hi[t] < p.order;
+    const PMS v = view_of(p.dims, p.order, p.lo[t], p.hi[t]);
+    if ((leaf || inner) && gram16_ok(v)) ++uses;
+    // a range finder's 2 + 2q skinny products (root-side nodes X_(t) with P == 1 or S == 1)
+    if (!leaf && !inner && (v.P == 1 ? v.M % 8 == 0 : v.S == 1 && v.P % 8 == 0)) uses += 2 + 2 * opt.power_iters;
+  }
+  if (uses < 2) return 0;
+  const Arena::Mark mk = arena->mark();
+  uint16_t* y = (uint16_t*)arena->get((size_t)p.N * sizeof(uint16_t));
+  double* part = y ? alloc<double>(cast16_part_size()) : nullptr;
+  if (!y || !part) {  // no room for the copy: the Grams read X
+    arena->rewind(mk);
+    (void)hipGetLastError();
+    return 0;
+  }
+  {
+    ProfScope ps(prof, st, PK_GRAM);
+    CK(cast16_sumsq_launch(st, x, p.N, prec, y, part, xsq));
+  }
+  x16 = y;
+  x16_prec = prec;
   passes += 1.0;
   return 0;
 }
@@ -545,7 +597,8 @@ static int large_block(int n, int k) { return std::min(std::min(64, std::max(2 *
 
 // Subspace iteration for n > 256 (multi-kernel, host-checked convergence). The starting
 // block is Philox stream `stream` (n x large_block(n, k) normals); Q0, when given, holds
-// it already (and is overwritten).
+// it already (and is overwritten). Each step multiplies by G and orthonormalises by
+// shifted CholeskyQR (zero columns refreshed from Philox stream `stream` + 2 + step).
 int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
                            double floor, double* Q0) {
   ProfScope ps(prof, st, PK_EIG);
@@ -563,7 +616,8 @@ int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, 
   double* ws = alloc<double>((size_t)dgemm_splitk_slices(b, b, n) * b * b);
   if (!Q || !Y || !T || !S || !th || !W || !res || !Zt || !Ri || !ws) return HT_ERR_ALLOC;
   if (!Q0) CK(fill_normals_f64_launch(st, Q, (long)n * b, opt.seed, stream));
-  CK(cholqr3(st, Q, Zt, n, b, W, Ri, info, opt.seed, stream + 1, ws));
+  // the Gaussian block itself starts the iteration (n x b normals are well conditioned:
+  // kappa ~ (sqrt n + sqrt b) / (sqrt n - sqrt b)); G Q is orthonormalised by the steps
   int it = 0, steps = 2;
   std::vector<double> hres(k), hth(b);
   bool done = false;
@@ -641,7 +695,9 @@ int Engine::tsqr(float* Y, long m, int b, float* Q, double* R) {
 // column the shift did not lift), the basis comes from TSQR instead (one host read of the
 // pivot flag). The randomised method orthonormalises every sketch this way, the
 // 1024-row co-range sketches of the 1024^3 workload as well as the million-row ones.
-int Engine::tall_orth(float* Y, long m, int b, float* Q) {
+// passes = 1: the shifted pass alone, a well-conditioned basis of the span (orthonormal
+// to about u_fp32 kappa(Y)^2) for an intermediate block of a subspace iteration.
+int Engine::tall_orth(float* Y, long m, int b, float* Q, int passes) {
   ProfScope ps(prof, st, PK_TSQR);
   double* W = alloc<double>((size_t)b * b);
   double* Ri = alloc<double>((size_t)b * b);
@@ -649,15 +705,17 @@ int Engine::tall_orth(float* Y, long m, int b, float* Q) {
   float* T = alloc<float>((size_t)m * b);
   int* info = alloc<int>(1);
   if (!W || !Ri || !part || !T || !info) return HT_ERR_ALLOC;
-  HT_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), st));
+  if (orth_defer) info = orth_flag;  // cleared once per run; read at its end
+  else HT_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), st));
   const float* src = Y;
-  for (int pass = 0; pass < 3; ++pass) {
+  for (int pass = 0; pass < passes; ++pass) {
     float* dst = (pass == 1) ? T : Q;  // Y -> Q -> T -> Q
     CK(tall_gram_launch(st, src, b, src, b, m, part, W));
     CK(chol_rinv_launch(st, W, b, m, pass == 0, Ri, info));
     CK(tall_small_launch(st, src, m, b, Ri, nullptr, nullptr, dst));
     src = dst;
   }
+  if (orth_defer) return 0;
   int hinfo = 0;
   CK(d2h(&hinfo, info, sizeof(int)));
   if (hinfo) {
@@ -674,23 +732,27 @@ int Engine::tall_orth(float* Y, long m, int b, float* Q) {
 
 // Skinny products with a (P,M,S) view where P == 1 (A = X_(t), lda = M) or S == 1
 // (A^T = X viewed as P x M, lda = P). op 0: Y = A B (B: cols x b); op 1: Y = A^T B.
+// 16-bit products read the run's 16-bit copy of X when there is one (the values they
+// would round X to; half the bytes).
 int Engine::view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y) {
   float* ws = nullptr;
   ProfScope ps(prof, st, PK_GEMM);
-  if (v.P == 1) {
-    const long M = v.M, K = v.S;
+  const bool h = x16 && prec == x16_prec;
+  if (v.P == 1 || v.S == 1) {
+    // P == 1: A = X_(t) (M x K, lda M); S == 1: X viewed as P x M, A = X^T
+    const long M = v.P == 1 ? v.M : v.P, K = v.P == 1 ? v.S : v.M;
+    const bool atb = (v.P == 1) == (op == 1);  // A^T B with A = X_(t), or A B = X^T B
     const size_t wsf = gemm_ws_floats(M, K, b);
     ws = alloc<float>(wsf);
     if (!ws) return HT_ERR_ALLOC;
-    if (op == 0) CK(gemm_ab_launch(st, x, M, K, B, b, prec, Y, ws, wsf));
-    else CK(gemm_atb_launch(st, x, M, K, B, b, prec, Y, ws, wsf));
-  } else if (v.S == 1) {
-    const long M = v.P, K = v.M;  // X viewed as P x M; A = X^T
-    const size_t wsf = gemm_ws_floats(M, K, b);
-    ws = alloc<float>(wsf);
-    if (!ws) return HT_ERR_ALLOC;
-    if (op == 0) CK(gemm_atb_launch(st, x, M, K, B, b, prec, Y, ws, wsf));  // A B = X^T B
-    else CK(gemm_ab_launch(st, x, M, K, B, b, prec, Y, ws, wsf));           // A^T B = X B
+    if (h && gemm16_ok(x16, M, B, atb ? 1 : 0)) {
+      if (atb) CK(gemm_atb16_launch(st, x16, M, K, B, b, prec, Y, ws, wsf));
+      else CK(gemm_ab16_launch(st, x16, M, K, B, b, prec, Y, ws, wsf));
+      passes += 0.5;
+      return 0;
+    }
+    if (atb) CK(gemm_atb_launch(st, x, M, K, B, b, prec, Y, ws, wsf));
+    else CK(gemm_ab_launch(st, x, M, K, B, b, prec, Y, ws, wsf));
   } else {
     return HT_ERR_UNSUPPORTED;
   }
@@ -927,13 +989,19 @@ bool Engine::side_ready() {
 // its Gram is done, beside the Grams that follow it on st (a Gram kernel leaves CUs free;
 // the solver's kernels are single work-groups and host-checked). The results equal those
 // of the same solves on st.
-int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
+int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev, bool defer) {
   const Problem& p = *prob;
   std::vector<int> leaves;
   for (int t = 0; t < p.n_nodes; ++t)
     if (p.left[t] < 0) leaves.push_back(t);
   std::vector<int> done(p.n_nodes, 0);
-  bool have_norm = false;
+  // with a 16-bit copy of X, its pass gave ||X||^2
+  bool have_norm = x16 != nullptr;
+  int ev_next = 0;  // side-stream events in use
+  if (have_norm) {
+    CK(early_finite_check(xsq_dev));
+    if (adaptive) CK(adapt_tau(xsq_dev, 2 * p.order - 3));
+  }
   for (int t : leaves) {
     if (done[t]) continue;
     const long n = p.dims[p.lo[t]];
@@ -947,7 +1015,9 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
     double* U = alloc<double>((size_t)nb * n * k);
     double* w = alloc<double>((size_t)nb * k);
     if (!G || !U || !w) return HT_ERR_ALLOC;
-    const bool overlap = n > 256 && nb <= HT_MAX_ORDER && side_ready();
+    const bool overlap = n > 256 && ev_next + nb <= HT_MAX_ORDER && side_ready();
+    const int ev0 = ev_next;
+    if (overlap) ev_next += nb;
     // the solvers' starting blocks are drawn on st first, while the GPU is free
     std::vector<double*> Q0(nb, nullptr);
     if (overlap)
@@ -966,26 +1036,20 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
         if (adaptive) CK(adapt_tau(xsq_dev, 2 * p.order - 3));
       }
       have_norm = true;
-      if (overlap) HT_HIP_CHECK(hipEventRecord(side_ev[i], st));
+      if (overlap) HT_HIP_CHECK(hipEventRecord(side_ev[ev0 + i], st));
     }
     if (overlap) {
-      // the same solves (and Philox streams) as sym_topk_batch's one-at-a-time path
-      const hipStream_t main_st = st;
-      st = *side;
-      int rc = 0;
-      for (int i = 0; i < nb && !rc; ++i) {
-        if (hipStreamWaitEvent(st, side_ev[i], 0) != hipSuccess) rc = HT_ERR_HIP;
-        else
-          rc = sym_topk_large(G + (size_t)i * n * n, (int)n, k, U + (size_t)i * n * k, w + (size_t)i * k, 1e-9,
-                              opt.max_eig_iters, 1000 + grp[0] * 16 + 8 * i, GRAM_FLOOR, Q0[i]);
-      }
-      const bool joined = hipEventRecord(side_ev[HT_MAX_ORDER], st) == hipSuccess;
-      st = main_st;
-      if (!joined || hipStreamWaitEvent(st, side_ev[HT_MAX_ORDER], 0) != hipSuccess) {
-        (void)hipStreamSynchronize(*side);
-        return rc ? rc : HT_ERR_HIP;
-      }
-      if (rc) return rc;
+      LeafSolves j;
+      j.grp = grp;
+      j.n = n;
+      j.k = k;
+      j.G = G;
+      j.U = U;
+      j.w = w;
+      j.Q0 = Q0;
+      j.ev0 = ev0;
+      if (defer && !adaptive) pending_leaves.push_back(j);
+      else CK(solve_large_leaves(j));
     } else {
       CK(sym_topk_batch(G, (int)n, k, nb, U, w, 1e-9, 2, 0, 1000 + grp[0] * 16, GRAM_FLOOR));
     }
@@ -996,6 +1060,37 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev) {
       done[grp[i]] = 1;
     }
   }
+  return 0;
+}
+
+// The side-stream solves of one group of large leaves (each after its Gram's event), then
+// st waits for them. The same solves (and Philox streams) as sym_topk_batch's
+// one-at-a-time path.
+int Engine::solve_large_leaves(const LeafSolves& j) {
+  const int nb = (int)j.grp.size();
+  const hipStream_t main_st = st;
+  st = *side;
+  int rc = 0;
+  for (int i = 0; i < nb && !rc; ++i) {
+    if (hipStreamWaitEvent(st, side_ev[j.ev0 + i], 0) != hipSuccess) rc = HT_ERR_HIP;
+    else
+      rc = sym_topk_large(j.G + (size_t)i * j.n * j.n, (int)j.n, j.k, j.U + (size_t)i * j.n * j.k,
+                          j.w + (size_t)i * j.k, 1e-9, opt.max_eig_iters, 1000 + j.grp[0] * 16 + 8 * i, GRAM_FLOOR,
+                          j.Q0[i]);
+  }
+  const bool joined = hipEventRecord(side_ev[HT_MAX_ORDER], st) == hipSuccess;
+  st = main_st;
+  if (!joined || hipStreamWaitEvent(st, side_ev[HT_MAX_ORDER], 0) != hipSuccess) {
+    (void)hipStreamSynchronize(*side);
+    return rc ? rc : HT_ERR_HIP;
+  }
+  return rc;
+}
+
+int Engine::solve_pending_leaves() {
+  std::vector<LeafSolves> jobs;
+  jobs.swap(pending_leaves);
+  for (const LeafSolves& j : jobs) CK(solve_large_leaves(j));
   return 0;
 }
 
@@ -1497,16 +1592,18 @@ int Engine::ltr_interior(double* C, int64_t* cd_in, const double* xsq, const flo
 int Engine::range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z,
                          float* Qz) {
   const long rows = v.M, cols = v.P * v.S;
-  // the sketches' bases come from shifted CholeskyQR3 (TSQR when it meets a non-positive
-  // pivot): only Q is needed
-  auto orth = [&](float* A, long m, float* Qo) { return tall_orth(A, m, l, Qo); };
+  // the sketches' bases come from shifted CholeskyQR (TSQR when it meets a non-positive
+  // pivot): only Q is needed. The power iteration's intermediate blocks only carry the
+  // span into the next product, so they take the shifted pass alone; the final block
+  // takes all three.
+  auto orth = [&](float* A, long m, float* Qo, bool last) { return tall_orth(A, m, l, Qo, last ? 3 : 1); };
   CK(view_gemm(x, v, 0, Om, l, prec, Y));
-  CK(orth(Y, rows, Q));
+  CK(orth(Y, rows, Q, opt.power_iters == 0));
   for (int q = 0; q < opt.power_iters; ++q) {
     CK(view_gemm(x, v, 1, Q, l, prec, Z));
-    CK(orth(Z, cols, Qz));
+    CK(orth(Z, cols, Qz, false));
     CK(view_gemm(x, v, 0, Qz, l, prec, Y));
-    CK(orth(Y, rows, Q));
+    CK(orth(Y, rows, Q, q + 1 == opt.power_iters));
   }
   return 0;
 }
@@ -1582,9 +1679,36 @@ int Engine::sketch_node(const float* x, int t, const float* S, const int64_t* of
   return 0;
 }
 
+// Randomised method. The large leaves' eigen problems are solved on the side stream while
+// st runs the range finders: their Grams are enqueued first, then every range finder with
+// its pivot checks deferred (no host read in between), then the host drives the leaf
+// solves. A run whose deferred check finds a non-positive pivot is repeated with the
+// checks made at once (each such sketch then takes the TSQR basis).
 int Engine::run_rsvd(const float* x) {
-  const Problem& p = *prob;
   if (adaptive) return HT_ERR_UNSUPPORTED;  // the range finder's width is fixed by the rank
+  const Arena::Mark mk = arena->mark();
+  // allocated either way, so that a fixed arena sees the same requests as a growable one
+  orth_flag = alloc<int>(1);
+  if (!orth_flag) return HT_ERR_ALLOC;
+  orth_defer = side_ready();
+  if (orth_defer) HT_HIP_CHECK(hipMemsetAsync(orth_flag, 0, sizeof(int), st));
+  int rc = rsvd_pass(x);
+  if (orth_defer) {
+    int flag = 0;
+    const int rd = rc ? 0 : d2h(&flag, orth_flag, sizeof(int));
+    orth_defer = false;
+    if (rd) return rd;
+    if (!rc && flag) {
+      arena->rewind(mk);
+      reset_run_state();
+      rc = rsvd_pass(x);
+    }
+  }
+  return rc;
+}
+
+int Engine::rsvd_pass(const float* x) {
+  const Problem& p = *prob;
   const int prec = opt.precision == HT_PREC_FP32 ? PREC_F32 : (opt.precision == HT_PREC_FP16 ? PREC_F16 : PREC_BF16);
   // FP8: the range finder's products only, kept when the a-posteriori estimate allows
   const bool fp8 = opt.precision == HT_PREC_FP8;
@@ -1593,7 +1717,9 @@ int Engine::run_rsvd(const float* x) {
   int64_t soff[HT_MAX_ORDER], stotal = 0;
   if (learned) CK(sketch_offsets(p.order, p.dims, opt.sketch_width, soff, &stotal));
   double* xsq = alloc<double>(1);
-  CK(leaves_from_x(x, prec, xsq));
+  if (!xsq) return HT_ERR_ALLOC;
+  CK(make_x16(x, prec, xsq));
+  CK(leaves_from_x(x, prec, xsq, orth_defer));
   std::vector<int> done(p.n_nodes, 0);
   for (int t = 1; t < p.n_nodes; ++t) {
     if (p.left[t] < 0 || done[t]) continue;
@@ -1703,6 +1829,7 @@ int Engine::run_rsvd(const float* x) {
     }
     done[t] = 1;
   }
+  CK(solve_pending_leaves());
   for (int t = 1; t < p.n_nodes; ++t) {
     if (p.left[t] < 0) continue;
     const int l = p.left[t], r = p.right[t];
@@ -1734,7 +1861,7 @@ int Engine::write_outputs(float* out) {
   return cast_multi_launch(st, items.data(), (int)items.size());
 }
 
-int Engine::run_once(const float* x, float* out) {
+void Engine::reset_run_state() {
   for (int t = 0; t < HT_MAX_NODES; ++t) {
     Ufull[t] = nullptr;
     Bd[t] = nullptr;
@@ -1748,6 +1875,12 @@ int Engine::run_once(const float* x, float* out) {
   sketch_fallbacks = 0;
   sketch_nodes = 0;
   sketch_resid = 0.0;
+  x16 = nullptr;
+  pending_leaves.clear();
+}
+
+int Engine::run_once(const float* x, float* out) {
+  reset_run_state();
   int s = 0;
   if (opt.method == HT_METHOD_PLAIN) s = run_plain(x);
   else if (opt.method == HT_METHOD_LTR) s = run_ltr(x);

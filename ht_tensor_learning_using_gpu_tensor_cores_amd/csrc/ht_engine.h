@@ -130,6 +130,28 @@ struct Engine {
   double passes = 0.0;
   int eig_iters_max = 0, eig_calls = 0, noconv = 0;
   int orth_fallbacks = 0;  // tall_orth calls that took the TSQR basis
+  // 16-bit copy of X (values rounded to precision x16_prec) that the Grams of the views
+  // gram16_ok accepts read instead of X; nullptr when the run made none
+  const uint16_t* x16 = nullptr;
+  int x16_prec = 0;
+  // Large-leaf eigen problems whose Grams are enqueued and whose side-stream solves wait
+  // for solve_pending_leaves (leaves_from_x with defer = true), so that the host enqueues
+  // the work after them on st first.
+  struct LeafSolves {
+    std::vector<int> grp;
+    long n = 0;
+    int k = 0;
+    double *G = nullptr, *U = nullptr, *w = nullptr;
+    std::vector<double*> Q0;
+    int ev0 = 0;  // side_ev[ev0 + i]: recorded after leaf i's Gram
+  };
+  std::vector<LeafSolves> pending_leaves;
+  int solve_pending_leaves();
+  int solve_large_leaves(const LeafSolves& j);
+  // orth_defer: tall_orth leaves its pivot check to the end of the run (one device flag,
+  // orth_flag, for every call; a run that finds it set is repeated with the checks)
+  bool orth_defer = false;
+  int* orth_flag = nullptr;
   int replayed = 0;  // 1: the run replayed the recorded plan and passed its verdicts
   int sketch_fallbacks = 0, sketch_nodes = 0;
   double sketch_resid = 0.0;
@@ -163,6 +185,7 @@ struct Engine {
   int d2h(void* dst, const void* src, size_t bytes);
   int gram(const float* x, PMS v, int prec, double* G, double* sumsq);
   int gram2(const float* x, PMS v1, PMS v2, int prec, double* G1, double* G2);
+  int make_x16(const float* x, int prec, double* xsq);
   // Top-k eigenpairs of symmetric G (FP64). A Ritz pair counts as converged when its
   // residual is <= tol * th_1 or its Ritz value is <= floor * th_1 (below what the Gram's
   // own accumulation error resolves, where any basis of the cluster serves).
@@ -175,7 +198,7 @@ struct Engine {
   int sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
                      double floor, double* Q0 = nullptr);
   int tsqr(float* Y, long m, int b, float* Q, double* R);
-  int tall_orth(float* Y, long m, int b, float* Q);
+  int tall_orth(float* Y, long m, int b, float* Q, int passes = 3);
   bool side_ready();
   int view_gemm(const float* x, PMS v, int op, const float* B, int b, int prec, float* Y);
   int exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U, double* Wr, double* sv, uint32_t stream);
@@ -185,7 +208,7 @@ struct Engine {
   int expand2(int ta, int tb, double** oa, double** ob);
   int residual(const float* x, double* err_sq, double* x_sq);
   int leaves_from_x_plain(const float* x, double* xsq_dev);
-  int leaves_from_x(const float* x, int prec, double* xsq_dev);
+  int leaves_from_x(const float* x, int prec, double* xsq_dev, bool defer = false);
   int eig_to_sv(const double* w, int t);
   int early_finite_check(const double* xsq_dev);
   int root_transfer(const float* x);
@@ -195,6 +218,8 @@ struct Engine {
   int ltr_interior(double* C, int64_t* cd, const double* xsq, const float* x);
   int small_side_svd(const double* C, PMS v, int k, double* U, const double** w, int* done);
   int run_rsvd(const float* x);
+  int rsvd_pass(const float* x);
+  void reset_run_state();
   int range_finder(const float* x, PMS v, const float* Om, int l, int prec, float* Y, float* Q, float* Z, float* Qz);
   int range_residual(const float* x, PMS v, int t, const float* Q, int l, const double* xsq, double* est);
   // One sketched node's loss (accumulated into loss_acc[0]) and, when grad != nullptr,

@@ -435,15 +435,18 @@ __global__ __launch_bounds__(256) void refresh_zero_cols_kernel(double* __restri
                                                                 uint32_t stream) {
   __shared__ double dmax;
   __shared__ double red[4];
+  __shared__ double dg[JMAX];  // the diagonal of W (a refresh leaves the others' in place)
   const int tid = threadIdx.x;
+  if (tid < b) dg[tid] = W[tid + (long)b * tid];
+  __syncthreads();
   if (tid == 0) {
     double d = 0.0;
-    for (int i = 0; i < b; ++i) d = fmax(d, W[i + (long)b * i]);
+    for (int i = 0; i < b; ++i) d = fmax(d, dg[i]);
     dmax = d;
   }
   __syncthreads();
   for (int j = 0; j < b; ++j) {
-    if (W[j + (long)b * j] > 1e-28 * dmax && dmax > 0.0) continue;  // uniform across the block
+    if (dg[j] > 1e-28 * dmax && dmax > 0.0) continue;  // uniform across the block
     for (long r = tid; r < m; r += 256) Z[r + m * j] = philox_normal(seed, stream, (uint64_t)j * m + r);
     __syncthreads();
     for (int i = 0; i < b; ++i) {

@@ -175,14 +175,17 @@ __device__ __forceinline__ void sk_write(const f32x16 (&acc)[NB], long c0, int w
 // multiplied; stored values with m >= M are zero, columns c >= K and j >= b are copies
 // whose outputs are never written.
 // VEC (M % 4 == 0, A and B 16-byte aligned): float4 loads along m.
-template <int PREC, int NB, bool VEC>
-__global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__ A, long M, long K,
+// TA = uint16_t: A is a 16-bit copy in the precision's format (M % 8 == 0, VEC): 16-byte
+// loads of 8 values along m, stored to LDS as they are.
+template <int PREC, int NB, bool VEC, typename TA = float>
+__global__ __launch_bounds__(256) void gemm_atb_kernel(const TA* __restrict__ A, long M, long K,
                                                        const float* __restrict__ B, int b, int n_split,
                                                        float* __restrict__ Y, float* __restrict__ part) {
   typedef typename SkT<PREC>::T T;
   constexpr int LD = SkT<PREC>::LD;
   constexpr int NBJ = 32 * NB;                    // j rows of the B tile
-  constexpr int RA = 32, RB = NBJ * SK_K / 256;   // floats per thread per chunk
+  constexpr int RA = 32, RB = NBJ * SK_K / 256;   // values per thread per chunk
+  constexpr bool A16 = sizeof(TA) == 2;
   __shared__ __attribute__((aligned(16))) T As[128 * LD];
   __shared__ __attribute__((aligned(16))) T Bs[NBJ * LD];
   __shared__ float smax[32];
@@ -191,25 +194,28 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
   const int split = blockIdx.y;
   const long n_chunks = (M + SK_K - 1) / SK_K;
   const long k_begin = (long)split * n_chunks / n_split, k_end = (long)(split + 1) * n_chunks / n_split;
-  float ra[RA], rb[RB];
+  float ra[A16 ? 1 : RA], rb[RB];
+  uint32_t rh[A16 ? RA / 2 : 1];
   // VEC: thread element q of A is (col f / 16, m 4 (f % 16) + 0..3), f = tid + 256 (q / 4);
-  // scalar: (col e / 64, m e % 64), e = tid + 256 q. B likewise with j for col.
+  // scalar: (col e / 64, m e % 64), e = tid + 256 q. B likewise with j for col. 16-bit A:
+  // group it is (col f / 8, m 8 (f % 8) + 0..7), f = tid + 256 it.
   auto load = [&](long ch) {
     const long m0 = ch * SK_K;
-    if constexpr (VEC) {
+    if constexpr (A16) {
+#pragma unroll
+      for (int it = 0; it < RA / 8; ++it) {
+        const int f = tid + 256 * it;
+        const long c = min(c0 + (f >> 3), K - 1), m = min(m0 + 8 * (f & 7), M - 8);
+        const uint4 v = *reinterpret_cast<const uint4*>(A + m + M * c);
+        rh[4 * it] = v.x; rh[4 * it + 1] = v.y; rh[4 * it + 2] = v.z; rh[4 * it + 3] = v.w;
+      }
+    } else if constexpr (VEC) {
 #pragma unroll
       for (int it = 0; it < RA / 4; ++it) {
         const int f = tid + 256 * it;
         const long c = min(c0 + (f >> 4), K - 1), m = min(m0 + 4 * (f & 15), M - 4);
         const float4 v = *reinterpret_cast<const float4*>(A + m + M * c);
         ra[4 * it] = v.x; ra[4 * it + 1] = v.y; ra[4 * it + 2] = v.z; ra[4 * it + 3] = v.w;
-      }
-#pragma unroll
-      for (int it = 0; it < RB / 4; ++it) {
-        const int f = tid + 256 * it;
-        const long j = min(f >> 4, b - 1), m = min(m0 + 4 * (f & 15), M - 4);
-        const float4 v = *reinterpret_cast<const float4*>(B + m + M * j);
-        rb[4 * it] = v.x; rb[4 * it + 1] = v.y; rb[4 * it + 2] = v.z; rb[4 * it + 3] = v.w;
       }
     } else {
 #pragma unroll
@@ -218,6 +224,16 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
         const long c = min(c0 + (e >> 6), K - 1), m = min(m0 + (e & 63), M - 1);
         ra[q] = A[m + M * c];
       }
+    }
+    if constexpr (VEC) {
+#pragma unroll
+      for (int it = 0; it < RB / 4; ++it) {
+        const int f = tid + 256 * it;
+        const long j = min(f >> 4, b - 1), m = min(m0 + 4 * (f & 15), M - 4);
+        const float4 v = *reinterpret_cast<const float4*>(B + m + M * j);
+        rb[4 * it] = v.x; rb[4 * it + 1] = v.y; rb[4 * it + 2] = v.z; rb[4 * it + 3] = v.w;
+      }
+    } else {
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
         const int e = tid + 256 * q;
@@ -228,20 +244,21 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
   };
   auto store = [&](long ch, float sa, float sb) {
     const long m0 = ch * SK_K;
-    if constexpr (VEC) {
+    if constexpr (A16) {
+#pragma unroll
+      for (int it = 0; it < RA / 8; ++it) {
+        const int f = tid + 256 * it;
+        const uint32_t keep = m0 + 8 * (f & 7) < M ? 0xffffffffu : 0u;
+        *reinterpret_cast<uint4*>(&As[(f >> 3) * LD + 8 * (f & 7)]) =
+            make_uint4(rh[4 * it] & keep, rh[4 * it + 1] & keep, rh[4 * it + 2] & keep, rh[4 * it + 3] & keep);
+      }
+    } else if constexpr (VEC) {
 #pragma unroll
       for (int it = 0; it < RA / 4; ++it) {
         const int f = tid + 256 * it;
         const bool ok = m0 + 4 * (f & 15) < M;
         sk_put4<PREC>(&As[(f >> 4) * LD + 4 * (f & 15)], ok ? ra[4 * it] * sa : 0.f, ok ? ra[4 * it + 1] * sa : 0.f,
                       ok ? ra[4 * it + 2] * sa : 0.f, ok ? ra[4 * it + 3] * sa : 0.f);
-      }
-#pragma unroll
-      for (int it = 0; it < RB / 4; ++it) {
-        const int f = tid + 256 * it;
-        const bool ok = m0 + 4 * (f & 15) < M;
-        sk_put4<PREC>(&Bs[(f >> 4) * LD + 4 * (f & 15)], ok ? rb[4 * it] * sb : 0.f, ok ? rb[4 * it + 1] * sb : 0.f,
-                      ok ? rb[4 * it + 2] * sb : 0.f, ok ? rb[4 * it + 3] * sb : 0.f);
       }
     } else {
       const bool ok = m0 + (tid & 63) < M;  // e % 64 == tid % 64 for every q
@@ -250,6 +267,17 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
         const int e = tid + 256 * q;
         As[(e >> 6) * LD + (e & 63)] = sk_one<PREC>(ok ? ra[q] * sa : 0.f);
       }
+    }
+    if constexpr (VEC) {
+#pragma unroll
+      for (int it = 0; it < RB / 4; ++it) {
+        const int f = tid + 256 * it;
+        const bool ok = m0 + 4 * (f & 15) < M;
+        sk_put4<PREC>(&Bs[(f >> 4) * LD + 4 * (f & 15)], ok ? rb[4 * it] * sb : 0.f, ok ? rb[4 * it + 1] * sb : 0.f,
+                      ok ? rb[4 * it + 2] * sb : 0.f, ok ? rb[4 * it + 3] * sb : 0.f);
+      }
+    } else {
+      const bool ok = m0 + (tid & 63) < M;
 #pragma unroll
       for (int q = 0; q < RB; ++q) {
         const int e = tid + 256 * q;
@@ -284,14 +312,18 @@ __global__ __launch_bounds__(256) void gemm_atb_kernel(const float* __restrict__
 // stores them as one row piece. Rows m >= M are copies whose outputs are never
 // written; columns k >= K are stored as zero.
 // VEC (K % 4 == 0, B 16-byte aligned): float4 loads of B along k.
-template <int PREC, int NB, bool VEC>
-__global__ __launch_bounds__(512) void gemm_ab_kernel(const float* __restrict__ A, long M, long K,
+// TA = uint16_t: A is a 16-bit copy (M % 4 == 0, 8-byte aligned): thread t reads rows
+// 4 (t % 64) .. + 3 of the 8 columns 8 (t / 64).. (8-byte loads; a wave reads 512
+// consecutive bytes of a column) and transposes them in registers.
+template <int PREC, int NB, bool VEC, typename TA = float>
+__global__ __launch_bounds__(512) void gemm_ab_kernel(const TA* __restrict__ A, long M, long K,
                                                       const float* __restrict__ B, int b, int n_split,
                                                       float* __restrict__ Y, float* __restrict__ part) {
   typedef typename SkT<PREC>::T T;
   constexpr int LD = SkT<PREC>::LD;
   constexpr int NBJ = 32 * NB;
   constexpr int RB = NBJ * SK_K / 512;
+  constexpr bool A16 = sizeof(TA) == 2;
   __shared__ __attribute__((aligned(16))) T As[256 * LD];
   __shared__ __attribute__((aligned(16))) T Bs[NBJ * LD];
   __shared__ float smax[32];
@@ -300,14 +332,23 @@ __global__ __launch_bounds__(512) void gemm_ab_kernel(const float* __restrict__ 
   const int split = blockIdx.y;
   const long n_chunks = (K + SK_K - 1) / SK_K;
   const long k_begin = (long)split * n_chunks / n_split, k_end = (long)(split + 1) * n_chunks / n_split;
-  const long arow = min(r0 + (tid & 255), M - 1);
-  const int kh = 32 * (tid >> 8);
-  float ra[32], rb[RB];
+  const long arow = A16 ? min(r0 + 4 * (tid & 63), M - 4) : min(r0 + (tid & 255), M - 1);
+  const int kh = A16 ? 8 * (tid >> 6) : 32 * (tid >> 8);
+  float ra[A16 ? 1 : 32], rb[RB];
+  uint32_t rh[A16 ? 16 : 1];
   auto load = [&](long ch) {
     const long k0 = ch * SK_K;
-    const float* a = A + arow;
+    const TA* a = A + arow;
+    if constexpr (A16) {
 #pragma unroll
-    for (int q = 0; q < 32; ++q) ra[q] = a[M * min(k0 + kh + q, K - 1)];
+      for (int q = 0; q < 8; ++q) {
+        const uint2 v = *reinterpret_cast<const uint2*>(a + M * min(k0 + kh + q, K - 1));
+        rh[2 * q] = v.x; rh[2 * q + 1] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 32; ++q) ra[q] = a[M * min(k0 + kh + q, K - 1)];
+    }
     if constexpr (VEC) {
 #pragma unroll
       for (int it = 0; it < RB / 4; ++it) {
@@ -327,11 +368,30 @@ __global__ __launch_bounds__(512) void gemm_ab_kernel(const float* __restrict__ 
   };
   auto store = [&](long ch, float sa, float sb) {
     const long k0 = ch * SK_K;
-    const long nk = K - (k0 + kh);  // valid columns among this thread's 32
+    const long nk = K - (k0 + kh);  // valid columns among this thread's
+    if constexpr (A16) {
+      // rh[2 q] = rows 0, 1 of column q (low half row 0), rh[2 q + 1] = rows 2, 3
+      uint32_t c[16];
 #pragma unroll
-    for (int q = 0; q < 32; q += 4)
-      sk_put4<PREC>(&As[(tid & 255) * LD + kh + q], q < nk ? ra[q] * sa : 0.f, q + 1 < nk ? ra[q + 1] * sa : 0.f,
-                    q + 2 < nk ? ra[q + 2] * sa : 0.f, q + 3 < nk ? ra[q + 3] * sa : 0.f);
+      for (int i = 0; i < 16; ++i) c[i] = (i >> 1) < nk ? rh[i] : 0u;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int h = r >> 1;
+        const bool hi = (r & 1) != 0;
+        uint32_t wv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t x0 = c[2 * (2 * j) + h], x1 = c[2 * (2 * j + 1) + h];
+          wv[j] = hi ? ((x0 >> 16) | (x1 & 0xffff0000u)) : ((x0 & 0xffffu) | (x1 << 16));
+        }
+        *reinterpret_cast<uint4*>(&As[(4 * (tid & 63) + r) * LD + kh]) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 32; q += 4)
+        sk_put4<PREC>(&As[(tid & 255) * LD + kh + q], q < nk ? ra[q] * sa : 0.f, q + 1 < nk ? ra[q + 1] * sa : 0.f,
+                      q + 2 < nk ? ra[q + 2] * sa : 0.f, q + 3 < nk ? ra[q + 3] * sa : 0.f);
+    }
     if constexpr (VEC) {
 #pragma unroll
       for (int it = 0; it < RB / 4; ++it) {
@@ -397,44 +457,45 @@ size_t gemm_ws_floats(long M, long K, int b) {
   return (size_t)std::max(std::max(a, c), 1L);
 }
 
-template <int PREC, int NB>
-static void atb_dispatch(hipStream_t st, dim3 grid, bool vec, const float* A, long M, long K, const float* B, int b,
+template <int PREC, int NB, typename TA>
+static void atb_dispatch(hipStream_t st, dim3 grid, bool vec, const TA* A, long M, long K, const float* B, int b,
                          int ns, float* Y, float* ws) {
   if (vec)
-    hipLaunchKernelGGL((gemm_atb_kernel<PREC, NB, true>), grid, dim3(256), 0, st, A, M, K, B, b, ns, Y, ws);
-  else
-    hipLaunchKernelGGL((gemm_atb_kernel<PREC, NB, false>), grid, dim3(256), 0, st, A, M, K, B, b, ns, Y, ws);
+    hipLaunchKernelGGL((gemm_atb_kernel<PREC, NB, true, TA>), grid, dim3(256), 0, st, A, M, K, B, b, ns, Y, ws);
+  else if constexpr (sizeof(TA) == 4)
+    hipLaunchKernelGGL((gemm_atb_kernel<PREC, NB, false, TA>), grid, dim3(256), 0, st, A, M, K, B, b, ns, Y, ws);
 }
-template <int PREC, int NB>
-static void ab_dispatch(hipStream_t st, dim3 grid, bool vec, const float* A, long M, long K, const float* B, int b,
+template <int PREC, int NB, typename TA>
+static void ab_dispatch(hipStream_t st, dim3 grid, bool vec, const TA* A, long M, long K, const float* B, int b,
                         int ns, float* Y, float* ws) {
   if (vec)
-    hipLaunchKernelGGL((gemm_ab_kernel<PREC, NB, true>), grid, dim3(512), 0, st, A, M, K, B, b, ns, Y, ws);
+    hipLaunchKernelGGL((gemm_ab_kernel<PREC, NB, true, TA>), grid, dim3(512), 0, st, A, M, K, B, b, ns, Y, ws);
   else
-    hipLaunchKernelGGL((gemm_ab_kernel<PREC, NB, false>), grid, dim3(512), 0, st, A, M, K, B, b, ns, Y, ws);
+    hipLaunchKernelGGL((gemm_ab_kernel<PREC, NB, false, TA>), grid, dim3(512), 0, st, A, M, K, B, b, ns, Y, ws);
 }
 
-int gemm_atb_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec, float* Y,
-                    float* ws, size_t ws_floats) {
-  if (b < 1 || b > SK_B || M < 1 || K < 1) return 8;
+template <typename TA>
+static int atb_launch(hipStream_t st, const TA* A, long M, long K, const float* B, int b, int prec, float* Y,
+                      float* ws, size_t ws_floats, bool vec) {
   const long tiles = (K + 127) / 128, chunks = (M + SK_K - 1) / SK_K;
   const int ns = pick_split(tiles, chunks, ATB_TARGET);
   if (ns > 1 && (size_t)ns * K * SK_B > ws_floats) return 4;
-  const bool vec = M % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
   dim3 grid((unsigned)tiles, ns);
   const bool nb1 = b <= 32;
   if (prec == PREC_BF16) {
     if (nb1) atb_dispatch<PREC_BF16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
     else atb_dispatch<PREC_BF16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
-  } else if (prec == PREC_FP8) {
-    if (nb1) atb_dispatch<PREC_FP8, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
-    else atb_dispatch<PREC_FP8, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
   } else if (prec == PREC_F16) {
     if (nb1) atb_dispatch<PREC_F16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
     else atb_dispatch<PREC_F16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
-  } else {
-    if (nb1) atb_dispatch<PREC_F32, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
-    else atb_dispatch<PREC_F32, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+  } else if constexpr (sizeof(TA) == 4) {
+    if (prec == PREC_FP8) {
+      if (nb1) atb_dispatch<PREC_FP8, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+      else atb_dispatch<PREC_FP8, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    } else {
+      if (nb1) atb_dispatch<PREC_F32, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+      else atb_dispatch<PREC_F32, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    }
   }
   if (ns > 1) {
     const long tot = K * b;
@@ -443,9 +504,9 @@ int gemm_atb_launch(hipStream_t st, const float* A, long M, long K, const float*
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
-int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec, float* Y,
-                   float* ws, size_t ws_floats) {
-  if (b < 1 || b > SK_B || M < 1 || K < 1) return 8;
+template <typename TA>
+static int ab_launch(hipStream_t st, const TA* A, long M, long K, const float* B, int b, int prec, float* Y,
+                     float* ws, size_t ws_floats) {
   const long tiles = (M + 255) / 256, chunks = (K + SK_K - 1) / SK_K;
   const int ns = pick_split(tiles, chunks, AB_TARGET);
   if (ns > 1 && (size_t)ns * M * SK_B > ws_floats) return 4;
@@ -455,21 +516,55 @@ int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* 
   if (prec == PREC_BF16) {
     if (nb1) ab_dispatch<PREC_BF16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
     else ab_dispatch<PREC_BF16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
-  } else if (prec == PREC_FP8) {
-    if (nb1) ab_dispatch<PREC_FP8, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
-    else ab_dispatch<PREC_FP8, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
   } else if (prec == PREC_F16) {
     if (nb1) ab_dispatch<PREC_F16, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
     else ab_dispatch<PREC_F16, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
-  } else {
-    if (nb1) ab_dispatch<PREC_F32, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
-    else ab_dispatch<PREC_F32, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+  } else if constexpr (sizeof(TA) == 4) {
+    if (prec == PREC_FP8) {
+      if (nb1) ab_dispatch<PREC_FP8, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+      else ab_dispatch<PREC_FP8, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    } else {
+      if (nb1) ab_dispatch<PREC_F32, 1>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+      else ab_dispatch<PREC_F32, 2>(st, grid, vec, A, M, K, B, b, ns, Y, ws);
+    }
   }
   if (ns > 1) {
     const long tot = M * b;
     hipLaunchKernelGGL(skinny_sum_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ws, M, b, ns, Y);
   }
   return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+int gemm_atb_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec, float* Y,
+                    float* ws, size_t ws_floats) {
+  if (b < 1 || b > SK_B || M < 1 || K < 1) return 8;
+  const bool vec = M % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+  return atb_launch(st, A, M, K, B, b, prec, Y, ws, ws_floats, vec);
+}
+
+int gemm_ab_launch(hipStream_t st, const float* A, long M, long K, const float* B, int b, int prec, float* Y,
+                   float* ws, size_t ws_floats) {
+  if (b < 1 || b > SK_B || M < 1 || K < 1) return 8;
+  return ab_launch(st, A, M, K, B, b, prec, Y, ws, ws_floats);
+}
+
+bool gemm16_ok(const uint16_t* A, long M, const float* B, int op_atb) {
+  if (op_atb) return M % 8 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0;
+  return M % 4 == 0 && ((uintptr_t)A & 7) == 0;
+}
+
+int gemm_atb16_launch(hipStream_t st, const uint16_t* A, long M, long K, const float* B, int b, int prec, float* Y,
+                      float* ws, size_t ws_floats) {
+  if (b < 1 || b > SK_B || M < 1 || K < 1 || (prec != PREC_BF16 && prec != PREC_F16) || !gemm16_ok(A, M, B, 1))
+    return 8;
+  return atb_launch(st, A, M, K, B, b, prec, Y, ws, ws_floats, true);
+}
+
+int gemm_ab16_launch(hipStream_t st, const uint16_t* A, long M, long K, const float* B, int b, int prec, float* Y,
+                     float* ws, size_t ws_floats) {
+  if (b < 1 || b > SK_B || M < 1 || K < 1 || (prec != PREC_BF16 && prec != PREC_F16) || !gemm16_ok(A, M, B, 0))
+    return 8;
+  return ab_launch(st, A, M, K, B, b, prec, Y, ws, ws_floats);
 }
 
 // ------------------------------------------------------------------ proj_pms

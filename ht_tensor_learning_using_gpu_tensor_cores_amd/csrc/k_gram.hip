@@ -456,8 +456,137 @@ __device__ __forceinline__ void gb_body(const float* __restrict__ x, PMS v, long
   }
 }
 
-template <int PREC, int LOADER>
-__global__ __launch_bounds__(512) void gram_big_kernel(const float* __restrict__ x, PMS v, long K, long n_chunks,
+// 16-bit copies of X (the values the FP32 loaders above convert to when they store to
+// LDS): a chunk is half the bytes, so the body keeps two chunks in flight in the same
+// registers. One 256 x 64 tile, 16 uint32 per thread.
+template <int LOADER>
+__device__ __forceinline__ void gb_load16(const uint16_t* __restrict__ x, PMS v, long K, long m0, long col0,
+                                          int tid, uint32_t (&r)[16]) {
+  if constexpr (LOADER == LD_COLS_CONTIG) {
+    // 8 lanes read a row's 128 bytes (16 each), rows tid / 8 + 64 i
+    const long s = col0 / v.P, p0 = col0 - s * v.P;
+    const uint16_t* base = x + p0 + v.P * (v.M * s + m0);
+    const unsigned off = (unsigned)v.P * (unsigned)(tid >> 3) + 8u * (unsigned)(tid & 7);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint4 q = *reinterpret_cast<const uint4*>(base + v.P * (64 * i) + off);
+      r[4 * i + 0] = q.x; r[4 * i + 1] = q.y; r[4 * i + 2] = q.z; r[4 * i + 3] = q.w;
+    }
+  } else {
+    // P == 1, M % 4 == 0. Thread owns rows m0 + 4 (tid % 64) .. + 3 and columns
+    // col0 + 8 (tid / 64) + i, i < 8 (clamped to K - 1; gb_store16 zeroes the copies): a
+    // wave's load i reads the 512 contiguous bytes of one column.
+    const long cb = col0 + 8 * __builtin_amdgcn_readfirstlane(tid >> 6);
+    const unsigned off = 4u * (unsigned)(tid & 63);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint2 q = *reinterpret_cast<const uint2*>(x + (v.M * min(cb + i, K - 1) + m0) + off);
+      r[2 * i] = q.x; r[2 * i + 1] = q.y;
+    }
+  }
+}
+
+// Registers -> LDS tile [row][k] (row stride GB_LD); columns past K, and every column
+// when keep == 0 (a chunk past the block's range), stored as zeros.
+template <int LOADER>
+__device__ __forceinline__ void gb_store16(uint16_t* __restrict__ T, int tid, const uint32_t (&r)[16], long K,
+                                           long col0, uint32_t keep) {
+  if constexpr (LOADER == LD_COLS_CONTIG) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      *reinterpret_cast<uint4*>(&T[((tid >> 3) + 64 * i) * GB_LD + 8 * (tid & 7)]) =
+          make_uint4(r[4 * i] & keep, r[4 * i + 1] & keep, r[4 * i + 2] & keep, r[4 * i + 3] & keep);
+  } else {
+    // r[2 i] = rows 0, 1 of column i (low half row 0), r[2 i + 1] = rows 2, 3: transpose
+    // to 8 consecutive columns of each of the 4 rows
+    const int row = 4 * (tid & 63), k0 = 8 * (tid >> 6);
+    const long nv = K - (col0 + k0);  // columns of this thread inside the matrix
+    uint32_t c[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) c[i] = (i >> 1) < nv ? (r[i] & keep) : 0u;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int h = q >> 1;            // word of the column: rows 0-1 or 2-3
+      const bool hi = (q & 1) != 0;    // upper half of that word
+      uint32_t w[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t a = c[2 * (2 * j) + h], b = c[2 * (2 * j + 1) + h];
+        w[j] = hi ? ((a >> 16) | (b & 0xffff0000u)) : ((a & 0xffffu) | (b << 16));
+      }
+      *reinterpret_cast<uint4*>(&T[(row + q) * GB_LD + k0]) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+}
+
+template <int PREC, bool DIAG>
+__device__ __forceinline__ void gb_mma(const uint16_t* __restrict__ Ai, const uint16_t* __restrict__ Aj, int wr,
+                                       int wc, int lane, f32x16 (&acc)[2][4]) {
+#pragma unroll
+  for (int kk = 0; kk < GB_K / 16; ++kk) {
+    u16x8 af[2], bf[4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+      af[a] = *reinterpret_cast<const u16x8*>(&Ai[(64 * wr + 32 * a + (lane & 31)) * GB_LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      bf[b] = *reinterpret_cast<const u16x8*>(&Aj[(128 * wc + 32 * b + (lane & 31)) * GB_LD + 16 * kk + 8 * (lane >> 5)]);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = mfma32x32x16<PREC>(af[a], bf[b], acc[a][b]);
+  }
+}
+
+// Two register stages: while chunk c is multiplied from one LDS buffer, chunk c + 1
+// (loaded one step earlier) is stored to the other and chunk c + 2 is being loaded. The
+// loop runs over pairs of chunks with no exit in between (an exit between the halves lets
+// the compiler move the first half's loads into the second and spill); an odd count ends
+// with one chunk of zeros. Loads past the range reread the last chunk.
+template <int PREC, int LOADER, bool DIAG>
+__device__ __forceinline__ void gb_body16(const uint16_t* __restrict__ x, PMS v, long K, long c_begin, long c_end,
+                                          long mi, long mj, uint16_t* __restrict__ lds, f32x16 (&acc)[2][4]) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  constexpr int BUF = DIAG ? GB_TILE : 2 * GB_TILE;
+  constexpr int OFF_J = DIAG ? 0 : GB_TILE;
+  uint32_t ri0[16], rj0[16], ri1[16], rj1[16];
+  const long last = c_end - 1;
+  const long c_stop = c_begin + ((c_end - c_begin + 1) & ~1L);
+  auto keep = [&](long c) { return c < c_end ? 0xffffffffu : 0u; };
+  gb_load16<LOADER>(x, v, K, mi, c_begin * GB_K, tid, ri0);
+  if constexpr (!DIAG) gb_load16<LOADER>(x, v, K, mj, c_begin * GB_K, tid, rj0);
+  gb_load16<LOADER>(x, v, K, mi, min(c_begin + 1, last) * GB_K, tid, ri1);
+  if constexpr (!DIAG) gb_load16<LOADER>(x, v, K, mj, min(c_begin + 1, last) * GB_K, tid, rj1);
+  gb_store16<LOADER>(lds, tid, ri0, K, c_begin * GB_K, ~0u);
+  if constexpr (!DIAG) gb_store16<LOADER>(lds + GB_TILE, tid, rj0, K, c_begin * GB_K, ~0u);
+  __syncthreads();
+  for (long c = c_begin; c < c_stop; c += 2) {
+    // buffer 0 holds chunk c, stage 1 chunk c + 1
+    gb_load16<LOADER>(x, v, K, mi, min(c + 2, last) * GB_K, tid, ri0);
+    if constexpr (!DIAG) gb_load16<LOADER>(x, v, K, mj, min(c + 2, last) * GB_K, tid, rj0);
+    __builtin_amdgcn_sched_barrier(0);
+    gb_mma<PREC, DIAG>(lds, lds + OFF_J, wr, wc, lane, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    gb_store16<LOADER>(lds + BUF, tid, ri1, K, min(c + 1, last) * GB_K, keep(c + 1));
+    if constexpr (!DIAG) gb_store16<LOADER>(lds + BUF + GB_TILE, tid, rj1, K, min(c + 1, last) * GB_K, keep(c + 1));
+    __syncthreads();
+    // buffer 1 holds chunk c + 1, stage 0 chunk c + 2
+    gb_load16<LOADER>(x, v, K, mi, min(c + 3, last) * GB_K, tid, ri1);
+    if constexpr (!DIAG) gb_load16<LOADER>(x, v, K, mj, min(c + 3, last) * GB_K, tid, rj1);
+    __builtin_amdgcn_sched_barrier(0);
+    gb_mma<PREC, DIAG>(lds + BUF, lds + BUF + OFF_J, wr, wc, lane, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    gb_store16<LOADER>(lds, tid, ri0, K, min(c + 2, last) * GB_K, keep(c + 2));
+    if constexpr (!DIAG) gb_store16<LOADER>(lds + GB_TILE, tid, rj0, K, min(c + 2, last) * GB_K, keep(c + 2));
+    __syncthreads();
+  }
+}
+
+// T = float: FP32 X converted as it is stored to LDS, norms = the diagonal pairs' sums of
+// squares. T = uint16_t: a 16-bit copy of X, no norms (the copy's pass takes them).
+template <int PREC, int LOADER, typename T>
+__global__ __launch_bounds__(512) void gram_big_kernel(const T* __restrict__ x, PMS v, long K, long n_chunks,
                                                        int n_split, int n_tiles, float* __restrict__ part,
                                                        double* __restrict__ norms) {
   __shared__ __attribute__((aligned(16))) uint16_t lds[4 * GB_TILE];
@@ -497,10 +626,16 @@ __global__ __launch_bounds__(512) void gram_big_kernel(const float* __restrict__
     for (int b = 0; b < 4; ++b) acc[a][b] = f32x16{};
   double sumsq = 0.0;
   if (c_begin < c_end) {
-    if (bi == bj)
-      gb_body<PREC, LOADER, true>(x, v, K, c_begin, c_end, mi, mi, (int)((long)GB_ROWS * bi - mi), lds, acc, sumsq);
-    else
-      gb_body<PREC, LOADER, false>(x, v, K, c_begin, c_end, mi, mj, 0, lds, acc, sumsq);
+    if constexpr (sizeof(T) == 2) {
+      if (bi == bj) gb_body16<PREC, LOADER, true>(x, v, K, c_begin, c_end, mi, mi, lds, acc);
+      else gb_body16<PREC, LOADER, false>(x, v, K, c_begin, c_end, mi, mj, lds, acc);
+    } else {
+      if (bi == bj)
+        gb_body<PREC, LOADER, true>(x, v, K, c_begin, c_end, mi, mi, (int)((long)GB_ROWS * bi - mi), lds, acc,
+                                    sumsq);
+      else
+        gb_body<PREC, LOADER, false>(x, v, K, c_begin, c_end, mi, mj, 0, lds, acc, sumsq);
+    }
   }
   float* out = part + ((long)pair * n_split + split) * (GB_ROWS * GB_ROWS);
 #pragma unroll
@@ -513,13 +648,15 @@ __global__ __launch_bounds__(512) void gram_big_kernel(const float* __restrict__
         const int col = 128 * wc + 32 * b + (lane & 31);
         out[row * GB_ROWS + col] = acc[a][b][e];
       }
-  sumsq = wave_sum(sumsq);
-  if (lane == 0) red[w] = sumsq;
-  __syncthreads();
-  if (tid == 0) {
-    double t = 0.0;
-    for (int q = 0; q < 8; ++q) t += red[q];
-    norms[(long)pair * n_split + split] = t;
+  if constexpr (sizeof(T) == 4) {
+    sumsq = wave_sum(sumsq);
+    if (lane == 0) red[w] = sumsq;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int q = 0; q < 8; ++q) t += red[q];
+      norms[(long)pair * n_split + split] = t;
+    }
   }
 }
 
@@ -562,10 +699,10 @@ static void gram_dispatch(hipStream_t st, const float* x, PMS v, int n_split, fl
       const int nt = (int)((v.M + GB_ROWS - 1) / GB_ROWS);
       dim3 grid(n_split * (nt * (nt + 1) / 2));
       if (v.P == 1)
-        hipLaunchKernelGGL((gram_big_kernel<PREC, LD_ROWS_CONTIG>), grid, dim3(512), 0, st, x, v, K, n_chunks,
+        hipLaunchKernelGGL((gram_big_kernel<PREC, LD_ROWS_CONTIG, float>), grid, dim3(512), 0, st, x, v, K, n_chunks,
                            n_split, nt, part, norms);
       else
-        hipLaunchKernelGGL((gram_big_kernel<PREC, LD_COLS_CONTIG>), grid, dim3(512), 0, st, x, v, K, n_chunks,
+        hipLaunchKernelGGL((gram_big_kernel<PREC, LD_COLS_CONTIG, float>), grid, dim3(512), 0, st, x, v, K, n_chunks,
                            n_split, nt, part, norms);
       return;
     }
@@ -698,6 +835,109 @@ __global__ void sum_f64_kernel(const double* __restrict__ a, int n, double* __re
 
 int gram_reduce_norms_launch(hipStream_t st, const double* norms, int n, double* out) {
   hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, st, norms, n, out);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// ------------------------------------------------------------------ 16-bit copy of X
+// Copy of X in 16-bit values (round to nearest even: the values the Gram and skinny GEMM
+// kernels form when they read FP32 X in 16-bit precision) with ||X||^2 of the FP32 values
+// (per block one FP64 partial, added by sum_f64_kernel). Each thread converts groups of 8
+// floats (two 16-byte loads, one 16-byte store), four groups in flight.
+constexpr int CAST16_BLOCKS = 2048;
+
+template <int PREC>
+__global__ __launch_bounds__(256) void cast16_sumsq_kernel(const float* __restrict__ x, long n,
+                                                           uint16_t* __restrict__ y, double* __restrict__ part) {
+  __shared__ double red[4];
+  const long n8 = n / 8, stride = (long)gridDim.x * 256;
+  double s = 0.0;
+  auto one = [&](long g, float4 a, float4 b) {
+    uint4 w;
+    w.x = cvt2<PREC>(a.x, a.y);
+    w.y = cvt2<PREC>(a.z, a.w);
+    w.z = cvt2<PREC>(b.x, b.y);
+    w.w = cvt2<PREC>(b.z, b.w);
+    *reinterpret_cast<uint4*>(y + 8 * g) = w;
+    s += (double)(a.x * a.x + a.y * a.y + a.z * a.z + a.w * a.w + b.x * b.x + b.y * b.y + b.z * b.z + b.w * b.w);
+  };
+  long g = blockIdx.x * 256L + threadIdx.x;
+  for (; g + 3 * stride < n8; g += 4 * stride) {
+    float4 a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = load_stream(x + 8 * (g + u * stride));
+      b[u] = load_stream(x + 8 * (g + u * stride) + 4);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) one(g + u * stride, a[u], b[u]);
+  }
+  for (; g < n8; g += stride) one(g, load_stream(x + 8 * g), load_stream(x + 8 * g + 4));
+  if (blockIdx.x == 0 && threadIdx.x < n - 8 * n8) {
+    const long i = 8 * n8 + threadIdx.x;
+    const float f = x[i];
+    y[i] = to16<PREC>(f);
+    s += (double)(f * f);
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+int cast16_part_size() { return CAST16_BLOCKS; }
+
+int cast16_sumsq_launch(hipStream_t st, const float* x, long n, int prec, uint16_t* y, double* part,
+                        double* sumsq) {
+  if (n <= 0 || ((uintptr_t)x & 15) || ((uintptr_t)y & 15)) return 8;
+  if (prec == PREC_F16)
+    hipLaunchKernelGGL(cast16_sumsq_kernel<PREC_F16>, dim3(CAST16_BLOCKS), dim3(256), 0, st, x, n, y, part);
+  else
+    hipLaunchKernelGGL(cast16_sumsq_kernel<PREC_BF16>, dim3(CAST16_BLOCKS), dim3(256), 0, st, x, n, y, part);
+  hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(256), 0, st, part, CAST16_BLOCKS, sumsq);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// Grams from the 16-bit copy (16-byte aligned): gram_big_kernel's tiles over
+// matricisations of 257 to 1024 rows whose columns are single values of a row's stretch
+// (P == 1, M % 4 == 0) or runs of 64 (P % 64 == 0). part: [pairs][n_split][256 * 256] FP32 (gram_pms_pairs and
+// gram_pms_nsplit of the same view in 16-bit precision).
+bool gram16_ok(PMS v) {
+  const bool rows = v.P == 1 && v.M % 4 == 0;
+  const bool cols = v.P % 64 == 0 && v.P <= (1L << 24);
+  return v.M > GB_ROWS && v.M <= 1024 && v.S > 0 && (rows || cols);
+}
+
+int gram16_pairs(PMS v) {
+  const int nt = (int)((v.M + GB_ROWS - 1) / GB_ROWS);
+  return nt * (nt + 1) / 2;
+}
+
+// as gram_pms_nsplit for 256-row tiles
+int gram16_nsplit(PMS v) {
+  const long n_chunks = (v.P * v.S + GB_K - 1) / GB_K;
+  long ns = 256 / gram16_pairs(v);
+  if (ns >= 8) ns -= ns % 8;
+  if (ns < 1) ns = 1;
+  if (ns > n_chunks) ns = n_chunks;
+  return (int)ns;
+}
+
+int gram16_launch(hipStream_t st, const uint16_t* x, PMS v, int prec, int n_split, float* part) {
+  if (!gram16_ok(v) || ((uintptr_t)x & 15) || n_split < 1 || (prec != PREC_BF16 && prec != PREC_F16)) return 8;
+  const long K = v.P * v.S;
+  const long n_chunks = (K + GB_K - 1) / GB_K;
+  const int nt = (int)((v.M + GB_ROWS - 1) / GB_ROWS);
+  dim3 grid(n_split * (nt * (nt + 1) / 2));
+#define HT_G16(P, L) \
+  hipLaunchKernelGGL((gram_big_kernel<P, L, uint16_t>), grid, dim3(512), 0, st, x, v, K, n_chunks, n_split, nt, part, nullptr)
+  if (prec == PREC_F16) {
+    if (v.P == 1) HT_G16(PREC_F16, LD_ROWS_CONTIG);
+    else HT_G16(PREC_F16, LD_COLS_CONTIG);
+  } else {
+    if (v.P == 1) HT_G16(PREC_BF16, LD_ROWS_CONTIG);
+    else HT_G16(PREC_BF16, LD_COLS_CONTIG);
+  }
+#undef HT_G16
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 

@@ -66,7 +66,8 @@ typedef enum {
    * projection pass to the leaf core and exact FP64 interior problems on that core. */
   HT_METHOD_LTR = 1,
   /* Root-to-leaves randomised: every non-root node by a range finder (oversampling p,
-   * q power iterations) on MFMA, shifted CholeskyQR3 orthonormalisation, FP64 projected SVD.
+   * q power iterations) on MFMA, shifted CholeskyQR orthonormalisation (three passes for
+   * the final block, one for the intermediate ones), FP64 projected SVD.
    * fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/kernel.cu:419-504. */
   HT_METHOD_RSVD = 2
 } ht_method;
@@ -218,8 +219,10 @@ ht_status ht_decompose_many(ht_handle_t handle, int32_t count, const float* cons
  * of ht_decompose); the workspace is a fixed library array of HT_NOALLOC_SCRATCH_BYTES
  * per GPU, held until the process ends (its address per device is taken by ht_create).
  * A problem whose workspace does not fit returns HT_ERR_WORKSPACE (ht_stats.
- * workspace_bytes of an ht_decompose call of the same problem gives its need; the 4th-order
- * 128^4 rank-16 leaves-to-root decomposition fits). Every call on one GPU shares that
+ * workspace_bytes of an ht_decompose call of the same problem gives its need, or more: for
+ * randomised runs in 16-bit precision on tensors of 2^22 elements and up, ht_decompose also
+ * holds a 16-bit copy of X, which this entry does without; the 4th-order 128^4 rank-16
+ * leaves-to-root decomposition fits). Every call on one GPU shares that
  * array: calls that overlap on one GPU (through different handles) race. Results equal
  * ht_decompose's. */
 #define HT_NOALLOC_SCRATCH_BYTES ((size_t)160 << 20)

@@ -30,6 +30,13 @@ ht_status ht_debug_mfma_probe(ht_handle_t h, const float* a, const float* b, flo
 ht_status ht_mode_gram(ht_handle_t h, const float* x, int32_t order, const int64_t* dims, int32_t lo, int32_t hi,
                        int32_t precision, double* G, double* sumsq);
 
+/* The same Gram in BF16/FP16 from a 16-bit copy of X made first (one pass that also gives
+ * *sumsq = ||X||^2), the path the randomised method takes for matricisations of 257-1024
+ * rows (P == 1 with rows_t % 4 == 0, or P % 64 == 0); HT_ERR_UNSUPPORTED for other views.
+ * x 16-byte aligned. Replaces the same call sites as ht_mode_gram. */
+ht_status ht_mode_gram16(ht_handle_t h, const float* x, int32_t order, const int64_t* dims, int32_t lo,
+                         int32_t hi, int32_t precision, double* G, double* sumsq);
+
 /* Grams of the two leading modes of an n1 x n2 x S tensor from one pass (BF16/FP16). */
 ht_status ht_slab_gram(ht_handle_t h, const float* x, int32_t n1, int32_t n2, int64_t S, int32_t precision,
                        double* G1, double* G2, double* sumsq);
@@ -65,6 +72,13 @@ ht_status ht_tall_orth(ht_handle_t h, float* Y, int64_t m, int32_t b, float* Q, 
  * fourth_order_tensor_decomposition/single_GPU/CUDA/Optimized/kernel.cu:434-444. */
 ht_status ht_skinny_gemm(ht_handle_t h, const float* A, int64_t M, int64_t K, int32_t transA, const float* B,
                          int32_t b, int32_t precision, float* Y);
+
+/* The same products (BF16/FP16) reading a 16-bit copy of A made first: the path the
+ * randomised method's range finders take once the run holds such a copy of X. A^T B needs
+ * M % 8 == 0 and 16-byte aligned A and B, A B needs M % 4 == 0; HT_ERR_UNSUPPORTED
+ * otherwise. Replaces the same call sites as ht_skinny_gemm. */
+ht_status ht_skinny_gemm16(ht_handle_t h, const float* A, int64_t M, int64_t K, int32_t transA, const float* B,
+                           int32_t b, int32_t precision, float* Y);
 
 /* n Philox4x32-10 standard normals of (seed, stream) on the device, and on the host
  * (out is host memory) -- the shared generator of every synthetic input. */

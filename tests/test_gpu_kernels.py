@@ -104,6 +104,49 @@ def test_mode_gram(lib, handle, rng, dims, lo, hi, precision):
     assert np.allclose(Gh, Gh.T)
 
 
+# The 16-bit copy path: the Gram must equal, bit for bit, the one that reads FP32 X and
+# rounds it on the way into LDS (same rounded values, same tiles, splits and chunk order;
+# an odd chunk count adds one chunk of zeros). Rows contiguous (P == 1, M % 4 == 0:
+# (300,20,18) mode 0 with a partial last column chunk, (260,7,9) with overlapped tiles,
+# (1024,3,5), (20,16,600) modes 0-1) and runs of 64 ((64,520,3) mode 1, (16,20,288) mode 2,
+# (64,1024,2) mode 1); odd tensor sizes exercise the copy's tail.
+@pytest.mark.parametrize("dims,lo,hi", [((300, 20, 18), 0, 1), ((260, 7, 9), 0, 1), ((1024, 3, 5), 0, 1),
+                                        ((20, 16, 600), 0, 2), ((64, 520, 3), 1, 2), ((16, 20, 288), 2, 3),
+                                        ((64, 1024, 2), 1, 2), ((292, 3, 3), 0, 1)])
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_mode_gram16_matches_fp32_read(lib, handle, rng, dims, lo, hi, precision):
+    import torch
+    x = rng.standard_normal(int(np.prod(dims))).astype(np.float32)
+    M = int(np.prod(dims[lo:hi]))
+    G = torch.zeros(M * M, dtype=torch.float64, device="cuda")
+    G16 = torch.zeros(M * M, dtype=torch.float64, device="cuda")
+    s, s16 = ctypes.c_double(), ctypes.c_double()
+    xd = dev(x)
+    args = (len(dims), _lib.i64_array(dims), lo, hi, _lib.PRECISION[precision])
+    print(f"mode_gram16 dims={dims} modes {lo}:{hi} {precision}", flush=True)
+    _lib.check(lib.ht_mode_gram(handle.ptr, ptr(xd), *args, ptr(G), ctypes.byref(s)), "gram")
+    _lib.check(lib.ht_mode_gram16(handle.ptr, ptr(xd), *args, ptr(G16), ctypes.byref(s16)), "gram16")
+    assert torch.equal(G, G16)
+    np.testing.assert_allclose(s16.value, float(np.sum(x.astype(np.float64) ** 2)), rtol=1e-6)
+    # and against the oracle's Gram of the rounded values
+    xr = bf16_round(x) if precision == "bf16" else x.astype(np.float16).astype(np.float32)
+    A = O.unfold(xr.astype(np.float64), list(dims), lo, hi)
+    ref = A @ A.T
+    np.testing.assert_allclose(host(G16).reshape((M, M), order="F"), ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+
+
+def test_mode_gram16_rejects_other_views(lib, handle, rng):
+    import torch
+    x = dev(rng.standard_normal(4 * 300 * 16).astype(np.float32))
+    G = torch.zeros(300 * 300, dtype=torch.float64, device="cuda")
+    # P == 4 (neither contiguous rows nor runs of 64) and FP32 take no 16-bit copy
+    for dims, lo, hi, prec in (((4, 300, 16), 1, 2, "bf16"), ((300, 4, 16), 0, 1, "fp32"),
+                               ((200, 6, 16), 0, 1, "bf16")):
+        rc = lib.ht_mode_gram16(handle.ptr, ptr(x), 3, _lib.i64_array(dims), lo, hi, _lib.PRECISION[prec],
+                                ptr(G), None)
+        assert _lib.STATUS_NAMES[rc] == "HT_ERR_UNSUPPORTED"
+
+
 @pytest.mark.parametrize("n1,n2,S", [(128, 128, 6), (64, 32, 10), (20, 12, 7), (128, 16, 3)])
 def test_slab_gram(lib, handle, rng, n1, n2, S):
     import torch
@@ -264,6 +307,47 @@ def test_skinny_gemm(lib, handle, rng, M, K, b, trans, precision):
     A64, B64 = A.astype(np.float64), B.astype(np.float64)
     ref = A64.T @ B64 if trans else A64 @ B64
     np.testing.assert_allclose(Yh, ref, rtol=1e-4, atol=1e-4 * np.abs(ref).max())
+
+
+# Reading a 16-bit copy of A instead of rounding FP32 A on its way to LDS gives the same
+# LDS tiles, splits and chunk order: the products must agree bit for bit. Shapes with one
+# and many splits, b in one or two tiles, partial chunks and tiles (M % 8 == 0 for A^T B,
+# M % 4 == 0 for A B; K arbitrary), and the randomised method's 1024 x 2^20 shapes.
+@pytest.mark.parametrize("M,K,b,trans", [(4096, 3000, 64, 1), (4096, 3000, 24, 0), (1000, 777, 17, 1),
+                                         (132, 5000, 8, 0), (16384, 512, 40, 1), (1024, 65537, 40, 0),
+                                         (780, 4099, 33, 0), (296, 64, 64, 0), (64, 70001, 40, 1),
+                                         (1024, 1 << 20, 40, 1), (1024, 1 << 20, 40, 0)])
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_skinny_gemm16_matches_fp32_read(lib, handle, rng, M, K, b, trans, precision):
+    import torch
+    print(f"skinny_gemm16 M={M} K={K} b={b} trans={trans} {precision}", flush=True)
+    A = torch.randn(M * K, device="cuda", generator=torch.Generator(device="cuda").manual_seed(M + K))
+    B = dev(rng.standard_normal(((M if trans else K), b)).astype(np.float32).reshape(-1, order="F"))
+    rows = K if trans else M
+    Y, Y16 = torch.zeros(rows * b, device="cuda"), torch.zeros(rows * b, device="cuda")
+    p = _lib.PRECISION[precision]
+    _lib.check(lib.ht_skinny_gemm(handle.ptr, ptr(A), M, K, trans, ptr(B), b, p, ptr(Y)), "gemm")
+    _lib.check(lib.ht_skinny_gemm16(handle.ptr, ptr(A), M, K, trans, ptr(B), b, p, ptr(Y16)), "gemm16")
+    assert torch.equal(Y, Y16)
+    if M * K <= 1 << 24:  # and the product of the rounded values
+        Ah = host(A).reshape((M, K), order="F")
+        Bh = host(B).reshape((-1, b), order="F")
+        r = bf16_round if precision == "bf16" else (lambda v: v.astype(np.float16).astype(np.float32))
+        A64, B64 = r(Ah).astype(np.float64), r(Bh).astype(np.float64)
+        ref = A64.T @ B64 if trans else A64 @ B64
+        np.testing.assert_allclose(host(Y16).reshape((rows, b), order="F"), ref, rtol=1e-4,
+                                   atol=1e-4 * np.abs(ref).max())
+
+
+def test_skinny_gemm16_rejects_unsupported(lib, handle, rng):
+    import torch
+    A = dev(rng.standard_normal(1020 * 30).astype(np.float32))
+    B = dev(rng.standard_normal(1020 * 8).astype(np.float32))
+    Y = torch.zeros(1020 * 8, device="cuda")
+    # A^T B with M % 8 != 0, and FP32
+    for M, K, trans, prec in ((1020, 30, 1, "bf16"), (1020, 30, 0, "fp32")):
+        rc = lib.ht_skinny_gemm16(handle.ptr, ptr(A), M, K, trans, ptr(B), 8, _lib.PRECISION[prec], ptr(Y))
+        assert _lib.STATUS_NAMES[rc] == "HT_ERR_UNSUPPORTED"
 
 
 # FP8 skinny products (OCP E4M3 with a power-of-two scale per operand tile and chunk).
