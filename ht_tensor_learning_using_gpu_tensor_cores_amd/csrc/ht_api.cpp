@@ -212,10 +212,10 @@ ht_status ht_destroy(ht_handle_t h) {
   if (h->xbuf) (void)hipFree(h->xbuf);
   if (h->obuf) (void)hipFree(h->obuf);
   for (auto& e : h->ev) (void)hipEventDestroy(e);
-  if (h->side) {
-    (void)hipStreamSynchronize(h->side);
+  if (h->side[0]) {
+    for (auto& s : h->side) (void)hipStreamSynchronize(s);
     for (auto& e : h->side_ev) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(h->side);
+    for (auto& s : h->side) (void)hipStreamDestroy(s);
   }
   (void)hipStreamDestroy(h->stream);
   delete h;
@@ -314,7 +314,7 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
   e.prof = &h->prof;
   e.pin = h->pinned;
   e.pin_bytes = h->pinned_bytes;
-  e.side = &h->side;
+  e.side = h->side;
   e.side_ev = h->side_ev;
   e.plan = &h->plan;
   e.done_ev = h->ev[1];  // recorded by the run before its final wait

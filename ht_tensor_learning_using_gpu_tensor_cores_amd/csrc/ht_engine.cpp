@@ -601,71 +601,111 @@ static int large_block(int n, int k) { return std::min(std::min(64, std::max(2 *
 // shifted CholeskyQR (zero columns refreshed from Philox stream `stream` + 2 + step).
 int Engine::sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
                            double floor, double* Q0) {
-  ProfScope ps(prof, st, PK_EIG);
-  int* info = alloc<int>(4);
+  LargeEig L;
+  L.st = st;
+  CK(sym_topk_large_begin(L, G, n, k, U, w, tol, maxit, stream, floor, Q0));
+  while (!L.done) CK(sym_topk_large_step(L));
+  return 0;
+}
+
+int Engine::sym_topk_large_begin(LargeEig& L, const double* G, int n, int k, double* U, double* w, double tol,
+                                 int maxit, uint32_t stream, double floor, double* Q0) {
   const int b = large_block(n, k);
-  double* Q = Q0 ? Q0 : alloc<double>((size_t)n * b);
-  double* Y = alloc<double>((size_t)n * b);
-  double* T = alloc<double>((size_t)b * b);
-  double* S = alloc<double>((size_t)b * b);
-  double* th = alloc<double>(b);
-  double* W = alloc<double>((size_t)b * b);
-  double* res = alloc<double>(k);
-  double* Zt = alloc<double>((size_t)n * b);
-  double* Ri = alloc<double>((size_t)b * b);
-  double* ws = alloc<double>((size_t)dgemm_splitk_slices(b, b, n) * b * b);
-  if (!Q || !Y || !T || !S || !th || !W || !res || !Zt || !Ri || !ws) return HT_ERR_ALLOC;
-  if (!Q0) CK(fill_normals_f64_launch(st, Q, (long)n * b, opt.seed, stream));
+  L.G = G;
+  L.U = U;
+  L.w = w;
+  L.n = n;
+  L.k = k;
+  L.b = b;
+  L.maxit = maxit;
+  L.stream = stream;
+  L.tol = tol;
+  L.floor = floor;
+  L.info = alloc<int>(4);
+  L.Q = Q0 ? Q0 : alloc<double>((size_t)n * b);
+  L.Y = alloc<double>((size_t)n * b);
+  L.T = alloc<double>((size_t)b * b);
+  L.S = alloc<double>((size_t)b * b);
+  L.th = alloc<double>(b);
+  L.W = alloc<double>((size_t)b * b);
+  L.res = alloc<double>(k);
+  L.Zt = alloc<double>((size_t)n * b);
+  L.Ri = alloc<double>((size_t)b * b);
+  L.ws = alloc<double>((size_t)dgemm_splitk_slices(b, b, n) * b * b);
+  L.QS = alloc<double>((size_t)n * b);
+  L.YS = alloc<double>((size_t)n * b);
+  if (!L.info || !L.Q || !L.Y || !L.T || !L.S || !L.th || !L.W || !L.res || !L.Zt || !L.Ri || !L.ws || !L.QS ||
+      !L.YS)
+    return HT_ERR_ALLOC;
+  const hipStream_t main_st = st;
+  st = L.st;
+  int rc = Q0 ? 0 : fill_normals_f64_launch(st, L.Q, (long)n * b, opt.seed, stream);
   // the Gaussian block itself starts the iteration (n x b normals are well conditioned:
   // kappa ~ (sqrt n + sqrt b) / (sqrt n - sqrt b)); G Q is orthonormalised by the steps
-  int it = 0, steps = 2;
-  std::vector<double> hres(k), hth(b);
-  bool done = false;
-  while (!done) {
-    for (int s = 0; s < steps; ++s) {
-      CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, G, n, Q, n, 0.0, Y, n));
-      // between Rayleigh-Ritz steps one shifted CholeskyQR pass keeps the block well
-      // conditioned (kappa ~ sqrt(u) kappa(Y)) for the next product; the block the
-      // Rayleigh-Ritz step reads is orthonormalised by all three passes
-      CK(cholqr3(st, Y, Zt, n, b, W, Ri, info, opt.seed, stream + 2 + it, ws, s + 1 == steps ? 3 : 1));
-      std::swap(Q, Y);
-      ++it;
-    }
-    // Rayleigh-Ritz: Y = G Q, T = Q^T Y, T = S diag(th) S^T, U_full = Q S, G U_full = Y S.
-    CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, G, n, Q, n, 0.0, Y, n));
-    CK(dgemm_splitk_launch(st, 1, 0, b, b, n, 1.0, Q, n, Y, n, 0.0, T, b, ws));
-    CK(jacobi_eig_launch(st, T, b, b, th, S, b, info, 60, 1e-15));
-    double* QS = alloc<double>((size_t)n * b);
-    double* YS = alloc<double>((size_t)n * b);
-    if (!QS || !YS) return HT_ERR_ALLOC;
-    CK(dgemm_small_launch(st, 0, 0, n, b, b, 1.0, Q, n, S, b, 0.0, QS, n));
-    CK(dgemm_small_launch(st, 0, 0, n, b, b, 1.0, Y, n, S, b, 0.0, YS, n));
-    CK(ritz_resid_launch(st, YS, QS, th, n, k, res));
-    CK(d2h(hres.data(), res, sizeof(double) * k));
-    CK(d2h(hth.data(), th, sizeof(double) * b));
+  if (!rc) rc = sym_topk_large_round(L);
+  st = main_st;
+  return rc;
+}
+
+// One round on L.st: `steps` products with G, each orthonormalised, then Rayleigh-Ritz
+// (Y = G Q, T = Q^T Y = S diag(th) S^T, QS = Q S, YS = Y S) and the Ritz residuals.
+int Engine::sym_topk_large_round(LargeEig& L) {
+  ProfScope ps(prof, st, PK_EIG);
+  const int n = L.n, b = L.b;
+  for (int s = 0; s < L.steps; ++s) {
+    CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, L.G, n, L.Q, n, 0.0, L.Y, n));
+    // between Rayleigh-Ritz steps one shifted CholeskyQR pass keeps the block well
+    // conditioned (kappa ~ sqrt(u) kappa(Y)) for the next product; the block the
+    // Rayleigh-Ritz step reads is orthonormalised by all three passes
+    CK(cholqr3(st, L.Y, L.Zt, n, b, L.W, L.Ri, L.info, opt.seed, L.stream + 2 + L.it, L.ws,
+               s + 1 == L.steps ? 3 : 1));
+    std::swap(L.Q, L.Y);
+    ++L.it;
+  }
+  CK(dgemm_small_launch(st, 0, 0, n, b, n, 1.0, L.G, n, L.Q, n, 0.0, L.Y, n));
+  CK(dgemm_splitk_launch(st, 1, 0, b, b, n, 1.0, L.Q, n, L.Y, n, 0.0, L.T, b, L.ws));
+  CK(jacobi_eig_launch(st, L.T, b, b, L.th, L.S, b, L.info, 60, 1e-15));
+  CK(dgemm_small_launch(st, 0, 0, n, b, b, 1.0, L.Q, n, L.S, b, 0.0, L.QS, n));
+  CK(dgemm_small_launch(st, 0, 0, n, b, b, 1.0, L.Y, n, L.S, b, 0.0, L.YS, n));
+  CK(ritz_resid_launch(st, L.YS, L.QS, L.th, n, L.k, L.res));
+  return 0;
+}
+
+int Engine::sym_topk_large_step(LargeEig& L) {
+  const hipStream_t main_st = st;
+  st = L.st;
+  auto body = [&]() -> int {
+    const int k = L.k, b = L.b, n = L.n;
+    std::vector<double> hres(k), hth(b);
+    CK(d2h(hres.data(), L.res, sizeof(double) * k));
+    CK(d2h(hth.data(), L.th, sizeof(double) * b));
     if (!std::isfinite(hth[0])) return HT_ERR_NONFINITE;  // a NaN/Inf Gram never converges
     const double scale = std::max(std::fabs(hth[0]), 1e-300);
     double rmax = 0.0;
     for (int i = 0; i < k; ++i)
-      if (std::fabs(hth[i]) > floor * scale) rmax = std::max(rmax, hres[i]);
-    done = (rmax <= tol * scale) || (it >= maxit) || (hth[0] == 0.0);
-    std::swap(Q, QS);  // continue from the Ritz basis
-    if (done) {
-      HT_HIP_CHECK(hipMemcpyAsync(U, Q, sizeof(double) * n * k, hipMemcpyDeviceToDevice, st));
-      HT_HIP_CHECK(hipMemcpyAsync(w, th, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
-      if (rmax > tol * scale && hth[0] != 0.0) {
+      if (std::fabs(hth[i]) > L.floor * scale) rmax = std::max(rmax, hres[i]);
+    L.done = (rmax <= L.tol * scale) || (L.it >= L.maxit) || (hth[0] == 0.0);
+    std::swap(L.Q, L.QS);  // continue from the Ritz basis
+    if (L.done) {
+      HT_HIP_CHECK(hipMemcpyAsync(L.U, L.Q, sizeof(double) * n * k, hipMemcpyDeviceToDevice, st));
+      HT_HIP_CHECK(hipMemcpyAsync(L.w, L.th, sizeof(double) * k, hipMemcpyDeviceToDevice, st));
+      if (rmax > L.tol * scale && hth[0] != 0.0) {
         noconv = 1;
         if (getenv("HT_DEBUG"))
           fprintf(stderr, "ht: sym_topk_large n=%d k=%d: residual %.3e > %.3e after %d iterations\n", n, k, rmax,
-                  tol * scale, it);
+                  L.tol * scale, L.it);
       }
+      CK(canon_signs(L.U, n, k, n));
+      eig_iters_max = std::max(eig_iters_max, L.it);
+      ++eig_calls;
+      return 0;
     }
-    steps = std::min(8, steps * 2);
-  }
-  CK(canon_signs(U, n, k, n));
-  eig_iters_max = std::max(eig_iters_max, it);
-  ++eig_calls;
-  return 0;
+    L.steps = std::min(8, L.steps * 2);
+    return sym_topk_large_round(L);
+  };
+  const int rc = body();
+  st = main_st;
+  return rc;
 }
 
 // Canonical signs of the k columns of U (largest-magnitude entry positive), with the
@@ -965,30 +1005,33 @@ int Engine::leaves_from_x_plain(const float* x, double* xsq_dev) {
   return leaves_from_x(x, PREC_F32, xsq_dev);
 }
 
-// The side stream and its events exist (made here on first use; never with a fixed
+// The side streams and their events exist (made here on first use; never with a fixed
 // arena, whose callers may not allow resources to be created).
 bool Engine::side_ready() {
   if (!side || !side_ev || arena->fixed) return false;
-  if (*side) return true;
-  if (hipStreamCreateWithFlags(side, hipStreamNonBlocking) != hipSuccess) {
-    *side = nullptr;
+  if (side[0]) return true;
+  auto undo = [&](int ns, int ne) {
+    for (int j = 0; j < ne; ++j) (void)hipEventDestroy(side_ev[j]);
+    for (int j = 0; j < ns; ++j) (void)hipStreamDestroy(side[j]);
+    for (int j = 0; j < NSIDE; ++j) side[j] = nullptr;
     return false;
-  }
-  for (int i = 0; i < SIDE_EVENTS; ++i)
-    if (hipEventCreateWithFlags(&side_ev[i], hipEventDisableTiming) != hipSuccess) {
-      for (int j = 0; j < i; ++j) (void)hipEventDestroy(side_ev[j]);
-      (void)hipStreamDestroy(*side);
-      *side = nullptr;
-      return false;
+  };
+  for (int i = 0; i < NSIDE; ++i)
+    if (hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking) != hipSuccess) {
+      side[i] = nullptr;
+      return undo(i, 0);
     }
+  for (int i = 0; i < SIDE_EVENTS; ++i)
+    if (hipEventCreateWithFlags(&side_ev[i], hipEventDisableTiming) != hipSuccess) return undo(NSIDE, i);
   return true;
 }
 
 // Leaf bases from Grams of X (one pass per leaf), eigen problems of equal shape batched.
-// Leaves of more than 256 rows are solved one by one on the side stream, each as soon as
-// its Gram is done, beside the Grams that follow it on st (a Gram kernel leaves CUs free;
-// the solver's kernels are single work-groups and host-checked). The results equal those
-// of the same solves on st.
+// Leaves of more than 256 rows are solved on the side streams, each starting when its
+// Gram is done, beside the Grams that follow it on st (a Gram kernel leaves CUs free;
+// the solver's kernels are single work-groups and host-checked). With defer, the solves
+// wait for solve_pending_leaves, so that the work after the Grams is enqueued on st
+// first. The results equal those of the same solves on st.
 int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev, bool defer) {
   const Problem& p = *prob;
   std::vector<int> leaves;
@@ -1063,28 +1106,40 @@ int Engine::leaves_from_x(const float* x, int prec, double* xsq_dev, bool defer)
   return 0;
 }
 
-// The side-stream solves of one group of large leaves (each after its Gram's event), then
-// st waits for them. The same solves (and Philox streams) as sym_topk_batch's
-// one-at-a-time path.
+// The side-stream solves of one group of large leaves, leaf i on side stream i % NSIDE
+// after its Gram's event, then st waits for them all. Every solve's first round is
+// enqueued before the host reads any residual, so the solves run side by side; the host
+// then steps each until it converges. The same solves (and Philox streams) as
+// sym_topk_batch's one-at-a-time path.
 int Engine::solve_large_leaves(const LeafSolves& j) {
   const int nb = (int)j.grp.size();
-  const hipStream_t main_st = st;
-  st = *side;
+  std::vector<LargeEig> L(nb);
   int rc = 0;
   for (int i = 0; i < nb && !rc; ++i) {
-    if (hipStreamWaitEvent(st, side_ev[j.ev0 + i], 0) != hipSuccess) rc = HT_ERR_HIP;
+    L[i].st = side[i % NSIDE];
+    if (hipStreamWaitEvent(L[i].st, side_ev[j.ev0 + i], 0) != hipSuccess) rc = HT_ERR_HIP;
     else
-      rc = sym_topk_large(j.G + (size_t)i * j.n * j.n, (int)j.n, j.k, j.U + (size_t)i * j.n * j.k,
-                          j.w + (size_t)i * j.k, 1e-9, opt.max_eig_iters, 1000 + j.grp[0] * 16 + 8 * i, GRAM_FLOOR,
-                          j.Q0[i]);
+      rc = sym_topk_large_begin(L[i], j.G + (size_t)i * j.n * j.n, (int)j.n, j.k, j.U + (size_t)i * j.n * j.k,
+                                j.w + (size_t)i * j.k, 1e-9, opt.max_eig_iters, 1000 + j.grp[0] * 16 + 8 * i,
+                                GRAM_FLOOR, j.Q0[i]);
   }
-  const bool joined = hipEventRecord(side_ev[HT_MAX_ORDER], st) == hipSuccess;
-  st = main_st;
-  if (!joined || hipStreamWaitEvent(st, side_ev[HT_MAX_ORDER], 0) != hipSuccess) {
-    (void)hipStreamSynchronize(*side);
+  for (bool more = !rc; more && !rc;) {
+    more = false;
+    for (int i = 0; i < nb && !rc; ++i)
+      if (!L[i].done) {
+        rc = sym_topk_large_step(L[i]);
+        more = more || !L[i].done;
+      }
+  }
+  bool joined = !rc;
+  for (int s = 0; s < std::min(nb, NSIDE) && joined; ++s)
+    joined = hipEventRecord(side_ev[HT_MAX_ORDER + s], side[s]) == hipSuccess &&
+             hipStreamWaitEvent(st, side_ev[HT_MAX_ORDER + s], 0) == hipSuccess;
+  if (!joined) {  // nothing of the side streams may outlive a failed call
+    for (int s = 0; s < NSIDE; ++s) (void)hipStreamSynchronize(side[s]);
     return rc ? rc : HT_ERR_HIP;
   }
-  return rc;
+  return 0;
 }
 
 int Engine::solve_pending_leaves() {

@@ -100,8 +100,26 @@ struct RunPlan {
   bool valid = false;
 };
 
-// Events of the side stream: one per leaf whose Gram it waits for, and the join.
-constexpr int SIDE_EVENTS = HT_MAX_ORDER + 1;
+// Side streams for the large leaves' eigen problems (leaf i on stream i % NSIDE) and
+// their events: one per leaf whose Gram a solve waits for, and one join per stream.
+constexpr int NSIDE = 3;
+constexpr int SIDE_EVENTS = HT_MAX_ORDER + NSIDE;
+
+// One large-leaf subspace iteration in progress (sym_topk_large_begin / _step): its
+// stream, buffers and the host state of its rounds.
+struct LargeEig {
+  hipStream_t st = nullptr;
+  const double* G = nullptr;
+  double *U = nullptr, *w = nullptr;
+  int n = 0, k = 0, b = 0, maxit = 0;
+  uint32_t stream = 0;
+  double tol = 0.0, floor = 0.0;
+  double *Q = nullptr, *Y = nullptr, *T = nullptr, *S = nullptr, *th = nullptr, *W = nullptr, *res = nullptr,
+         *Zt = nullptr, *Ri = nullptr, *ws = nullptr, *QS = nullptr, *YS = nullptr;
+  int* info = nullptr;
+  int it = 0, steps = 2;
+  bool done = false;
+};
 
 struct Engine {
   Profiler* prof = nullptr;
@@ -160,9 +178,9 @@ struct Engine {
   // copies into pageable memory complete synchronously one by one
   char* pin = nullptr;
   size_t pin_bytes = 0;
-  // The handle's side stream and its events (SIDE_EVENTS), made on first use: the eigen
-  // problems of large leaves run there beside the remaining Grams on st. Null: no side
-  // stream (they run on st).
+  // The handle's NSIDE side streams and their events (SIDE_EVENTS), made on first use:
+  // the eigen problems of large leaves run there beside the work that follows their
+  // Grams on st. Null: no side streams (they run on st).
   hipStream_t* side = nullptr;
   hipEvent_t* side_ev = nullptr;
   // record / replay of the host decisions (leaves-to-root only)
@@ -197,6 +215,13 @@ struct Engine {
   int canon_signs(double* U, int m, int k, int ldu);
   int sym_topk_large(const double* G, int n, int k, double* U, double* w, double tol, int maxit, uint32_t stream,
                      double floor, double* Q0 = nullptr);
+  // The same solve in pieces, on stream L.st: _begin allocates and enqueues the first
+  // round; _step reads a round's residuals (waits for L.st) and enqueues the next round,
+  // or the result copies once converged (L.done).
+  int sym_topk_large_begin(LargeEig& L, const double* G, int n, int k, double* U, double* w, double tol, int maxit,
+                           uint32_t stream, double floor, double* Q0);
+  int sym_topk_large_step(LargeEig& L);
+  int sym_topk_large_round(LargeEig& L);
   int tsqr(float* Y, long m, int b, float* Q, double* R);
   int tall_orth(float* Y, long m, int b, float* Q, int passes = 3);
   bool side_ready();
@@ -238,8 +263,8 @@ struct ht_handle_s {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {};
-  // side stream for large leaves' eigen problems (Engine::side), made on first use
-  hipStream_t side = nullptr;
+  // side streams for large leaves' eigen problems (Engine::side), made on first use
+  hipStream_t side[ht::NSIDE] = {};
   hipEvent_t side_ev[ht::SIDE_EVENTS] = {};
   ht::Arena arena;
   float* xbuf = nullptr;   // device copy of host inputs
