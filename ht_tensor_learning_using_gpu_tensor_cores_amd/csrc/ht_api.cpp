@@ -8,6 +8,7 @@
 #include <chrono>
 #include <mutex>
 #include <thread>
+#include <memory>
 #include <vector>
 
 #include "../../include/ht_amd_debug.h"
@@ -212,6 +213,9 @@ ht_status ht_destroy(ht_handle_t h) {
   if (h->xbuf) (void)hipFree(h->xbuf);
   if (h->obuf) (void)hipFree(h->obuf);
   for (auto& e : h->ev) (void)hipEventDestroy(e);
+  for (auto& ew : h->stagger_ev)
+    for (auto& e : ew)
+      if (e) (void)hipEventDestroy(e);
   if (h->side[0]) {
     for (auto& s : h->side) (void)hipStreamSynchronize(s);
     for (auto& e : h->side_ev) (void)hipEventDestroy(e);
@@ -261,7 +265,8 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
                                 const int32_t* left, const int32_t* right, const int32_t* ranks,
                                 const ht_options* opt_in, float* out, int32_t out_on_device, double* rel_err,
                                 ht_stats* stats, Arena* arena, double rel_eps = 0.0, int32_t max_rank = 0,
-                                int32_t* ranks_out = nullptr) {
+                                int32_t* ranks_out = nullptr, hipEvent_t pass_ev = nullptr,
+                                std::atomic<int>* pass_flag = nullptr, int pass_mark = 1) {
   if (!h || !x) return HT_ERR_INVALID_ARG;
   if (rel_err) *rel_err = NAN;
   ht_options opt;
@@ -320,6 +325,9 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
   e.done_ev = h->ev[1];  // recorded by the run before its final wait
   e.adaptive = adaptive;
   e.adapt_eps = rel_eps;
+  e.pass_ev = pass_ev;
+  e.pass_flag = pass_flag;
+  e.pass_mark = pass_mark;
   if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
   int s = e.run(xd, od);
   if (s && s != HT_ERR_NOCONV) {
@@ -406,6 +414,16 @@ ht_status ht_decompose_many(ht_handle_t h, int32_t count, const float* const* xs
   if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
   for (int w = 0; w + 1 < inflight; ++w)
     if (hipStreamWaitEvent(h->workers[w]->stream, h->ev[0], 0) != hipSuccess) return HT_ERR_HIP;
+  for (int w = 0; w + 1 < inflight; ++w)
+    for (auto& ev : h->stagger_ev[w])
+      if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return HT_ERR_HIP;
+  // Staggered workers: worker w's decomposition of round r (its r-th) waits for the first
+  // pass over X of worker w - 1's decomposition of round r, so that one worker's passes
+  // run beside another's solver phases. Workers left to themselves fall into step (every
+  // pass at once, then every solver with HBM idle). passed[w] = rounds of worker w whose
+  // first pass is enqueued (or that ended).
+  std::unique_ptr<std::atomic<int>[]> passed(new std::atomic<int>[inflight]);
+  for (int w = 0; w < inflight; ++w) passed[w].store(0);
   std::atomic<int> first_err{0};
   std::vector<ht_stats> wst(inflight);
   const auto t0 = std::chrono::steady_clock::now();
@@ -421,8 +439,16 @@ ht_status ht_decompose_many(ht_handle_t h, int32_t count, const float* const* xs
       double e = NAN;
       ht_stats st;
       std::memset(&st, 0, sizeof(st));
+      const int r = i / inflight;
+      if (w > 0) {
+        while (passed[w - 1].load() <= r && first_err.load() == 0) std::this_thread::yield();
+        (void)hipStreamWaitEvent(wh->stream, h->stagger_ev[w - 1][r % 4], 0);
+      }
+      hipEvent_t pev = w + 1 < inflight ? h->stagger_ev[w][r % 4] : nullptr;
       const ht_status s = decompose_impl(wh, xs[i], 1, order, dims, n_nodes, lo, hi, left, right, ranks, opt, outs[i],
-                                         1, &e, &st, &wh->arena);
+                                         1, &e, &st, &wh->arena, 0.0, 0, nullptr, pev, pev ? &passed[w] : nullptr,
+                                         r + 1);
+      if (passed[w].load() < r + 1) passed[w].store(r + 1);
       if (rel_errs) rel_errs[i] = e;
       if (s != HT_OK && s != HT_ERR_NOCONV) {
         int z = 0;
@@ -434,6 +460,7 @@ ht_status ht_decompose_many(ht_handle_t h, int32_t count, const float* const* xs
       wst[w].workspace_bytes = std::max(wst[w].workspace_bytes, st.workspace_bytes);
       wst[w].replayed += st.replayed;
     }
+    passed[w].store(1 << 30);  // a worker that is done never holds the next one back
   };
   std::vector<std::thread> threads;
   for (int w = 1; w < inflight; ++w) threads.emplace_back(work, w);

@@ -250,6 +250,15 @@ int Engine::d2h(void* dst, const void* src, size_t bytes) {
   return 0;
 }
 
+void Engine::note_pass(double amount) {
+  passes += amount;
+  if (pass_ev) {
+    (void)hipEventRecord(pass_ev, st);
+    if (pass_flag) pass_flag->store(pass_mark);
+    pass_ev = nullptr;
+  }
+}
+
 // Eigenvalues of a Gram accumulated in FP32 (or from 16-bit inputs) below this fraction
 // of the largest are not resolved by it: their Ritz vectors are not refined further.
 static constexpr double GRAM_FLOOR = 1e-6;
@@ -275,7 +284,7 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
       CK(gram16_launch(st, x16, v, prec, ns, part));
     }
     CK(gram_reduce_launch(st, part, (int)v.M, ns, G, 256));
-    passes += 0.5;
+    note_pass(0.5);
     return 0;
   }
   const int ns = gram_pms_nsplit(x, v, prec);
@@ -290,7 +299,7 @@ int Engine::gram(const float* x, PMS v, int prec, double* G, double* sumsq) {
   }
   CK(gram_reduce_launch(st, part, (int)v.M, ns, G, tile));
   if (sumsq) CK(gram_reduce_norms_launch(st, norms, pairs * ns, sumsq));
-  passes += 1.0;
+  note_pass(1.0);
   return 0;
 }
 
@@ -329,7 +338,7 @@ int Engine::make_x16(const float* x, int prec, double* xsq) {
   }
   x16 = y;
   x16_prec = prec;
-  passes += 1.0;
+  note_pass(1.0);
   return 0;
 }
 
@@ -350,7 +359,7 @@ int Engine::gram2(const float* x, PMS v1, PMS v2, int prec, double* G1, double* 
   }
   CK(gram_reduce2_launch(st, part[0], (int)v1.M, ns[0], G1, part[1], (int)v2.M, ns[1], G2, nullptr, 0, nullptr,
                          tile[0], tile[1]));
-  passes += 2.0;
+  note_pass(2.0);
   return 0;
 }
 
@@ -788,7 +797,7 @@ int Engine::view_gemm(const float* x, PMS v, int op, const float* B, int b, int 
     if (h && gemm16_ok(x16, M, B, atb ? 1 : 0)) {
       if (atb) CK(gemm_atb16_launch(st, x16, M, K, B, b, prec, Y, ws, wsf));
       else CK(gemm_ab16_launch(st, x16, M, K, B, b, prec, Y, ws, wsf));
-      passes += 0.5;
+      note_pass(0.5);
       return 0;
     }
     if (atb) CK(gemm_atb_launch(st, x, M, K, B, b, prec, Y, ws, wsf));
@@ -796,7 +805,7 @@ int Engine::view_gemm(const float* x, PMS v, int op, const float* B, int b, int 
   } else {
     return HT_ERR_UNSUPPORTED;
   }
-  passes += 1.0;
+  note_pass(1.0);
   return 0;
 }
 
@@ -983,7 +992,7 @@ int Engine::residual(const float* x, double* err_sq, double* x_sq) {
     ProfScope ps(prof, st, PK_RESIDUAL);
     CK(lowrank_launch(st, 1, Lf, Rf, nl, nr, kl, x, nullptr, acc, &nb));
   }
-  passes += 1.0;
+  note_pass(1.0);
   if (deferred) {
     // into the head of the verdict block, read with it at the end of the run
     CK(sum_strided_launch(st, acc, nb, reinterpret_cast<double*>(verdict) - 4));
@@ -1393,7 +1402,7 @@ int Engine::run_ltr(const float* x) {
       CK(slab_gram_launch(st, x, (int)n0, (int)n1, S, prec, nb, p1, p2, norms));
     }
     CK(gram_reduce2_launch(st, p1, (int)n0, nb, G0, p2, (int)n1, nb, G1, norms, nb, xsq));
-    passes += 1.0;
+    note_pass(1.0);
   } else {
     if (n0 > 1024 || n1 > 1024) return HT_ERR_UNSUPPORTED;
     CK(gram(x, view_of(p.dims, d, 0, 1), prec, G0, xsq));
@@ -1438,7 +1447,7 @@ int Engine::run_ltr(const float* x) {
     CK(proj_pms_f32_launch(st, x, PMS{1, n0, n1 * S}, U0f, r0, Z1));
     CK(proj_pms_f32_launch(st, Z1, PMS{r0, n1, S}, U1f, r1, Z));
   }
-  passes += 1.0;
+  note_pass(1.0);
   // Remaining leaves from Z; then the core C (FP64).
   int64_t zd[HT_MAX_ORDER];
   zd[0] = r0;

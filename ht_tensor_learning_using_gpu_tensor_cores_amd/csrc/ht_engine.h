@@ -1,5 +1,6 @@
 // Internal host-side types of the HT engine.
 #pragma once
+#include <atomic>
 #include <map>
 #include <utility>
 #include <vector>
@@ -146,6 +147,13 @@ struct Engine {
   double* SVd[HT_MAX_NODES];
   int SVsq[HT_MAX_NODES] = {};  // 1: SVd[t] holds eigenvalues (squared singular values)
   double passes = 0.0;
+  // recorded on st after the run's first pass over X when set, then *pass_flag =
+  // pass_mark (ht_decompose_many starts each decomposition of a worker after the first
+  // pass of the previous worker's decomposition of the same round)
+  hipEvent_t pass_ev = nullptr;
+  std::atomic<int>* pass_flag = nullptr;
+  int pass_mark = 1;
+  void note_pass(double amount);
   int eig_iters_max = 0, eig_calls = 0, noconv = 0;
   int orth_fallbacks = 0;  // tall_orth calls that took the TSQR basis
   // 16-bit copy of X (values rounded to precision x16_prec) that the Grams of the views
@@ -265,6 +273,9 @@ struct ht_handle_s {
   hipEvent_t ev[4] = {};
   // side streams for large leaves' eigen problems (Engine::side), made on first use
   hipStream_t side[ht::NSIDE] = {};
+  // ht_decompose_many: worker w records stagger_ev[w][r % 4] after the first pass over X
+  // of its decomposition of round r
+  hipEvent_t stagger_ev[HT_MAX_INFLIGHT][4] = {};
   hipEvent_t side_ev[ht::SIDE_EVENTS] = {};
   ht::Arena arena;
   float* xbuf = nullptr;   // device copy of host inputs
