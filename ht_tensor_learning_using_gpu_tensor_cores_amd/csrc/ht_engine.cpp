@@ -1329,6 +1329,31 @@ int Engine::root_transfer(const float* x) {
   return 0;
 }
 
+// B_root = U_l^T X_(l) U_r from the co-range product of the root child c whose range
+// finder ran: with A = X_(c), Z = A^T Q and U_c = Q S' (S' = Q^T U_c, l x k_c: the
+// eigenvectors of Z^T Z with U_c's canonical signs), A^T U_c = Z S'. c = r gives
+// X_(l) U_r = Z S', so B = U_l^T (Z S'); c = l gives X_(l)^T U_l = Z S', so
+// B = (Z S')^T U_r. Used when Z is an FP32 product (the FP32 precision), in place of the
+// pass over X that root_transfer makes.
+int Engine::root_transfer_from_z() {
+  const Problem& p = *prob;
+  const int l = p.left[0], r = p.right[0], c = rz_t;
+  const int kl = p.rank[l], kr = p.rank[r], kc = p.rank[c];
+  const long no = rz_rows;  // rows of the other child = rows of Z
+  double* Zd = alloc<double>((size_t)no * rz_l);
+  double* W = alloc<double>((size_t)no * kc);
+  double* ws = alloc<double>((size_t)dgemm_splitk_slices(kl, kr, (int)no) * kl * kr);
+  Bd[0] = alloc<double>((size_t)kl * kr);
+  if (!Zd || !W || !ws || !Bd[0]) return HT_ERR_ALLOC;
+  CK(cast_f32_f64_launch(st, rz_Z, Zd, no * rz_l));
+  CK(dgemm_small_launch(st, 0, 0, (int)no, kc, rz_l, 1.0, Zd, (int)no, rz_Sv, rz_l, 0.0, W, (int)no));
+  if (c == r)
+    CK(dgemm_splitk_launch(st, 1, 0, kl, kr, (int)no, 1.0, Ufull[l], (int)no, W, (int)no, 0.0, Bd[0], kl, ws));
+  else
+    CK(dgemm_splitk_launch(st, 1, 0, kl, kr, (int)no, 1.0, W, (int)no, Ufull[r], (int)no, 0.0, Bd[0], kl, ws));
+  return 0;
+}
+
 int Engine::finish_error(const float* x, const double* xsq_dev) {
   rel_err = NAN;
   if (deferred) {
@@ -1866,12 +1891,27 @@ int Engine::rsvd_pass(const float* x) {
     CK(tall_gram_launch(st, Z, l, Z, l, cols, gpart, M));
     CK(jacobi_eig_launch(st, M, l, l, th, Sv, l, info, 60, 1e-15));
     ++eig_calls;
+
     // Left vectors of A: Q Sv; right vectors: Qz Rz Sv / sigma.
     double* Qd = alloc<double>((size_t)rows * l);
     Ufull[t] = alloc<double>((size_t)rows * k);
     CK(cast_f32_f64_launch(st, Q, Qd, rows * l));
     CK(dgemm_small_launch(st, 0, 0, (int)rows, k, l, 1.0, Qd, (int)rows, Sv, l, 0.0, Ufull[t], (int)rows));
     CK(canon_signs(Ufull[t], (int)rows, k, (int)rows));
+    if (prec == PREC_F32 && p.parent[t] == 0 && cols <= INT32_MAX && rows <= INT32_MAX) {
+      // an FP32 co-range product serves the root transfer (16-bit ones leave B_root at
+      // their rounding, which small tensors do not average out: it takes its own FP32 pass)
+      // U_t = Q S' with S' = Q^T U_t (l x k): Sv's first k columns with the canonical signs
+      double* Sp = alloc<double>((size_t)l * k);
+      double* ws = alloc<double>((size_t)dgemm_splitk_slices(l, k, (int)rows) * l * k);
+      if (!Sp || !ws) return HT_ERR_ALLOC;
+      CK(dgemm_splitk_launch(st, 1, 0, l, k, (int)rows, 1.0, Qd, (int)rows, Ufull[t], (int)rows, 0.0, Sp, l, ws));
+      rz_t = t;
+      rz_l = l;
+      rz_rows = cols;
+      rz_Z = Z;
+      rz_Sv = Sp;
+    }
     double* sv = alloc<double>(l);
     CK(sqrt_clamp_launch(st, th, l, sv));
     SVd[t] = sv;
@@ -1900,7 +1940,7 @@ int Engine::rsvd_pass(const float* x) {
     Bd[t] = alloc<double>((size_t)p.rank[l] * p.rank[r] * p.rank[t]);
     CK(contract_transfer(Ufull[t], Ufull[l], Ufull[r], p.rows[l], p.rows[r], p.rank[l], p.rank[r], p.rank[t], Bd[t]));
   }
-  CK(root_transfer(x));
+  CK(rz_t >= 0 ? root_transfer_from_z() : root_transfer(x));
   return finish_error(x, xsq);
 }
 
@@ -1941,6 +1981,7 @@ void Engine::reset_run_state() {
   sketch_resid = 0.0;
   x16 = nullptr;
   pending_leaves.clear();
+  rz_t = -1;
 }
 
 int Engine::run_once(const float* x, float* out) {

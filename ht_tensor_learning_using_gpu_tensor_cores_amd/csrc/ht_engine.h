@@ -245,6 +245,14 @@ struct Engine {
   int eig_to_sv(const double* w, int t);
   int early_finite_check(const double* xsq_dev);
   int root_transfer(const float* x);
+  // Randomised method in FP32: the root child rz_t whose range finder ran keeps its
+  // co-range product Z = X_(rz_t)^T Q (rz_rows x rz_l) and S' = Q^T U_rz_t (rz_l x k);
+  // the root transfer then needs no pass over X.
+  int rz_t = -1, rz_l = 0;
+  long rz_rows = 0;
+  const float* rz_Z = nullptr;
+  const double* rz_Sv = nullptr;
+  int root_transfer_from_z();
   int finish_error(const float* x, const double* xsq_dev);
   int run_plain(const float* x);
   int run_ltr(const float* x);

@@ -125,6 +125,31 @@ def test_3d_against_oracle(dims, core, kind, ranks, method):
     check_against_oracle(method, x, list(dims), make_tree(kind), ranks, list(core))
 
 
+# In FP32 the randomised method takes the root transfer from the root child's co-range
+# product Z = X_(c)^T Q (B_root = U_l^T (Z Q^T U_r) or its transpose) instead of a pass
+# over X: it must equal the projection U_l^T X_(l) U_r onto the returned bases (FP64 on
+# the CPU), for an interior root child on the left, on the right and on both sides. The
+# ranks are the signal ranks and the noise 1e-6, so that the interior bases lie in the
+# span of their children's (the returned factors carry that projection of them).
+@pytest.mark.parametrize("dims,core,kind,ranks", [((32, 24, 20), (4, 5, 3), "ref", [1, 3, 3, 4, 5]),
+                                                  ((48, 32, 16), (8, 8, 8), "first", [1, 8, 8, 8, 8]),
+                                                  ((20, 24, 28, 16), (4, 4, 4, 4), "bal4",
+                                                   [1, 16, 16, 4, 4, 4, 4])])
+def test_rsvd_fp32_root_transfer_is_the_projection(dims, core, kind, ranks):
+    x = G.tucker(list(dims), list(core), seed=3, noise_rel=1e-6)
+    otr = as_oracle_tree(make_tree(kind))
+    print(f"rsvd fp32 root transfer dims={dims} tree={kind}", flush=True)
+    res, _ = run(x, list(dims), make_tree(kind), ranks, method="rsvd", precision="fp32")
+    h = oracle_from_result(res, list(dims), otr)
+    l, r = otr.left[0], otr.right[0]
+    Ul, Ur = h.full_basis(l), h.full_basis(r)
+    proj = Ul.T @ O.unfold(x.astype(np.float64), list(dims), otr.lo[l], otr.hi[l]) @ Ur
+    Bg = h.B[0][:, :, 0]
+    np.testing.assert_allclose(Bg, proj, rtol=0, atol=1e-5 * np.abs(proj).max())
+    # and the error the engine reports is that of the returned factors
+    assert abs(O.rel_error(x, h.reconstruct()) - res.rel_err) <= 1e-6
+
+
 @pytest.mark.parametrize("n,core,r", [(24, 8, 8), (32, 8, 6), (40, 16, 16)])
 @pytest.mark.parametrize("method", ["plain", "ltr", "rsvd"])
 def test_4d_against_oracle(n, core, r, method):
