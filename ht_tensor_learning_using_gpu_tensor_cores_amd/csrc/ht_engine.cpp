@@ -1309,11 +1309,29 @@ int Engine::run_plain(const float* x) {
   return finish_error(x, xsq);
 }
 
+// B_root = U_l^T X_(l) U_r with one FP32 pass over X. The pass produces the product on
+// the shorter side: X_(l) U_r (nl x kr) when the right child has more rows, then
+// B = U_l^T (X_(l) U_r) over nl rows; else X_(l)^T U_l (nr x kl), then B = (X^T U_l)^T U_r
+// over nr rows.
 int Engine::root_transfer(const float* x) {
   const Problem& p = *prob;
   const int l = p.left[0], r = p.right[0];
   const long nl = p.rows[l], nr = p.rows[r];
   const int kl = p.rank[l], kr = p.rank[r];
+  if (nr > nl) {
+    float* Urf = alloc<float>((size_t)nr * kr);
+    float* W = alloc<float>((size_t)nl * kr);
+    double* Wd = alloc<double>((size_t)nl * kr);
+    double* ws = alloc<double>((size_t)dgemm_splitk_slices(kl, kr, (int)nl) * kl * kr);
+    Bd[0] = alloc<double>((size_t)kl * kr);
+    if (!Urf || !W || !Wd || !ws || !Bd[0]) return HT_ERR_ALLOC;
+    CK(cast_f64_f32_launch(st, Ufull[r], Urf, nr * kr));
+    const PMS v{1, nl, nr};  // X as (nl x nr), A U_r
+    CK(view_gemm(x, v, 0, Urf, kr, PREC_F32, W));
+    CK(cast_f32_f64_launch(st, W, Wd, nl * kr));
+    CK(dgemm_splitk_launch(st, 1, 0, kl, kr, (int)nl, 1.0, Ufull[l], (int)nl, Wd, (int)nl, 0.0, Bd[0], kl, ws));
+    return 0;
+  }
   float* Ulf = alloc<float>((size_t)nl * kl);
   float* Y = alloc<float>((size_t)nr * kl);
   double* Yd = alloc<double>((size_t)nr * kl);
