@@ -266,7 +266,8 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
                                 const ht_options* opt_in, float* out, int32_t out_on_device, double* rel_err,
                                 ht_stats* stats, Arena* arena, double rel_eps = 0.0, int32_t max_rank = 0,
                                 int32_t* ranks_out = nullptr, hipEvent_t pass_ev = nullptr,
-                                std::atomic<int>* pass_flag = nullptr, int pass_mark = 1) {
+                                std::atomic<int>* pass_flag = nullptr, int pass_mark = 1,
+                                bool shared_gpu = false) {
   if (!h || !x) return HT_ERR_INVALID_ARG;
   if (rel_err) *rel_err = NAN;
   ht_options opt;
@@ -328,6 +329,7 @@ static ht_status decompose_impl(ht_handle_t h, const float* x, int32_t x_on_devi
   e.pass_ev = pass_ev;
   e.pass_flag = pass_flag;
   e.pass_mark = pass_mark;
+  e.shared_gpu = shared_gpu;
   if (hipEventRecord(h->ev[0], h->stream) != hipSuccess) return HT_ERR_HIP;
   int s = e.run(xd, od);
   if (s && s != HT_ERR_NOCONV) {
@@ -447,7 +449,7 @@ ht_status ht_decompose_many(ht_handle_t h, int32_t count, const float* const* xs
       hipEvent_t pev = w + 1 < inflight ? h->stagger_ev[w][r % 4] : nullptr;
       const ht_status s = decompose_impl(wh, xs[i], 1, order, dims, n_nodes, lo, hi, left, right, ranks, opt, outs[i],
                                          1, &e, &st, &wh->arena, 0.0, 0, nullptr, pev, pev ? &passed[w] : nullptr,
-                                         r + 1);
+                                         r + 1, inflight > 1);
       if (passed[w].load() < r + 1) passed[w].store(r + 1);
       if (rel_errs) rel_errs[i] = e;
       if (s != HT_OK && s != HT_ERR_NOCONV) {

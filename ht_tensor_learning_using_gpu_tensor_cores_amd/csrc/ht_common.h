@@ -188,8 +188,9 @@ int gram16_launch(hipStream_t st, const uint16_t* x, PMS v, int prec, int n_spli
 int slab_gram_launch(hipStream_t st, const float* x, int n1, int n2, long S, int prec, int n_blocks,
                      float* part1, float* part2, double* norms);
 // PROJ: z[a + r1*b + r1*r2*s] = sum_ij U1[i,a] X[i,j,s] U2[j,b]  (r1, r2 <= 64, FP32).
+// max_blocks: the grid's cap (3 blocks per CU fit; 768 fills the chip)
 int slab_proj_launch(hipStream_t st, const float* x, int n1, int n2, long S, const float* U1, int r1,
-                     const float* U2, int r2, float* z);
+                     const float* U2, int r2, float* z, int max_blocks = 768);
 
 // Generic mode projection y = x x_t U^T on a (P,M,S) view: y[p + P*j + P*r*s] =
 // sum_m U[m + M*j] x[p + P*m + P*M*s]. T = float or double.

@@ -1435,7 +1435,7 @@ int Engine::run_ltr(const float* x) {
   double* G1 = G0 + (size_t)n0 * n0;
   if (!G0 || !G1) return HT_ERR_ALLOC;
   if (slab) {
-    int nb = (int)std::min<long>(S, 256);
+    int nb = (int)std::min<long>(S, shared_gpu ? 224 : 256);
     float* p1 = alloc<float>((size_t)nb * 128 * 128);
     float* p2 = alloc<float>((size_t)nb * 128 * 128);
     double* norms = alloc<double>(nb);
@@ -1483,7 +1483,7 @@ int Engine::run_ltr(const float* x) {
   }
   if (slab) {
     ProfScope ps(prof, st, PK_SLAB_PROJ);
-    CK(slab_proj_launch(st, x, (int)n0, (int)n1, S, U0f, r0, U1f, r1, Z));
+    CK(slab_proj_launch(st, x, (int)n0, (int)n1, S, U0f, r0, U1f, r1, Z, shared_gpu ? 672 : 768));
   } else {
     float* Z1 = alloc<float>((size_t)r0 * n1 * S);
     if (!Z1) return HT_ERR_ALLOC;

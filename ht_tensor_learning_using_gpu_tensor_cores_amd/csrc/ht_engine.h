@@ -153,6 +153,10 @@ struct Engine {
   hipEvent_t pass_ev = nullptr;
   std::atomic<int>* pass_flag = nullptr;
   int pass_mark = 1;
+  // other decompositions run beside this one (ht_decompose_many): the slab passes, one
+  // work-group per CU for their whole length, leave an eighth of the CUs to the others'
+  // single-work-group solver kernels, which cannot share a CU with them
+  bool shared_gpu = false;
   void note_pass(double amount);
   int eig_iters_max = 0, eig_calls = 0, noconv = 0;
   int orth_fallbacks = 0;  // tall_orth calls that took the TSQR basis

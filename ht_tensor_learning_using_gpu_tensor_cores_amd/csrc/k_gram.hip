@@ -1256,14 +1256,15 @@ __global__ __launch_bounds__(256) void slab_proj_kernel(const float* __restrict_
 }
 
 int slab_proj_launch(hipStream_t st, const float* x, int n1, int n2, long S, const float* U1, int r1,
-                     const float* U2, int r2, float* z) {
+                     const float* U2, int r2, float* z, int max_blocks) {
   if (n1 < 16 || n2 < 16 || n1 % 16 || n2 % 16 || n1 > 128 || n2 > 128 || r1 < 1 || r2 < 1 || r1 > 32 || r2 > 32)
     return 8;
   if (((uintptr_t)x & 15) != 0) return 8;
   // three blocks per CU fit the LDS (U tiles and four 8.4 KB stages each)
   long waves = S;
   long blocks = (waves + 3) / 4;
-  if (blocks > 768) blocks = 768;
+  if (blocks > max_blocks) blocks = max_blocks;
+  if (blocks < 1) blocks = 1;
   dim3 grid((unsigned)blocks);
   if (r1 <= 16 && r2 <= 16)
     hipLaunchKernelGGL((slab_proj_kernel<1, 1>), grid, dim3(256), 0, st, x, n1, n2, S, U1, r1, U2, r2, z);

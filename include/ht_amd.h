@@ -201,7 +201,11 @@ ht_status ht_decompose_adaptive(ht_handle_t handle, const float* x, int32_t x_on
  * queue of its own for that (HIP's GPU_MAX_HW_QUEUES, 4 by default, counts every stream
  * of the process, torch's included); streams that share a queue run one after the other,
  * with the same results. Outputs equal those of
- * ht_decompose one tensor at a time. The work starts after what the handle's stream holds
+ * ht_decompose one tensor at a time (to rounding where, with several in flight, the slab
+ * passes of the leaves-to-root method leave an eighth of the CUs to the other decompositions
+ * and split their partial sums differently; bit for bit otherwise). Worker w starts each
+ * decomposition after the first pass over X of worker w-1's of the same round, so the
+ * workers' passes and solver phases interleave. The work starts after what the handle's stream holds
  * and the call returns when every decomposition is done; stats (optional) receives the
  * host wall time of the whole call in total_ms. No counterpart in the reference, whose
  * drivers decompose one tensor per call (third_order_tensor_decompsition/CUDA/Optimized/
