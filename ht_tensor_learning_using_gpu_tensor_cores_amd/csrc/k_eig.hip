@@ -792,7 +792,9 @@ int cast_multi_launch(hipStream_t st, const CastItem* items, int n) {
       tab.it[i] = items[s0 + i];
       mx = std::max(mx, tab.it[i].n);
     }
-    hipLaunchKernelGGL(cast_multi_kernel, dim3(std::min<unsigned>(grid_for(mx), 256), cnt), dim3(256), 0, st, tab);
+    // up to 8 blocks per CU for the largest item (a million-row basis of a 1024^3 run is
+    // 256 MB: a thread loops over it, and 256 blocks kept too few loads in flight)
+    hipLaunchKernelGGL(cast_multi_kernel, dim3(std::min<unsigned>(grid_for(mx), 2048), cnt), dim3(256), 0, st, tab);
     if (hipGetLastError() != hipSuccess) return 5;
   }
   return 0;
