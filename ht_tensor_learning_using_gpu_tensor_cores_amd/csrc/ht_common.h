@@ -235,6 +235,9 @@ int jacobi_eig_batched_launch(hipStream_t st, const double* A, int n, long strid
 // C(m x n) = op(A) op(B) with op = transpose flags; multi-block; all FP64, column-major.
 int dgemm_small_launch(hipStream_t st, int ta, int tb, int m, int n, int k, double alpha, const double* A,
                        int lda, const double* B, int ldb, double beta, double* C, int ldc);
+// The same product (no transposes) with A in FP32, widened to FP64 as it is loaded.
+int dgemm_small_f32a_launch(hipStream_t st, int m, int n, int k, double alpha, const float* A, int lda,
+                            const double* B, int ldb, double beta, double* C, int ldc);
 // The same with the k range split over work-groups (small C, long k): ws holds
 // dgemm_splitk_slices(m, n, k) * m * n doubles; the slices are summed in a fixed order.
 int dgemm_splitk_slices(int m, int n, int k);

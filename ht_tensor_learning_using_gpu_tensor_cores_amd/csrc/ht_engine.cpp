@@ -869,10 +869,7 @@ int Engine::exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U,
   eig_iters_max = std::max(eig_iters_max, it);
   ++eig_calls;
   // U = Q S (left vectors of R: R R^T S = S Th), W = Qz R^T S Th^{-1/2}.
-  double* Qd = alloc<double>((size_t)rows * b);
-  if (!Qd) return HT_ERR_ALLOC;
-  CK(cast_f32_f64_launch(st, Q, Qd, rows * b));
-  CK(dgemm_small_launch(st, 0, 0, (int)rows, kU, b, 1.0, Qd, (int)rows, Sv, b, 0.0, U, (int)rows));
+  CK(dgemm_small_f32a_launch(st, (int)rows, kU, b, 1.0, Q, (int)rows, Sv, b, 0.0, U, (int)rows));
   CK(canon_signs(U, (int)rows, kU, (int)rows));
   std::vector<double> hs(b);
   for (int i = 0; i < b; ++i) hs[i] = std::sqrt(std::max(cur[i], 0.0));
@@ -882,8 +879,7 @@ int Engine::exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U,
   if (Wr && kW > 0) {
     // V_R = R^T S diag(1/sigma) (b x kW), W = Qz V_R.
     double* RtS = alloc<double>((size_t)b * kW);
-    double* Qzd = alloc<double>((size_t)cols * b);
-    if (!RtS || !Qzd) return HT_ERR_ALLOC;
+    if (!RtS) return HT_ERR_ALLOC;
     std::vector<double> inv(b * kW, 0.0);
     for (int i = 0; i < kW; ++i) inv[i + (size_t)b * i] = hs[i] > 0 ? 1.0 / hs[i] : 0.0;
     double* D = alloc<double>((size_t)b * kW);
@@ -891,8 +887,7 @@ int Engine::exact_svd_subspace(const float* x, PMS v, int kU, int kW, double* U,
     double* RtSfull = alloc<double>((size_t)b * b);
     CK(dgemm_small_launch(st, 1, 0, b, b, b, 1.0, R, b, Sv, b, 0.0, RtSfull, b));
     CK(dgemm_small_launch(st, 0, 0, b, kW, b, 1.0, RtSfull, b, D, b, 0.0, RtS, b));
-    CK(cast_f32_f64_launch(st, Qz, Qzd, cols * b));
-    CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, b, 1.0, Qzd, (int)cols, RtS, b, 0.0, Wr, (int)cols));
+    CK(dgemm_small_f32a_launch(st, (int)cols, kW, b, 1.0, Qz, (int)cols, RtS, b, 0.0, Wr, (int)cols));
     CK(canon_signs(Wr, (int)cols, kW, (int)cols));
   }
   return sync();
@@ -1358,13 +1353,11 @@ int Engine::root_transfer_from_z() {
   const int l = p.left[0], r = p.right[0], c = rz_t;
   const int kl = p.rank[l], kr = p.rank[r], kc = p.rank[c];
   const long no = rz_rows;  // rows of the other child = rows of Z
-  double* Zd = alloc<double>((size_t)no * rz_l);
   double* W = alloc<double>((size_t)no * kc);
   double* ws = alloc<double>((size_t)dgemm_splitk_slices(kl, kr, (int)no) * kl * kr);
   Bd[0] = alloc<double>((size_t)kl * kr);
-  if (!Zd || !W || !ws || !Bd[0]) return HT_ERR_ALLOC;
-  CK(cast_f32_f64_launch(st, rz_Z, Zd, no * rz_l));
-  CK(dgemm_small_launch(st, 0, 0, (int)no, kc, rz_l, 1.0, Zd, (int)no, rz_Sv, rz_l, 0.0, W, (int)no));
+  if (!W || !ws || !Bd[0]) return HT_ERR_ALLOC;
+  CK(dgemm_small_f32a_launch(st, (int)no, kc, rz_l, 1.0, rz_Z, (int)no, rz_Sv, rz_l, 0.0, W, (int)no));
   if (c == r)
     CK(dgemm_splitk_launch(st, 1, 0, kl, kr, (int)no, 1.0, Ufull[l], (int)no, W, (int)no, 0.0, Bd[0], kl, ws));
   else
@@ -1910,19 +1903,20 @@ int Engine::rsvd_pass(const float* x) {
     CK(jacobi_eig_launch(st, M, l, l, th, Sv, l, info, 60, 1e-15));
     ++eig_calls;
 
-    // Left vectors of A: Q Sv; right vectors: Qz Rz Sv / sigma.
-    double* Qd = alloc<double>((size_t)rows * l);
+    // Left vectors of A: Q Sv (Q widened to FP64 as it is read); right vectors: Qz Rz Sv / sigma.
     Ufull[t] = alloc<double>((size_t)rows * k);
-    CK(cast_f32_f64_launch(st, Q, Qd, rows * l));
-    CK(dgemm_small_launch(st, 0, 0, (int)rows, k, l, 1.0, Qd, (int)rows, Sv, l, 0.0, Ufull[t], (int)rows));
+    if (!Ufull[t]) return HT_ERR_ALLOC;
+    CK(dgemm_small_f32a_launch(st, (int)rows, k, l, 1.0, Q, (int)rows, Sv, l, 0.0, Ufull[t], (int)rows));
     CK(canon_signs(Ufull[t], (int)rows, k, (int)rows));
     if (prec == PREC_F32 && p.parent[t] == 0 && cols <= INT32_MAX && rows <= INT32_MAX) {
       // an FP32 co-range product serves the root transfer (16-bit ones leave B_root at
       // their rounding, which small tensors do not average out: it takes its own FP32 pass)
       // U_t = Q S' with S' = Q^T U_t (l x k): Sv's first k columns with the canonical signs
+      double* Qd = alloc<double>((size_t)rows * l);
       double* Sp = alloc<double>((size_t)l * k);
       double* ws = alloc<double>((size_t)dgemm_splitk_slices(l, k, (int)rows) * l * k);
-      if (!Sp || !ws) return HT_ERR_ALLOC;
+      if (!Qd || !Sp || !ws) return HT_ERR_ALLOC;
+      CK(cast_f32_f64_launch(st, Q, Qd, rows * l));
       CK(dgemm_splitk_launch(st, 1, 0, l, k, (int)rows, 1.0, Qd, (int)rows, Ufull[t], (int)rows, 0.0, Sp, l, ws));
       rz_t = t;
       rz_l = l;
@@ -1936,14 +1930,12 @@ int Engine::rsvd_pass(const float* x) {
     SVsq[t] = 0;
     if (kW) {
       double* RS = alloc<double>((size_t)l * kW);
-      double* Zd = alloc<double>((size_t)cols * l);
       Ufull[sib] = alloc<double>((size_t)cols * kW);
-      if (!RS || !Zd || !Ufull[sib]) return HT_ERR_ALLOC;
+      if (!RS || !Ufull[sib]) return HT_ERR_ALLOC;
       HT_HIP_CHECK(hipMemcpy2DAsync(RS, sizeof(double) * l, Sv, sizeof(double) * l, sizeof(double) * l, kW,
                                     hipMemcpyDeviceToDevice, st));
       CK(scale_cols_inv_launch(st, RS, l, kW, sv));
-      CK(cast_f32_f64_launch(st, Z, Zd, cols * l));
-      CK(dgemm_small_launch(st, 0, 0, (int)cols, kW, l, 1.0, Zd, (int)cols, RS, l, 0.0, Ufull[sib], (int)cols));
+      CK(dgemm_small_f32a_launch(st, (int)cols, kW, l, 1.0, Z, (int)cols, RS, l, 0.0, Ufull[sib], (int)cols));
       CK(canon_signs(Ufull[sib], (int)cols, kW, (int)cols));
       SVd[sib] = sv;
       SVsq[sib] = 0;

@@ -195,9 +195,9 @@ int jacobi_eig_batched_launch(hipStream_t st, const double* A, int n, long strid
 constexpr int DG_T = 64, DG_K = 16;
 // Split-K: slice z = blockIdx.z covers k in [z kchunk, (z+1) kchunk) and writes its tile
 // to C + z cstride (kchunk = k, cstride = 0 without a split).
-template <int TA, int TB>
+template <int TA, int TB, typename TE = double>
 __global__ __launch_bounds__(256) void dgemm_tiled_kernel(int m, int n, int k, double alpha,
-                                                          const double* __restrict__ A, int lda,
+                                                          const TE* __restrict__ A, int lda,
                                                           const double* __restrict__ B, int ldb, double beta,
                                                           double* __restrict__ C, int ldc, int kchunk, long cstride) {
   __shared__ double As[DG_K][DG_T + 1];
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void dgemm_tiled_kernel(int m, int n, int k, d
       if (TA == 0) { ii = e % DG_T; kk = e / DG_T; } else { kk = e % DG_K; ii = e / DG_K; }
       const long gi = min(i0 + ii, (long)m - 1);
       const int gk = min(k0 + kk, ke - 1);
-      ra[q] = TA == 0 ? A[gi + (long)lda * gk] : A[gk + (long)lda * gi];
+      ra[q] = (double)(TA == 0 ? A[gi + (long)lda * gk] : A[gk + (long)lda * gi]);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -317,6 +317,17 @@ int dgemm_small_launch(hipStream_t st, int ta, int tb, int m, int n, int k, doub
   const dim3 grid((unsigned)((m + DG_T - 1) / DG_T), (unsigned)((n + DG_T - 1) / DG_T), 1);
   if (grid.y > 65535) return 8;
   dgemm_any(st, ta, tb, grid, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, k < 1 ? 1 : k, 0);
+  return hipGetLastError() == hipSuccess ? 0 : 5;
+}
+
+// A in FP32 (widened as it is loaded): C = alpha A B + beta C, A m x k column-major.
+int dgemm_small_f32a_launch(hipStream_t st, int m, int n, int k, double alpha, const float* A, int lda,
+                            const double* B, int ldb, double beta, double* C, int ldc) {
+  if (m < 1 || n < 1) return 0;
+  const dim3 grid((unsigned)((m + DG_T - 1) / DG_T), (unsigned)((n + DG_T - 1) / DG_T), 1);
+  if (grid.y > 65535) return 8;
+  hipLaunchKernelGGL((dgemm_tiled_kernel<0, 0, float>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta,
+                     C, ldc, k < 1 ? 1 : k, 0L);
   return hipGetLastError() == hipSuccess ? 0 : 5;
 }
 
