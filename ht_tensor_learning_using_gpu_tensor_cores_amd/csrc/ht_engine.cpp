@@ -898,7 +898,13 @@ int Engine::contract_transfer(const double* Ut, const double* Ul, const double* 
                               int kr, int kt, double* B) {
   double* T1 = alloc<double>((size_t)kl * nr * kt);
   if (!T1) return HT_ERR_ALLOC;
-  CK(proj_pms_f64_launch(st, Ut, PMS{1, nl, nr * kt}, Ul, kl, T1));       // (kl, nr, kt)
+  if (nl >= 512 && nr * kt >= 4096 && nr * kt <= INT32_MAX) {
+    // a tall first contraction (the million-row basis of a 1024^3 run): U_l^T times U_t
+    // viewed as nl x (nr kt), through the LDS-tiled FP64 GEMM (coalesced reads of U_t)
+    CK(dgemm_small_launch(st, 1, 0, kl, (int)(nr * kt), (int)nl, 1.0, Ul, (int)nl, Ut, (int)nl, 0.0, T1, kl));
+  } else {
+    CK(proj_pms_f64_launch(st, Ut, PMS{1, nl, nr * kt}, Ul, kl, T1));     // (kl, nr, kt)
+  }
   CK(proj_pms_f64_launch(st, T1, PMS{kl, nr, kt}, Ur, kr, B));           // (kl, kr, kt)
   return 0;
 }
