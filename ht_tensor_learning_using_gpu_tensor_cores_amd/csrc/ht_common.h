@@ -198,6 +198,8 @@ int proj_pms_f32_launch(hipStream_t st, const float* x, PMS v, const float* U, i
 int proj_pms_f64_launch(hipStream_t st, const double* x, PMS v, const double* U, int r, double* y);
 // The same with U given transposed: Ut is r x M column-major (U[m, j] = Ut[j + r m]).
 int proj_pms_f64t_launch(hipStream_t st, const double* x, PMS v, const double* Ut, int r, double* y);
+// The same with an FP32 result (FP64 products, rounded once at the store).
+int proj_pms_f64t_f32out_launch(hipStream_t st, const double* x, PMS v, const double* Ut, int r, float* y);
 // Two independent proj_pms_f64t products in one launch.
 int proj_pms_f64t_pair_launch(hipStream_t st, const double* x0, PMS v0, const double* Ut0, int r0, double* y0,
                               const double* x1, PMS v1, const double* Ut1, int r1, double* y1);

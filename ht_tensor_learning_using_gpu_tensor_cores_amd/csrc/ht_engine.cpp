@@ -974,7 +974,10 @@ int Engine::residual(const float* x, double* err_sq, double* x_sq) {
   const long nl = p.rows[l], nr = p.rows[r];
   const int kl = p.rank[l], kr = p.rank[r];
   double *Ul, *Ur;
-  // Rr = Ur B_root^T (nr x kl)
+  float* Lf = alloc<float>((size_t)nl * kl);
+  float* Rf = alloc<float>((size_t)nr * kl);
+  if (!Lf || !Rf) return HT_ERR_ALLOC;
+  // Rr = Ur B_root^T (nr x kl), in FP32 for the residual pass (Rf)
   double* Rr = nullptr;
   if (p.left[l] < 0 && p.left[r] >= 0) {
     // a leaf beside an interior node (the 1024^3 tree {1},{2,3}): B_root goes into the
@@ -989,23 +992,19 @@ int Engine::residual(const float* x, double* err_sq, double* x_sq) {
     CK(expand(rr, &Urr));
     double* Bm = alloc<double>((size_t)krl * krr * kl);
     double* T1 = alloc<double>((size_t)nrl * krr * kl);
-    Rr = alloc<double>((size_t)nr * kl);
-    if (!Bm || !T1 || !Rr) return HT_ERR_ALLOC;
+    if (!Bm || !T1) return HT_ERR_ALLOC;
     CK(dgemm_small_launch(st, 0, 1, krl * krr, kl, kr, 1.0, Bd[r], krl * krr, Bd[0], kl, 0.0, Bm, krl * krr));
     CK(proj_pms_f64t_launch(st, Bm, PMS{1, krl, (long)krr * kl}, Url, (int)nrl, T1));  // (nrl, krr, kl)
-    CK(proj_pms_f64t_launch(st, T1, PMS{nrl, krr, kl}, Urr, (int)nrr, Rr));             // (nrl, nrr, kl)
+    CK(proj_pms_f64t_f32out_launch(st, T1, PMS{nrl, krr, kl}, Urr, (int)nrr, Rf));     // (nrl, nrr, kl)
   } else {
     CK(expand2(l, r, &Ul, &Ur));
     Rr = alloc<double>((size_t)nr * kl);
     if (!Rr) return HT_ERR_ALLOC;
     CK(dgemm_small_launch(st, 0, 1, (int)nr, kl, kr, 1.0, Ur, (int)nr, Bd[0], kl, 0.0, Rr, (int)nr));
   }
-  float* Lf = alloc<float>((size_t)nl * kl);
-  float* Rf = alloc<float>((size_t)nr * kl);
-  if (!Lf || !Rf) return HT_ERR_ALLOC;
   {
     const CastItem it[2] = {{Ul, Lf, nl * kl, 0}, {Rr, Rf, nr * kl, 0}};
-    CK(cast_multi_launch(st, it, 2));
+    CK(cast_multi_launch(st, it, Rr ? 2 : 1));
   }
   const long nbx = (nl + 127) / 128, nby = (nr + 127) / 128;
   double* acc = alloc<double>((size_t)2 * nbx * nby);

@@ -684,6 +684,9 @@ int proj_pms_f64_launch(hipStream_t st, const double* x, PMS v, const double* U,
 int proj_pms_f64t_launch(hipStream_t st, const double* x, PMS v, const double* Ut, int r, double* y) {
   return proj_launch(st, x, v, Ut, r, 1, r, y);
 }
+int proj_pms_f64t_f32out_launch(hipStream_t st, const double* x, PMS v, const double* Ut, int r, float* y) {
+  return proj_launch(st, x, v, Ut, r, 1, r, y);
+}
 int proj_pms_f64t_pair_launch(hipStream_t st, const double* x0, PMS v0, const double* Ut0, int r0, double* y0,
                               const double* x1, PMS v1, const double* Ut1, int r1, double* y1) {
   ProjJob j[2] = {{x0, v0, Ut0, r0, 1, r0, y0, 0, 0}, {x1, v1, Ut1, r1, 1, r1, y1, 0, 0}};
